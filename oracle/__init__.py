@@ -1,0 +1,272 @@
+"""CPU oracle for the deep-zoom perturbation path -- TEST INFRASTRUCTURE ONLY.
+
+A ctypes binding to ``liboracle.so`` built from ``fdo.c``: a plain-C restatement of
+fractodactyl's CPU renderer (crates/fd-kernel/src/{grid,sample,bla,reference,view,store}.rs,
+crates/fd-fixed, crates/fd-samples).  Only tests, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may use it, and only as the checker or the timed CPU
+baseline; the product library never routes through it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+COL_CLASS, COL_NU, COL_DE, COL_NORMAL = 1, 2, 4, 8
+TIER_AUTO, TIER_F64, TIER_FIXED, TIER_SCALED = -1, 0, 1, 2
+ESCAPED, INTERIOR, UNRESOLVED = 0, 1, 2
+EPS_EXACT = 1.0 / 1125899906842624.0  # 2^-50, fd-cli/src/orbit.rs:16
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+class _Plane(C.Structure):
+    _fields_ = [("c_re", C.c_double), ("c_im", C.c_double), ("h_m", C.c_double),
+                ("h_e", C.c_int64), ("tier", C.c_int32), ("bits", C.c_uint64),
+                ("cos_r", C.c_double), ("sin_r", C.c_double), ("half_x", C.c_double),
+                ("half_y", C.c_double)]
+
+
+class _Stats(C.Structure):
+    _fields_ = [("reference_len", C.c_uint64), ("iterations", C.c_uint64),
+                ("equivalent", C.c_uint64), ("blocks", C.c_uint64), ("skipped", C.c_uint64),
+                ("fallback_samples", C.c_uint64), ("closed_form_samples", C.c_uint64),
+                ("bla_levels", C.c_uint32), ("reference_seconds", C.c_double),
+                ("bla_seconds", C.c_double), ("render_seconds", C.c_double)]
+
+
+class _Outcomes(C.Structure):
+    _fields_ = [("kind", C.c_void_p), ("n", C.c_void_p), ("zr", C.c_void_p),
+                ("zi", C.c_void_p), ("dr", C.c_void_p), ("di", C.c_void_p),
+                ("dexp", C.c_void_p)]
+
+
+class _Columns(C.Structure):
+    _fields_ = [("class_", C.c_void_p), ("nu", C.c_void_p), ("de", C.c_void_p),
+                ("normal", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.fdo_exp2i.restype = C.c_double
+        L.fdo_exp2i.argtypes = [C.c_int64]
+        L.fdo_plane_new.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_double, C.c_uint32,
+                                    C.c_uint32, C.POINTER(_Plane), C.c_char_p, C.c_size_t]
+        L.fdo_reference.restype = C.c_int64
+        L.fdo_reference.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_double, C.c_uint32,
+                                    C.c_uint32, C.c_uint64, C.c_double, C.c_int, C.c_void_p,
+                                    C.c_void_p, C.c_uint64, C.POINTER(C.c_uint32), C.c_char_p,
+                                    C.c_size_t]
+        L.fdo_reference_fixed.restype = C.c_int64
+        L.fdo_reference_fixed.argtypes = [C.c_char_p, C.c_char_p, C.c_uint64, C.c_uint64,
+                                          C.c_void_p, C.c_void_p, C.c_uint64]
+        L.fdo_nucleus_period.restype = C.c_uint64
+        L.fdo_nucleus_period.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
+        L.fdo_bla_build.restype = C.c_int
+        L.fdo_bla_build.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_double, C.c_double,
+                                    C.c_int64, C.c_void_p, C.c_uint64]
+        L.fdo_render.restype = C.c_int
+        L.fdo_render.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_double, C.c_uint32,
+                                 C.c_uint32, C.c_uint32, C.c_uint64, C.c_double, C.c_uint32,
+                                 C.c_int, C.c_int, C.c_double, C.c_int, C.c_void_p,
+                                 C.POINTER(_Outcomes), C.POINTER(_Columns), C.POINTER(_Stats),
+                                 C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t]
+        L.fdo_sample.restype = C.c_int
+        L.fdo_sample.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int, C.c_int,
+                                 C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                 C.c_int64, C.c_int, C.c_uint64, C.c_double, C.c_uint64,
+                                 C.POINTER(C.c_uint64), C.c_void_p, C.c_void_p,
+                                 C.POINTER(C.c_int64), C.POINTER(C.c_uint64),
+                                 C.POINTER(C.c_uint64)]
+        L.fdo_write_fds.restype = C.c_uint64
+        L.fdo_write_fds.argtypes = [C.c_void_p, C.c_uint64, C.c_char_p, C.c_char_p, C.c_char_p,
+                                    C.c_double, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
+                                    C.c_double, C.c_uint32, C.c_char_p, C.POINTER(_Columns)]
+        L.fdo_bla_find_many.restype = C.c_int
+        L.fdo_bla_find_many.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_double,
+                                        C.c_double, C.c_int64, C.c_void_p, C.c_uint64,
+                                        C.c_double, C.c_uint64, C.c_void_p]
+        L.fdo_fixed_parse.restype = C.c_int
+        L.fdo_fixed_parse.argtypes = [C.c_char_p, C.c_uint64, C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def exp2i(k):
+    return lib().fdo_exp2i(k)
+
+
+def plane(re, im, width, rotation, nx, ny):
+    p = _Plane()
+    err = C.create_string_buffer(256)
+    if lib().fdo_plane_new(re.encode(), im.encode(), width.encode(), rotation, nx, ny,
+                           C.byref(p), err, 256) != 0:
+        raise ValueError(err.value.decode())
+    return {k: getattr(p, k) for k, _ in _Plane._fields_}
+
+
+def reference(re, im, width, rotation, nx, ny, max_iter, escape_radius, tier=TIER_AUTO):
+    """grid.rs `reference`: (re, im, bits) of the orbit the CPU renderer uses."""
+    err = C.create_string_buffer(256)
+    bits = C.c_uint32()
+    n = lib().fdo_reference(re.encode(), im.encode(), width.encode(), rotation, nx, ny, max_iter,
+                            escape_radius, tier, None, None, 0, C.byref(bits), err, 256)
+    if n < 0:
+        raise ValueError(err.value.decode())
+    zr = np.zeros(n, np.float64)
+    zi = np.zeros(n, np.float64)
+    lib().fdo_reference(re.encode(), im.encode(), width.encode(), rotation, nx, ny, max_iter,
+                        escape_radius, tier, _p(zr), _p(zi), n, C.byref(bits), err, 256)
+    return zr, zi, bits.value
+
+
+def reference_fixed(re, im, bits, max_iter):
+    cap = min(max_iter, (1 << 20) - 1) + 1
+    zr = np.zeros(cap, np.float64)
+    zi = np.zeros(cap, np.float64)
+    n = lib().fdo_reference_fixed(re.encode(), im.encode(), bits, max_iter, _p(zr), _p(zi), cap)
+    if n < 0:
+        raise ValueError("bad decimal")
+    return zr[:n].copy(), zi[:n].copy()
+
+
+def nucleus_period(zr, zi):
+    zr = np.ascontiguousarray(zr, np.float64)
+    zi = np.ascontiguousarray(zi, np.float64)
+    return int(lib().fdo_nucleus_period(_p(zr), _p(zi), len(zr)))
+
+
+def bla_build(zr, zi, eps, dc_max, dc_exp=0):
+    """Bla::build_scaled: list of (count, 7) float64 arrays, one per level 1, 2, ..."""
+    zr = np.ascontiguousarray(zr, np.float64)
+    zi = np.ascontiguousarray(zi, np.float64)
+    steps = max(len(zr) - 2, 0)
+    total = sum(steps >> j for j in range(1, 64))
+    out = np.zeros((max(total, 1), 7), np.float64)
+    nl = lib().fdo_bla_build(_p(zr), _p(zi), len(zr), eps, dc_max, dc_exp, _p(out), total)
+    if nl < 0:
+        raise ValueError("BLA needs 0 < eps <= 2^-24 and a finite dc_max > 0")
+    levels, off = [], 0
+    for j in range(1, nl + 1):
+        c = steps >> j
+        levels.append(out[off:off + c].copy())
+        off += c
+    return levels
+
+
+def sample(zr, zi, ar, ai, max_iter, r2, c=None, extended=False, offset_exp=0,
+           derivative=True, bla_levels=None, nucleus=0):
+    """sample.rs iterate for one sample; returns a dict with the outcome."""
+    zr = np.ascontiguousarray(zr, np.float64)
+    zi = np.ascontiguousarray(zi, np.float64)
+    blocks = None
+    nl = 0
+    if bla_levels:
+        blocks = np.ascontiguousarray(np.concatenate(bla_levels, axis=0))
+        nl = len(bla_levels)
+    n = C.c_uint64()
+    z = np.zeros(2)
+    d = np.zeros(2)
+    dexp = C.c_int64()
+    nb = C.c_uint64()
+    ns = C.c_uint64()
+    kind = lib().fdo_sample(_p(zr), _p(zi), len(zr), _p(blocks), nl, c is not None,
+                            c[0] if c else 0.0, c[1] if c else 0.0, ar, ai, int(extended),
+                            offset_exp, int(derivative), max_iter, r2, nucleus, C.byref(n),
+                            _p(z), _p(d), C.byref(dexp), C.byref(nb), C.byref(ns))
+    return dict(kind=kind, n=n.value, zr=z[0], zi=z[1], dr=d[0], di=d[1], dexp=dexp.value,
+                blocks=nb.value, skipped=ns.value)
+
+
+def render(re, im, width, rotation, nx, ny, max_iter, escape_radius=1e10,
+           columns=COL_NU | COL_DE | COL_NORMAL, ss=1, tier=TIER_AUTO, bla=False,
+           bla_eps=EPS_EXACT, threads=None, mask=None, outcomes=True):
+    """grid.rs render_with / render_bla (table built per frame as `fd control --bla
+    per-frame` does, control.rs:356-366).  Returns a dict of numpy columns, outcomes,
+    stats and the header kernel id."""
+    n = nx * ny
+    threads = threads or os.cpu_count() or 1
+    cols = {"class": np.zeros(n, np.uint8)}
+    if columns & COL_NU:
+        cols["nu"] = np.zeros(n, np.float64)
+    if columns & COL_DE:
+        cols["de"] = np.zeros(n, np.float32)
+    if columns & COL_NORMAL:
+        cols["normal"] = np.zeros(n, np.uint16)
+    cs = _Columns(_p(cols["class"]), _p(cols.get("nu")), _p(cols.get("de")),
+                  _p(cols.get("normal")))
+    oc = None
+    res = {}
+    if outcomes:
+        res = {"kind": np.zeros(n, np.uint8), "n": np.zeros(n, np.uint64),
+               "zr": np.zeros(n), "zi": np.zeros(n), "dr": np.zeros(n), "di": np.zeros(n),
+               "dexp": np.zeros(n, np.int64)}
+        oc = _Outcomes(*[_p(res[k]) for k in ("kind", "n", "zr", "zi", "dr", "di", "dexp")])
+    st = _Stats()
+    kid = C.create_string_buffer(128)
+    err = C.create_string_buffer(256)
+    mk = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+    rc = lib().fdo_render(re.encode(), im.encode(), width.encode(), rotation, nx, ny, ss,
+                          max_iter, escape_radius, columns, tier, int(bla), bla_eps, threads,
+                          _p(mk), C.byref(oc) if oc is not None else None, C.byref(cs),
+                          C.byref(st), kid, 128, err, 256)
+    if rc != 0:
+        raise ValueError(err.value.decode())
+    return dict(columns=cols, outcomes=res, kernel=kid.value.decode(),
+                stats={k: getattr(st, k) for k, _ in _Stats._fields_},
+                view=(re, im, width, rotation), grid=(nx, ny, ss), max_iter=max_iter,
+                escape_radius=escape_radius, colmask=columns | COL_CLASS)
+
+
+def write_fds(frame, minor=1):
+    """fd_samples::write of a render() result; minor rewrites bytes 10..12."""
+    re, im, width, rot = frame["view"]
+    nx, ny, ss = frame["grid"]
+    cols = frame["columns"]
+    cs = _Columns(_p(cols["class"]), _p(cols.get("nu")), _p(cols.get("de")),
+                  _p(cols.get("normal")))
+    args = (re.encode(), im.encode(), width.encode(), rot, nx, ny, ss, frame["max_iter"],
+            frame["escape_radius"], frame["colmask"], frame["kernel"].encode(), C.byref(cs))
+    size = lib().fdo_write_fds(None, 0, *args)
+    buf = np.zeros(size, np.uint8)
+    lib().fdo_write_fds(_p(buf), size, *args)
+    b = bytearray(buf.tobytes())
+    b[10:12] = int(minor).to_bytes(2, "little")
+    return bytes(b)
+
+
+def bla_find(zr, zi, eps, dc_max, ms, d2, budget, dc_exp=0):
+    """Bla::find at each m: the block length (2 << level), or None."""
+    zr = np.ascontiguousarray(zr, np.float64)
+    zi = np.ascontiguousarray(zi, np.float64)
+    ms = np.ascontiguousarray(ms, np.uint64)
+    out = np.zeros(len(ms), np.int32)
+    if lib().fdo_bla_find_many(_p(zr), _p(zi), len(zr), eps, dc_max, dc_exp, _p(ms), len(ms),
+                               d2, min(int(budget), (1 << 64) - 1), _p(out)) < 0:
+        raise ValueError("bad eps / dc_max")
+    return [None if v < 0 else 2 << int(v) for v in out]
+
+
+def fixed_parse(s, limbs):
+    """Fixed::parse: (neg, little-endian limbs)."""
+    out = np.zeros(limbs, np.uint64)
+    neg = lib().fdo_fixed_parse(s.encode(), limbs, _p(out))
+    if neg < 0:
+        raise ValueError("not a decimal number / too large: %r" % s)
+    return bool(neg), [int(v) for v in out]
