@@ -1,0 +1,346 @@
+"""fractodactyl_amd: fractodactyl's deep-zoom perturbation/BLA path on AMD MI355X.
+
+A ctypes mirror of the C ABI in ``include/fdg.h`` (``libfdg.so``, built in-tree by
+``python -m fractodactyl_amd.build``).  The API follows the reference's GPU route and
+sample API:
+
+* :func:`enabled` -- ``perturb_gpu::enabled`` (crates/fd-cli/src/perturb_gpu/frame.rs:62-68)
+* :func:`render` / :func:`render_bla` -- ``perturb_gpu::render`` / ``render_bla``
+  (frame.rs:70-84): ``(header, samples, stats)`` for a :class:`View` and :class:`Params`
+* :func:`reference` -- ``fd_kernel::reference`` (crates/fd-kernel/src/grid.rs:84-87)
+* :func:`build_bla` -- ``Bla::build_scaled`` (crates/fd-kernel/src/bla.rs:161-181), on the GPU
+* :func:`write` -- ``fd_samples::write`` (crates/fd-samples/src/write.rs:8-57)
+
+There is no CPU fallback: without the HIP library or a device these raise.
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfdg.so")
+
+COL_CLASS, COL_NU, COL_DE, COL_NORMAL, COL_BOUND, COL_STRIPE = 1, 2, 4, 8, 16, 32
+TIER_AUTO, TIER_F64, TIER_FIXED, TIER_SCALED = -1, 0, 1, 2
+ROUTE_PLAIN, ROUTE_BLA = 0, 1
+FORM_F64 = 0
+ESCAPED, INTERIOR, UNRESOLVED = 0, 1, 2
+EPS = 1.0 / 1125899906842624.0          # 2^-50, crates/fd-cli/src/orbit.rs:16
+FILM_EPS = 1.0 / (1 << 26)              # orbit.rs:22
+ERRORS = {-1: "invalid request", -2: "declined", -3: "device failure", -4: "out of memory"}
+
+
+class FdgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+@dataclass
+class View:
+    """fd_samples::View (crates/fd-samples/src/header.rs:263-273)."""
+    center_re: str
+    center_im: str
+    width: str
+    rotation: float = 0.0
+
+
+@dataclass
+class Params:
+    """fd_kernel::Params (crates/fd-kernel/src/grid.rs:17-38)."""
+    nx: int
+    ny: int
+    ss: int = 1
+    max_iter: int = 100_000
+    escape_radius: float = 1e10
+    columns: int = COL_NU | COL_DE | COL_NORMAL
+    tier: int = TIER_AUTO
+
+
+class _View(C.Structure):
+    _fields_ = [("center_re", C.c_char_p), ("center_im", C.c_char_p), ("width", C.c_char_p),
+                ("rotation", C.c_double)]
+
+
+class _Params(C.Structure):
+    _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32), ("ss", C.c_uint32),
+                ("max_iter", C.c_uint64), ("escape_radius", C.c_double),
+                ("columns", C.c_uint32), ("tier", C.c_int32)]
+
+
+class _Columns(C.Structure):
+    _fields_ = [("class_", C.c_void_p), ("nu", C.c_void_p), ("de", C.c_void_p),
+                ("normal", C.c_void_p)]
+
+
+class _Outcomes(C.Structure):
+    _fields_ = [("kind", C.c_void_p), ("n", C.c_void_p), ("zr", C.c_void_p),
+                ("zi", C.c_void_p), ("dr", C.c_void_p), ("di", C.c_void_p),
+                ("dexp", C.c_void_p)]
+
+
+class _Stats(C.Structure):
+    _fields_ = [("reference_len", C.c_uint64), ("reference_bits", C.c_uint32),
+                ("bla_levels", C.c_uint32), ("nucleus_period", C.c_uint64),
+                ("iterations", C.c_uint64), ("equivalent", C.c_uint64),
+                ("blocks", C.c_uint64), ("skipped", C.c_uint64), ("plain_steps", C.c_uint64),
+                ("fallback_samples", C.c_uint64), ("closed_form_samples", C.c_uint64),
+                ("reference_seconds", C.c_double), ("bla_seconds", C.c_double),
+                ("kernel_seconds", C.c_double), ("transfer_seconds", C.c_double),
+                ("encode_seconds", C.c_double)]
+
+
+# Every entry point of include/fdg.h with its ctypes signature.
+SIGNATURES = {
+    "fdg_abi_version": (C.c_int, []),
+    "fdg_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
+    "fdg_destroy": (None, [C.c_void_p]),
+    "fdg_last_error": (C.c_char_p, [C.c_void_p]),
+    "fdg_enabled": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "fdg_render": (C.c_int, [C.c_void_p, C.POINTER(_View), C.POINTER(_Params), C.c_int, C.c_int,
+                             C.c_double, C.POINTER(_Columns), C.POINTER(_Outcomes),
+                             C.POINTER(_Stats), C.c_char_p, C.c_size_t]),
+    "fdg_reference": (C.c_int, [C.c_void_p, C.POINTER(_View), C.POINTER(_Params), C.c_void_p,
+                                C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64),
+                                C.POINTER(C.c_uint32)]),
+    "fdg_build_bla": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_double,
+                                C.c_double, C.c_int64, C.c_void_p, C.c_uint64,
+                                C.POINTER(C.c_uint32)]),
+    "fdg_frame_open": (C.c_int, [C.c_void_p, C.POINTER(_View), C.POINTER(_Params), C.c_int,
+                                 C.c_int, C.c_double, C.POINTER(C.c_void_p)]),
+    "fdg_frame_run": (C.c_int, [C.c_void_p, C.POINTER(_Stats)]),
+    "fdg_frame_read": (C.c_int, [C.c_void_p, C.POINTER(_Columns), C.POINTER(_Outcomes),
+                                 C.POINTER(_Stats), C.c_char_p, C.c_size_t]),
+    "fdg_frame_close": (None, [C.c_void_p]),
+    "fdg_write_fds": (C.c_int64, [C.POINTER(_View), C.POINTER(_Params), C.c_char_p,
+                                  C.POINTER(_Columns), C.c_void_p, C.c_uint64]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded libfdg.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FdgError(-3, f"{LIB_PATH} is missing: run python -m fractodactyl_amd.build")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _view(v):
+    return _View(v.center_re.encode(), v.center_im.encode(), v.width.encode(), float(v.rotation))
+
+
+def _params(p):
+    return _Params(p.nx, p.ny, p.ss, p.max_iter, p.escape_radius, p.columns, p.tier)
+
+
+class Context:
+    """One HIP device, one stream and its resident buffers (fdg_create/fdg_destroy)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        rc = lib().fdg_create(C.byref(h), device)
+        if rc != 0:
+            raise FdgError(rc, lib().fdg_last_error(None).decode())
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fdg_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc < 0:
+            raise FdgError(rc, lib().fdg_last_error(self._h).decode())
+        return rc
+
+
+_default = None
+
+
+def default_context():
+    global _default
+    if _default is None:
+        _default = Context(0)
+    return _default
+
+
+def enabled(value):
+    """perturb_gpu::enabled: ``(on, route)`` for "on" / "bla" / "off" / None."""
+    e, r = C.c_int(0), C.c_int(0)
+    if lib().fdg_enabled(None if value is None else value.encode(), C.byref(e), C.byref(r)) != 0:
+        raise ValueError(f"--gpu-perturb: expected on, bla or off, got {value!r}")
+    return bool(e.value), r.value
+
+
+def render(view, params, route=ROUTE_PLAIN, form=FORM_F64, bla_eps=EPS, ctx=None, raw=False):
+    """perturb_gpu::render (frame.rs:70-76): ``(header, samples, stats)``.
+
+    ``header`` is a dict of the .fds header fields, ``samples`` a dict of numpy columns
+    (``class`` always; ``nu``/``de``/``normal`` as requested) plus the raw outcomes under
+    ``"raw"`` when ``raw`` is set."""
+    ctx = ctx or default_context()
+    n = params.nx * params.ny
+    cols = {"class": np.zeros(n, np.uint8)}
+    if params.columns & COL_NU:
+        cols["nu"] = np.zeros(n, np.float64)
+    if params.columns & COL_DE:
+        cols["de"] = np.zeros(n, np.float32)
+    if params.columns & COL_NORMAL:
+        cols["normal"] = np.zeros(n, np.uint16)
+    cs = _Columns(_p(cols["class"]), _p(cols.get("nu")), _p(cols.get("de")),
+                  _p(cols.get("normal")))
+    oc = None
+    if raw:
+        rw = {"kind": np.zeros(n, np.uint8), "n": np.zeros(n, np.uint64), "zr": np.zeros(n),
+              "zi": np.zeros(n), "dr": np.zeros(n), "di": np.zeros(n),
+              "dexp": np.zeros(n, np.int64)}
+        oc = _Outcomes(*[_p(rw[k]) for k in ("kind", "n", "zr", "zi", "dr", "di", "dexp")])
+        cols["raw"] = rw
+    st = _Stats()
+    kid = C.create_string_buffer(256)
+    ctx._check(lib().fdg_render(ctx._h, C.byref(_view(view)), C.byref(_params(params)), route,
+                                form, bla_eps, C.byref(cs),
+                                C.byref(oc) if oc is not None else None, C.byref(st), kid, 256))
+    header = dict(view=view, params=params, kernel=kid.value.decode(),
+                  columns=params.columns | COL_CLASS)
+    return header, cols, {k: getattr(st, k) for k, _ in _Stats._fields_}
+
+
+class ResidentFrame:
+    """A frame whose orbit, table and outcomes stay in HBM between passes
+    (fdg_frame_open / run / read / close)."""
+
+    def __init__(self, view, params, route=ROUTE_BLA, form=FORM_F64, bla_eps=EPS, ctx=None):
+        self.ctx = ctx or default_context()
+        self.view, self.params = view, params
+        self._v, self._p = _view(view), _params(params)
+        h = C.c_void_p()
+        self.ctx._check(lib().fdg_frame_open(self.ctx._h, C.byref(self._v), C.byref(self._p),
+                                             route, form, bla_eps, C.byref(h)))
+        self._h = h
+
+    def run(self):
+        """One pass of the hot path (table build + perturbation kernel); returns stats."""
+        st = _Stats()
+        self.ctx._check(lib().fdg_frame_run(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in _Stats._fields_}
+
+    def read(self, raw=False):
+        n = self.params.nx * self.params.ny
+        p = self.params
+        cols = {"class": np.zeros(n, np.uint8)}
+        if p.columns & COL_NU:
+            cols["nu"] = np.zeros(n, np.float64)
+        if p.columns & COL_DE:
+            cols["de"] = np.zeros(n, np.float32)
+        if p.columns & COL_NORMAL:
+            cols["normal"] = np.zeros(n, np.uint16)
+        cs = _Columns(_p(cols["class"]), _p(cols.get("nu")), _p(cols.get("de")),
+                      _p(cols.get("normal")))
+        oc = None
+        if raw:
+            rw = {"kind": np.zeros(n, np.uint8), "n": np.zeros(n, np.uint64),
+                  "zr": np.zeros(n), "zi": np.zeros(n), "dr": np.zeros(n), "di": np.zeros(n),
+                  "dexp": np.zeros(n, np.int64)}
+            oc = _Outcomes(*[_p(rw[k]) for k in ("kind", "n", "zr", "zi", "dr", "di", "dexp")])
+            cols["raw"] = rw
+        st = _Stats()
+        kid = C.create_string_buffer(256)
+        self.ctx._check(lib().fdg_frame_read(self._h, C.byref(cs),
+                                             C.byref(oc) if oc is not None else None,
+                                             C.byref(st), kid, 256))
+        header = dict(view=self.view, params=p, kernel=kid.value.decode(),
+                      columns=p.columns | COL_CLASS)
+        return header, cols, {k: getattr(st, k) for k, _ in _Stats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fdg_frame_close(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def render_bla(view, params, form=FORM_F64, bla_eps=EPS, ctx=None, raw=False):
+    """perturb_gpu::render_bla (frame.rs:78-84)."""
+    return render(view, params, ROUTE_BLA, form, bla_eps, ctx, raw)
+
+
+def reference(view, params, ctx=None):
+    """fd_kernel::reference: ``(re, im, bits)`` of the orbit the renderer uses."""
+    ctx = ctx or default_context()
+    ln, bits = C.c_uint64(), C.c_uint32()
+    v, p = _view(view), _params(params)
+    ctx._check(lib().fdg_reference(ctx._h, C.byref(v), C.byref(p), None, None, 0, C.byref(ln),
+                                   C.byref(bits)))
+    re = np.zeros(ln.value)
+    im = np.zeros(ln.value)
+    ctx._check(lib().fdg_reference(ctx._h, C.byref(v), C.byref(p), _p(re), _p(im), ln.value,
+                                   C.byref(ln), C.byref(bits)))
+    return re, im, bits.value
+
+
+def build_bla(re, im, eps, dc_max, dc_exp=0, ctx=None):
+    """Bla::build_scaled on the GPU: a list of (count, 7) arrays for levels 1, 2, ..."""
+    ctx = ctx or default_context()
+    re = np.ascontiguousarray(re, np.float64)
+    im = np.ascontiguousarray(im, np.float64)
+    steps = max(len(re) - 2, 0)
+    total = sum(steps >> j for j in range(1, 64))
+    out = np.zeros((max(total, 1), 7))
+    nl = C.c_uint32()
+    ctx._check(lib().fdg_build_bla(ctx._h, _p(re), _p(im), len(re), eps, dc_max, dc_exp,
+                                   _p(out), total, C.byref(nl)))
+    levels, off = [], 0
+    for j in range(1, nl.value + 1):
+        c = steps >> j
+        levels.append(out[off:off + c].copy())
+        off += c
+    return levels
+
+
+def write(header, samples, minor=1):
+    """fd_samples::write: the canonical .fds bytes (``minor`` rewrites bytes 10..12)."""
+    v, p = _view(header["view"]), _params(header["params"])
+    cs = _Columns(_p(samples["class"]), _p(samples.get("nu")), _p(samples.get("de")),
+                  _p(samples.get("normal")))
+    kid = header["kernel"].encode()
+    size = lib().fdg_write_fds(C.byref(v), C.byref(p), kid, C.byref(cs), None, 0)
+    if size < 0:
+        raise FdgError(size, "write failed")
+    buf = np.zeros(size, np.uint8)
+    lib().fdg_write_fds(C.byref(v), C.byref(p), kid, C.byref(cs), _p(buf), size)
+    b = bytearray(buf.tobytes())
+    b[10:12] = int(minor).to_bytes(2, "little")
+    return bytes(b)

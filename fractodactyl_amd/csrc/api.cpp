@@ -1,0 +1,627 @@
+// C ABI (include/fdg.h): frame setup on the host, resident device buffers, kernel
+// launches on one HIP stream, and the column encoding.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/fdg.h"
+#include "host.hpp"
+#include "kernels.hpp"
+
+namespace fdg {
+hipError_t launch_bla_build(const double* orbit, double* r, double* r2, double* ab, double* alpha,
+                            double* beta, uint32_t* valid, uint64_t steps, uint64_t stride,
+                            double eps, double dc_max, int64_t dc_exp, uint32_t max_levels,
+                            hipStream_t stream);
+hipError_t launch_perturb_f64(const FrameDev& f, bool ext, bool deriv, bool bla, int grid,
+                              hipStream_t stream);
+}  // namespace fdg
+
+using namespace fdg;
+
+namespace {
+
+constexpr uint32_t kMaxLevels = 48;
+constexpr double kEpsLimit = 1.0 / 16777216.0;  // bla.rs:288-292 EPS_LIMIT = 2^-24
+
+std::string g_create_error;
+
+// A grow-only device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= bytes) return hipSuccess;
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+}  // namespace
+
+struct fdg_ctx {
+    int device = 0;
+    int cus = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::string err;
+
+    int fail(int code, const std::string& m) {
+        err = m;
+        return code;
+    }
+    int hip(hipError_t e, const char* what) {
+        if (e == hipSuccess) return FDG_OK;
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? FDG_ENOMEM : FDG_EDEVICE;
+    }
+};
+
+namespace {
+// Device residency of one frame: the orbit, its BLA table and the per-sample outcomes.
+struct Buffers {
+    DevBuf orbit, bla_r, bla_r2, bla_ab, bla_alpha, bla_beta, bla_valid;
+    DevBuf o_kind, o_n, o_zr, o_zi, o_dr, o_di, o_dexp, counters;
+    uint64_t bla_stride = 0, bla_steps = 0;
+    void release() {
+        for (DevBuf* b : {&orbit, &bla_r, &bla_r2, &bla_ab, &bla_alpha, &bla_beta, &bla_valid,
+                          &o_kind, &o_n, &o_zr, &o_zi, &o_dr, &o_di, &o_dexp, &counters})
+            b->release();
+    }
+    int outputs(fdg_ctx* ctx, size_t n);
+};
+}  // namespace
+
+#define FDG_TRY(ctx, call, what)                       \
+    do {                                               \
+        int rc_ = (ctx)->hip((call), (what));          \
+        if (rc_ != FDG_OK) return rc_;                 \
+    } while (0)
+
+namespace {
+
+struct Frame {
+    Plane pl;
+    Tier tier;
+};
+
+int Buffers::outputs(fdg_ctx* ctx, size_t n) {
+    FDG_TRY(ctx, o_kind.ensure(n), "hipMalloc");
+    FDG_TRY(ctx, o_n.ensure(n * 8), "hipMalloc");
+    FDG_TRY(ctx, o_zr.ensure(n * 8), "hipMalloc");
+    FDG_TRY(ctx, o_zi.ensure(n * 8), "hipMalloc");
+    FDG_TRY(ctx, o_dr.ensure(n * 8), "hipMalloc");
+    FDG_TRY(ctx, o_di.ensure(n * 8), "hipMalloc");
+    FDG_TRY(ctx, o_dexp.ensure(n * 8), "hipMalloc");
+    FDG_TRY(ctx, counters.ensure(256), "hipMalloc");
+    return FDG_OK;
+}
+
+int setup(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, Frame* fr) {
+    if (!v || !p || !v->center_re || !v->center_im || !v->width)
+        return ctx->fail(FDG_EINVAL, "null view or params");
+    // grid.rs:779-799 setup
+    if (p->ss == 0 || p->nx % p->ss != 0 || p->ny % p->ss != 0)
+        return ctx->fail(FDG_EINVAL, "grid must be a whole number of pixels");
+    std::string e;
+    if (!Plane::make(v->center_re, v->center_im, v->width, v->rotation, p->nx, p->ny, &fr->pl, &e))
+        return ctx->fail(FDG_EINVAL, e);
+    Tier t = p->tier < 0 ? fr->pl.tier : (Tier)p->tier;
+    if (p->tier > 2) return ctx->fail(FDG_EINVAL, "unknown tier");
+    bool allows = t == TIER_F64 ? fr->pl.tier == TIER_F64
+                                : (t == TIER_FIXED ? fr->pl.tier != TIER_SCALED : true);
+    if (!allows) return ctx->fail(FDG_EINVAL, "tier is not valid at this depth");
+    fr->tier = t;
+    return FDG_OK;
+}
+
+int compute_orbit(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, const Frame& fr,
+                  Orbit* o) {
+    // grid.rs:802-815 compute
+    if (fr.tier == TIER_F64) {
+        *o = Orbit::f64(fr.pl.c_re, fr.pl.c_im, p->max_iter, p->escape_radius);
+        return FDG_OK;
+    }
+    if (!Orbit::fixed(v->center_re, v->center_im, fr.pl.bits, p->max_iter, o))
+        return ctx->fail(FDG_EINVAL, "bad centre for the fixed-point reference");
+    return FDG_OK;
+}
+
+uint64_t next_pow2_host(uint64_t x) {
+    uint64_t s = 1;
+    while (s < x) s <<= 1;
+    return s;
+}
+
+// Upload the orbit interleaved (re, im) and build the BLA table on the GPU.  Returns the
+// level count (0 when no level has a valid block) in *levels.
+int build_table(fdg_ctx* ctx, Buffers* bf, uint64_t len, double eps, double dc_max,
+                int64_t dc_exp, uint32_t* levels, double* seconds) {
+    Buffers& B = *bf;
+    if (!(eps > 0.0 && eps <= kEpsLimit && dc_max > 0.0 && std::isfinite(dc_max)))
+        return ctx->fail(FDG_EINVAL,
+                         "BLA needs 0 < eps <= 2^-24 and a finite dc_max > 0");
+    uint64_t steps = len >= 2 ? len - 2 : 0;
+    uint64_t stride = next_pow2_host(std::max<uint64_t>(steps, 2));
+    B.bla_stride = stride;
+    B.bla_steps = steps;
+    *levels = 0;
+    if (steps < 2) return FDG_OK;
+    FDG_TRY(ctx, B.bla_r.ensure(stride * 8), "hipMalloc");
+    FDG_TRY(ctx, B.bla_r2.ensure(stride * 8), "hipMalloc");
+    FDG_TRY(ctx, B.bla_ab.ensure(stride * 32), "hipMalloc");
+    FDG_TRY(ctx, B.bla_alpha.ensure(stride * 8), "hipMalloc");
+    FDG_TRY(ctx, B.bla_beta.ensure(stride * 8), "hipMalloc");
+    FDG_TRY(ctx, B.bla_valid.ensure(kMaxLevels * 4), "hipMalloc");
+    FDG_TRY(ctx, hipMemsetAsync(B.bla_valid.p, 0, kMaxLevels * 4, ctx->stream), "memset");
+    FDG_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream), "event");
+    FDG_TRY(ctx,
+            launch_bla_build(B.orbit.as<double>(), B.bla_r.as<double>(),
+                             B.bla_r2.as<double>(), B.bla_ab.as<double>(),
+                             B.bla_alpha.as<double>(), B.bla_beta.as<double>(),
+                             B.bla_valid.as<uint32_t>(), steps, stride, eps, dc_max, dc_exp,
+                             kMaxLevels, ctx->stream),
+            "BLA build launch");
+    FDG_TRY(ctx, hipEventRecord(ctx->ev[1], ctx->stream), "event");
+    uint32_t valid[kMaxLevels];
+    FDG_TRY(ctx,
+            hipMemcpyAsync(valid, B.bla_valid.p, sizeof(valid), hipMemcpyDeviceToHost,
+                           ctx->stream),
+            "readback");
+    FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "BLA build");
+    float ms = 0;
+    hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+    *seconds = ms * 1e-3;
+    // bla.rs:442-445: levels end before the first level with no valid block.
+    uint32_t n = 0;
+    while (n < kMaxLevels && (steps >> (n + 1)) > 0 && valid[n]) n++;
+    *levels = n;
+    return FDG_OK;
+}
+
+int upload_orbit(fdg_ctx* ctx, Buffers* B, const Orbit& o, double* seconds) {
+    double t0 = now_s();
+    std::vector<double> inter(2 * o.len());
+    for (uint64_t k = 0; k < o.len(); k++) {
+        inter[2 * k] = o.re[k];
+        inter[2 * k + 1] = o.im[k];
+    }
+    FDG_TRY(ctx, B->orbit.ensure(inter.size() * 8 + 64), "hipMalloc orbit");
+    FDG_TRY(ctx,
+            hipMemcpyAsync(B->orbit.p, inter.data(), inter.size() * 8, hipMemcpyHostToDevice,
+                           ctx->stream),
+            "orbit upload");
+    FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "orbit upload");
+    *seconds += now_s() - t0;
+    return FDG_OK;
+}
+
+}  // namespace
+
+struct fdg_frame {
+    fdg_ctx* ctx = nullptr;
+    std::string re, im, width;
+    fdg_view view;
+    fdg_params params;
+    Frame fr;
+    int route = 0, form = 0;
+    double eps = 0;
+    uint32_t count = 0;
+    uint64_t len = 0, nucleus = 0;
+    uint32_t bits = 0;
+    double dc = 0;
+    int64_t dc_exp = 0;
+    double reference_seconds = 0, transfer_seconds = 0;
+    bool use_bla = false, ran = false;
+    fdg_stats last;
+    Buffers buf;
+};
+
+extern "C" {
+
+int fdg_abi_version(void) { return FDG_ABI_VERSION; }
+
+int fdg_create(fdg_ctx** out, int device) {
+    if (!out) return FDG_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        g_create_error = "no HIP device";
+        return FDG_EDEVICE;
+    }
+    if (device < 0 || device >= n) {
+        g_create_error = "device index out of range";
+        return FDG_EINVAL;
+    }
+    fdg_ctx* c = new fdg_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        g_create_error = "cannot open the HIP device";
+        delete c;
+        return FDG_EDEVICE;
+    }
+    for (auto& e : c->ev) hipEventCreate(&e);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
+    *out = c;
+    return FDG_OK;
+}
+
+void fdg_destroy(fdg_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    for (auto& e : c->ev)
+        if (e) hipEventDestroy(e);
+    hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* fdg_last_error(const fdg_ctx* c) {
+    return c ? c->err.c_str() : g_create_error.c_str();
+}
+
+int fdg_enabled(const char* value, int* enabled, int* route) {
+    std::string v = value ? value : "off";
+    if (v == "off") {
+        if (enabled) *enabled = 0;
+        return FDG_OK;
+    }
+    if (v == "on" || v == "bla") {
+        if (enabled) *enabled = 1;
+        if (route) *route = v == "bla" ? FDG_ROUTE_BLA : FDG_ROUTE_PLAIN;
+        return FDG_OK;
+    }
+    return FDG_EINVAL;
+}
+
+int fdg_reference(fdg_ctx* ctx_in, const fdg_view* v, const fdg_params* p, double* re,
+                  double* im, uint64_t cap, uint64_t* len, uint32_t* bits) {
+    // Host-only: a NULL context is allowed (errors then go to fdg_last_error(NULL)).
+    fdg_ctx local;
+    fdg_ctx* ctx = ctx_in ? ctx_in : &local;
+    struct Report {
+        fdg_ctx* c;
+        bool own;
+        ~Report() {
+            if (own && !c->err.empty()) g_create_error = c->err;
+        }
+    } report{ctx, ctx_in == nullptr};
+    Frame fr;
+    int rc = setup(ctx, v, p, &fr);
+    if (rc) return rc;
+    Orbit o;
+    rc = compute_orbit(ctx, v, p, fr, &o);
+    if (rc) return rc;
+    if (len) *len = o.len();
+    if (bits) *bits = o.bits;
+    if (re && im && cap >= o.len()) {
+        std::memcpy(re, o.re.data(), o.len() * 8);
+        std::memcpy(im, o.im.data(), o.len() * 8);
+    }
+    return FDG_OK;
+}
+
+int fdg_build_bla(fdg_ctx* ctx, const double* re, const double* im, uint64_t len, double eps,
+                  double dc_max, int64_t dc_exp, double* blocks, uint64_t cap_blocks,
+                  uint32_t* levels) {
+    if (!ctx || !re || !im) return FDG_EINVAL;
+    FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    Orbit o;
+    o.re.assign(re, re + len);
+    o.im.assign(im, im + len);
+    double t = 0;
+    Buffers B;
+    struct Free {
+        Buffers* b;
+        ~Free() { b->release(); }
+    } guard{&B};
+    int rc = upload_orbit(ctx, &B, o, &t);
+    if (rc) return rc;
+    uint32_t nl = 0;
+    rc = build_table(ctx, &B, len, eps, dc_max, dc_exp, &nl, &t);
+    if (rc) return rc;
+    if (levels) *levels = nl;
+    if (!blocks) return FDG_OK;
+    uint64_t steps = B.bla_steps, total = 0;
+    for (uint32_t i = 0; i < nl; i++) total += steps >> (i + 1);
+    if (total > cap_blocks) return ctx->fail(FDG_EINVAL, "block buffer too small");
+    uint64_t stride = B.bla_stride;
+    std::vector<double> r(stride), ab(4 * stride), al(stride), be(stride);
+    if (nl) {
+        FDG_TRY(ctx, hipMemcpy(r.data(), B.bla_r.p, stride * 8, hipMemcpyDeviceToHost), "copy");
+        FDG_TRY(ctx, hipMemcpy(ab.data(), B.bla_ab.p, stride * 32, hipMemcpyDeviceToHost), "copy");
+        FDG_TRY(ctx, hipMemcpy(al.data(), B.bla_alpha.p, stride * 8, hipMemcpyDeviceToHost), "copy");
+        FDG_TRY(ctx, hipMemcpy(be.data(), B.bla_beta.p, stride * 8, hipMemcpyDeviceToHost), "copy");
+    }
+    uint64_t w = 0;
+    for (uint32_t i = 0; i < nl; i++) {
+        uint64_t off = bla_level_offset(stride, i), c = steps >> (i + 1);
+        for (uint64_t k = 0; k < c; k++, w++) {
+            double* b = blocks + 7 * w;
+            b[0] = ab[4 * (off + k)];
+            b[1] = ab[4 * (off + k) + 1];
+            b[2] = ab[4 * (off + k) + 2];
+            b[3] = ab[4 * (off + k) + 3];
+            b[4] = r[off + k];
+            b[5] = al[off + k];
+            b[6] = be[off + k];
+        }
+    }
+    return FDG_OK;
+}
+
+int fdg_frame_open(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int route, int form,
+                   double bla_eps, fdg_frame** out) {
+    if (!ctx || !out) return FDG_EINVAL;
+    *out = nullptr;
+    FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    Frame fr;
+    int rc = setup(ctx, v, p, &fr);
+    if (rc) return rc;
+    if (p->columns & (FDG_COL_BOUND | FDG_COL_STRIPE))
+        return ctx->fail(FDG_EDECLINED,
+                         "bound/stripe columns are CPU-kernel columns (frame.rs:133-144)");
+    if (form != FDG_FORM_F64) return ctx->fail(FDG_EINVAL, "unknown delta form");
+    if (route != FDG_ROUTE_PLAIN && route != FDG_ROUTE_BLA)
+        return ctx->fail(FDG_EINVAL, "unknown route");
+    uint64_t count64 = (uint64_t)p->nx * p->ny;
+    if (count64 >= (1ull << 32)) return ctx->fail(FDG_EINVAL, "grid too large");
+    if (route == FDG_ROUTE_BLA) {
+        // grid.rs:378-413 bla_checks, and bla.rs:27-33 check
+        if (std::isnan(p->escape_radius) || p->escape_radius < 4.0)
+            return ctx->fail(FDG_EINVAL, "escape radius is below 4: BLA needs >= 4");
+        if (!(bla_eps > 0.0 && bla_eps <= kEpsLimit))
+            return ctx->fail(FDG_EINVAL, "BLA needs 0 < eps <= 2^-24");
+    }
+    fdg_frame* f = new fdg_frame();
+    f->ctx = ctx;
+    f->re = v->center_re;
+    f->im = v->center_im;
+    f->width = v->width;
+    f->view = *v;
+    f->view.center_re = f->re.c_str();
+    f->view.center_im = f->im.c_str();
+    f->view.width = f->width.c_str();
+    f->params = *p;
+    f->fr = fr;
+    f->route = route;
+    f->form = form;
+    f->eps = bla_eps;
+    f->count = (uint32_t)count64;
+    double t0 = now_s();
+    Orbit o;
+    rc = compute_orbit(ctx, v, p, fr, &o);
+    if (rc) {
+        delete f;
+        return rc;
+    }
+    f->reference_seconds = now_s() - t0;
+    f->len = o.len();
+    f->bits = o.bits;
+    f->nucleus = o.nucleus_period();
+    rc = upload_orbit(ctx, &f->buf, o, &f->transfer_seconds);
+    if (!rc) rc = f->buf.outputs(ctx, f->count);
+    if (rc) {
+        f->buf.release();
+        delete f;
+        return rc;
+    }
+    dc_bound(fr.pl, fr.tier, p->nx, p->ny, &f->dc, &f->dc_exp);
+    *out = f;
+    return FDG_OK;
+}
+
+int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
+    if (!f) return FDG_EINVAL;
+    fdg_ctx* ctx = f->ctx;
+    FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    fdg_stats s;
+    std::memset(&s, 0, sizeof(s));
+    s.reference_seconds = f->reference_seconds;
+    s.reference_len = f->len;
+    s.reference_bits = f->bits;
+    s.nucleus_period = f->nucleus;
+    s.transfer_seconds = f->transfer_seconds;
+    uint32_t levels = 0;
+    if (f->route == FDG_ROUTE_BLA) {
+        int rc = build_table(ctx, &f->buf, f->len, f->eps, f->dc, f->dc_exp, &levels,
+                             &s.bla_seconds);
+        if (rc) return rc;
+    }
+    s.bla_levels = levels;
+    f->use_bla = levels > 0;
+    Buffers& b = f->buf;
+    FDG_TRY(ctx, hipMemsetAsync(b.counters.p, 0, 256, ctx->stream), "memset");
+    size_t n = f->count;
+    for (DevBuf* d : {&b.o_zr, &b.o_zi, &b.o_dr, &b.o_di, &b.o_dexp})
+        FDG_TRY(ctx, hipMemsetAsync(d->p, 0, n * 8, ctx->stream), "memset");
+    FrameDev d;
+    std::memset(&d, 0, sizeof(d));
+    const Plane& pl = f->fr.pl;
+    const fdg_params* p = &f->params;
+    d.orbit = b.orbit.as<double>();
+    d.len = f->len;
+    d.nucleus = f->nucleus;
+    d.bla.r2 = b.bla_r2.as<double>();
+    d.bla.ab = b.bla_ab.as<double>();
+    d.bla.steps = b.bla_steps;
+    d.bla.stride = b.bla_stride;
+    d.bla.nlevels = f->use_bla ? levels : 0;
+    d.c_re = pl.c_re;
+    d.c_im = pl.c_im;
+    d.h = pl.h();
+    d.h_m = pl.h_m;
+    d.h_e = pl.h_e;
+    d.cos_r = pl.cos_r;
+    d.sin_r = pl.sin_r;
+    d.half_x = pl.half_x;
+    d.half_y = pl.half_y;
+    d.nx = p->nx;
+    d.ny = p->ny;
+    d.count = f->count;
+    d.tier = f->fr.tier;
+    d.max_iter = p->max_iter;
+    d.r2 = p->escape_radius * p->escape_radius;
+    d.mask = nullptr;
+    d.kind = b.o_kind.as<uint8_t>();
+    d.n = b.o_n.as<uint64_t>();
+    d.zr = b.o_zr.as<double>();
+    d.zi = b.o_zi.as<double>();
+    d.dr = b.o_dr.as<double>();
+    d.di = b.o_di.as<double>();
+    d.dexp = b.o_dexp.as<int64_t>();
+    d.next = b.counters.as<uint32_t>();
+    d.stats = reinterpret_cast<unsigned long long*>(b.counters.as<uint8_t>() + 64);
+    bool deriv = (p->columns & (FDG_COL_DE | FDG_COL_NORMAL)) != 0;
+    int grid = (int)std::min<uint64_t>((f->count + 255) / 256, (uint64_t)ctx->cus * 8);
+    if (grid < 1) grid = 1;
+    FDG_TRY(ctx, hipEventRecord(ctx->ev[2], ctx->stream), "event");
+    FDG_TRY(ctx,
+            launch_perturb_f64(d, f->fr.tier == TIER_SCALED, deriv, f->use_bla, grid, ctx->stream),
+            "perturbation launch");
+    FDG_TRY(ctx, hipEventRecord(ctx->ev[3], ctx->stream), "event");
+    unsigned long long cnt[ST_COUNT];
+    FDG_TRY(ctx,
+            hipMemcpyAsync(cnt, d.stats, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream),
+            "readback");
+    FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "perturbation kernel");
+    float ms = 0;
+    hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
+    s.kernel_seconds = ms * 1e-3;
+    s.iterations = cnt[ST_ITERATIONS];
+    s.equivalent = cnt[ST_EQUIVALENT];
+    s.blocks = cnt[ST_BLOCKS];
+    s.skipped = cnt[ST_SKIPPED];
+    s.fallback_samples = cnt[ST_FALLBACK];
+    s.closed_form_samples = cnt[ST_CLOSED];
+    s.plain_steps = cnt[ST_PLAIN];
+    f->last = s;
+    f->ran = true;
+    if (st) *st = s;
+    return FDG_OK;
+}
+
+int fdg_frame_read(fdg_frame* f, fdg_columns* out, fdg_outcomes* raw, fdg_stats* st,
+                   char* kernel_id, size_t kernel_cap) {
+    if (!f) return FDG_EINVAL;
+    fdg_ctx* ctx = f->ctx;
+    if (!f->ran) return ctx->fail(FDG_EINVAL, "frame has not been run");
+    const fdg_params* p = &f->params;
+    if (!out || !out->class_) return ctx->fail(FDG_EINVAL, "the class column is required");
+    if ((p->columns & FDG_COL_NU && !out->nu) || (p->columns & FDG_COL_DE && !out->de) ||
+        (p->columns & FDG_COL_NORMAL && !out->normal))
+        return ctx->fail(FDG_EINVAL, "a requested column has no buffer");
+    FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    fdg_stats s = f->last;
+    size_t n = f->count;
+    Buffers& b = f->buf;
+    double t1 = now_s();
+    std::vector<uint8_t> kind(n);
+    std::vector<uint64_t> nn(n);
+    std::vector<double> zr(n), zi(n), dr(n), di(n);
+    std::vector<int64_t> dexp(n);
+    FDG_TRY(ctx, hipMemcpy(kind.data(), b.o_kind.p, n, hipMemcpyDeviceToHost), "readback");
+    FDG_TRY(ctx, hipMemcpy(nn.data(), b.o_n.p, n * 8, hipMemcpyDeviceToHost), "readback");
+    FDG_TRY(ctx, hipMemcpy(zr.data(), b.o_zr.p, n * 8, hipMemcpyDeviceToHost), "readback");
+    FDG_TRY(ctx, hipMemcpy(zi.data(), b.o_zi.p, n * 8, hipMemcpyDeviceToHost), "readback");
+    FDG_TRY(ctx, hipMemcpy(dr.data(), b.o_dr.p, n * 8, hipMemcpyDeviceToHost), "readback");
+    FDG_TRY(ctx, hipMemcpy(di.data(), b.o_di.p, n * 8, hipMemcpyDeviceToHost), "readback");
+    FDG_TRY(ctx, hipMemcpy(dexp.data(), b.o_dexp.p, n * 8, hipMemcpyDeviceToHost), "readback");
+    s.transfer_seconds += now_s() - t1;
+    double t2 = now_s();
+    Encoder enc(f->fr.pl, p->ss);
+    double* nu = (p->columns & FDG_COL_NU) ? out->nu : nullptr;
+    float* de = (p->columns & FDG_COL_DE) ? out->de : nullptr;
+    uint16_t* nm = (p->columns & FDG_COL_NORMAL) ? out->normal : nullptr;
+    unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned w = 0; w < nt; w++)
+        th.emplace_back([&, w] {
+            size_t lo = n * w / nt, hi = n * (w + 1) / nt;
+            for (size_t i = lo; i < hi; i++) {
+                if (nu) nu[i] = 0.0;
+                if (de) de[i] = 0.0f;
+                if (nm) nm[i] = 0;
+                enc.put(i, kind[i], nn[i], zr[i], zi[i], dr[i], di[i], dexp[i], out->class_, nu,
+                        de, nm);
+            }
+        });
+    for (auto& t : th) t.join();
+    s.encode_seconds = now_s() - t2;
+    if (raw) {
+        if (raw->kind) std::memcpy(raw->kind, kind.data(), n);
+        if (raw->n) std::memcpy(raw->n, nn.data(), n * 8);
+        if (raw->zr) std::memcpy(raw->zr, zr.data(), n * 8);
+        if (raw->zi) std::memcpy(raw->zi, zi.data(), n * 8);
+        if (raw->dr) std::memcpy(raw->dr, dr.data(), n * 8);
+        if (raw->di) std::memcpy(raw->di, di.data(), n * 8);
+        if (raw->dexp) std::memcpy(raw->dexp, dexp.data(), n * 8);
+    }
+    if (st) *st = s;
+    if (kernel_id && kernel_cap) {
+        std::string k = f->fr.pl.kernel(f->fr.tier);
+        if (f->use_bla) k += " bla/1";
+        k += " gpu-hip-f64/1";
+        std::snprintf(kernel_id, kernel_cap, "%s", k.c_str());
+    }
+    return FDG_OK;
+}
+
+void fdg_frame_close(fdg_frame* f) {
+    if (!f) return;
+    hipSetDevice(f->ctx->device);
+    hipStreamSynchronize(f->ctx->stream);
+    f->buf.release();
+    delete f;
+}
+
+int fdg_render(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int route, int form,
+               double bla_eps, fdg_columns* out, fdg_outcomes* raw, fdg_stats* st,
+               char* kernel_id, size_t kernel_cap) {
+    if (!ctx) return FDG_EINVAL;
+    if (!out || !out->class_) return ctx->fail(FDG_EINVAL, "the class column is required");
+    fdg_frame* f = nullptr;
+    int rc = fdg_frame_open(ctx, v, p, route, form, bla_eps, &f);
+    if (rc) return rc;
+    rc = fdg_frame_run(f, nullptr);
+    if (!rc) rc = fdg_frame_read(f, out, raw, st, kernel_id, kernel_cap);
+    fdg_frame_close(f);
+    return rc;
+}
+
+int64_t fdg_write_fds(const fdg_view* v, const fdg_params* p, const char* kernel_id,
+                      const fdg_columns* c, uint8_t* buf, uint64_t cap) {
+    if (!v || !p || !c || !c->class_ || !kernel_id) return FDG_EINVAL;
+    std::vector<uint8_t> b = write_fds(v->center_re, v->center_im, v->width, v->rotation, p->nx,
+                                       p->ny, p->ss, p->max_iter, p->escape_radius, p->columns,
+                                       kernel_id, c->class_, c->nu, c->de, c->normal);
+    if (buf && cap >= b.size()) std::memcpy(buf, b.data(), b.size());
+    return (int64_t)b.size();
+}
+
+}  // extern "C"

@@ -1,0 +1,71 @@
+// Device helpers whose bits must equal the host's (and the reference CPU kernel's).
+// Compiled with -ffp-contract=off; gfx950's f64 add/mul/div/sqrt are IEEE
+// round-to-nearest, so these reproduce the host results exactly.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace fdg {
+namespace dm {
+
+// 2^k: exact where representable, 0 below the subnormals, inf above
+// (crates/fd-fixed/src/lib.rs:30-42).
+__device__ inline double exp2i(int64_t k) {
+    if (k > 1023) return __longlong_as_double(0x7ff0000000000000ll);
+    if (k >= -1022) return __longlong_as_double((long long)((uint64_t)(k + 1023) << 52));
+    if (k >= -1074) return __longlong_as_double((long long)(1ull << (k + 1074)));
+    return 0.0;
+}
+
+// floor(log2(a)) for finite a > 0, as the reference computes it (a.log2().floor()).
+// Only ever used to pick a power-of-two rescaling, which is exact whichever of two
+// adjacent exponents is chosen.
+__device__ inline int64_t floor_log2(double a) { return (int64_t)floor(log2(a)); }
+
+__device__ inline uint64_t next_pow2(uint64_t x) {
+    if (x <= 1) return 1;
+    return 1ull << (64 - __builtin_clzll(x - 1));
+}
+
+// glibc's hypot (C. F. Borges, "An Improved Algorithm for hypot(a,b)", 2019, unfused
+// correction), which Rust's f64::hypot calls on Linux: the BLA radii depend on its bits.
+__device__ inline double hypot_kernel(double ax, double ay) {
+    double h = sqrt(ax * ax + ay * ay);
+    double t1, t2;
+    if (h <= 2.0 * ay) {
+        double delta = h - ay;
+        t1 = ax * (2.0 * delta - ax);
+        t2 = (delta - 2.0 * (ax - ay)) * delta;
+    } else {
+        double delta = h - ax;
+        t1 = 2.0 * delta * (ax - 2.0 * ay);
+        t2 = (4.0 * delta - ay) * ay + delta * delta;
+    }
+    h -= (t1 + t2) / (2.0 * h);
+    return h;
+}
+
+__device__ inline double hypot_ref(double x, double y) {
+    if (!isfinite(x) || !isfinite(y)) {
+        if (isinf(x) || isinf(y)) return __longlong_as_double(0x7ff0000000000000ll);
+        return x + y;
+    }
+    x = fabs(x);
+    y = fabs(y);
+    double ax = x < y ? y : x, ay = x < y ? x : y;
+    const double kScale = 0x1p-600, kLarge = 0x1p+511, kTiny = 0x1p-511, kEps = 0x1p-54;
+    if (ax > kLarge) {
+        if (ay <= ax * kEps) return ax + ay;
+        return hypot_kernel(ax * kScale, ay * kScale) / kScale;
+    }
+    if (ay < kTiny) {
+        if (ax >= ay / kEps) return ax + ay;
+        return hypot_kernel(ax / kScale, ay / kScale) * kScale;
+    }
+    if (ay <= ax * kEps) return ax + ay;
+    return hypot_kernel(ax, ay);
+}
+
+}  // namespace dm
+}  // namespace fdg

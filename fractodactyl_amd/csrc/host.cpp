@@ -1,0 +1,566 @@
+// Host side of the deep-zoom path; see host.hpp.  Built with -ffp-contract=off: every
+// floating-point expression keeps the reference's operation order.
+#include "host.hpp"
+
+#include <cmath>
+#include <cstring>
+
+namespace fdg {
+
+using u128 = unsigned __int128;
+
+double exp2i(int64_t k) {
+    uint64_t b;
+    if (k > 1023) return INFINITY;
+    if (k >= -1022)
+        b = (uint64_t)(k + 1023) << 52;
+    else if (k >= -1074)
+        b = 1ull << (k + 1074);
+    else
+        return 0.0;
+    double d;
+    std::memcpy(&d, &b, 8);
+    return d;
+}
+
+static int64_t biased_exponent(double v) {
+    uint64_t b;
+    std::memcpy(&b, &v, 8);
+    return (int64_t)((b >> 52) & 0x7ff) - 1023;
+}
+
+// Borges' unfused correction step on ax >= ay > 0 in range.
+static double hypot_kernel(double ax, double ay) {
+    double h = std::sqrt(ax * ax + ay * ay);
+    double t1, t2;
+    if (h <= 2.0 * ay) {
+        double delta = h - ay;
+        t1 = ax * (2.0 * delta - ax);
+        t2 = (delta - 2.0 * (ax - ay)) * delta;
+    } else {
+        double delta = h - ax;
+        t1 = 2.0 * delta * (ax - 2.0 * ay);
+        t2 = (4.0 * delta - ay) * ay + delta * delta;
+    }
+    h -= (t1 + t2) / (2.0 * h);
+    return h;
+}
+
+double hypot_ref(double x, double y) {
+    if (!std::isfinite(x) || !std::isfinite(y)) {
+        if (std::isinf(x) || std::isinf(y)) return INFINITY;
+        return x + y;
+    }
+    x = std::fabs(x);
+    y = std::fabs(y);
+    double ax = x < y ? y : x, ay = x < y ? x : y;
+    const double kScale = 0x1p-600, kLarge = 0x1p+511, kTiny = 0x1p-511, kEps = 0x1p-54;
+    if (ax > kLarge) {
+        if (ay <= ax * kEps) return ax + ay;
+        return hypot_kernel(ax * kScale, ay * kScale) / kScale;
+    }
+    if (ay < kTiny) {
+        if (ax >= ay / kEps) return ax + ay;
+        return hypot_kernel(ax / kScale, ay / kScale) * kScale;
+    }
+    if (ay <= ax * kEps) return ax + ay;
+    return hypot_kernel(ax, ay);
+}
+
+// ------------------------------------------------------------------ limbs
+
+namespace limbs {
+static int cmp(const uint64_t* a, const uint64_t* b, size_t l) {
+    for (size_t i = l; i-- > 0;)
+        if (a[i] != b[i]) return a[i] < b[i] ? -1 : 1;
+    return 0;
+}
+static void add(const uint64_t* a, const uint64_t* b, uint64_t* o, size_t l) {
+    uint64_t carry = 0;
+    for (size_t i = 0; i < l; i++) {
+        u128 s = (u128)a[i] + b[i] + carry;
+        o[i] = (uint64_t)s;
+        carry = (uint64_t)(s >> 64);
+    }
+}
+static void sub(const uint64_t* a, const uint64_t* b, uint64_t* o, size_t l) {
+    uint64_t borrow = 0;
+    for (size_t i = 0; i < l; i++) {
+        u128 d = (u128)a[i] - b[i] - borrow;
+        o[i] = (uint64_t)d;
+        borrow = (uint64_t)(d >> 64) & 1;
+    }
+}
+static bool nonzero(const uint64_t* a, size_t l) {
+    for (size_t i = 0; i < l; i++)
+        if (a[i]) return true;
+    return false;
+}
+// Signed sum; returns true when the result is negative (zero is never negative).
+static bool add_signed(bool an, const uint64_t* a, bool bn, const uint64_t* b, uint64_t* o,
+                       size_t l) {
+    if (an == bn) {
+        add(a, b, o, l);
+        return an && nonzero(o, l);
+    }
+    if (cmp(a, b, l) < 0) {
+        sub(b, a, o, l);
+        return bn;
+    }
+    sub(a, b, o, l);
+    return an && nonzero(o, l);
+}
+// Exact square into a 2l-limb product, then truncation to the top l limbs below the
+// integer limb: o = floor(a^2) at 64(l-1) fraction bits.
+static void sqr(const uint64_t* a, uint64_t* prod, uint64_t* o, size_t l) {
+    std::memset(prod, 0, 2 * l * sizeof(uint64_t));
+    for (size_t i = 0; i < l; i++) {
+        uint64_t ai = a[i];
+        if (!ai) continue;
+        uint64_t carry = 0;
+        for (size_t j = i + 1; j < l; j++) {
+            u128 t = (u128)ai * a[j] + prod[i + j] + carry;
+            prod[i + j] = (uint64_t)t;
+            carry = (uint64_t)(t >> 64);
+        }
+        prod[i + l] = carry;
+    }
+    uint64_t top = 0;
+    for (size_t i = 0; i < 2 * l; i++) {
+        uint64_t nt = prod[i] >> 63;
+        prod[i] = (prod[i] << 1) | top;
+        top = nt;
+    }
+    uint64_t carry = 0;
+    for (size_t i = 0; i < l; i++) {
+        u128 sq = (u128)a[i] * a[i];
+        u128 t = (u128)prod[2 * i] + (uint64_t)sq + carry;
+        prod[2 * i] = (uint64_t)t;
+        u128 t2 = (u128)prod[2 * i + 1] + (uint64_t)(sq >> 64) + (uint64_t)(t >> 64);
+        prod[2 * i + 1] = (uint64_t)t2;
+        carry = (uint64_t)(t2 >> 64);
+    }
+    std::memcpy(o, prod + (l - 1), l * sizeof(uint64_t));
+}
+}  // namespace limbs
+
+static bool frexp_limbs(bool neg, const uint64_t* mag, size_t l, double* m, int64_t* e) {
+    size_t k = l;
+    while (k > 0 && mag[k - 1] == 0) k--;
+    if (k == 0) return false;
+    k--;
+    double below1 = k >= 1 ? (double)mag[k - 1] : 0.0;
+    double below2 = k >= 2 ? (double)mag[k - 2] : 0.0;
+    double v = (double)mag[k] + below1 * exp2i(-64) + below2 * exp2i(-128);
+    int64_t ve = biased_exponent(v);
+    double mm = v * exp2i(-ve);
+    *m = neg ? -mm : mm;
+    *e = ve + 64 * ((int64_t)k - ((int64_t)l - 1));
+    return true;
+}
+
+static double to_f64_limbs(bool neg, const uint64_t* mag, size_t l) {
+    double m;
+    int64_t e;
+    if (!frexp_limbs(neg, mag, l, &m, &e)) return 0.0;
+    if (e < -1000) return m * exp2i(e + 100) * exp2i(-100);
+    return m * exp2i(e);
+}
+
+bool Fixed::frexp(double* m, int64_t* e) const { return frexp_limbs(neg, mag.data(), mag.size(), m, e); }
+double Fixed::to_f64() const { return to_f64_limbs(neg, mag.data(), mag.size()); }
+
+// ------------------------------------------------------------------ decimal
+
+static bool is_ws(char c) { return c == ' ' || (c >= '\t' && c <= '\r'); }
+
+bool Decimal::parse(const std::string& s, Decimal* out) {
+    size_t b = 0, e = s.size();
+    while (b < e && is_ws(s[b])) b++;
+    while (e > b && is_ws(s[e - 1])) e--;
+    std::string t = s.substr(b, e - b);
+    bool neg = false;
+    if (!t.empty() && (t[0] == '-' || t[0] == '+')) {
+        neg = t[0] == '-';
+        t.erase(0, 1);
+    }
+    size_t epos = t.find_first_of("eE");
+    int64_t exp = 0;
+    std::string mant = t;
+    if (epos != std::string::npos) {
+        std::string es = t.substr(epos + 1);
+        mant = t.substr(0, epos);
+        // Rust's i64 parse: optional sign, at least one digit, no overflow.
+        size_t i = 0;
+        bool eneg = false;
+        if (!es.empty() && (es[0] == '+' || es[0] == '-')) {
+            eneg = es[0] == '-';
+            i = 1;
+        }
+        if (i == es.size()) return false;
+        u128 v = 0;
+        for (; i < es.size(); i++) {
+            if (es[i] < '0' || es[i] > '9') return false;
+            v = v * 10 + (unsigned)(es[i] - '0');
+            if (v > ((u128)1 << 63)) return false;
+        }
+        if (!eneg && v > (u128)INT64_MAX) return false;
+        exp = eneg ? (int64_t)(0 - (uint64_t)v) : (int64_t)v;
+    }
+    size_t dot = mant.find('.');
+    std::string ip = dot == std::string::npos ? mant : mant.substr(0, dot);
+    std::string fp = dot == std::string::npos ? std::string() : mant.substr(dot + 1);
+    Decimal d;
+    for (char c : ip + fp) {
+        uint8_t v = (uint8_t)(c - '0');
+        if (v > 9) return false;
+        d.digits.push_back(v);
+    }
+    if (d.digits.empty()) return false;
+    bool zero = true;
+    for (uint8_t v : d.digits) zero = zero && v == 0;
+    if (exp < INT64_MIN + (int64_t)fp.size()) return false;
+    d.neg = neg && !zero;
+    d.exp10 = exp - (int64_t)fp.size();
+    *out = std::move(d);
+    return true;
+}
+
+double Decimal::log2_abs() const {
+    size_t first = 0;
+    while (first < digits.size() && digits[first] == 0) first++;
+    if (first == digits.size()) return -INFINITY;
+    size_t sig = digits.size() - first;
+    double head = 0.0;
+    for (size_t i = 0; i < sig && i < 17; i++) head = head * 10.0 + (double)digits[first + i];
+    double rest = sig > 17 ? (double)(sig - 17) : 0.0;
+    return (std::log10(head) + rest + (double)exp10) * 3.32192809488736234787031942948939018;
+}
+
+static void mul_small(std::vector<uint64_t>& v, uint64_t m) {
+    uint64_t c = 0;
+    for (auto& x : v) {
+        u128 t = (u128)x * m + c;
+        x = (uint64_t)t;
+        c = (uint64_t)(t >> 64);
+    }
+    if (c) v.push_back(c);
+}
+static void add_small(std::vector<uint64_t>& v, uint64_t a) {
+    uint64_t c = a;
+    for (auto& x : v) {
+        uint64_t s = x + c;
+        c = s < x;
+        x = s;
+        if (!c) return;
+    }
+    if (c) v.push_back(c);
+}
+static void div_small(std::vector<uint64_t>& v, uint64_t d) {
+    u128 r = 0;
+    for (size_t i = v.size(); i-- > 0;) {
+        u128 cur = (r << 64) | v[i];
+        v[i] = (uint64_t)(cur / d);
+        r = cur % d;
+    }
+}
+static uint64_t pow10(unsigned k) {
+    uint64_t p = 1;
+    while (k--) p *= 10;
+    return p;
+}
+
+bool Decimal::to_fixed(size_t limbs, bool* neg_out, std::vector<uint64_t>* mag) const {
+    std::vector<uint64_t> n{0};
+    for (size_t c = 0; c < digits.size(); c += 19) {
+        size_t len = std::min<size_t>(19, digits.size() - c);
+        uint64_t val = 0;
+        for (size_t i = 0; i < len; i++) val = val * 10 + digits[c + i];
+        mul_small(n, pow10((unsigned)len));
+        add_small(n, val);
+    }
+    int64_t e = exp10;
+    while (e > 0) {
+        unsigned k = (unsigned)std::min<int64_t>(e, 19);
+        mul_small(n, pow10(k));
+        e -= k;
+        if (n.size() > 1) return false;
+    }
+    std::vector<uint64_t> v(limbs - 1, 0);
+    v.insert(v.end(), n.begin(), n.end());
+    while (e < 0) {
+        unsigned k = (unsigned)std::min<int64_t>(-e, 19);
+        div_small(v, pow10(k));
+        e += k;
+        if (!limbs::nonzero(v.data(), v.size())) break;
+    }
+    if (v.size() > limbs && limbs::nonzero(v.data() + limbs, v.size() - limbs)) return false;
+    v.resize(limbs);
+    *neg_out = neg && limbs::nonzero(v.data(), limbs);
+    *mag = std::move(v);
+    return true;
+}
+
+bool Fixed::parse(const std::string& s, size_t limbs, Fixed* out) {
+    Decimal d;
+    if (!Decimal::parse(s, &d)) return false;
+    return d.to_fixed(limbs, &out->neg, &out->mag);
+}
+
+// ------------------------------------------------------------------ plane
+
+bool Plane::make(const std::string& re, const std::string& im, const std::string& width,
+                 double rotation, uint32_t nx, uint32_t ny, Plane* p, std::string* err) {
+    if (nx == 0 || ny == 0) {
+        *err = "empty sample grid";
+        return false;
+    }
+    Decimal w;
+    if (!Decimal::parse(width, &w)) {
+        *err = "not a decimal number: \"" + width + "\"";
+        return false;
+    }
+    double lw = w.log2_abs();
+    if (w.neg || !std::isfinite(lw)) {
+        *err = "view width must be positive";
+        return false;
+    }
+    Fixed wf;
+    if (!w.to_fixed(Fixed::limbs_for((uint64_t)std::ceil(std::fmax(64.0 - lw, 64.0))), &wf.neg,
+                    &wf.mag)) {
+        *err = "value too large for fixed point";
+        return false;
+    }
+    double wm;
+    int64_t we;
+    if (!wf.frexp(&wm, &we)) {
+        *err = "view width underflowed";
+        return false;
+    }
+    double q = wm / (double)nx;
+    int64_t qe = biased_exponent(q);
+    p->h_m = q * exp2i(-qe);
+    p->h_e = we + qe;
+    Decimal tmp;
+    if (!Decimal::parse(re, &tmp) || !Decimal::parse(im, &tmp)) {
+        *err = "not a decimal number";
+        return false;
+    }
+    p->c_re = std::strtod(re.c_str(), nullptr);
+    p->c_im = std::strtod(im.c_str(), nullptr);
+    if (!(std::isfinite(p->c_re) && std::isfinite(p->c_im))) {
+        *err = "view centre out of range";
+        return false;
+    }
+    double c_mag = std::fmax(std::fmax(std::fabs(p->c_re), std::fabs(p->c_im)), exp2i(-1022));
+    bool f64_ok = std::log2(c_mag) - 52.0 <= (double)p->h_e - 10.0 && p->h_e > -1000;
+    p->tier = f64_ok ? TIER_F64 : (p->h_e >= -900 ? TIER_FIXED : TIER_SCALED);
+    p->bits = (uint64_t)std::max<int64_t>(128 - p->h_e, 128);
+    p->sin_r = std::sin(rotation);
+    p->cos_r = std::cos(rotation);
+    p->half_x = (double)nx / 2.0;
+    p->half_y = (double)ny / 2.0;
+    return true;
+}
+
+void Plane::unit_offset(uint32_t i, uint32_t j, double* ux, double* uy) const {
+    double x = (double)i + 0.5 - half_x;
+    double y = (double)j + 0.5 - half_y;
+    *ux = cos_r * x + sin_r * y;
+    *uy = sin_r * x - cos_r * y;
+}
+
+std::string Plane::kernel(Tier t) const {
+    switch (t) {
+        case TIER_F64: return "pert-f64/1";
+        case TIER_FIXED: return "pert-fx/1 bits=" + std::to_string(bits);
+        default: return "pert-fx-scaled/1 bits=" + std::to_string(bits);
+    }
+}
+
+// ------------------------------------------------------------------ orbit
+
+Orbit Orbit::f64(double c_re, double c_im, uint64_t max_iter, double radius, uint64_t max_len) {
+    Orbit o;
+    uint64_t steps = std::min<uint64_t>(max_iter, max_len - 1);
+    o.re.reserve(std::min<uint64_t>(steps, 1 << 16) + 1);
+    o.im.reserve(std::min<uint64_t>(steps, 1 << 16) + 1);
+    o.re.push_back(0.0);
+    o.im.push_back(0.0);
+    double x = 0.0, y = 0.0;
+    for (uint64_t k = 0; k < steps; k++) {
+        double t = x * x - y * y + c_re;
+        y = 2.0 * x * y + c_im;
+        x = t;
+        o.re.push_back(x);
+        o.im.push_back(y);
+        if (x * x + y * y > radius * radius) break;
+    }
+    o.bits = 53;
+    return o;
+}
+
+bool Orbit::fixed(const std::string& re, const std::string& im, uint64_t bits, uint64_t max_iter,
+                  Orbit* out, uint64_t max_len) {
+    size_t l = Fixed::limbs_for(bits);
+    Fixed cr, ci;
+    if (!Fixed::parse(re, l, &cr) || !Fixed::parse(im, l, &ci)) return false;
+    uint64_t steps = std::min<uint64_t>(max_iter, max_len - 1);
+    Orbit o;
+    o.re.reserve(steps + 1);
+    o.im.reserve(steps + 1);
+    o.re.push_back(0.0);
+    o.im.push_back(0.0);
+    std::vector<uint64_t> buf(9 * l, 0);
+    uint64_t *x = &buf[0], *y = &buf[l], *x2 = &buf[2 * l], *y2 = &buf[3 * l], *s = &buf[4 * l],
+             *s2 = &buf[5 * l], *t = &buf[6 * l], *prod = &buf[7 * l];
+    bool xn = false, yn = false;
+    for (uint64_t k = 0; k < steps; k++) {
+        // x' = x^2 - y^2 + cr;  y' = (x + y)^2 - (x^2 + y^2) + ci
+        limbs::sqr(x, prod, x2, l);
+        limbs::sqr(y, prod, y2, l);
+        limbs::add_signed(xn, x, yn, y, s, l);
+        limbs::sqr(s, prod, s2, l);
+        bool tn = limbs::add_signed(false, x2, true, y2, t, l);
+        xn = limbs::add_signed(tn, t, cr.neg, cr.mag.data(), x, l);
+        limbs::add(x2, y2, s, l);
+        tn = limbs::add_signed(false, s2, true, s, t, l);
+        yn = limbs::add_signed(tn, t, ci.neg, ci.mag.data(), y, l);
+        double fx = to_f64_limbs(xn, x, l), fy = to_f64_limbs(yn, y, l);
+        o.re.push_back(fx);
+        o.im.push_back(fy);
+        if (fx * fx + fy * fy > 4294967296.0) break;
+    }
+    o.bits = (uint32_t)bits;
+    *out = std::move(o);
+    return true;
+}
+
+uint64_t Orbit::nucleus_period() const {
+    if (len() < 2) return 0;
+    double record = hypot_ref(re[1], im[1]);
+    double log_beta = 0.0;
+    for (uint64_t p = 2; p < len() / 2; p++) {
+        double previous = hypot_ref(re[p - 1], im[p - 1]);
+        log_beta += 1.0 + std::log2(previous);
+        double radius = hypot_ref(re[p], im[p]);
+        if (radius > 1e-140 && radius < 1e-24 && radius < record * 1e-12 &&
+            1.0 + log_beta + std::log2(radius) < -40.0 &&
+            hypot_ref(re[2 * p], im[2 * p]) <= 2.0 * radius)
+            return p;
+        record = std::fmin(record, radius);
+    }
+    return 0;
+}
+
+void dc_bound(const Plane& pl, Tier tier, uint32_t nx, uint32_t ny, double* dc, int64_t* e) {
+    double half = hypot_ref((double)nx / 2.0, (double)ny / 2.0);
+    if (tier == TIER_SCALED) {
+        *dc = pl.h_m * half;
+        *e = pl.h_e;
+    } else {
+        *dc = pl.h() * half;
+        *e = 0;
+    }
+}
+
+// ------------------------------------------------------------------ encoder
+
+static uint16_t quantised_angle(double x, double y) {
+    double t = std::atan2(y, x) * (65536.0 / 6.283185307179586);
+    double r = std::round(t);
+    if (std::isnan(r)) return 0;
+    int64_t v = r >= 9.2233720368547758e18 ? INT64_MAX
+                : r <= -9.2233720368547758e18 ? INT64_MIN : (int64_t)r;
+    return (uint16_t)(uint64_t)v;
+}
+
+void Encoder::put(size_t idx, uint8_t kind, uint64_t n, double zr, double zi, double dr,
+                  double di, int64_t dexp, uint8_t* cls, double* nu, float* de,
+                  uint16_t* normal) const {
+    if (kind != KIND_ESCAPED) {
+        if (cls) cls[idx] = kind == KIND_UNRESOLVED ? 2 : 1;
+        return;
+    }
+    double z2 = zr * zr + zi * zi;
+    double log2z = 0.5 * std::log2(z2);
+    if (cls) cls[idx] = 0;
+    if (nu) nu[idx] = (double)n + 1.0 - std::log2(log2z);
+    if (de) {
+        // store.rs:141-150: 2|z| ln|z| / |dz/dc| in output pixels, exponent-safe.
+        double az = std::sqrt(z2);
+        double lnz = log2z * 0.693147180559945309417232121458176568;
+        double base = 2.0 * az * lnz / (hypot_ref(dr, di) * px_m);
+        int64_t k = std::min<int64_t>(std::max<int64_t>(-(dexp + px_e), -4000), 4000);
+        de[idx] = (float)(base * exp2i(k / 2) * exp2i(k - k / 2));
+    }
+    if (normal) {
+        double a = zr * dr + zi * di, b = zi * dr - zr * di;
+        normal[idx] = quantised_angle(pl.cos_r * a + pl.sin_r * b, pl.sin_r * a - pl.cos_r * b);
+    }
+}
+
+// ------------------------------------------------------------------ .fds
+
+namespace {
+struct Out {
+    std::vector<uint8_t> b;
+    template <class T>
+    void put(T v) {
+        const uint8_t* p = reinterpret_cast<const uint8_t*>(&v);
+        b.insert(b.end(), p, p + sizeof(T));
+    }
+    void bytes(const void* p, size_t n) {
+        const uint8_t* q = static_cast<const uint8_t*>(p);
+        b.insert(b.end(), q, q + n);
+    }
+    void str(const std::string& s) {
+        put<uint16_t>((uint16_t)s.size());
+        bytes(s.data(), s.size());
+    }
+    void pad8() { b.resize((b.size() + 7) & ~size_t(7), 0); }
+};
+}  // namespace
+
+std::vector<uint8_t> write_fds(const std::string& re, const std::string& im,
+                               const std::string& width, double rotation, uint32_t nx,
+                               uint32_t ny, uint32_t ss, uint64_t max_iter, double escape_radius,
+                               uint32_t columns, const std::string& kernel, const uint8_t* cls,
+                               const double* nu, const float* de, const uint16_t* normal) {
+    Out o;
+    uint32_t cols = columns | COL_CLASS;
+    o.bytes("FDSAMPLE", 8);
+    o.put<uint16_t>(1);  // MAJOR
+    o.put<uint16_t>(1);  // MINOR
+    o.put<uint32_t>(cols);
+    o.put<uint32_t>(nx);
+    o.put<uint32_t>(ny);
+    o.put<uint32_t>(ss);
+    o.put<uint32_t>(0);
+    o.put<uint64_t>(max_iter);
+    o.put<double>(escape_radius);
+    o.put<double>(rotation);
+    o.str(re);
+    o.str(im);
+    o.str(width);
+    o.str(kernel);
+    o.pad8();
+    size_t n = (size_t)nx * ny;
+    o.bytes(cls, n);
+    o.pad8();
+    if (cols & COL_NU) {
+        o.bytes(nu, 8 * n);
+        o.pad8();
+    }
+    if (cols & COL_DE) {
+        o.bytes(de, 4 * n);
+        o.pad8();
+    }
+    if (cols & COL_NORMAL) {
+        o.bytes(normal, 2 * n);
+        o.pad8();
+    }
+    return o.b;
+}
+
+}  // namespace fdg

@@ -1,0 +1,106 @@
+// Host side of the MI355X deep-zoom path: exact decimal views, the arbitrary-precision
+// reference orbit, sample-grid geometry, and the column encoder / .fds writer.
+//
+// These restate the reference's host-side pieces (crates/fd-fixed, crates/fd-kernel/src/
+// {view,reference,store}.rs, crates/fd-samples) with the same floating-point operation
+// order, so a frame computed here and on the GPU encodes to the reference's bytes.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace fdg {
+
+// 2^k as a double: exact where representable, 0 below the subnormals, inf above
+// (crates/fd-fixed/src/lib.rs:30-42).
+double exp2i(int64_t k);
+
+// glibc's hypot (An Improved Algorithm for hypot(a,b), C. F. Borges 2019, unfused
+// correction), which the reference's f64::hypot resolves to on Linux.  Restated here so
+// host and device produce the same bits.
+double hypot_ref(double x, double y);
+
+enum Tier : int32_t { TIER_F64 = 0, TIER_FIXED = 1, TIER_SCALED = 2 };
+
+// An exact decimal +-digits * 10^exp10 (crates/fd-fixed/src/decimal.rs:6-50).
+struct Decimal {
+    bool neg = false;
+    std::vector<uint8_t> digits;
+    int64_t exp10 = 0;
+    static bool parse(const std::string& s, Decimal* out);
+    double log2_abs() const;
+    // Truncated toward zero to `limbs` little-endian u64 limbs (one integer limb).
+    bool to_fixed(size_t limbs, bool* neg_out, std::vector<uint64_t>* mag) const;
+};
+
+// Sign-magnitude fixed point, one integer limb (crates/fd-fixed/src/lib.rs:16-23).
+struct Fixed {
+    bool neg = false;
+    std::vector<uint64_t> mag;
+    static size_t limbs_for(uint64_t bits) { return 1 + (size_t)((bits + 63) / 64); }
+    static bool parse(const std::string& s, size_t limbs, Fixed* out);
+    bool frexp(double* m, int64_t* e) const;
+    double to_f64() const;
+};
+
+// Sample-grid geometry and tier choice (crates/fd-kernel/src/view.rs:18-135).
+struct Plane {
+    double c_re = 0, c_im = 0, h_m = 1;
+    int64_t h_e = 0;
+    Tier tier = TIER_F64;
+    uint64_t bits = 128;
+    double cos_r = 1, sin_r = 0, half_x = 0, half_y = 0;
+    static bool make(const std::string& re, const std::string& im, const std::string& width,
+                     double rotation, uint32_t nx, uint32_t ny, Plane* out, std::string* err);
+    double h() const { return h_m * exp2i(h_e); }
+    void unit_offset(uint32_t i, uint32_t j, double* ux, double* uy) const;
+    std::string kernel(Tier t) const;
+};
+
+// Reference orbit Z_0 = 0 .. Z_{len-1}, structure of arrays in f64
+// (crates/fd-kernel/src/reference.rs:5-96).
+struct Orbit {
+    std::vector<double> re, im;
+    uint32_t bits = 53;
+    static constexpr uint64_t kMaxLen = 1ull << 20;
+    // f64 orbit at an f64 centre, stopping at the escape radius.
+    static Orbit f64(double c_re, double c_im, uint64_t max_iter, double radius,
+                     uint64_t max_len = kMaxLen);
+    // Exact-centre fixed-point orbit (3 truncated squarings per step), stopping at
+    // |Z|^2 > 2^32.
+    static bool fixed(const std::string& re, const std::string& im, uint64_t bits,
+                      uint64_t max_iter, Orbit* out, uint64_t max_len = kMaxLen);
+    uint64_t len() const { return re.size(); }
+    // Reference::nucleus_period (reference.rs:102-120); 0 for None.
+    uint64_t nucleus_period() const;
+};
+
+// Upper bound m 2^e on |dc| over a view's samples (grid.rs:423-444).
+void dc_bound(const Plane& pl, Tier tier, uint32_t nx, uint32_t ny, double* dc, int64_t* e);
+
+// Raw per-sample result of the kernel (sample.rs:21-40).
+enum Kind : uint8_t { KIND_ESCAPED = 0, KIND_INTERIOR = 1, KIND_UNRESOLVED = 2 };
+
+// Column bits (crates/fd-samples/src/column.rs:9-23).
+enum : uint32_t { COL_CLASS = 1, COL_NU = 2, COL_DE = 4, COL_NORMAL = 8, COL_BOUND = 16,
+                  COL_STRIPE = 32 };
+
+// Store::put for heuristic evidence (store.rs:66-112): one escaped/interior/unresolved
+// outcome into the class/nu/de/normal columns.
+struct Encoder {
+    Plane pl;
+    double px_m;
+    int64_t px_e;
+    Encoder(const Plane& p, uint32_t ss) : pl(p), px_m(p.h_m * (double)ss), px_e(p.h_e) {}
+    void put(size_t idx, uint8_t kind, uint64_t n, double zr, double zi, double dr, double di,
+             int64_t dexp, uint8_t* cls, double* nu, float* de, uint16_t* normal) const;
+};
+
+// fd_samples::write (write.rs:8-57, header.rs:311-330): the canonical .fds bytes.
+std::vector<uint8_t> write_fds(const std::string& re, const std::string& im,
+                               const std::string& width, double rotation, uint32_t nx,
+                               uint32_t ny, uint32_t ss, uint64_t max_iter, double escape_radius,
+                               uint32_t columns, const std::string& kernel, const uint8_t* cls,
+                               const double* nu, const float* de, const uint16_t* normal);
+
+}  // namespace fdg

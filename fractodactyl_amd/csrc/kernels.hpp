@@ -1,0 +1,63 @@
+// Kernel-argument layouts shared by the host launch code and the HIP kernels.
+#pragma once
+#include <cstdint>
+
+namespace fdg {
+
+// BLA table in HBM.  Level i (block length 2 << i, i = 0 .. nlevels-1) holds
+// steps >> (i + 1) blocks, stored from offset stride - (stride >> i) where stride is the
+// next power of two >= steps; block t of level i starts at reference index 1 + t 2^(i+1)
+// (crates/fd-kernel/src/bla.rs:399-419).  Arrays are structure-of-arrays so the radius
+// walk touches only r2.
+struct BlaDev {
+    const double* r2;       // squared validity radius (0 when never valid)
+    const double* ab;       // 4 doubles per block: a.re, a.im, b.re, b.im
+    uint64_t steps;         // reference points - 2
+    uint64_t stride;        // next power of two >= steps (level-offset base)
+    uint32_t nlevels;       // levels actually used by lookups
+};
+
+__host__ __device__ inline uint64_t bla_level_offset(uint64_t stride, uint32_t i) {
+    return stride - (stride >> i);
+}
+
+// Statistics accumulated by one perturbation launch (grid.rs:40-70 Stats / BlaStats).
+enum StatSlot : int {
+    ST_ITERATIONS = 0,  // Stats::iterations: blocks count as one iterate
+    ST_EQUIVALENT = 1,  // sum of Outcome::iterations (BLA-skipped steps included)
+    ST_BLOCKS = 2,
+    ST_SKIPPED = 3,
+    ST_FALLBACK = 4,    // samples that applied no block (excluding closed-form)
+    ST_CLOSED = 5,      // closed-form main-cardioid / period-2 samples
+    ST_PLAIN = 6,       // plain perturbation steps executed
+    ST_COUNT = 8
+};
+
+// One frame for the perturbation kernels.
+struct FrameDev {
+    const double* orbit;    // 2 doubles per point: Z_m re, im
+    uint64_t len;           // reference points
+    uint64_t nucleus;       // Reference::nucleus_period, 0 = none
+    BlaDev bla;
+    // sample geometry (view.rs:117-128)
+    double c_re, c_im, h, h_m;
+    int64_t h_e;
+    double cos_r, sin_r, half_x, half_y;
+    uint32_t nx, ny, count;
+    int32_t tier;           // 0 F64, 1 Fixed, 2 Scaled
+    uint64_t max_iter;
+    double r2;              // escape radius squared
+    const uint8_t* mask;    // nullable: compute only mask[i] != 0
+    // outputs (structure of arrays, count each)
+    uint8_t* kind;
+    uint64_t* n;
+    double* zr;
+    double* zi;
+    double* dr;
+    double* di;
+    int64_t* dexp;
+    uint32_t* next;                 // work-stealing counter (zeroed by the host)
+    unsigned long long* stats;      // ST_COUNT counters (zeroed by the host)
+};
+
+}  // namespace fdg

@@ -1,0 +1,720 @@
+// FP64 deep-zoom kernels for gfx950: the BLA table builder (a level-by-level tree
+// reduction) and the persistent per-sample perturbation kernel.
+//
+// Built with -ffp-contract=off: every floating-point expression keeps the reference's
+// operation order (crates/fd-kernel/src/{sample,number,bla,interior}.rs), and gfx950's
+// IEEE add/mul/div/sqrt then give the CPU kernel's exact bits.  The "FP64" form is the
+// ordinary tiers' Binary64 arithmetic and the scaled tier's Binary64Exponent (f64 mantissa
+// times a carried power of two), selected at compile time.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device_math.hpp"
+#include "kernels.hpp"
+
+namespace fdg {
+namespace {
+
+using namespace fdg::dm;
+
+// ------------------------------------------------------------------ BLA table build
+
+struct Block {
+    double ar, ai, br, bi, r, alpha, beta;
+};
+
+// Block::single (bla.rs:72-85): one step at Z_k.
+__device__ inline Block block_single(double zr, double zi, double eps) {
+    Block b = {0, 0, 0, 0, 0, 0, 0};
+    double az = hypot_ref(zr, zi);
+    double r = 2.0 * eps * az;
+    if (!(r > 0.0 && az <= 2.0)) return b;
+    b.ar = 2.0 * zr + 0.0;
+    b.ai = 2.0 * zi + 0.0;
+    b.br = 1.0;
+    b.bi = 0.0;
+    b.r = r;
+    b.alpha = r;
+    b.beta = 0.0;
+    return b;
+}
+
+// v 2^e for e far below the f64 range (bla.rs:114-121).
+__device__ inline double times_exp2(double v, int64_t e) {
+    if (e >= -1022) return v * exp2i(e);
+    return v * exp2i(-1022) * exp2i(e + 1022);
+}
+
+// Block::merge (bla.rs:91-111): x then y, for |dc| <= c 2^ce.
+__device__ inline Block block_merge(const Block& x, const Block& y, double c, int64_t ce) {
+    Block z = {0, 0, 0, 0, 0, 0, 0};
+    if (x.r == 0.0 || y.r == 0.0) return z;
+    double ax = hypot_ref(x.ar, x.ai), ay = hypot_ref(y.ar, y.ai);
+    double ahat = ax + x.alpha;
+    double bhat = hypot_ref(x.br, x.bi) + x.beta;
+    double r = fmin(x.r, (y.r - times_exp2(bhat * c, ce)) / ahat);
+    Block m;
+    m.ar = y.ar * x.ar - y.ai * x.ai + 0.0 + 0.0;
+    m.ai = y.ar * x.ai + y.ai * x.ar + 0.0 + 0.0;
+    m.br = y.ar * x.br - y.ai * x.bi + y.br + 0.0;
+    m.bi = y.ar * x.bi + y.ai * x.br + y.bi + 0.0;
+    m.r = r;
+    m.alpha = ay * x.alpha + y.alpha * ahat;
+    m.beta = ay * x.beta + y.alpha * bhat + y.beta;
+    if (r > 0.0 && isfinite(m.ar) && isfinite(m.ai) && isfinite(m.br) && isfinite(m.bi) &&
+        isfinite(m.r) && isfinite(m.alpha) && isfinite(m.beta))
+        return m;
+    return z;
+}
+
+struct BlaBuild {
+    double* r;        // validity radius
+    double* r2;
+    double* ab;       // 4 per block
+    double* alpha;
+    double* beta;
+    uint64_t steps, stride;
+    double eps, dc_max;
+    int64_t dc_exp;
+    uint32_t* valid;  // per level: 1 if any block is valid
+};
+
+__device__ inline void store_block(const BlaBuild& t, uint64_t at, const Block& b) {
+    t.r[at] = b.r;
+    t.r2[at] = b.r * b.r;
+    reinterpret_cast<double4*>(t.ab)[at] = make_double4(b.ar, b.ai, b.br, b.bi);
+    t.alpha[at] = b.alpha;
+    t.beta[at] = b.beta;
+}
+
+__device__ inline Block load_block(const BlaBuild& t, uint64_t at) {
+    double4 ab = reinterpret_cast<const double4*>(t.ab)[at];
+    return Block{ab.x, ab.y, ab.z, ab.w, t.r[at], t.alpha[at], t.beta[at]};
+}
+
+__device__ inline void flag_valid(const BlaBuild& t, uint32_t level, bool v) {
+    if (__ballot(v) && __lane_id() == 0) atomicOr(&t.valid[level], 1u);
+}
+
+// Level 0 (2-step blocks) straight from the orbit: singles at k = 1 + 2t and 2 + 2t.
+__global__ void __launch_bounds__(256) bla_level0(const double* orbit, BlaBuild t) {
+    uint64_t cnt = t.steps >> 1;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < cnt;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t k = 1 + 2 * i;
+        Block x = block_single(orbit[2 * k], orbit[2 * k + 1], t.eps);
+        Block y = block_single(orbit[2 * k + 2], orbit[2 * k + 3], t.eps);
+        Block m = block_merge(x, y, t.dc_max, t.dc_exp);
+        store_block(t, bla_level_offset(t.stride, 0) + i, m);
+        flag_valid(t, 0, m.r > 0.0);
+    }
+}
+
+// Level `lv` >= 1: pairs of level lv-1 blocks.
+__global__ void __launch_bounds__(256) bla_level(BlaBuild t, uint32_t lv) {
+    uint64_t cnt = t.steps >> (lv + 1);
+    uint64_t src = bla_level_offset(t.stride, lv - 1), dst = bla_level_offset(t.stride, lv);
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < cnt;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        Block m = block_merge(load_block(t, src + 2 * i), load_block(t, src + 2 * i + 1),
+                              t.dc_max, t.dc_exp);
+        store_block(t, dst + i, m);
+        flag_valid(t, lv, m.r > 0.0);
+    }
+}
+
+// ------------------------------------------------------------------ perturbation
+
+constexpr double kHi = 4294967296.0;
+constexpr double kLo = 1.0 / 4294967296.0;
+constexpr double kSquareOk = 1e-290;
+constexpr double kTiny = 1.0 / 1e270;
+constexpr double kU = 2.220446049250313e-16 / 2.0;
+constexpr double kResolvable = 1e-12;
+
+// interior.rs:6-11
+__device__ inline bool in_main_components(double cr, double ci) {
+    double y2 = ci * ci;
+    double q = (cr - 0.25) * (cr - 0.25) + y2;
+    return q * (q + (cr - 0.25)) <= 0.25 * y2 || (cr + 1.0) * (cr + 1.0) + y2 <= 0.0625;
+}
+
+// interior.rs:16-64 map + contracting: `period` steps from phase m0 with delta x0.
+__device__ bool contracting(const double* orbit, uint64_t len, uint64_t m0, double x0r,
+                            double x0i, double ar, double ai, uint64_t period) {
+    uint64_t last = len - 1, m = m0;
+    double xr = x0r, xi = x0i, dr = 1.0, di = 0.0;
+    for (uint64_t s = 0; s < period; s++) {
+        double Zr = orbit[2 * m], Zi = orbit[2 * m + 1];
+        double fr = Zr + xr, fi = Zi + xi;
+        double t = 2.0 * (fr * dr - fi * di);
+        di = 2.0 * (fr * di + fi * dr);
+        dr = t;
+        t = 2.0 * (Zr * xr - Zi * xi) + xr * xr - xi * xi + ar;
+        xi = 2.0 * (Zr * xi + Zi * xr + xr * xi) + ai;
+        xr = t;
+        m += 1;
+        fr = orbit[2 * m] + xr;
+        fi = orbit[2 * m + 1] + xi;
+        double f2 = fr * fr + fi * fi;
+        if (f2 > 4.0) return false;
+        if (f2 < xr * xr + xi * xi || m == last) {
+            xr = fr;
+            xi = fi;
+            m = 0;
+        }
+    }
+    return dr * dr + di * di < 1.0;
+}
+
+// sample.rs:597-652 Multiplier and sample.rs:657-698 Contraction: a complex product with
+// a separate binary exponent.
+struct Scaled {
+    double re, im;
+    int64_t exp;
+};
+
+__device__ inline void mu_multiply(Scaled& s, double re, double im) {
+    double nr = s.re * re - s.im * im, ni = s.re * im + s.im * re;
+    s.re = nr;
+    s.im = ni;
+    double a = fmax(fabs(nr), fabs(ni));
+    if (!(a > 1e-120 && a < 1e120) && a > 0.0 && isfinite(a)) {
+        int64_t e = floor_log2(a);
+        double sc = exp2i(-e);
+        s.re *= sc;
+        s.im *= sc;
+        s.exp += e;
+    }
+}
+
+__device__ inline bool mu_below_one(const Scaled& s) {
+    return isfinite(s.re) && isfinite(s.im) &&
+           (s.exp < -64 || (s.exp <= 64 && (s.re * s.re + s.im * s.im) * exp2i(2 * s.exp) < 1.0));
+}
+
+__device__ inline void con_multiply(Scaled& s, bool& started, double re, double im) {
+    if (!started) {
+        s.re = 1.0;
+        started = true;
+    }
+    int64_t exp = 0;
+    double a = fmax(fabs(re), fabs(im));
+    if (a != 0.0 && !(kLo <= a && a <= kHi)) {
+        int64_t e = floor_log2(a);
+        double sc = exp2i(-e);
+        re *= sc;
+        im *= sc;
+        exp += e;
+    }
+    double nr = s.re * re - s.im * im, ni = s.re * im + s.im * re;
+    s.re = nr;
+    s.im = ni;
+    s.exp += exp;
+    a = fmax(fabs(s.re), fabs(s.im));
+    if (a != 0.0 && !(kLo <= a && a <= kHi)) {
+        int64_t e = floor_log2(a);
+        double sc = exp2i(-e);
+        s.re *= sc;
+        s.im *= sc;
+        s.exp += e;
+    }
+}
+
+__device__ inline bool con_small(const Scaled& s, bool started) {
+    return started && isfinite(s.re) && isfinite(s.im) &&
+           (s.exp < -64 ||
+            (s.exp <= 64 && (s.re * s.re + s.im * s.im) * exp2i(2 * s.exp) < 1e-6));
+}
+
+// sample.rs:586-595
+__device__ inline void normalise(double& re, double& im, int64_t& e, int64_t max_e) {
+    double a = fmax(fabs(re), fabs(im));
+    if (a == 0.0 || !isfinite(a)) return;
+    int64_t k = floor_log2(a);
+    int64_t lim = max_e - e;
+    if (lim < k) k = lim;
+    double h = exp2i(-(k / 2)), l = exp2i(-(k - k / 2));
+    re = re * h * l;
+    im = im * h * l;
+    e = e + k;
+}
+
+// sample.rs:1094-1114 sum2 over two terms
+__device__ inline void sum2(double t0r, double t0i, int64_t t0e, double t1r, double t1i,
+                            int64_t t1e, int64_t zero_e, double& ore, double& oim, int64_t& oe) {
+    bool have = false;
+    int64_t top = 0;
+    double a0 = fmax(fabs(t0r), fabs(t0i)), a1 = fmax(fabs(t1r), fabs(t1i));
+    if (a0 != 0.0) {
+        top = floor_log2(a0) + t0e;
+        have = true;
+    }
+    if (a1 != 0.0) {
+        int64_t g = floor_log2(a1) + t1e;
+        if (!have || g > top) top = g;
+        have = true;
+    }
+    if (!have) {
+        ore = 0.0;
+        oim = 0.0;
+        oe = zero_e;
+        return;
+    }
+    int64_t k0 = t0e - top, k1 = t1e - top;
+    double r = 0.0, i = 0.0;
+    r = r + (k0 < -2046 ? 0.0 : t0r * exp2i(k0 / 2) * exp2i(k0 - k0 / 2));
+    i = i + (k0 < -2046 ? 0.0 : t0i * exp2i(k0 / 2) * exp2i(k0 - k0 / 2));
+    r = r + (k1 < -2046 ? 0.0 : t1r * exp2i(k1 / 2) * exp2i(k1 - k1 / 2));
+    i = i + (k1 < -2046 ? 0.0 : t1i * exp2i(k1 / 2) * exp2i(k1 - k1 / 2));
+    ore = r;
+    oim = i;
+    oe = top;
+}
+
+// bla.rs:234-249 Bla::find: level index of the longest valid block at m, or -1.
+__device__ inline int bla_find(const BlaDev& t, uint64_t m, double d2, uint64_t budget) {
+    uint64_t k = m - 1;
+    int tz = k == 0 ? 64 : __builtin_ctzll(k);
+    int top = tz < (int)t.nlevels ? tz : (int)t.nlevels;
+    if (top == 0) return -1;
+    int i = -1;
+    for (int lv = 0; lv < top; lv++) {
+        uint64_t idx = k >> (lv + 1);
+        if (!((2ull << lv) <= budget && idx < (t.steps >> (lv + 1)) &&
+              d2 < t.r2[bla_level_offset(t.stride, lv) + idx]))
+            break;
+        i = lv;
+    }
+    return i;
+}
+
+// One lane's sample state (sample.rs:190-236).
+struct Lane {
+    double ar, ai, xr, xi, dr, di, dc_r, dc_i, scale, inverse;
+    double sr, si, qr, qi, qstep;
+    Scaled mu, con;
+    int64_t delta_exp, derivative_exp, offset_exp;
+    uint64_t m, n, chk, sn, hold, nucleus, blocks, skipped;
+    int returns;
+    bool pending, predicted, con_started;
+};
+
+template <bool EXT, bool D>
+__device__ inline void lane_start(Lane& s, const FrameDev& f, double ar, double ai, int64_t oe) {
+    s.ar = ar;
+    s.ai = ai;
+    s.xr = 0.0;
+    s.xi = 0.0;
+    s.offset_exp = oe;
+    s.delta_exp = EXT ? oe : 0;
+    s.scale = EXT ? exp2i(s.delta_exp) : 1.0;
+    s.dc_r = ar;
+    s.dc_i = ai;
+    s.dr = 0.0;
+    s.di = 0.0;
+    s.derivative_exp = 0;
+    s.inverse = 1.0;
+    s.m = 0;
+    s.n = 0;
+    s.sr = 0.0;
+    s.si = 0.0;
+    s.chk = 16;
+    s.sn = EXT ? 0 : 8;
+    s.returns = 0;
+    s.pending = false;
+    s.mu = Scaled{1.0, 0.0, 0};
+    s.hold = 0;
+    s.predicted = false;
+    s.qr = s.qi = s.qstep = 0.0;
+    s.nucleus = f.nucleus;
+    s.con = Scaled{0.0, 0.0, 0};
+    s.con_started = false;
+    s.blocks = 0;
+    s.skipped = 0;
+}
+
+// Re-derive dc in the delta's units after its exponent changed (sample.rs:300-302).
+template <bool EXT>
+__device__ inline void refresh_dc(Lane& s) {
+    if (EXT) {
+        s.scale = exp2i(s.delta_exp);
+        double factor = exp2i(s.offset_exp - s.delta_exp);
+        s.dc_r = s.ar * factor;
+        s.dc_i = s.ai * factor;
+    }
+}
+
+// Outcome codes returned by lane_advance.
+enum : int { RUNNING = -1 };
+
+// One block or one plain step plus the checks after it (sample.rs:237-575).  Returns
+// RUNNING, or the finished outcome kind with the escaped state left in the lane.
+template <bool EXT, bool D, bool L>
+__device__ inline int lane_advance(Lane& s, const FrameDev& f, double& ofr, double& ofi) {
+    // sample.rs:237 `while n < limit`: the budget ran out.
+    if (s.n >= f.max_iter) return 2;  // Unresolved
+    const double* orbit = f.orbit;
+    const uint64_t last = f.len - 1;
+    int lvl = -1;
+    if (L && (s.nucleus == 0 || s.m != 0)) {
+        double d2 = EXT ? (s.xr * s.xr + s.xi * s.xi) * s.scale * s.scale
+                        : s.xr * s.xr + s.xi * s.xi;
+        uint64_t b = s.hold > 0 ? s.sn + s.hold : (s.pending ? ~0ull : s.chk);
+        b = b < f.max_iter ? b : f.max_iter;
+        uint64_t nb = s.nucleus ? (s.n == 0 ? 1 : s.nucleus + 1) : ~0ull;
+        b = b < nb ? b : nb;
+        b = b > s.n ? b - s.n : 0;
+        lvl = bla_find(f.bla, s.m, d2, b);
+    }
+    if (lvl >= 0) {
+        uint64_t at = bla_level_offset(f.bla.stride, lvl) + ((s.m - 1) >> (lvl + 1));
+        double4 ab = reinterpret_cast<const double4*>(f.bla.ab)[at];
+        double a_r = ab.x, a_i = ab.y, b_r = ab.z, b_i = ab.w;
+        uint64_t blen = 2ull << lvl;
+        if (s.nucleus && s.n >= 1) con_multiply(s.con, s.con_started, a_r, a_i);
+        if (!EXT && s.hold > 0) mu_multiply(s.mu, a_r, a_i);
+        if (EXT) {
+            normalise(s.xr, s.xi, s.delta_exp, 0);
+            if (D) {
+                normalise(s.dr, s.di, s.derivative_exp, INT64_MAX / 4);
+                s.inverse = exp2i(-s.derivative_exp);
+            }
+            refresh_dc<EXT>(s);
+        }
+        if (D) {
+            double t = a_r * s.dr - a_i * s.di + b_r * s.inverse;
+            s.di = a_r * s.di + a_i * s.dr + b_i * s.inverse;
+            s.dr = t;
+        }
+        double t = a_r * s.xr - a_i * s.xi + b_r * s.dc_r - b_i * s.dc_i;
+        s.xi = a_r * s.xi + a_i * s.xr + b_r * s.dc_i + b_i * s.dc_r;
+        s.xr = t;
+        if (EXT) {
+            normalise(s.xr, s.xi, s.delta_exp, 0);
+            if (D) {
+                normalise(s.dr, s.di, s.derivative_exp, INT64_MAX / 4);
+                s.inverse = exp2i(-s.derivative_exp);
+            }
+            refresh_dc<EXT>(s);
+        }
+        s.m += blen;
+        s.n += blen;
+        s.blocks += 1;
+        s.skipped += blen;
+    } else {
+        double Zr = orbit[2 * s.m], Zi = orbit[2 * s.m + 1];
+        double fr = Zr + s.scale * s.xr, fi = Zi + s.scale * s.xi;
+        if (s.nucleus && s.n >= 1) con_multiply(s.con, s.con_started, 2.0 * fr, 2.0 * fi);
+        if (!EXT && s.hold > 0) mu_multiply(s.mu, 2.0 * fr, 2.0 * fi);
+        if (D) {
+            double t = 2.0 * (fr * s.dr - fi * s.di) + s.inverse;
+            s.di = 2.0 * (fr * s.di + fi * s.dr);
+            s.dr = t;
+            if (EXT && fmax(fabs(s.dr), fabs(s.di)) > kHi) {
+                s.dr = s.dr * kLo;
+                s.di = s.di * kLo;
+                s.derivative_exp += 32;
+                s.inverse = exp2i(-s.derivative_exp);
+            }
+        }
+        double xr = s.xr, xi = s.xi;
+        if (EXT) {
+            s.xr = 2.0 * (Zr * xr - Zi * xi) + s.scale * (xr * xr - xi * xi) + s.dc_r;
+            s.xi = 2.0 * (Zr * xi + Zi * xr) + s.scale * (2.0 * xr * xi) + s.dc_i;
+        } else {
+            s.xr = 2.0 * (Zr * xr - Zi * xi) + xr * xr - xi * xi + s.dc_r;
+            s.xi = 2.0 * (Zr * xi + Zi * xr + xr * xi) + s.dc_i;
+        }
+        s.m += 1;
+        s.n += 1;
+    }
+    double pr = s.scale * s.xr, pi = s.scale * s.xi;
+    double fr = orbit[2 * s.m] + pr, fi = orbit[2 * s.m + 1] + pi;
+    double f2 = fr * fr + fi * fi;
+    if (f2 > f.r2) {
+        ofr = fr;
+        ofi = fi;
+        return 0;  // Escaped
+    }
+    if (s.nucleus && s.n == s.nucleus + 1) {
+        if (con_small(s.con, s.con_started)) return 1;  // Interior
+        s.nucleus = 0;
+    }
+    double delta2 = pr * pr + pi * pi;
+    bool rebase = EXT ? f2 <= delta2 : f2 < delta2;
+    if (rebase || s.m == last) {
+        if (EXT) {
+            if (f2 > kSquareOk) {
+                s.xr = fr;
+                s.xi = fi;
+                s.delta_exp = 0;
+            } else {
+                sum2(orbit[2 * s.m], orbit[2 * s.m + 1], 0, s.xr, s.xi, s.delta_exp, s.delta_exp,
+                     s.xr, s.xi, s.delta_exp);
+            }
+            s.scale = exp2i(s.delta_exp);
+            s.m = 0;
+            if (s.scale < kTiny) {
+                s.nucleus = 0;
+                if (D) {
+                    // zero_step_dz (sample.rs:1124-1134)
+                    double qr = 2.0 * (s.xr * s.dr - s.xi * s.di);
+                    double qi = 2.0 * (s.xr * s.di + s.xi * s.dr);
+                    sum2(qr, qi, s.delta_exp + s.derivative_exp, 1.0, 0.0, 0, s.derivative_exp,
+                         s.dr, s.di, s.derivative_exp);
+                    s.inverse = exp2i(-s.derivative_exp);
+                }
+                // zero_step (sample.rs:1116-1122)
+                double wr = s.xr, wi = s.xi;
+                sum2(wr * wr - wi * wi, 2.0 * wr * wi, 2 * s.delta_exp, s.ar, s.ai, s.offset_exp,
+                     s.delta_exp, s.xr, s.xi, s.delta_exp);
+                s.scale = exp2i(s.delta_exp);
+                s.m = 1;
+                s.n = s.n + 1;
+            }
+            refresh_dc<EXT>(s);
+        } else {
+            s.xr = fr;
+            s.xi = fi;
+            s.m = 0;
+        }
+    } else if (EXT) {
+        double magnitude = fmax(fabs(s.xr), fabs(s.xi));
+        int64_t k = 0;
+        if (magnitude > kHi && s.delta_exp < 0)
+            k = -s.delta_exp < 32 ? -s.delta_exp : 32;
+        else if (magnitude < kLo && magnitude != 0.0)
+            k = -32;
+        if (k != 0) {
+            double factor = exp2i(-k);
+            s.xr = s.xr * factor;
+            s.xi = s.xi * factor;
+            s.delta_exp = s.delta_exp + k;
+            refresh_dc<EXT>(s);
+        }
+    }
+    double cr = EXT ? pr : s.xr, ci = EXT ? pi : s.xi;
+    bool res = cr * cr + ci * ci > kResolvable * kResolvable * f2;
+    bool judge = EXT ? (s.returns < 4 && res) : res;
+    if (!judge) {
+        if (EXT) {
+            if (s.n >= s.chk) {
+                s.sr = fr;
+                s.si = fi;
+                s.sn = s.n;
+                s.chk = s.chk * 2;
+            }
+        } else if (s.hold > 0) {
+            if (s.n >= s.sn + s.hold) {
+                s.hold = 0;
+                s.predicted = false;
+                s.pending = true;
+            }
+        } else if (s.n >= s.chk) {
+            s.pending = true;
+        }
+        return RUNNING;
+    }
+    double dist = fabs(fr - s.sr) + fabs(fi - s.si);
+    double size = fabs(s.sr) + fabs(s.si);
+    if (EXT && s.returns < 4 && dist < 1e-13 * size + 1e-300) {
+        if (contracting(orbit, f.len, s.m, cr, ci, s.ar * exp2i(s.offset_exp),
+                        s.ai * exp2i(s.offset_exp), s.n - s.sn))
+            return 2;  // Unresolved
+        s.returns += 1;
+    }
+    if (!EXT) {
+        if (dist < 1e-3 * size) {
+            uint64_t period = s.n - s.sn;
+            if (s.hold == period && mu_below_one(s.mu)) {
+                double sc = s.mu.exp < -1000 ? 0.0 : exp2i(s.mu.exp < 1000 ? s.mu.exp : 1000);
+                double mr = s.mu.re * sc, mi = s.mu.im * sc;
+                double gr = fr - s.sr, gi = fi - s.si;
+                double er = mr - 1.0, ei = mi;
+                double den = er * er + ei * ei;
+                double pr2 = s.sr - (gr * er + gi * ei) / den;
+                double pi2 = s.si - (gi * er - gr * ei) / den;
+                double tol = fmin(0.1 * sqrt(den), 1e-2) * s.qstep + 16.0 * kU * size;
+                if (s.predicted && fabs(pr2 - s.qr) + fabs(pi2 - s.qi) <= tol) return 1;
+                s.qr = pr2;
+                s.qi = pi2;
+                s.qstep = (fabs(gr) + fabs(gi)) / sqrt(den);
+                s.predicted = true;
+            } else {
+                s.predicted = false;
+            }
+            s.hold = period;
+            s.sr = fr;
+            s.si = fi;
+            s.sn = s.n;
+            s.pending = false;
+            s.mu = Scaled{1.0, 0.0, 0};
+            return RUNNING;
+        }
+        if (s.hold > 0) {
+            if (s.n < s.sn + s.hold) return RUNNING;
+            s.hold = 0;
+            s.predicted = false;
+            s.pending = true;
+        }
+        if (s.pending || s.n >= s.chk) {
+            uint64_t np = next_pow2(s.n + 1);
+            s.chk = s.chk > np ? s.chk : np;
+            s.sr = fr;
+            s.si = fi;
+            s.sn = s.n;
+            s.pending = false;
+        }
+        return RUNNING;
+    }
+    if (s.n == s.chk || (EXT && s.n > s.chk)) {
+        s.sr = fr;
+        s.si = fi;
+        s.sn = s.n;
+        s.chk = s.chk * 2;
+    }
+    return RUNNING;
+}
+
+__device__ inline uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ inline unsigned long long wave_sum(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Persistent work-stealing kernel: every lane owns one sample at a time; lanes whose
+// sample finished are refilled together from one atomic per wave, so the wave keeps
+// 64 samples in flight however uneven their iteration counts are.
+template <bool EXT, bool D, bool L>
+__global__ void __launch_bounds__(256) perturb_f64(FrameDev f) {
+    const uint32_t lane = __lane_id();
+    Lane s;
+    bool active = false;
+    bool drained = false;
+    uint32_t idx = 0;
+    unsigned long long its = 0, eq = 0, blocks = 0, skipped = 0, fallback = 0, closed = 0,
+                       plain = 0;
+    for (;;) {
+        uint64_t idle = __ballot(!active);
+        if (idle && !drained) {
+            uint32_t want = __popcll(idle);
+            uint32_t leader = __ffsll((unsigned long long)idle) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(f.next, want);
+            base = __shfl(base, leader);
+            if (base + want >= f.count) drained = true;
+            if (!active) {
+                uint32_t my = base + lanes_below(idle);
+                if (my < f.count && (!f.mask || f.mask[my])) {
+                    idx = my;
+                    uint32_t i = idx % f.nx, j = idx / f.nx;
+                    double x = (double)i + 0.5 - f.half_x;
+                    double y = (double)j + 0.5 - f.half_y;
+                    double ux = f.cos_r * x + f.sin_r * y;
+                    double uy = f.sin_r * x - f.cos_r * y;
+                    if (f.tier == 0) {
+                        double ar = ux * f.h, ai = uy * f.h;
+                        if (in_main_components(f.c_re + ar, f.c_im + ai)) {
+                            f.kind[idx] = 1;
+                            f.n[idx] = 0;
+                            closed += L ? 1 : 0;
+                        } else {
+                            lane_start<EXT, D>(s, f, ar, ai, 0);
+                            active = true;
+                        }
+                    } else if (f.tier == 1) {
+                        lane_start<EXT, D>(s, f, ux * f.h, uy * f.h, 0);
+                        active = true;
+                    } else {
+                        lane_start<EXT, D>(s, f, ux * f.h_m, uy * f.h_m, f.h_e);
+                        active = true;
+                    }
+                }
+            }
+        }
+        if (!__ballot(active)) {
+            if (drained) break;
+            continue;
+        }
+        if (active) {
+            double fr = 0.0, fi = 0.0;
+            bool was_block = false;
+            uint64_t nb = s.blocks;
+            int k = lane_advance<EXT, D, L>(s, f, fr, fi);
+            was_block = s.blocks != nb;
+            plain += was_block ? 0 : 1;
+            if (k != RUNNING) {
+                uint64_t n = k == 2 ? 0 : s.n;
+                f.kind[idx] = (uint8_t)k;
+                f.n[idx] = n;
+                if (k == 0) {
+                    f.zr[idx] = fr;
+                    f.zi[idx] = fi;
+                    f.dr[idx] = s.dr;
+                    f.di[idx] = s.di;
+                    f.dexp[idx] = s.derivative_exp;
+                }
+                uint64_t spent = k == 2 ? f.max_iter : s.n;
+                eq += spent;
+                its += L ? spent - s.skipped + s.blocks : spent;
+                blocks += s.blocks;
+                skipped += s.skipped;
+                fallback += (L && s.blocks == 0) ? 1 : 0;
+                active = false;
+            }
+        }
+    }
+    unsigned long long v[7] = {its, eq, blocks, skipped, fallback, closed, plain};
+    for (int q = 0; q < 7; q++) {
+        unsigned long long t = wave_sum(v[q]);
+        if (lane == 0 && t) atomicAdd(&f.stats[q], t);
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+
+hipError_t launch_bla_build(const double* orbit, double* r, double* r2, double* ab, double* alpha,
+                            double* beta, uint32_t* valid, uint64_t steps, uint64_t stride,
+                            double eps, double dc_max, int64_t dc_exp, uint32_t max_levels,
+                            hipStream_t stream) {
+    BlaBuild t{r, r2, ab, alpha, beta, steps, stride, eps, dc_max, dc_exp, valid};
+    uint64_t cnt = steps >> 1;
+    if (cnt == 0) return hipSuccess;
+    auto grid = [](uint64_t c) { return (unsigned)((c + 255) / 256 < 65536 ? (c + 255) / 256 : 65536); };
+    hipLaunchKernelGGL(bla_level0, dim3(grid(cnt)), dim3(256), 0, stream, orbit, t);
+    for (uint32_t lv = 1; lv < max_levels; lv++) {
+        uint64_t c = steps >> (lv + 1);
+        if (c == 0) break;
+        hipLaunchKernelGGL(bla_level, dim3(grid(c)), dim3(256), 0, stream, t, lv);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_perturb_f64(const FrameDev& f, bool ext, bool deriv, bool bla, int grid,
+                              hipStream_t stream) {
+#define FDG_LAUNCH(E, D, L) \
+    hipLaunchKernelGGL((perturb_f64<E, D, L>), dim3(grid), dim3(256), 0, stream, f)
+    int key = (ext ? 4 : 0) | (deriv ? 2 : 0) | (bla ? 1 : 0);
+    switch (key) {
+        case 0: FDG_LAUNCH(false, false, false); break;
+        case 1: FDG_LAUNCH(false, false, true); break;
+        case 2: FDG_LAUNCH(false, true, false); break;
+        case 3: FDG_LAUNCH(false, true, true); break;
+        case 4: FDG_LAUNCH(true, false, false); break;
+        case 5: FDG_LAUNCH(true, false, true); break;
+        case 6: FDG_LAUNCH(true, true, false); break;
+        default: FDG_LAUNCH(true, true, true); break;
+    }
+#undef FDG_LAUNCH
+    return hipGetLastError();
+}
+
+}  // namespace fdg

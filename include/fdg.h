@@ -1,0 +1,163 @@
+/*
+ * fdg.h - C ABI of the MI355X deep-zoom perturbation/BLA path.
+ *
+ * The drop-in boundary for fractodactyl's GPU perturbation route: the Rust host
+ * (crates/fd-cli/src/perturb_gpu.rs, perturb_gpu/frame.rs) binds these entry points
+ * through a thin FFI crate instead of dispatching WGSL through wgpu.  Plain C types
+ * only; no exception crosses the boundary; every call returns 0 on success or a
+ * negative FDG_E* code with a message from fdg_last_error().
+ *
+ * Threading: a context owns one HIP device, one stream and its resident buffers; use a
+ * context from one host thread at a time.  Several contexts may coexist.
+ */
+#ifndef FDG_H
+#define FDG_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FDG_ABI_VERSION 1
+
+enum {
+    FDG_OK = 0,
+    FDG_EINVAL = -1,    /* invalid request (the reference's Err(String) cases) */
+    FDG_EDECLINED = -2, /* a request this route does not serve (Bound/Stripe columns) */
+    FDG_EDEVICE = -3,   /* HIP runtime / device failure */
+    FDG_ENOMEM = -4
+};
+
+/* Columns: fd_samples::ColumnSet bits (crates/fd-samples/src/column.rs:9-23). */
+enum { FDG_COL_CLASS = 1, FDG_COL_NU = 2, FDG_COL_DE = 4, FDG_COL_NORMAL = 8,
+       FDG_COL_BOUND = 16, FDG_COL_STRIPE = 32 };
+/* Tiers: fd_kernel::Tier (crates/fd-kernel/src/view.rs:6-14); -1 = cheapest valid. */
+enum { FDG_TIER_AUTO = -1, FDG_TIER_F64 = 0, FDG_TIER_FIXED = 1, FDG_TIER_SCALED = 2 };
+/* Routes: `--gpu-perturb on|bla` (crates/fd-cli/src/perturb_gpu/frame.rs:62-84). */
+enum { FDG_ROUTE_PLAIN = 0, FDG_ROUTE_BLA = 1 };
+/* Delta arithmetic.  F64 reproduces the reference CPU kernel bit for bit (Binary64 on
+ * the f64/fx tiers, Binary64Exponent on the scaled tier). */
+enum { FDG_FORM_F64 = 0 };
+/* Outcome kinds (crates/fd-kernel/src/sample.rs:21-40). */
+enum { FDG_ESCAPED = 0, FDG_INTERIOR = 1, FDG_UNRESOLVED = 2 };
+
+typedef struct fdg_ctx fdg_ctx;
+
+/* fd_samples::View (crates/fd-samples/src/header.rs:263-273): exact decimal strings. */
+typedef struct {
+    const char *center_re;
+    const char *center_im;
+    const char *width;
+    double rotation;
+} fdg_view;
+
+/* fd_kernel::Params (crates/fd-kernel/src/grid.rs:17-38); worker threads and stripe
+ * parameters have no meaning here. */
+typedef struct {
+    uint32_t nx, ny, ss;
+    uint64_t max_iter;
+    double escape_radius;
+    uint32_t columns;
+    int32_t tier;
+} fdg_params;
+
+/* fd_samples::Samples columns (crates/fd-samples/src/samples.rs:6-21), caller-allocated
+ * with nx*ny elements each; a column not requested may be NULL. */
+typedef struct {
+    uint8_t *class_;
+    double *nu;
+    float *de;
+    uint16_t *normal;
+} fdg_columns;
+
+/* Raw per-sample outcomes before quantisation (sample.rs:21-40); any member may be NULL.
+ * z and dz/dc = (dr, di) 2^dexp are meaningful for escaped samples only; n is 0 for
+ * unresolved ones. */
+typedef struct {
+    uint8_t *kind;
+    uint64_t *n;
+    double *zr, *zi, *dr, *di;
+    int64_t *dexp;
+} fdg_outcomes;
+
+/* Work counters (grid.rs:40-70 Stats + BlaStats) and timings. */
+typedef struct {
+    uint64_t reference_len;
+    uint32_t reference_bits;
+    uint32_t bla_levels;
+    uint64_t nucleus_period;
+    uint64_t iterations;          /* Stats::iterations (an applied block counts once) */
+    uint64_t equivalent;          /* plain-equivalent iterations (skipped steps counted) */
+    uint64_t blocks, skipped, plain_steps;
+    uint64_t fallback_samples, closed_form_samples;
+    double reference_seconds;     /* host arbitrary-precision orbit */
+    double bla_seconds;           /* GPU table build (device time) */
+    double kernel_seconds;        /* GPU perturbation kernel (device time) */
+    double transfer_seconds;      /* uploads + outcome readback */
+    double encode_seconds;        /* Store::put into the columns (host) */
+} fdg_stats;
+
+int fdg_abi_version(void);
+
+/* Open a context on HIP device `device`.  No counterpart in the reference beyond the
+ * wgpu device open in crates/fd-shade/src/gpu/testkit.rs:12-47. */
+int fdg_create(fdg_ctx **out, int device);
+void fdg_destroy(fdg_ctx *ctx);
+/* Message for the last non-zero status on this context (or of fdg_create when ctx is
+ * NULL). */
+const char *fdg_last_error(const fdg_ctx *ctx);
+
+/* perturb_gpu::enabled (frame.rs:62-68): "on" -> 1 route plain, "bla" -> 1 route BLA,
+ * "off"/NULL -> 0; anything else is FDG_EINVAL. */
+int fdg_enabled(const char *value, int *enabled, int *route);
+
+/* perturb_gpu::render / render_bla (frame.rs:70-84, render_inner :119-355).  Renders every
+ * sample of the frame on the GPU: the reference orbit (the exact-centre fixed-point
+ * orbit of Reference::from_fixed, or Reference::new on the f64 tier) is computed on the
+ * host; route FDG_ROUTE_BLA builds a Bla table on the GPU over it with `bla_eps` for the
+ * frame's |dc| bound (as `fd control --bla per-frame` does, control.rs:356-366) and
+ * applies its blocks.  Writes the requested columns, exactly fd_kernel::render_with /
+ * render_bla's values for form FDG_FORM_F64, optionally the raw outcomes, and the header
+ * kernel id (plane kernel, " bla/1" when blocks were used, then " gpu-hip-f64/1"). */
+int fdg_render(fdg_ctx *ctx, const fdg_view *view, const fdg_params *params, int route, int form,
+               double bla_eps, fdg_columns *out, fdg_outcomes *raw, fdg_stats *stats,
+               char *kernel_id, size_t kernel_cap);
+
+/* The same frame split so that its inputs stay resident in HBM between passes, for
+ * callers that render a view repeatedly (benchmarks, progressive refinement, films): no
+ * separate counterpart in the reference, where render_inner (frame.rs:119-355) does all
+ * four steps in one call.  fdg_frame_open computes the reference orbit on the host and
+ * uploads it; fdg_frame_run builds the BLA table (route BLA) and runs the perturbation
+ * kernel, leaving the outcomes on the device; fdg_frame_read copies them back and encodes
+ * the columns (as fdg_render); fdg_frame_close frees the frame's device memory. */
+typedef struct fdg_frame fdg_frame;
+int fdg_frame_open(fdg_ctx *ctx, const fdg_view *view, const fdg_params *params, int route,
+                   int form, double bla_eps, fdg_frame **out);
+int fdg_frame_run(fdg_frame *frame, fdg_stats *stats);
+int fdg_frame_read(fdg_frame *frame, fdg_columns *out, fdg_outcomes *raw, fdg_stats *stats,
+                   char *kernel_id, size_t kernel_cap);
+void fdg_frame_close(fdg_frame *frame);
+
+/* fd_kernel::reference (grid.rs:84-87): the orbit the renderer uses and its precision in
+ * bits.  With cap too small, only *len and *bits are written.  Host-only: ctx may be NULL. */
+int fdg_reference(fdg_ctx *ctx, const fdg_view *view, const fdg_params *params, double *re,
+                  double *im, uint64_t cap, uint64_t *len, uint32_t *bits);
+
+/* Bla::build_scaled (crates/fd-kernel/src/bla.rs:161-181) computed on the GPU as a
+ * level-by-level tree reduction.  Blocks of levels 1.. are written level-major, 7 doubles
+ * each (a.re a.im b.re b.im r alpha beta, Block::to_array order, bla.rs:321-325), level j
+ * holding (len-2)>>j blocks; *levels receives the stored level count. */
+int fdg_build_bla(fdg_ctx *ctx, const double *re, const double *im, uint64_t len, double eps,
+                  double dc_max, int64_t dc_exp, double *blocks, uint64_t cap_blocks,
+                  uint32_t *levels);
+
+/* fd_samples::write (crates/fd-samples/src/write.rs:8-57): canonical .fds bytes of the
+ * columns.  Returns the byte count (the size needed when cap is too small), or < 0. */
+int64_t fdg_write_fds(const fdg_view *view, const fdg_params *params, const char *kernel_id,
+                      const fdg_columns *columns, uint8_t *buf, uint64_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,106 @@
+"""CPU checks of the C ABI (include/fdg.h) and the host logic of libfdg.so: the library
+loads, exports every declared entry point, and its host-only pieces (argument parsing,
+the arbitrary-precision reference orbit, the .fds writer) agree with the oracle."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import fractodactyl_amd as fa
+import oracle
+from fdsfile import kern05, parse_fds
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    text = open(os.path.join(ROOT, "include", "fdg.h")).read()
+    return sorted(set(re.findall(r"\b(fdg_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = fa.lib()
+    names = declared()
+    assert "fdg_render" in names and "fdg_build_bla" in names
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(fa.SIGNATURES), set(names) ^ set(fa.SIGNATURES)
+    assert L.fdg_abi_version() == 1
+
+
+def test_enabled_parses_the_route_flag():
+    # frame.rs:62-68
+    assert fa.enabled(None) == (False, 0)
+    assert fa.enabled("off") == (False, 0)
+    assert fa.enabled("on") == (True, fa.ROUTE_PLAIN)
+    assert fa.enabled("bla") == (True, fa.ROUTE_BLA)
+    with pytest.raises(ValueError):
+        fa.enabled("yes")
+
+
+def test_create_without_a_device_fails_cleanly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    with pytest.raises(fa.FdgError) as e:
+        fa.Context(0)
+    assert e.value.code == -3
+
+
+def _host_reference(view, params):
+    ln, bits = C.c_uint64(), C.c_uint32()
+    v, p = fa._view(view), fa._params(params)
+    assert fa.lib().fdg_reference(None, C.byref(v), C.byref(p), None, None, 0, C.byref(ln),
+                                  C.byref(bits)) == 0
+    re_ = np.zeros(ln.value)
+    im_ = np.zeros(ln.value)
+    assert fa.lib().fdg_reference(None, C.byref(v), C.byref(p), fa._p(re_), fa._p(im_),
+                                  ln.value, C.byref(ln), C.byref(bits)) == 0
+    return re_, im_, bits.value
+
+
+@pytest.mark.parametrize("re_,im_,width,iters,tier", [
+    ("-0.7453", "0.1127", "0.0065", 20000, -1),
+    ("-0.7453", "0.1127", "0.0065", 20000, 2),
+    ("-0.743643887037158704752191506114774", "0.131825904205311970493132056385139", "1e-28",
+     50000, -1),
+    ("0", "1", "1e-1000", 20000, -1),
+    ("-2", "0", "1e-200", 20000, -1),
+])
+def test_host_reference_orbit_matches_oracle(re_, im_, width, iters, tier):
+    v = fa.View(re_, im_, width, 0.0)
+    p = fa.Params(192, 108, max_iter=iters, tier=tier)
+    a_re, a_im, a_bits = _host_reference(v, p)
+    b_re, b_im, b_bits = oracle.reference(re_, im_, width, 0.0, 192, 108, iters, 1e10, tier)
+    assert a_bits == b_bits
+    assert np.array_equal(a_re.view(np.int64), b_re.view(np.int64))
+    assert np.array_equal(a_im.view(np.int64), b_im.view(np.int64))
+
+
+def test_host_reference_rejects_bad_requests():
+    v, p = fa._view(fa.View("abc", "0", "1e-3")), fa._params(fa.Params(8, 8))
+    ln, bits = C.c_uint64(), C.c_uint32()
+    assert fa.lib().fdg_reference(None, C.byref(v), C.byref(p), None, None, 0, C.byref(ln),
+                                  C.byref(bits)) == -1
+    assert b"decimal" in fa.lib().fdg_last_error(None)
+    v = fa._view(fa.View("0", "0", "1e-3"))
+    p = fa._params(fa.Params(8, 8, ss=3))
+    assert fa.lib().fdg_reference(None, C.byref(v), C.byref(p), None, None, 0, C.byref(ln),
+                                  C.byref(bits)) == -1
+    p = fa._params(fa.Params(8, 8, tier=fa.TIER_F64))
+    v = fa._view(fa.View("0", "1", "1e-300"))
+    assert fa.lib().fdg_reference(None, C.byref(v), C.byref(p), None, None, 0, C.byref(ln),
+                                  C.byref(bits)) == -1
+
+
+def test_writer_matches_golden_bytes():
+    # write.rs / header.rs through the product's writer, on the golden frames' own columns
+    for name, data in kern05()[:4]:
+        h, cols = parse_fds(data)
+        header = dict(view=fa.View(h["re"], h["im"], h["width"], h["rotation"]),
+                      params=fa.Params(h["nx"], h["ny"], h["ss"], h["max_iter"],
+                                       h["escape_radius"], h["columns"] & ~1),
+                      kernel=h["kernel"])
+        assert fa.write(header, cols, minor=0) == data, name
