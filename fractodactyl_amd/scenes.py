@@ -144,8 +144,9 @@ def find_nucleus(seed, pmin, pmax, bits, box=((-1.8, 0.4), (0.05, 0.9)), tries=4
             nr, ni, ok = newton(out[0], out[1], p, b)
             if not ok:
                 continue
+            # Newton's last step was below 2^(24 - b): c is the nucleus to ~bits bits.
             lz, div, lsize = nucleus_check(nr, ni, p, b)
-            if div is not None or lz > -bits + 16:
+            if div is not None:
                 continue
             if max_log10_size is not None and lsize > max_log10_size:
                 continue
@@ -161,7 +162,7 @@ CONFIGS = {
     1: dict(pmin=200, pmax=2000, bits=256, width="1e-20", nx=480, ny=270, max_iter=20_000,
             escape_radius=1e3, bla=False, shift=0.0, max_log10_size=-21),
     2: dict(pmin=1000, pmax=10000, bits=512, width="1e-100", nx=1920, ny=1080,
-            max_iter=100_000, escape_radius=1e10, bla=True, shift=0.3, max_log10_size=-101),
+            max_iter=100_000, escape_radius=1e10, bla=True, shift=0.3),
 }
 
 
