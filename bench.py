@@ -109,6 +109,8 @@ def main():
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--frames", default="config,escape",
+                    help="which frames to run: config (the JSON line), escape (companion)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this path runs on one GPU; --gpus {a.gpus} is not supported "
@@ -124,13 +126,23 @@ def main():
     params = fa.Params(cfg["nx"], cfg["ny"], 1, cfg["max_iter"], cfg["escape_radius"], cols)
     route = fa.ROUTE_BLA if cfg["bla"] else fa.ROUTE_PLAIN
     fa.default_context()
+    frames = a.frames.split(",")
+    if "config" not in frames:
+        # profiling aid: the escape companion alone, no JSON line
+        ka = json.load(open(os.path.join(ROOT, "tests", "golden", "known_answers.json")))
+        v0 = fa.View(ka["v0_centre"]["re"], ka["v0_centre"]["im"], "2e-49", 0.0)
+        p0 = fa.Params(1920, 1080, 1, 100_000, 1e10, cols)
+        m0 = measure(fa, v0, p0, fa.ROUTE_BLA, a.steps, a.warmup, "v0-2e-49")
+        log(json.dumps(m0))
+        return
     m = measure(fa, view, params, route, a.steps, a.warmup, f"config{a.config}")
     # Escape-dominated companion: the reference's own deep-zoom path (bench/path-atlas-v0.txt,
     # the period-764 minibrot's neighbourhood) at its frame-749 width, at 1080p.
     ka = json.load(open(os.path.join(ROOT, "tests", "golden", "known_answers.json")))
     v0 = fa.View(ka["v0_centre"]["re"], ka["v0_centre"]["im"], "2e-49", 0.0)
     p0 = fa.Params(1920, 1080, 1, 100_000, 1e10, cols)
-    m0 = measure(fa, v0, p0, fa.ROUTE_BLA, a.steps, a.warmup, "v0-2e-49")
+    m0 = (measure(fa, v0, p0, fa.ROUTE_BLA, a.steps, a.warmup, "v0-2e-49")
+          if "escape" in frames else {})
     out = {
         "metric": "effective pixel-iter/s at 1080p deep zoom (BLA-skipped steps included)",
         "value": m["value"],
@@ -161,8 +173,9 @@ def main():
         t = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline(view, params, route == fa.ROUTE_BLA, a.cpu_seconds,
                                            threads)
-        out["config"]["escape_frame"]["cpu_baseline"] = cpu_baseline(v0, p0, True,
-                                                                     a.cpu_seconds, threads)
+        if "escape" in frames:
+            out["config"]["escape_frame"]["cpu_baseline"] = cpu_baseline(
+                v0, p0, True, a.cpu_seconds, threads)
         log(f"cpu baselines: {time.perf_counter() - t:.1f}s")
     print(json.dumps(out), flush=True)
 

@@ -391,6 +391,8 @@ int fdg_frame_open(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int rou
         return ctx->fail(FDG_EINVAL, "unknown route");
     uint64_t count64 = (uint64_t)p->nx * p->ny;
     if (count64 >= (1ull << 32)) return ctx->fail(FDG_EINVAL, "grid too large");
+    // The kernels count iterates in 32 bits.
+    if (p->max_iter >= (1ull << 31)) return ctx->fail(FDG_EINVAL, "max_iter must be < 2^31");
     if (route == FDG_ROUTE_BLA) {
         // grid.rs:378-413 bla_checks, and bla.rs:27-33 check
         if (std::isnan(p->escape_radius) || p->escape_radius < 4.0)

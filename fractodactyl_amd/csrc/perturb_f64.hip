@@ -169,67 +169,57 @@ __device__ bool contracting(const double* orbit, uint64_t len, uint64_t m0, doub
 }
 
 // sample.rs:597-652 Multiplier and sample.rs:657-698 Contraction: a complex product with
-// a separate binary exponent.
-struct Scaled {
-    double re, im;
-    int64_t exp;
-};
-
-__device__ inline void mu_multiply(Scaled& s, double re, double im) {
-    double nr = s.re * re - s.im * im, ni = s.re * im + s.im * re;
-    s.re = nr;
-    s.im = ni;
+// a separate binary exponent.  The exponent choice only picks an exact power-of-two split.
+__device__ inline void mu_multiply(double& re_, double& im_, int32_t& ex, double re, double im) {
+    double nr = re_ * re - im_ * im, ni = re_ * im + im_ * re;
+    re_ = nr;
+    im_ = ni;
     double a = fmax(fabs(nr), fabs(ni));
     if (!(a > 1e-120 && a < 1e120) && a > 0.0 && isfinite(a)) {
-        int64_t e = floor_log2(a);
+        int32_t e = (int32_t)floor_log2(a);
         double sc = exp2i(-e);
-        s.re *= sc;
-        s.im *= sc;
-        s.exp += e;
+        re_ *= sc;
+        im_ *= sc;
+        ex += e;
     }
 }
 
-__device__ inline bool mu_below_one(const Scaled& s) {
-    return isfinite(s.re) && isfinite(s.im) &&
-           (s.exp < -64 || (s.exp <= 64 && (s.re * s.re + s.im * s.im) * exp2i(2 * s.exp) < 1.0));
+__device__ inline bool scaled_below(double re, double im, int32_t ex, double bound) {
+    return isfinite(re) && isfinite(im) &&
+           (ex < -64 || (ex <= 64 && (re * re + im * im) * exp2i(2 * (int64_t)ex) < bound));
 }
 
-__device__ inline void con_multiply(Scaled& s, bool& started, double re, double im) {
+__device__ inline void con_multiply(double& re_, double& im_, int32_t& ex, bool& started,
+                                    double re, double im) {
     if (!started) {
-        s.re = 1.0;
+        re_ = 1.0;
         started = true;
     }
-    int64_t exp = 0;
+    int32_t exp = 0;
     double a = fmax(fabs(re), fabs(im));
     if (a != 0.0 && !(kLo <= a && a <= kHi)) {
-        int64_t e = floor_log2(a);
+        int32_t e = (int32_t)floor_log2(a);
         double sc = exp2i(-e);
         re *= sc;
         im *= sc;
         exp += e;
     }
-    double nr = s.re * re - s.im * im, ni = s.re * im + s.im * re;
-    s.re = nr;
-    s.im = ni;
-    s.exp += exp;
-    a = fmax(fabs(s.re), fabs(s.im));
+    double nr = re_ * re - im_ * im, ni = re_ * im + im_ * re;
+    re_ = nr;
+    im_ = ni;
+    ex += exp;
+    a = fmax(fabs(re_), fabs(im_));
     if (a != 0.0 && !(kLo <= a && a <= kHi)) {
-        int64_t e = floor_log2(a);
+        int32_t e = (int32_t)floor_log2(a);
         double sc = exp2i(-e);
-        s.re *= sc;
-        s.im *= sc;
-        s.exp += e;
+        re_ *= sc;
+        im_ *= sc;
+        ex += e;
     }
 }
 
-__device__ inline bool con_small(const Scaled& s, bool started) {
-    return started && isfinite(s.re) && isfinite(s.im) &&
-           (s.exp < -64 ||
-            (s.exp <= 64 && (s.re * s.re + s.im * s.im) * exp2i(2 * s.exp) < 1e-6));
-}
-
 // sample.rs:586-595
-__device__ inline void normalise(double& re, double& im, int64_t& e, int64_t max_e) {
+__device__ inline void normalise(double& re, double& im, int32_t& e, int64_t max_e) {
     double a = fmax(fabs(re), fabs(im));
     if (a == 0.0 || !isfinite(a)) return;
     int64_t k = floor_log2(a);
@@ -238,12 +228,12 @@ __device__ inline void normalise(double& re, double& im, int64_t& e, int64_t max
     double h = exp2i(-(k / 2)), l = exp2i(-(k - k / 2));
     re = re * h * l;
     im = im * h * l;
-    e = e + k;
+    e = (int32_t)(e + k);
 }
 
 // sample.rs:1094-1114 sum2 over two terms
 __device__ inline void sum2(double t0r, double t0i, int64_t t0e, double t1r, double t1i,
-                            int64_t t1e, int64_t zero_e, double& ore, double& oim, int64_t& oe) {
+                            int64_t t1e, int64_t zero_e, double& ore, double& oim, int32_t& oe) {
     bool have = false;
     int64_t top = 0;
     double a0 = fmax(fabs(t0r), fabs(t0i)), a1 = fmax(fabs(t1r), fabs(t1i));
@@ -259,7 +249,7 @@ __device__ inline void sum2(double t0r, double t0i, int64_t t0e, double t1r, dou
     if (!have) {
         ore = 0.0;
         oim = 0.0;
-        oe = zero_e;
+        oe = (int32_t)zero_e;
         return;
     }
     int64_t k0 = t0e - top, k1 = t1e - top;
@@ -270,43 +260,30 @@ __device__ inline void sum2(double t0r, double t0i, int64_t t0e, double t1r, dou
     i = i + (k1 < -2046 ? 0.0 : t1i * exp2i(k1 / 2) * exp2i(k1 - k1 / 2));
     ore = r;
     oim = i;
-    oe = top;
+    oe = (int32_t)top;
 }
 
-// bla.rs:234-249 Bla::find: level index of the longest valid block at m, or -1.
-__device__ inline int bla_find(const BlaDev& t, uint64_t m, double d2, uint64_t budget) {
-    uint64_t k = m - 1;
-    int tz = k == 0 ? 64 : __builtin_ctzll(k);
-    int top = tz < (int)t.nlevels ? tz : (int)t.nlevels;
-    if (top == 0) return -1;
-    int i = -1;
-    for (int lv = 0; lv < top; lv++) {
-        uint64_t idx = k >> (lv + 1);
-        if (!((2ull << lv) <= budget && idx < (t.steps >> (lv + 1)) &&
-              d2 < t.r2[bla_level_offset(t.stride, lv) + idx]))
-            break;
-        i = lv;
-    }
-    return i;
-}
-
-// One lane's sample state (sample.rs:190-236).
+// One lane's sample state (sample.rs:190-236), kept in registers.  (Zr, Zi) is Z_m,
+// carried from the previous escape test, so a step loads the orbit once (its landing
+// point) and the load is issued before the step's arithmetic.
 struct Lane {
-    double ar, ai, xr, xi, dr, di, dc_r, dc_i, scale, inverse;
+    double xr, xi, dr, di, ar, ai, Zr, Zi;
     double sr, si, qr, qi, qstep;
-    Scaled mu, con;
-    int64_t delta_exp, derivative_exp, offset_exp;
-    uint64_t m, n, chk, sn, hold, nucleus, blocks, skipped;
-    int returns;
+    double mur, mui, conr, coni;
+    double scale, dc_r, dc_i, inverse;  // scaled tier only
+    int32_t delta_exp, derivative_exp, offset_exp, mue, cone;
+    uint32_t m, n, sn, hold, chk, nucleus, blocks, skipped, plain, returns;
     bool pending, predicted, con_started;
 };
 
 template <bool EXT, bool D>
-__device__ inline void lane_start(Lane& s, const FrameDev& f, double ar, double ai, int64_t oe) {
+__device__ inline void lane_start(Lane& s, const FrameDev& f, double ar, double ai, int32_t oe) {
     s.ar = ar;
     s.ai = ai;
     s.xr = 0.0;
     s.xi = 0.0;
+    s.Zr = 0.0;  // Z_0 = 0
+    s.Zi = 0.0;
     s.offset_exp = oe;
     s.delta_exp = EXT ? oe : 0;
     s.scale = EXT ? exp2i(s.delta_exp) : 1.0;
@@ -324,15 +301,20 @@ __device__ inline void lane_start(Lane& s, const FrameDev& f, double ar, double 
     s.sn = EXT ? 0 : 8;
     s.returns = 0;
     s.pending = false;
-    s.mu = Scaled{1.0, 0.0, 0};
+    s.mur = 1.0;
+    s.mui = 0.0;
+    s.mue = 0;
     s.hold = 0;
     s.predicted = false;
     s.qr = s.qi = s.qstep = 0.0;
-    s.nucleus = f.nucleus;
-    s.con = Scaled{0.0, 0.0, 0};
+    s.nucleus = (uint32_t)f.nucleus;
+    s.conr = 0.0;
+    s.coni = 0.0;
+    s.cone = 0;
     s.con_started = false;
     s.blocks = 0;
     s.skipped = 0;
+    s.plain = 0;
 }
 
 // Re-derive dc in the delta's units after its exponent changed (sample.rs:300-302).
@@ -340,46 +322,62 @@ template <bool EXT>
 __device__ inline void refresh_dc(Lane& s) {
     if (EXT) {
         s.scale = exp2i(s.delta_exp);
-        double factor = exp2i(s.offset_exp - s.delta_exp);
+        double factor = exp2i((int64_t)s.offset_exp - s.delta_exp);
         s.dc_r = s.ar * factor;
         s.dc_i = s.ai * factor;
     }
 }
 
-// Outcome codes returned by lane_advance.
 enum : int { RUNNING = -1 };
 
 // One block or one plain step plus the checks after it (sample.rs:237-575).  Returns
-// RUNNING, or the finished outcome kind with the escaped state left in the lane.
+// RUNNING, or the finished outcome kind (0 escaped with z in ofr/ofi, 1 interior,
+// 2 unresolved).
 template <bool EXT, bool D, bool L>
 __device__ inline int lane_advance(Lane& s, const FrameDev& f, double& ofr, double& ofi) {
     // sample.rs:237 `while n < limit`: the budget ran out.
-    if (s.n >= f.max_iter) return 2;  // Unresolved
-    const double* orbit = f.orbit;
-    const uint64_t last = f.len - 1;
+    if (s.n >= (uint32_t)f.max_iter) return 2;
+    const double2* orbit = reinterpret_cast<const double2*>(f.orbit);
+    const uint32_t last = (uint32_t)f.len - 1;
+    double2 Zn = orbit[s.m + 1];  // landing point of a plain step (s.m < last here)
     int lvl = -1;
     if (L && (s.nucleus == 0 || s.m != 0)) {
-        double d2 = EXT ? (s.xr * s.xr + s.xi * s.xi) * s.scale * s.scale
-                        : s.xr * s.xr + s.xi * s.xi;
-        uint64_t b = s.hold > 0 ? s.sn + s.hold : (s.pending ? ~0ull : s.chk);
-        b = b < f.max_iter ? b : f.max_iter;
-        uint64_t nb = s.nucleus ? (s.n == 0 ? 1 : s.nucleus + 1) : ~0ull;
-        b = b < nb ? b : nb;
-        b = b > s.n ? b - s.n : 0;
-        lvl = bla_find(f.bla, s.m, d2, b);
+        // Bla::find (bla.rs:234-249): blocks of 2^(i+1) steps start at k = m - 1 divisible
+        // by 2^(i+1), so an odd k (m = 0 included, k wraps) has none.
+        uint32_t k = s.m - 1;
+        if ((k & 1u) == 0) {
+            int top = k == 0 ? (int)f.bla.nlevels : __builtin_ctz(k);
+            top = top < (int)f.bla.nlevels ? top : (int)f.bla.nlevels;
+            double d2 = EXT ? (s.xr * s.xr + s.xi * s.xi) * s.scale * s.scale
+                            : s.xr * s.xr + s.xi * s.xi;
+            uint32_t b = s.hold > 0 ? s.sn + s.hold : (s.pending ? 0xffffffffu : s.chk);
+            uint32_t mi = (uint32_t)f.max_iter;
+            b = b < mi ? b : mi;
+            uint32_t nb = s.nucleus ? (s.n == 0 ? 1u : s.nucleus + 1) : 0xffffffffu;
+            b = b < nb ? b : nb;
+            b = b > s.n ? b - s.n : 0;
+            for (int lv = 0; lv < top; lv++) {
+                uint32_t idx = k >> (lv + 1);
+                if (!((2ull << lv) <= (uint64_t)b && (uint64_t)idx < (f.bla.steps >> (lv + 1)) &&
+                      d2 < f.bla.r2[bla_level_offset(f.bla.stride, lv) + idx]))
+                    break;
+                lvl = lv;
+            }
+        }
     }
     if (lvl >= 0) {
         uint64_t at = bla_level_offset(f.bla.stride, lvl) + ((s.m - 1) >> (lvl + 1));
         double4 ab = reinterpret_cast<const double4*>(f.bla.ab)[at];
         double a_r = ab.x, a_i = ab.y, b_r = ab.z, b_i = ab.w;
-        uint64_t blen = 2ull << lvl;
-        if (s.nucleus && s.n >= 1) con_multiply(s.con, s.con_started, a_r, a_i);
-        if (!EXT && s.hold > 0) mu_multiply(s.mu, a_r, a_i);
+        uint32_t blen = 2u << lvl;
+        Zn = orbit[s.m + blen];
+        if (s.nucleus && s.n >= 1) con_multiply(s.conr, s.coni, s.cone, s.con_started, a_r, a_i);
+        if (!EXT && s.hold > 0) mu_multiply(s.mur, s.mui, s.mue, a_r, a_i);
         if (EXT) {
             normalise(s.xr, s.xi, s.delta_exp, 0);
             if (D) {
                 normalise(s.dr, s.di, s.derivative_exp, INT64_MAX / 4);
-                s.inverse = exp2i(-s.derivative_exp);
+                s.inverse = exp2i(-(int64_t)s.derivative_exp);
             }
             refresh_dc<EXT>(s);
         }
@@ -395,7 +393,7 @@ __device__ inline int lane_advance(Lane& s, const FrameDev& f, double& ofr, doub
             normalise(s.xr, s.xi, s.delta_exp, 0);
             if (D) {
                 normalise(s.dr, s.di, s.derivative_exp, INT64_MAX / 4);
-                s.inverse = exp2i(-s.derivative_exp);
+                s.inverse = exp2i(-(int64_t)s.derivative_exp);
             }
             refresh_dc<EXT>(s);
         }
@@ -404,10 +402,11 @@ __device__ inline int lane_advance(Lane& s, const FrameDev& f, double& ofr, doub
         s.blocks += 1;
         s.skipped += blen;
     } else {
-        double Zr = orbit[2 * s.m], Zi = orbit[2 * s.m + 1];
+        double Zr = s.Zr, Zi = s.Zi;
         double fr = Zr + s.scale * s.xr, fi = Zi + s.scale * s.xi;
-        if (s.nucleus && s.n >= 1) con_multiply(s.con, s.con_started, 2.0 * fr, 2.0 * fi);
-        if (!EXT && s.hold > 0) mu_multiply(s.mu, 2.0 * fr, 2.0 * fi);
+        if (s.nucleus && s.n >= 1)
+            con_multiply(s.conr, s.coni, s.cone, s.con_started, 2.0 * fr, 2.0 * fi);
+        if (!EXT && s.hold > 0) mu_multiply(s.mur, s.mui, s.mue, 2.0 * fr, 2.0 * fi);
         if (D) {
             double t = 2.0 * (fr * s.dr - fi * s.di) + s.inverse;
             s.di = 2.0 * (fr * s.di + fi * s.dr);
@@ -416,7 +415,7 @@ __device__ inline int lane_advance(Lane& s, const FrameDev& f, double& ofr, doub
                 s.dr = s.dr * kLo;
                 s.di = s.di * kLo;
                 s.derivative_exp += 32;
-                s.inverse = exp2i(-s.derivative_exp);
+                s.inverse = exp2i(-(int64_t)s.derivative_exp);
             }
         }
         double xr = s.xr, xi = s.xi;
@@ -429,9 +428,12 @@ __device__ inline int lane_advance(Lane& s, const FrameDev& f, double& ofr, doub
         }
         s.m += 1;
         s.n += 1;
+        s.plain += 1;
     }
+    s.Zr = Zn.x;
+    s.Zi = Zn.y;
     double pr = s.scale * s.xr, pi = s.scale * s.xi;
-    double fr = orbit[2 * s.m] + pr, fi = orbit[2 * s.m + 1] + pi;
+    double fr = Zn.x + pr, fi = Zn.y + pi;
     double f2 = fr * fr + fi * fi;
     if (f2 > f.r2) {
         ofr = fr;
@@ -439,7 +441,7 @@ __device__ inline int lane_advance(Lane& s, const FrameDev& f, double& ofr, doub
         return 0;  // Escaped
     }
     if (s.nucleus && s.n == s.nucleus + 1) {
-        if (con_small(s.con, s.con_started)) return 1;  // Interior
+        if (s.con_started && scaled_below(s.conr, s.coni, s.cone, 1e-6)) return 1;  // Interior
         s.nucleus = 0;
     }
     double delta2 = pr * pr + pi * pi;
@@ -451,38 +453,45 @@ __device__ inline int lane_advance(Lane& s, const FrameDev& f, double& ofr, doub
                 s.xi = fi;
                 s.delta_exp = 0;
             } else {
-                sum2(orbit[2 * s.m], orbit[2 * s.m + 1], 0, s.xr, s.xi, s.delta_exp, s.delta_exp,
-                     s.xr, s.xi, s.delta_exp);
+                sum2(Zn.x, Zn.y, 0, s.xr, s.xi, s.delta_exp, s.delta_exp, s.xr, s.xi,
+                     s.delta_exp);
             }
             s.scale = exp2i(s.delta_exp);
             s.m = 0;
+            s.Zr = 0.0;
+            s.Zi = 0.0;
             if (s.scale < kTiny) {
                 s.nucleus = 0;
                 if (D) {
                     // zero_step_dz (sample.rs:1124-1134)
                     double qr = 2.0 * (s.xr * s.dr - s.xi * s.di);
                     double qi = 2.0 * (s.xr * s.di + s.xi * s.dr);
-                    sum2(qr, qi, s.delta_exp + s.derivative_exp, 1.0, 0.0, 0, s.derivative_exp,
-                         s.dr, s.di, s.derivative_exp);
-                    s.inverse = exp2i(-s.derivative_exp);
+                    sum2(qr, qi, (int64_t)s.delta_exp + s.derivative_exp, 1.0, 0.0, 0,
+                         s.derivative_exp, s.dr, s.di, s.derivative_exp);
+                    s.inverse = exp2i(-(int64_t)s.derivative_exp);
                 }
                 // zero_step (sample.rs:1116-1122)
                 double wr = s.xr, wi = s.xi;
-                sum2(wr * wr - wi * wi, 2.0 * wr * wi, 2 * s.delta_exp, s.ar, s.ai, s.offset_exp,
-                     s.delta_exp, s.xr, s.xi, s.delta_exp);
+                sum2(wr * wr - wi * wi, 2.0 * wr * wi, 2 * (int64_t)s.delta_exp, s.ar, s.ai,
+                     s.offset_exp, s.delta_exp, s.xr, s.xi, s.delta_exp);
                 s.scale = exp2i(s.delta_exp);
                 s.m = 1;
                 s.n = s.n + 1;
+                double2 z1 = orbit[1];
+                s.Zr = z1.x;
+                s.Zi = z1.y;
             }
             refresh_dc<EXT>(s);
         } else {
             s.xr = fr;
             s.xi = fi;
             s.m = 0;
+            s.Zr = 0.0;
+            s.Zi = 0.0;
         }
     } else if (EXT) {
         double magnitude = fmax(fabs(s.xr), fabs(s.xi));
-        int64_t k = 0;
+        int32_t k = 0;
         if (magnitude > kHi && s.delta_exp < 0)
             k = -s.delta_exp < 32 ? -s.delta_exp : 32;
         else if (magnitude < kLo && magnitude != 0.0)
@@ -520,17 +529,17 @@ __device__ inline int lane_advance(Lane& s, const FrameDev& f, double& ofr, doub
     double dist = fabs(fr - s.sr) + fabs(fi - s.si);
     double size = fabs(s.sr) + fabs(s.si);
     if (EXT && s.returns < 4 && dist < 1e-13 * size + 1e-300) {
-        if (contracting(orbit, f.len, s.m, cr, ci, s.ar * exp2i(s.offset_exp),
+        if (contracting(f.orbit, f.len, s.m, cr, ci, s.ar * exp2i(s.offset_exp),
                         s.ai * exp2i(s.offset_exp), s.n - s.sn))
             return 2;  // Unresolved
         s.returns += 1;
     }
     if (!EXT) {
         if (dist < 1e-3 * size) {
-            uint64_t period = s.n - s.sn;
-            if (s.hold == period && mu_below_one(s.mu)) {
-                double sc = s.mu.exp < -1000 ? 0.0 : exp2i(s.mu.exp < 1000 ? s.mu.exp : 1000);
-                double mr = s.mu.re * sc, mi = s.mu.im * sc;
+            uint32_t period = s.n - s.sn;
+            if (s.hold == period && scaled_below(s.mur, s.mui, s.mue, 1.0)) {
+                double sc = s.mue < -1000 ? 0.0 : exp2i(s.mue < 1000 ? s.mue : 1000);
+                double mr = s.mur * sc, mi = s.mui * sc;
                 double gr = fr - s.sr, gi = fi - s.si;
                 double er = mr - 1.0, ei = mi;
                 double den = er * er + ei * ei;
@@ -550,7 +559,9 @@ __device__ inline int lane_advance(Lane& s, const FrameDev& f, double& ofr, doub
             s.si = fi;
             s.sn = s.n;
             s.pending = false;
-            s.mu = Scaled{1.0, 0.0, 0};
+            s.mur = 1.0;
+            s.mui = 0.0;
+            s.mue = 0;
             return RUNNING;
         }
         if (s.hold > 0) {
@@ -560,7 +571,7 @@ __device__ inline int lane_advance(Lane& s, const FrameDev& f, double& ofr, doub
             s.pending = true;
         }
         if (s.pending || s.n >= s.chk) {
-            uint64_t np = next_pow2(s.n + 1);
+            uint32_t np = (uint32_t)next_pow2((uint64_t)s.n + 1);
             s.chk = s.chk > np ? s.chk : np;
             s.sr = fr;
             s.si = fi;
@@ -632,7 +643,7 @@ __global__ void __launch_bounds__(256) perturb_f64(FrameDev f) {
                         lane_start<EXT, D>(s, f, ux * f.h, uy * f.h, 0);
                         active = true;
                     } else {
-                        lane_start<EXT, D>(s, f, ux * f.h_m, uy * f.h_m, f.h_e);
+                        lane_start<EXT, D>(s, f, ux * f.h_m, uy * f.h_m, (int32_t)f.h_e);
                         active = true;
                     }
                 }
@@ -642,33 +653,32 @@ __global__ void __launch_bounds__(256) perturb_f64(FrameDev f) {
             if (drained) break;
             continue;
         }
-        if (active) {
-            double fr = 0.0, fi = 0.0;
-            bool was_block = false;
-            uint64_t nb = s.blocks;
-            int k = lane_advance<EXT, D, L>(s, f, fr, fi);
-            was_block = s.blocks != nb;
-            plain += was_block ? 0 : 1;
-            if (k != RUNNING) {
-                uint64_t n = k == 2 ? 0 : s.n;
-                f.kind[idx] = (uint8_t)k;
-                f.n[idx] = n;
-                if (k == 0) {
-                    f.zr[idx] = fr;
-                    f.zi[idx] = fi;
-                    f.dr[idx] = s.dr;
-                    f.di[idx] = s.di;
-                    f.dexp[idx] = s.derivative_exp;
+        // Step until some lane finishes (then refill) or the wave has drained.
+        do {
+            if (active) {
+                double fr = 0.0, fi = 0.0;
+                int k = lane_advance<EXT, D, L>(s, f, fr, fi);
+                if (k != RUNNING) {
+                    f.kind[idx] = (uint8_t)k;
+                    f.n[idx] = k == 2 ? 0 : s.n;
+                    if (k == 0) {
+                        f.zr[idx] = fr;
+                        f.zi[idx] = fi;
+                        f.dr[idx] = s.dr;
+                        f.di[idx] = s.di;
+                        f.dexp[idx] = s.derivative_exp;
+                    }
+                    uint64_t spent = k == 2 ? f.max_iter : s.n;
+                    eq += spent;
+                    its += L ? spent - s.skipped + s.blocks : spent;
+                    blocks += s.blocks;
+                    skipped += s.skipped;
+                    plain += s.plain;
+                    fallback += (L && s.blocks == 0) ? 1 : 0;
+                    active = false;
                 }
-                uint64_t spent = k == 2 ? f.max_iter : s.n;
-                eq += spent;
-                its += L ? spent - s.skipped + s.blocks : spent;
-                blocks += s.blocks;
-                skipped += s.skipped;
-                fallback += (L && s.blocks == 0) ? 1 : 0;
-                active = false;
             }
-        }
+        } while (!__ballot(!active) || (drained && __ballot(active)));
     }
     unsigned long long v[7] = {its, eq, blocks, skipped, fallback, closed, plain};
     for (int q = 0; q < 7; q++) {
