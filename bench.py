@@ -74,8 +74,11 @@ def measure(fa, view, params, route, steps, warmup, label):
         stats.append(frame.run())
         log(f"{label} step {i}: kernel {stats[-1]['kernel_seconds'] * 1e3:.2f} ms")
     t1 = time.perf_counter()
-    h, cols, _ = frame.read()
+    h, cols, _ = frame.read(raw=True)
     frame.close()
+    import numpy as np
+    ops = cols["raw"]["ops"].astype(np.float64)
+    ops_pct = {f"p{q}": float(np.percentile(ops, q)) for q in (50, 90, 99, 99.9, 100)}
     st = stats[-1]
     step_s = (t1 - t0) / steps
     kernel_s = sum(s["kernel_seconds"] for s in stats) / len(stats)
@@ -94,6 +97,7 @@ def measure(fa, view, params, route, steps, warmup, label):
                 nucleus_period=st["nucleus_period"], plain_steps=st["plain_steps"],
                 blocks=st["blocks"], skipped=st["skipped"],
                 equivalent_iterations=st["equivalent"], classes=classes,
+                sample_ops_percentiles=ops_pct, max_sample_ops=st["max_sample_ops"],
                 reference_seconds=st["reference_seconds"],
                 roofline={"bound": "valu-fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                           "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,

@@ -78,7 +78,7 @@ class _Columns(C.Structure):
 class _Outcomes(C.Structure):
     _fields_ = [("kind", C.c_void_p), ("n", C.c_void_p), ("zr", C.c_void_p),
                 ("zi", C.c_void_p), ("dr", C.c_void_p), ("di", C.c_void_p),
-                ("dexp", C.c_void_p)]
+                ("dexp", C.c_void_p), ("ops", C.c_void_p)]
 
 
 class _Stats(C.Structure):
@@ -87,7 +87,7 @@ class _Stats(C.Structure):
                 ("iterations", C.c_uint64), ("equivalent", C.c_uint64),
                 ("blocks", C.c_uint64), ("skipped", C.c_uint64), ("plain_steps", C.c_uint64),
                 ("fallback_samples", C.c_uint64), ("closed_form_samples", C.c_uint64),
-                ("reference_seconds", C.c_double), ("bla_seconds", C.c_double),
+                ("max_sample_ops", C.c_uint64), ("reference_seconds", C.c_double), ("bla_seconds", C.c_double),
                 ("kernel_seconds", C.c_double), ("transfer_seconds", C.c_double),
                 ("encode_seconds", C.c_double)]
 
@@ -220,8 +220,9 @@ def render(view, params, route=ROUTE_PLAIN, form=FORM_F64, bla_eps=EPS, ctx=None
     if raw:
         rw = {"kind": np.zeros(n, np.uint8), "n": np.zeros(n, np.uint64), "zr": np.zeros(n),
               "zi": np.zeros(n), "dr": np.zeros(n), "di": np.zeros(n),
-              "dexp": np.zeros(n, np.int64)}
-        oc = _Outcomes(*[_p(rw[k]) for k in ("kind", "n", "zr", "zi", "dr", "di", "dexp")])
+              "dexp": np.zeros(n, np.int64), "ops": np.zeros(n, np.uint32)}
+        oc = _Outcomes(*[_p(rw[k]) for k in ("kind", "n", "zr", "zi", "dr", "di", "dexp",
+                                             "ops")])
         cols["raw"] = rw
     st = _Stats()
     kid = C.create_string_buffer(256)
@@ -268,8 +269,9 @@ class ResidentFrame:
         if raw:
             rw = {"kind": np.zeros(n, np.uint8), "n": np.zeros(n, np.uint64),
                   "zr": np.zeros(n), "zi": np.zeros(n), "dr": np.zeros(n), "di": np.zeros(n),
-                  "dexp": np.zeros(n, np.int64)}
-            oc = _Outcomes(*[_p(rw[k]) for k in ("kind", "n", "zr", "zi", "dr", "di", "dexp")])
+                  "dexp": np.zeros(n, np.int64), "ops": np.zeros(n, np.uint32)}
+            oc = _Outcomes(*[_p(rw[k]) for k in ("kind", "n", "zr", "zi", "dr", "di", "dexp",
+                                                 "ops")])
             cols["raw"] = rw
         st = _Stats()
         kid = C.create_string_buffer(256)

@@ -85,11 +85,12 @@ namespace {
 // Device residency of one frame: the orbit, its BLA table and the per-sample outcomes.
 struct Buffers {
     DevBuf orbit, bla_r, bla_r2, bla_ab, bla_alpha, bla_beta, bla_valid;
-    DevBuf o_kind, o_n, o_zr, o_zi, o_dr, o_di, o_dexp, counters;
+    DevBuf o_kind, o_n, o_zr, o_zi, o_dr, o_di, o_dexp, o_ops, counters;
     uint64_t bla_stride = 0, bla_steps = 0;
     void release() {
         for (DevBuf* b : {&orbit, &bla_r, &bla_r2, &bla_ab, &bla_alpha, &bla_beta, &bla_valid,
-                          &o_kind, &o_n, &o_zr, &o_zi, &o_dr, &o_di, &o_dexp, &counters})
+                          &o_kind, &o_n, &o_zr, &o_zi, &o_dr, &o_di, &o_dexp, &o_ops,
+                          &counters})
             b->release();
     }
     int outputs(fdg_ctx* ctx, size_t n);
@@ -117,6 +118,7 @@ int Buffers::outputs(fdg_ctx* ctx, size_t n) {
     FDG_TRY(ctx, o_dr.ensure(n * 8), "hipMalloc");
     FDG_TRY(ctx, o_di.ensure(n * 8), "hipMalloc");
     FDG_TRY(ctx, o_dexp.ensure(n * 8), "hipMalloc");
+    FDG_TRY(ctx, o_ops.ensure(n * 4), "hipMalloc");
     FDG_TRY(ctx, counters.ensure(256), "hipMalloc");
     return FDG_OK;
 }
@@ -462,6 +464,7 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
     size_t n = f->count;
     for (DevBuf* d : {&b.o_zr, &b.o_zi, &b.o_dr, &b.o_di, &b.o_dexp})
         FDG_TRY(ctx, hipMemsetAsync(d->p, 0, n * 8, ctx->stream), "memset");
+    FDG_TRY(ctx, hipMemsetAsync(b.o_ops.p, 0, n * 4, ctx->stream), "memset");
     FrameDev d;
     std::memset(&d, 0, sizeof(d));
     const Plane& pl = f->fr.pl;
@@ -497,6 +500,7 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
     d.dr = b.o_dr.as<double>();
     d.di = b.o_di.as<double>();
     d.dexp = b.o_dexp.as<int64_t>();
+    d.ops = b.o_ops.as<uint32_t>();
     d.next = b.counters.as<uint32_t>();
     d.stats = reinterpret_cast<unsigned long long*>(b.counters.as<uint8_t>() + 64);
     bool deriv = (p->columns & (FDG_COL_DE | FDG_COL_NORMAL)) != 0;
@@ -522,6 +526,7 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
     s.fallback_samples = cnt[ST_FALLBACK];
     s.closed_form_samples = cnt[ST_CLOSED];
     s.plain_steps = cnt[ST_PLAIN];
+    s.max_sample_ops = cnt[ST_MAXOPS];
     f->last = s;
     f->ran = true;
     if (st) *st = s;
@@ -554,6 +559,8 @@ int fdg_frame_read(fdg_frame* f, fdg_columns* out, fdg_outcomes* raw, fdg_stats*
     FDG_TRY(ctx, hipMemcpy(dr.data(), b.o_dr.p, n * 8, hipMemcpyDeviceToHost), "readback");
     FDG_TRY(ctx, hipMemcpy(di.data(), b.o_di.p, n * 8, hipMemcpyDeviceToHost), "readback");
     FDG_TRY(ctx, hipMemcpy(dexp.data(), b.o_dexp.p, n * 8, hipMemcpyDeviceToHost), "readback");
+    if (raw && raw->ops)
+        FDG_TRY(ctx, hipMemcpy(raw->ops, b.o_ops.p, n * 4, hipMemcpyDeviceToHost), "readback");
     s.transfer_seconds += now_s() - t1;
     double t2 = now_s();
     Encoder enc(f->fr.pl, p->ss);

@@ -30,6 +30,7 @@ enum StatSlot : int {
     ST_FALLBACK = 4,    // samples that applied no block (excluding closed-form)
     ST_CLOSED = 5,      // closed-form main-cardioid / period-2 samples
     ST_PLAIN = 6,       // plain perturbation steps executed
+    ST_MAXOPS = 7,      // most operations (plain steps + blocks) one sample executed
     ST_COUNT = 8
 };
 
@@ -56,6 +57,7 @@ struct FrameDev {
     double* dr;
     double* di;
     int64_t* dexp;
+    uint32_t* ops;                  // executed plain steps + applied blocks per sample
     uint32_t* next;                 // work-stealing counter (zeroed by the host)
     unsigned long long* stats;      // ST_COUNT counters (zeroed by the host)
 };

@@ -610,7 +610,7 @@ __global__ void __launch_bounds__(256) perturb_f64(FrameDev f) {
     bool drained = false;
     uint32_t idx = 0;
     unsigned long long its = 0, eq = 0, blocks = 0, skipped = 0, fallback = 0, closed = 0,
-                       plain = 0;
+                       plain = 0, maxops = 0;
     for (;;) {
         uint64_t idle = __ballot(!active);
         if (idle && !drained) {
@@ -624,6 +624,7 @@ __global__ void __launch_bounds__(256) perturb_f64(FrameDev f) {
                 uint32_t my = base + lanes_below(idle);
                 if (my < f.count && (!f.mask || f.mask[my])) {
                     idx = my;
+                    f.ops[idx] = 0;
                     uint32_t i = idx % f.nx, j = idx / f.nx;
                     double x = (double)i + 0.5 - f.half_x;
                     double y = (double)j + 0.5 - f.half_y;
@@ -661,6 +662,8 @@ __global__ void __launch_bounds__(256) perturb_f64(FrameDev f) {
                 if (k != RUNNING) {
                     f.kind[idx] = (uint8_t)k;
                     f.n[idx] = k == 2 ? 0 : s.n;
+                    f.ops[idx] = s.plain + s.blocks;
+                    maxops = max(maxops, (unsigned long long)(s.plain + s.blocks));
                     if (k == 0) {
                         f.zr[idx] = fr;
                         f.zi[idx] = fi;
@@ -685,6 +688,8 @@ __global__ void __launch_bounds__(256) perturb_f64(FrameDev f) {
         unsigned long long t = wave_sum(v[q]);
         if (lane == 0 && t) atomicAdd(&f.stats[q], t);
     }
+    for (int off = 32; off > 0; off >>= 1) maxops = max(maxops, __shfl_xor(maxops, off));
+    if (lane == 0 && maxops) atomicMax(&f.stats[ST_MAXOPS], maxops);
 }
 
 }  // namespace

@@ -79,6 +79,7 @@ typedef struct {
     uint64_t *n;
     double *zr, *zi, *dr, *di;
     int64_t *dexp;
+    uint32_t *ops; /* operations executed: plain steps + applied BLA blocks */
 } fdg_outcomes;
 
 /* Work counters (grid.rs:40-70 Stats + BlaStats) and timings. */
@@ -91,6 +92,7 @@ typedef struct {
     uint64_t equivalent;          /* plain-equivalent iterations (skipped steps counted) */
     uint64_t blocks, skipped, plain_steps;
     uint64_t fallback_samples, closed_form_samples;
+    uint64_t max_sample_ops;      /* most operations executed by one sample */
     double reference_seconds;     /* host arbitrary-precision orbit */
     double bla_seconds;           /* GPU table build (device time) */
     double kernel_seconds;        /* GPU perturbation kernel (device time) */
