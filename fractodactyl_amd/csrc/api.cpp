@@ -19,7 +19,7 @@ hipError_t launch_bla_build(const double* orbit, double* r, double* r2, double* 
                             double* beta, uint32_t* valid, uint64_t steps, uint64_t stride,
                             double eps, double dc_max, int64_t dc_exp, uint32_t max_levels,
                             hipStream_t stream);
-hipError_t launch_perturb_f64(const FrameDev& f, bool ext, bool deriv, bool bla, int grid,
+hipError_t launch_perturb_f64(const FrameDev* f, bool ext, bool deriv, bool bla, int grid,
                               hipStream_t stream);
 }  // namespace fdg
 
@@ -85,12 +85,12 @@ namespace {
 // Device residency of one frame: the orbit, its BLA table and the per-sample outcomes.
 struct Buffers {
     DevBuf orbit, bla_r, bla_r2, bla_ab, bla_alpha, bla_beta, bla_valid;
-    DevBuf o_kind, o_n, o_zr, o_zi, o_dr, o_di, o_dexp, o_ops, counters;
+    DevBuf o_kind, o_n, o_zr, o_zi, o_dr, o_di, o_dexp, o_ops, counters, fdev;
     uint64_t bla_stride = 0, bla_steps = 0;
     void release() {
         for (DevBuf* b : {&orbit, &bla_r, &bla_r2, &bla_ab, &bla_alpha, &bla_beta, &bla_valid,
                           &o_kind, &o_n, &o_zr, &o_zi, &o_dr, &o_di, &o_dexp, &o_ops,
-                          &counters})
+                          &counters, &fdev})
             b->release();
     }
     int outputs(fdg_ctx* ctx, size_t n);
@@ -120,6 +120,7 @@ int Buffers::outputs(fdg_ctx* ctx, size_t n) {
     FDG_TRY(ctx, o_dexp.ensure(n * 8), "hipMalloc");
     FDG_TRY(ctx, o_ops.ensure(n * 4), "hipMalloc");
     FDG_TRY(ctx, counters.ensure(256), "hipMalloc");
+    FDG_TRY(ctx, fdev.ensure(sizeof(FrameDev)), "hipMalloc");
     return FDG_OK;
 }
 
@@ -506,9 +507,12 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
     bool deriv = (p->columns & (FDG_COL_DE | FDG_COL_NORMAL)) != 0;
     int grid = (int)std::min<uint64_t>((f->count + 255) / 256, (uint64_t)ctx->cus * 8);
     if (grid < 1) grid = 1;
+    FDG_TRY(ctx, hipMemcpyAsync(b.fdev.p, &d, sizeof(d), hipMemcpyHostToDevice, ctx->stream),
+            "frame upload");
     FDG_TRY(ctx, hipEventRecord(ctx->ev[2], ctx->stream), "event");
     FDG_TRY(ctx,
-            launch_perturb_f64(d, f->fr.tier == TIER_SCALED, deriv, f->use_bla, grid, ctx->stream),
+            launch_perturb_f64(b.fdev.as<FrameDev>(), f->fr.tier == TIER_SCALED, deriv,
+                               f->use_bla, grid, ctx->stream),
             "perturbation launch");
     FDG_TRY(ctx, hipEventRecord(ctx->ev[3], ctx->stream), "event");
     unsigned long long cnt[ST_COUNT];

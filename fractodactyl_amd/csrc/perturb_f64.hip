@@ -263,28 +263,52 @@ __device__ inline void sum2(double t0r, double t0i, int64_t t0e, double t1r, dou
     oe = (int32_t)top;
 }
 
-// One lane's sample state (sample.rs:190-236), kept in registers.  (Zr, Zi) is Z_m,
-// carried from the previous escape test, so a step loads the orbit once (its landing
-// point) and the load is issued before the step's arithmetic.
-struct Lane {
-    double xr, xi, dr, di, ar, ai, Zr, Zi;
-    double sr, si, qr, qi, qstep;
-    double mur, mui, conr, coni;
-    double scale, dc_r, dc_i, inverse;  // scaled tier only
-    int32_t delta_exp, derivative_exp, offset_exp, mue, cone;
-    uint32_t m, n, sn, hold, chk, nucleus, blocks, skipped, plain, returns;
-    bool pending, predicted, con_started;
+// Per-lane state that only the periodicity confirmation and the nucleus heuristic use
+// (sample.rs:216-230): kept in LDS so the hot loop's registers hold just the iterate.
+constexpr int kBlock = 256;
+
+// Structure of arrays in LDS (one slot per lane: no bank conflicts between lanes).
+struct RareLds {
+    double qr[kBlock], qi[kBlock], qstep[kBlock], mur[kBlock], mui[kBlock], conr[kBlock],
+        coni[kBlock];
+    int32_t mue[kBlock], cone[kBlock];
 };
 
-template <bool EXT, bool D>
-__device__ inline void lane_start(Lane& s, const FrameDev& f, double ar, double ai, int32_t oe) {
+// One lane's view of its LDS slot.
+struct Rare {
+    double &qr, &qi, &qstep, &mur, &mui, &conr, &coni;
+    int32_t &mue, &cone;
+    __device__ Rare(RareLds& l, int t)
+        : qr(l.qr[t]), qi(l.qi[t]), qstep(l.qstep[t]), mur(l.mur[t]), mui(l.mui[t]),
+          conr(l.conr[t]), coni(l.coni[t]), mue(l.mue[t]), cone(l.cone[t]) {}
+};
+
+// One lane's sample state (sample.rs:190-236) in registers.  (Zr, Zi) is Z_m, carried
+// from the previous escape test, so a step loads the orbit once (its landing point).
+struct Lane {
+    double xr, xi, dr, di, ar, ai, Zr, Zi, sr, si;
+    double scale, dc_r, dc_i, inverse;  // scaled tier only
+    int32_t delta_exp, derivative_exp;  // scaled tier only
+    uint32_t m, n, sn, hold, chk, nucleus, blocks, skipped, plain;
+    uint32_t flags;                     // kPending | kPredicted | kConStarted | returns << 4
+};
+
+constexpr uint32_t kPending = 1, kPredicted = 2, kConStarted = 4;
+
+__device__ inline uint32_t np2_above(uint32_t n) {
+    // next_power_of_two(n + 1) for 1 <= n < 2^31
+    return 1u << (32 - __builtin_clz(n));
+}
+
+template <bool EXT>
+__device__ inline void lane_start(Lane& s, Rare& r, uint32_t nucleus, double ar, double ai,
+                                  int32_t oe) {
     s.ar = ar;
     s.ai = ai;
     s.xr = 0.0;
     s.xi = 0.0;
     s.Zr = 0.0;  // Z_0 = 0
     s.Zi = 0.0;
-    s.offset_exp = oe;
     s.delta_exp = EXT ? oe : 0;
     s.scale = EXT ? exp2i(s.delta_exp) : 1.0;
     s.dc_r = ar;
@@ -299,87 +323,169 @@ __device__ inline void lane_start(Lane& s, const FrameDev& f, double ar, double 
     s.si = 0.0;
     s.chk = 16;
     s.sn = EXT ? 0 : 8;
-    s.returns = 0;
-    s.pending = false;
-    s.mur = 1.0;
-    s.mui = 0.0;
-    s.mue = 0;
     s.hold = 0;
-    s.predicted = false;
-    s.qr = s.qi = s.qstep = 0.0;
-    s.nucleus = (uint32_t)f.nucleus;
-    s.conr = 0.0;
-    s.coni = 0.0;
-    s.cone = 0;
-    s.con_started = false;
+    s.nucleus = nucleus;
+    s.flags = 0;
     s.blocks = 0;
     s.skipped = 0;
     s.plain = 0;
+    r.qr = r.qi = r.qstep = 0.0;
+    r.mur = 1.0;
+    r.mui = 0.0;
+    r.mue = 0;
+    r.conr = 0.0;
+    r.coni = 0.0;
+    r.cone = 0;
 }
 
 // Re-derive dc in the delta's units after its exponent changed (sample.rs:300-302).
 template <bool EXT>
-__device__ inline void refresh_dc(Lane& s) {
+__device__ inline void refresh_dc(Lane& s, int32_t oe) {
     if (EXT) {
         s.scale = exp2i(s.delta_exp);
-        double factor = exp2i((int64_t)s.offset_exp - s.delta_exp);
+        double factor = exp2i((int64_t)oe - s.delta_exp);
         s.dc_r = s.ar * factor;
         s.dc_i = s.ai * factor;
     }
 }
 
+// Frame constants the loop reads, copied once into registers / SGPRs.
+struct Hot {
+    const double2* orbit;
+    const double* r2;
+    const double4* ab;
+    uint32_t last, max_iter, nlevels, steps;
+    uint64_t stride;
+    double esc2;
+    int32_t oe;
+    uint32_t len;
+};
+
 enum : int { RUNNING = -1 };
+
+// Rare path of a near-return on the ordinary tiers (sample.rs:524-558): Newton's cycle
+// point prediction from the return multiplier; returns true when two consecutive
+// predictions agree (Interior).
+__device__ bool near_return(Lane& s, Rare& r, double fr, double fi, double size) {
+    uint32_t period = s.n - s.sn;
+    if (s.hold == period && scaled_below(r.mur, r.mui, r.mue, 1.0)) {
+        double sc = r.mue < -1000 ? 0.0 : exp2i(r.mue < 1000 ? r.mue : 1000);
+        double mr = r.mur * sc, mi = r.mui * sc;
+        double gr = fr - s.sr, gi = fi - s.si;
+        double er = mr - 1.0, ei = mi;
+        double den = er * er + ei * ei;
+        double pr2 = s.sr - (gr * er + gi * ei) / den;
+        double pi2 = s.si - (gi * er - gr * ei) / den;
+        double tol = fmin(0.1 * sqrt(den), 1e-2) * r.qstep + 16.0 * kU * size;
+        if ((s.flags & kPredicted) && fabs(pr2 - r.qr) + fabs(pi2 - r.qi) <= tol) return true;
+        r.qr = pr2;
+        r.qi = pi2;
+        r.qstep = (fabs(gr) + fabs(gi)) / sqrt(den);
+        s.flags |= kPredicted;
+    } else {
+        s.flags &= ~kPredicted;
+    }
+    s.hold = period;
+    s.sr = fr;
+    s.si = fi;
+    s.sn = s.n;
+    s.flags &= ~kPending;
+    r.mur = 1.0;
+    r.mui = 0.0;
+    r.mue = 0;
+    return false;
+}
+
+// The scaled tier's rebase (sample.rs:422-449): exact two-term sums below 1e-290 and the
+// exponent-safe step from Z_0 when the delta is below 1e-270.
+template <bool D>
+__device__ void ext_rebase(Lane& s, const Hot& h, double fr, double fi, double f2,
+                           double Zmr, double Zmi) {
+    if (f2 > kSquareOk) {
+        s.xr = fr;
+        s.xi = fi;
+        s.delta_exp = 0;
+    } else {
+        sum2(Zmr, Zmi, 0, s.xr, s.xi, s.delta_exp, s.delta_exp, s.xr, s.xi, s.delta_exp);
+    }
+    s.scale = exp2i(s.delta_exp);
+    s.m = 0;
+    s.Zr = 0.0;
+    s.Zi = 0.0;
+    if (s.scale < kTiny) {
+        s.nucleus = 0;
+        if (D) {
+            double qr = 2.0 * (s.xr * s.dr - s.xi * s.di);
+            double qi = 2.0 * (s.xr * s.di + s.xi * s.dr);
+            sum2(qr, qi, (int64_t)s.delta_exp + s.derivative_exp, 1.0, 0.0, 0, s.derivative_exp,
+                 s.dr, s.di, s.derivative_exp);
+            s.inverse = exp2i(-(int64_t)s.derivative_exp);
+        }
+        double wr = s.xr, wi = s.xi;
+        sum2(wr * wr - wi * wi, 2.0 * wr * wi, 2 * (int64_t)s.delta_exp, s.ar, s.ai, h.oe,
+             s.delta_exp, s.xr, s.xi, s.delta_exp);
+        s.scale = exp2i(s.delta_exp);
+        s.m = 1;
+        s.n = s.n + 1;
+        double2 z1 = h.orbit[1];
+        s.Zr = z1.x;
+        s.Zi = z1.y;
+    }
+    refresh_dc<true>(s, h.oe);
+}
 
 // One block or one plain step plus the checks after it (sample.rs:237-575).  Returns
 // RUNNING, or the finished outcome kind (0 escaped with z in ofr/ofi, 1 interior,
 // 2 unresolved).
 template <bool EXT, bool D, bool L>
-__device__ inline int lane_advance(Lane& s, const FrameDev& f, double& ofr, double& ofi) {
+__device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, double& ofi) {
     // sample.rs:237 `while n < limit`: the budget ran out.
-    if (s.n >= (uint32_t)f.max_iter) return 2;
-    const double2* orbit = reinterpret_cast<const double2*>(f.orbit);
-    const uint32_t last = (uint32_t)f.len - 1;
-    double2 Zn = orbit[s.m + 1];  // landing point of a plain step (s.m < last here)
+    if (s.n >= h.max_iter) return 2;
+    double2 Zn = h.orbit[s.m + 1];  // a plain step's landing point (s.m < last here)
     int lvl = -1;
-    if (L && (s.nucleus == 0 || s.m != 0)) {
-        // Bla::find (bla.rs:234-249): blocks of 2^(i+1) steps start at k = m - 1 divisible
-        // by 2^(i+1), so an odd k (m = 0 included, k wraps) has none.
+    if (L) {
+        // Bla::find (bla.rs:234-249): a block of 2^(i+1) steps starts at k = m - 1 divisible
+        // by 2^(i+1), so an odd k (m = 0 included: k wraps) has none.
         uint32_t k = s.m - 1;
-        if ((k & 1u) == 0) {
-            int top = k == 0 ? (int)f.bla.nlevels : __builtin_ctz(k);
-            top = top < (int)f.bla.nlevels ? top : (int)f.bla.nlevels;
+        if ((k & 1u) == 0 && (s.nucleus == 0 || s.m != 0)) {
+            int top = k == 0 ? (int)h.nlevels : __builtin_ctz(k);
+            top = top < (int)h.nlevels ? top : (int)h.nlevels;
             double d2 = EXT ? (s.xr * s.xr + s.xi * s.xi) * s.scale * s.scale
                             : s.xr * s.xr + s.xi * s.xi;
-            uint32_t b = s.hold > 0 ? s.sn + s.hold : (s.pending ? 0xffffffffu : s.chk);
-            uint32_t mi = (uint32_t)f.max_iter;
-            b = b < mi ? b : mi;
+            uint32_t b = s.hold > 0 ? s.sn + s.hold : ((s.flags & kPending) ? 0xffffffffu : s.chk);
+            b = b < h.max_iter ? b : h.max_iter;
             uint32_t nb = s.nucleus ? (s.n == 0 ? 1u : s.nucleus + 1) : 0xffffffffu;
             b = b < nb ? b : nb;
             b = b > s.n ? b - s.n : 0;
             for (int lv = 0; lv < top; lv++) {
                 uint32_t idx = k >> (lv + 1);
-                if (!((2ull << lv) <= (uint64_t)b && (uint64_t)idx < (f.bla.steps >> (lv + 1)) &&
-                      d2 < f.bla.r2[bla_level_offset(f.bla.stride, lv) + idx]))
+                if (!((2ull << lv) <= (uint64_t)b && idx < (h.steps >> (lv + 1)) &&
+                      d2 < h.r2[bla_level_offset(h.stride, lv) + idx]))
                     break;
                 lvl = lv;
             }
         }
     }
+    double Zmr = s.Zr, Zmi = s.Zi;
     if (lvl >= 0) {
-        uint64_t at = bla_level_offset(f.bla.stride, lvl) + ((s.m - 1) >> (lvl + 1));
-        double4 ab = reinterpret_cast<const double4*>(f.bla.ab)[at];
+        uint64_t at = bla_level_offset(h.stride, lvl) + ((s.m - 1) >> (lvl + 1));
+        double4 ab = h.ab[at];
         double a_r = ab.x, a_i = ab.y, b_r = ab.z, b_i = ab.w;
         uint32_t blen = 2u << lvl;
-        Zn = orbit[s.m + blen];
-        if (s.nucleus && s.n >= 1) con_multiply(s.conr, s.coni, s.cone, s.con_started, a_r, a_i);
-        if (!EXT && s.hold > 0) mu_multiply(s.mur, s.mui, s.mue, a_r, a_i);
+        Zn = h.orbit[s.m + blen];
+        if (s.nucleus && s.n >= 1) {
+            bool st = s.flags & kConStarted;
+            con_multiply(r.conr, r.coni, r.cone, st, a_r, a_i);
+            s.flags |= kConStarted;
+        }
+        if (!EXT && s.hold > 0) mu_multiply(r.mur, r.mui, r.mue, a_r, a_i);
         if (EXT) {
             normalise(s.xr, s.xi, s.delta_exp, 0);
             if (D) {
                 normalise(s.dr, s.di, s.derivative_exp, INT64_MAX / 4);
                 s.inverse = exp2i(-(int64_t)s.derivative_exp);
             }
-            refresh_dc<EXT>(s);
+            refresh_dc<EXT>(s, h.oe);
         }
         if (D) {
             double t = a_r * s.dr - a_i * s.di + b_r * s.inverse;
@@ -395,18 +501,20 @@ __device__ inline int lane_advance(Lane& s, const FrameDev& f, double& ofr, doub
                 normalise(s.dr, s.di, s.derivative_exp, INT64_MAX / 4);
                 s.inverse = exp2i(-(int64_t)s.derivative_exp);
             }
-            refresh_dc<EXT>(s);
+            refresh_dc<EXT>(s, h.oe);
         }
         s.m += blen;
         s.n += blen;
         s.blocks += 1;
         s.skipped += blen;
     } else {
-        double Zr = s.Zr, Zi = s.Zi;
-        double fr = Zr + s.scale * s.xr, fi = Zi + s.scale * s.xi;
-        if (s.nucleus && s.n >= 1)
-            con_multiply(s.conr, s.coni, s.cone, s.con_started, 2.0 * fr, 2.0 * fi);
-        if (!EXT && s.hold > 0) mu_multiply(s.mur, s.mui, s.mue, 2.0 * fr, 2.0 * fi);
+        double fr = Zmr + s.scale * s.xr, fi = Zmi + s.scale * s.xi;
+        if (s.nucleus && s.n >= 1) {
+            bool st = s.flags & kConStarted;
+            con_multiply(r.conr, r.coni, r.cone, st, 2.0 * fr, 2.0 * fi);
+            s.flags |= kConStarted;
+        }
+        if (!EXT && s.hold > 0) mu_multiply(r.mur, r.mui, r.mue, 2.0 * fr, 2.0 * fi);
         if (D) {
             double t = 2.0 * (fr * s.dr - fi * s.di) + s.inverse;
             s.di = 2.0 * (fr * s.di + fi * s.dr);
@@ -420,171 +528,103 @@ __device__ inline int lane_advance(Lane& s, const FrameDev& f, double& ofr, doub
         }
         double xr = s.xr, xi = s.xi;
         if (EXT) {
-            s.xr = 2.0 * (Zr * xr - Zi * xi) + s.scale * (xr * xr - xi * xi) + s.dc_r;
-            s.xi = 2.0 * (Zr * xi + Zi * xr) + s.scale * (2.0 * xr * xi) + s.dc_i;
+            s.xr = 2.0 * (Zmr * xr - Zmi * xi) + s.scale * (xr * xr - xi * xi) + s.dc_r;
+            s.xi = 2.0 * (Zmr * xi + Zmi * xr) + s.scale * (2.0 * xr * xi) + s.dc_i;
         } else {
-            s.xr = 2.0 * (Zr * xr - Zi * xi) + xr * xr - xi * xi + s.dc_r;
-            s.xi = 2.0 * (Zr * xi + Zi * xr + xr * xi) + s.dc_i;
+            s.xr = 2.0 * (Zmr * xr - Zmi * xi) + xr * xr - xi * xi + s.dc_r;
+            s.xi = 2.0 * (Zmr * xi + Zmi * xr + xr * xi) + s.dc_i;
         }
         s.m += 1;
         s.n += 1;
         s.plain += 1;
     }
-    s.Zr = Zn.x;
-    s.Zi = Zn.y;
     double pr = s.scale * s.xr, pi = s.scale * s.xi;
     double fr = Zn.x + pr, fi = Zn.y + pi;
     double f2 = fr * fr + fi * fi;
-    if (f2 > f.r2) {
+    if (f2 > h.esc2) {
         ofr = fr;
         ofi = fi;
         return 0;  // Escaped
     }
     if (s.nucleus && s.n == s.nucleus + 1) {
-        if (s.con_started && scaled_below(s.conr, s.coni, s.cone, 1e-6)) return 1;  // Interior
+        if ((s.flags & kConStarted) && scaled_below(r.conr, r.coni, r.cone, 1e-6)) return 1;
         s.nucleus = 0;
     }
     double delta2 = pr * pr + pi * pi;
-    bool rebase = EXT ? f2 <= delta2 : f2 < delta2;
-    if (rebase || s.m == last) {
-        if (EXT) {
-            if (f2 > kSquareOk) {
-                s.xr = fr;
-                s.xi = fi;
-                s.delta_exp = 0;
-            } else {
-                sum2(Zn.x, Zn.y, 0, s.xr, s.xi, s.delta_exp, s.delta_exp, s.xr, s.xi,
-                     s.delta_exp);
-            }
-            s.scale = exp2i(s.delta_exp);
-            s.m = 0;
-            s.Zr = 0.0;
-            s.Zi = 0.0;
-            if (s.scale < kTiny) {
-                s.nucleus = 0;
-                if (D) {
-                    // zero_step_dz (sample.rs:1124-1134)
-                    double qr = 2.0 * (s.xr * s.dr - s.xi * s.di);
-                    double qi = 2.0 * (s.xr * s.di + s.xi * s.dr);
-                    sum2(qr, qi, (int64_t)s.delta_exp + s.derivative_exp, 1.0, 0.0, 0,
-                         s.derivative_exp, s.dr, s.di, s.derivative_exp);
-                    s.inverse = exp2i(-(int64_t)s.derivative_exp);
-                }
-                // zero_step (sample.rs:1116-1122)
-                double wr = s.xr, wi = s.xi;
-                sum2(wr * wr - wi * wi, 2.0 * wr * wi, 2 * (int64_t)s.delta_exp, s.ar, s.ai,
-                     s.offset_exp, s.delta_exp, s.xr, s.xi, s.delta_exp);
-                s.scale = exp2i(s.delta_exp);
-                s.m = 1;
-                s.n = s.n + 1;
-                double2 z1 = orbit[1];
-                s.Zr = z1.x;
-                s.Zi = z1.y;
-            }
-            refresh_dc<EXT>(s);
+    bool rebase = (EXT ? f2 <= delta2 : f2 < delta2) || s.m == h.last;
+    if (EXT) {
+        s.Zr = Zn.x;
+        s.Zi = Zn.y;
+        if (rebase) {
+            ext_rebase<D>(s, h, fr, fi, f2, Zn.x, Zn.y);
         } else {
-            s.xr = fr;
-            s.xi = fi;
-            s.m = 0;
-            s.Zr = 0.0;
-            s.Zi = 0.0;
-        }
-    } else if (EXT) {
-        double magnitude = fmax(fabs(s.xr), fabs(s.xi));
-        int32_t k = 0;
-        if (magnitude > kHi && s.delta_exp < 0)
-            k = -s.delta_exp < 32 ? -s.delta_exp : 32;
-        else if (magnitude < kLo && magnitude != 0.0)
-            k = -32;
-        if (k != 0) {
-            double factor = exp2i(-k);
-            s.xr = s.xr * factor;
-            s.xi = s.xi * factor;
-            s.delta_exp = s.delta_exp + k;
-            refresh_dc<EXT>(s);
-        }
-    }
-    double cr = EXT ? pr : s.xr, ci = EXT ? pi : s.xi;
-    bool res = cr * cr + ci * ci > kResolvable * kResolvable * f2;
-    bool judge = EXT ? (s.returns < 4 && res) : res;
-    if (!judge) {
-        if (EXT) {
-            if (s.n >= s.chk) {
-                s.sr = fr;
-                s.si = fi;
-                s.sn = s.n;
-                s.chk = s.chk * 2;
+            double magnitude = fmax(fabs(s.xr), fabs(s.xi));
+            int32_t k = 0;
+            if (magnitude > kHi && s.delta_exp < 0)
+                k = -s.delta_exp < 32 ? -s.delta_exp : 32;
+            else if (magnitude < kLo && magnitude != 0.0)
+                k = -32;
+            if (k != 0) {
+                double factor = exp2i(-k);
+                s.xr = s.xr * factor;
+                s.xi = s.xi * factor;
+                s.delta_exp = s.delta_exp + k;
+                refresh_dc<EXT>(s, h.oe);
             }
-        } else if (s.hold > 0) {
-            if (s.n >= s.sn + s.hold) {
-                s.hold = 0;
-                s.predicted = false;
-                s.pending = true;
+        }
+        // extended tier: judged on the physical delta before rebase/rescale
+        uint32_t returns = s.flags >> 4;
+        bool res = returns < 4 && pr * pr + pi * pi > kResolvable * kResolvable * f2;
+        if (res) {
+            double dist = fabs(fr - s.sr) + fabs(fi - s.si);
+            double size = fabs(s.sr) + fabs(s.si);
+            if (dist < 1e-13 * size + 1e-300) {
+                if (contracting(reinterpret_cast<const double*>(h.orbit), h.len, s.m, pr, pi,
+                                s.ar * exp2i(h.oe), s.ai * exp2i(h.oe), s.n - s.sn))
+                    return 2;  // Unresolved
+                s.flags += 16;
             }
-        } else if (s.n >= s.chk) {
-            s.pending = true;
+        }
+        bool save = res ? s.n >= s.chk : s.n >= s.chk;  // n == chk || n > chk
+        if (save) {
+            s.sr = fr;
+            s.si = fi;
+            s.sn = s.n;
+            s.chk = s.chk * 2;
         }
         return RUNNING;
     }
+    // ordinary tiers: rebase onto Z_0 = 0 (sample.rs:451), by selects
+    s.xr = rebase ? fr : s.xr;
+    s.xi = rebase ? fi : s.xi;
+    s.m = rebase ? 0u : s.m;
+    s.Zr = rebase ? 0.0 : Zn.x;
+    s.Zi = rebase ? 0.0 : Zn.y;
+    bool res = s.xr * s.xr + s.xi * s.xi > kResolvable * kResolvable * f2;
     double dist = fabs(fr - s.sr) + fabs(fi - s.si);
     double size = fabs(s.sr) + fabs(s.si);
-    if (EXT && s.returns < 4 && dist < 1e-13 * size + 1e-300) {
-        if (contracting(f.orbit, f.len, s.m, cr, ci, s.ar * exp2i(s.offset_exp),
-                        s.ai * exp2i(s.offset_exp), s.n - s.sn))
-            return 2;  // Unresolved
-        s.returns += 1;
-    }
-    if (!EXT) {
-        if (dist < 1e-3 * size) {
-            uint32_t period = s.n - s.sn;
-            if (s.hold == period && scaled_below(s.mur, s.mui, s.mue, 1.0)) {
-                double sc = s.mue < -1000 ? 0.0 : exp2i(s.mue < 1000 ? s.mue : 1000);
-                double mr = s.mur * sc, mi = s.mui * sc;
-                double gr = fr - s.sr, gi = fi - s.si;
-                double er = mr - 1.0, ei = mi;
-                double den = er * er + ei * ei;
-                double pr2 = s.sr - (gr * er + gi * ei) / den;
-                double pi2 = s.si - (gi * er - gr * ei) / den;
-                double tol = fmin(0.1 * sqrt(den), 1e-2) * s.qstep + 16.0 * kU * size;
-                if (s.predicted && fabs(pr2 - s.qr) + fabs(pi2 - s.qi) <= tol) return 1;
-                s.qr = pr2;
-                s.qi = pi2;
-                s.qstep = (fabs(gr) + fabs(gi)) / sqrt(den);
-                s.predicted = true;
-            } else {
-                s.predicted = false;
-            }
-            s.hold = period;
-            s.sr = fr;
-            s.si = fi;
-            s.sn = s.n;
-            s.pending = false;
-            s.mur = 1.0;
-            s.mui = 0.0;
-            s.mue = 0;
-            return RUNNING;
-        }
-        if (s.hold > 0) {
-            if (s.n < s.sn + s.hold) return RUNNING;
-            s.hold = 0;
-            s.predicted = false;
-            s.pending = true;
-        }
-        if (s.pending || s.n >= s.chk) {
-            uint32_t np = (uint32_t)next_pow2((uint64_t)s.n + 1);
-            s.chk = s.chk > np ? s.chk : np;
-            s.sr = fr;
-            s.si = fi;
-            s.sn = s.n;
-            s.pending = false;
-        }
+    if (res && dist < 1e-3 * size) {
+        if (near_return(s, r, fr, fi, size)) return 1;  // Interior
         return RUNNING;
     }
-    if (s.n == s.chk || (EXT && s.n > s.chk)) {
+    uint32_t h0 = s.hold;
+    bool expired = h0 > 0 && s.n >= s.sn + h0;
+    if (expired) {
+        s.hold = 0;
+        s.flags = (s.flags & ~kPredicted) | kPending;
+    }
+    // !res: a save that falls on an unjudgeable iterate is deferred (pending)
+    bool defer = !res && h0 == 0 && s.n >= s.chk;
+    s.flags |= defer ? kPending : 0u;
+    // res: save at the deferred or scheduled point unless a return is being awaited
+    bool save = res && (h0 == 0 || expired) && ((s.flags & kPending) || s.n >= s.chk);
+    if (save) {
+        uint32_t np = np2_above(s.n);
+        s.chk = s.chk > np ? s.chk : np;
         s.sr = fr;
         s.si = fi;
         s.sn = s.n;
-        s.chk = s.chk * 2;
+        s.flags &= ~kPending;
     }
     return RUNNING;
 }
@@ -603,8 +643,24 @@ __device__ inline unsigned long long wave_sum(unsigned long long v) {
 // sample finished are refilled together from one atomic per wave, so the wave keeps
 // 64 samples in flight however uneven their iteration counts are.
 template <bool EXT, bool D, bool L>
-__global__ void __launch_bounds__(256) perturb_f64(FrameDev f) {
+__global__ void __launch_bounds__(kBlock) perturb_f64(const FrameDev* __restrict__ fp) {
+    __shared__ RareLds rare;
     const uint32_t lane = __lane_id();
+    Rare r(rare, threadIdx.x);
+    Hot h;
+    h.orbit = reinterpret_cast<const double2*>(fp->orbit);
+    h.r2 = fp->bla.r2;
+    h.ab = reinterpret_cast<const double4*>(fp->bla.ab);
+    h.last = (uint32_t)fp->len - 1;
+    h.len = (uint32_t)fp->len;
+    h.max_iter = (uint32_t)fp->max_iter;
+    h.nlevels = fp->bla.nlevels;
+    h.steps = (uint32_t)fp->bla.steps;
+    h.stride = fp->bla.stride;
+    h.esc2 = fp->r2;
+    h.oe = (int32_t)fp->h_e;
+    const uint32_t count = fp->count, nx = fp->nx, nucleus = (uint32_t)fp->nucleus;
+    uint32_t* next = fp->next;
     Lane s;
     bool active = false;
     bool drained = false;
@@ -617,34 +673,34 @@ __global__ void __launch_bounds__(256) perturb_f64(FrameDev f) {
             uint32_t want = __popcll(idle);
             uint32_t leader = __ffsll((unsigned long long)idle) - 1;
             uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(f.next, want);
+            if (lane == leader) base = atomicAdd(next, want);
             base = __shfl(base, leader);
-            if (base + want >= f.count) drained = true;
+            if (base + want >= count) drained = true;
             if (!active) {
                 uint32_t my = base + lanes_below(idle);
-                if (my < f.count && (!f.mask || f.mask[my])) {
+                if (my < count && (!fp->mask || fp->mask[my])) {
                     idx = my;
-                    f.ops[idx] = 0;
-                    uint32_t i = idx % f.nx, j = idx / f.nx;
-                    double x = (double)i + 0.5 - f.half_x;
-                    double y = (double)j + 0.5 - f.half_y;
-                    double ux = f.cos_r * x + f.sin_r * y;
-                    double uy = f.sin_r * x - f.cos_r * y;
-                    if (f.tier == 0) {
-                        double ar = ux * f.h, ai = uy * f.h;
-                        if (in_main_components(f.c_re + ar, f.c_im + ai)) {
-                            f.kind[idx] = 1;
-                            f.n[idx] = 0;
+                    uint32_t i = idx % nx, j = idx / nx;
+                    double x = (double)i + 0.5 - fp->half_x;
+                    double y = (double)j + 0.5 - fp->half_y;
+                    double ux = fp->cos_r * x + fp->sin_r * y;
+                    double uy = fp->sin_r * x - fp->cos_r * y;
+                    fp->ops[idx] = 0;
+                    if (fp->tier == 0) {
+                        double ar = ux * fp->h, ai = uy * fp->h;
+                        if (in_main_components(fp->c_re + ar, fp->c_im + ai)) {
+                            fp->kind[idx] = 1;
+                            fp->n[idx] = 0;
                             closed += L ? 1 : 0;
                         } else {
-                            lane_start<EXT, D>(s, f, ar, ai, 0);
+                            lane_start<EXT>(s, r, nucleus, ar, ai, 0);
                             active = true;
                         }
-                    } else if (f.tier == 1) {
-                        lane_start<EXT, D>(s, f, ux * f.h, uy * f.h, 0);
+                    } else if (fp->tier == 1) {
+                        lane_start<EXT>(s, r, nucleus, ux * fp->h, uy * fp->h, 0);
                         active = true;
                     } else {
-                        lane_start<EXT, D>(s, f, ux * f.h_m, uy * f.h_m, (int32_t)f.h_e);
+                        lane_start<EXT>(s, r, nucleus, ux * fp->h_m, uy * fp->h_m, h.oe);
                         active = true;
                     }
                 }
@@ -654,29 +710,29 @@ __global__ void __launch_bounds__(256) perturb_f64(FrameDev f) {
             if (drained) break;
             continue;
         }
-        // Step until some lane finishes (then refill) or the wave has drained.
+        // Step until some lane finishes (then refill) or, once drained, all are done.
         do {
             if (active) {
                 double fr = 0.0, fi = 0.0;
-                int k = lane_advance<EXT, D, L>(s, f, fr, fi);
+                int k = lane_advance<EXT, D, L>(s, r, h, fr, fi);
                 if (k != RUNNING) {
-                    f.kind[idx] = (uint8_t)k;
-                    f.n[idx] = k == 2 ? 0 : s.n;
-                    f.ops[idx] = s.plain + s.blocks;
-                    maxops = max(maxops, (unsigned long long)(s.plain + s.blocks));
+                    fp->kind[idx] = (uint8_t)k;
+                    fp->n[idx] = k == 2 ? 0 : s.n;
+                    fp->ops[idx] = s.plain + s.blocks;
                     if (k == 0) {
-                        f.zr[idx] = fr;
-                        f.zi[idx] = fi;
-                        f.dr[idx] = s.dr;
-                        f.di[idx] = s.di;
-                        f.dexp[idx] = s.derivative_exp;
+                        fp->zr[idx] = fr;
+                        fp->zi[idx] = fi;
+                        fp->dr[idx] = s.dr;
+                        fp->di[idx] = s.di;
+                        fp->dexp[idx] = s.derivative_exp;
                     }
-                    uint64_t spent = k == 2 ? f.max_iter : s.n;
+                    uint64_t spent = k == 2 ? h.max_iter : s.n;
                     eq += spent;
                     its += L ? spent - s.skipped + s.blocks : spent;
                     blocks += s.blocks;
                     skipped += s.skipped;
                     plain += s.plain;
+                    maxops = max(maxops, (unsigned long long)(s.plain + s.blocks));
                     fallback += (L && s.blocks == 0) ? 1 : 0;
                     active = false;
                 }
@@ -686,10 +742,10 @@ __global__ void __launch_bounds__(256) perturb_f64(FrameDev f) {
     unsigned long long v[7] = {its, eq, blocks, skipped, fallback, closed, plain};
     for (int q = 0; q < 7; q++) {
         unsigned long long t = wave_sum(v[q]);
-        if (lane == 0 && t) atomicAdd(&f.stats[q], t);
+        if (lane == 0 && t) atomicAdd(&fp->stats[q], t);
     }
     for (int off = 32; off > 0; off >>= 1) maxops = max(maxops, __shfl_xor(maxops, off));
-    if (lane == 0 && maxops) atomicMax(&f.stats[ST_MAXOPS], maxops);
+    if (lane == 0 && maxops) atomicMax(&fp->stats[ST_MAXOPS], maxops);
 }
 
 }  // namespace
@@ -713,10 +769,10 @@ hipError_t launch_bla_build(const double* orbit, double* r, double* r2, double* 
     return hipGetLastError();
 }
 
-hipError_t launch_perturb_f64(const FrameDev& f, bool ext, bool deriv, bool bla, int grid,
+hipError_t launch_perturb_f64(const FrameDev* f, bool ext, bool deriv, bool bla, int grid,
                               hipStream_t stream) {
 #define FDG_LAUNCH(E, D, L) \
-    hipLaunchKernelGGL((perturb_f64<E, D, L>), dim3(grid), dim3(256), 0, stream, f)
+    hipLaunchKernelGGL((perturb_f64<E, D, L>), dim3(grid), dim3(kBlock), 0, stream, f)
     int key = (ext ? 4 : 0) | (deriv ? 2 : 0) | (bla ? 1 : 0);
     switch (key) {
         case 0: FDG_LAUNCH(false, false, false); break;
