@@ -98,6 +98,8 @@ def measure(fa, view, params, route, steps, warmup, label):
                 blocks=st["blocks"], skipped=st["skipped"],
                 equivalent_iterations=st["equivalent"], classes=classes,
                 sample_ops_percentiles=ops_pct, max_sample_ops=st["max_sample_ops"],
+                wave_iterations=st["wave_iterations"],
+                lane_utilisation=st["lane_iterations"] / max(1, 64 * st["wave_iterations"]),
                 reference_seconds=st["reference_seconds"],
                 roofline={"bound": "valu-fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                           "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,

@@ -87,7 +87,8 @@ class _Stats(C.Structure):
                 ("iterations", C.c_uint64), ("equivalent", C.c_uint64),
                 ("blocks", C.c_uint64), ("skipped", C.c_uint64), ("plain_steps", C.c_uint64),
                 ("fallback_samples", C.c_uint64), ("closed_form_samples", C.c_uint64),
-                ("max_sample_ops", C.c_uint64), ("reference_seconds", C.c_double), ("bla_seconds", C.c_double),
+                ("max_sample_ops", C.c_uint64), ("wave_iterations", C.c_uint64),
+                ("lane_iterations", C.c_uint64), ("reference_seconds", C.c_double), ("bla_seconds", C.c_double),
                 ("kernel_seconds", C.c_double), ("transfer_seconds", C.c_double),
                 ("encode_seconds", C.c_double)]
 

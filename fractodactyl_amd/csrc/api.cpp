@@ -531,6 +531,8 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
     s.closed_form_samples = cnt[ST_CLOSED];
     s.plain_steps = cnt[ST_PLAIN];
     s.max_sample_ops = cnt[ST_MAXOPS];
+    s.wave_iterations = cnt[ST_WAVE_ITERS];
+    s.lane_iterations = cnt[ST_LANE_ITERS];
     f->last = s;
     f->ran = true;
     if (st) *st = s;

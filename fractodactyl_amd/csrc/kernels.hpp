@@ -31,7 +31,9 @@ enum StatSlot : int {
     ST_CLOSED = 5,      // closed-form main-cardioid / period-2 samples
     ST_PLAIN = 6,       // plain perturbation steps executed
     ST_MAXOPS = 7,      // most operations (plain steps + blocks) one sample executed
-    ST_COUNT = 8
+    ST_WAVE_ITERS = 8,  // loop iterations summed over waves
+    ST_LANE_ITERS = 9,  // active lanes summed over those iterations (utilisation)
+    ST_COUNT = 16
 };
 
 // One frame for the perturbation kernels.

@@ -170,13 +170,18 @@ __device__ bool contracting(const double* orbit, uint64_t len, uint64_t m0, doub
 
 // sample.rs:597-652 Multiplier and sample.rs:657-698 Contraction: a complex product with
 // a separate binary exponent.  The exponent choice only picks an exact power-of-two split.
+// The exponent e with a = m 2^e, 1 <= m < 2 (a finite, > 0).  Where the reference takes
+// floor(log2(a)) only to pick a power-of-two rescaling, any exponent gives the same value
+// (the scaling is exact), so the frexp exponent replaces the libm log2.
+__device__ inline int32_t exponent_of(double a) { return __builtin_amdgcn_frexp_exp(a) - 1; }
+
 __device__ inline void mu_multiply(double& re_, double& im_, int32_t& ex, double re, double im) {
     double nr = re_ * re - im_ * im, ni = re_ * im + im_ * re;
     re_ = nr;
     im_ = ni;
     double a = fmax(fabs(nr), fabs(ni));
     if (!(a > 1e-120 && a < 1e120) && a > 0.0 && isfinite(a)) {
-        int32_t e = (int32_t)floor_log2(a);
+        int32_t e = exponent_of(a);
         double sc = exp2i(-e);
         re_ *= sc;
         im_ *= sc;
@@ -197,8 +202,8 @@ __device__ inline void con_multiply(double& re_, double& im_, int32_t& ex, bool&
     }
     int32_t exp = 0;
     double a = fmax(fabs(re), fabs(im));
-    if (a != 0.0 && !(kLo <= a && a <= kHi)) {
-        int32_t e = (int32_t)floor_log2(a);
+    if (a != 0.0 && !(kLo <= a && a <= kHi) && isfinite(a)) {
+        int32_t e = exponent_of(a);
         double sc = exp2i(-e);
         re *= sc;
         im *= sc;
@@ -209,8 +214,8 @@ __device__ inline void con_multiply(double& re_, double& im_, int32_t& ex, bool&
     im_ = ni;
     ex += exp;
     a = fmax(fabs(re_), fabs(im_));
-    if (a != 0.0 && !(kLo <= a && a <= kHi)) {
-        int32_t e = (int32_t)floor_log2(a);
+    if (a != 0.0 && !(kLo <= a && a <= kHi) && isfinite(a)) {
+        int32_t e = exponent_of(a);
         double sc = exp2i(-e);
         re_ *= sc;
         im_ *= sc;
@@ -467,7 +472,54 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         }
     }
     double Zmr = s.Zr, Zmi = s.Zi;
-    if (lvl >= 0) {
+    if (!EXT) {
+        // One instruction stream for both a block (delta' = A delta + B dc, D' = A D + B)
+        // and a plain step (delta' = 2 Z delta + delta^2 + dc, D' = 2 z D + 1): the
+        // operands are selected per lane so that each expression rounds exactly as the
+        // reference's (multiplying by c = 1 or by 1.0, and adding -0.0, are exact).
+        bool blk = lvl >= 0;
+        double a_r = 0.0, a_i = 0.0, b_r = 1.0, b_i = 0.0;
+        uint32_t blen = 1;
+        if (blk) {
+            uint64_t at = bla_level_offset(h.stride, lvl) + ((s.m - 1) >> (lvl + 1));
+            double4 ab = h.ab[at];
+            a_r = ab.x;
+            a_i = ab.y;
+            b_r = ab.z;
+            b_i = ab.w;
+            blen = 2u << lvl;
+            Zn = h.orbit[s.m + blen];
+        }
+        double xr = s.xr, xi = s.xi;
+        double fr = Zmr + xr, fi = Zmi + xi;           // z = Z_m + delta (plain step)
+        double c = blk ? 1.0 : 2.0;
+        double mr = blk ? a_r : 2.0 * fr, mi = blk ? a_i : 2.0 * fi;  // step multiplier
+        if (s.nucleus && s.n >= 1) {
+            bool st = s.flags & kConStarted;
+            con_multiply(r.conr, r.coni, r.cone, st, mr, mi);
+            s.flags |= kConStarted;
+        }
+        if (s.hold > 0) mu_multiply(r.mur, r.mui, r.mue, mr, mi);
+        if (D) {
+            double p2r = blk ? a_r : fr, p2i = blk ? a_i : fi;
+            double dr = s.dr, di = s.di;
+            s.dr = c * (p2r * dr - p2i * di) + b_r * s.inverse;
+            s.di = c * (p2r * di + p2i * dr) + (blk ? b_i : -0.0) * s.inverse;
+        }
+        double pr_ = blk ? a_r : Zmr, pi_ = blk ? a_i : Zmi;
+        double ur = blk ? b_r : xr, vr = blk ? s.dc_r : xr;
+        double wr = blk ? b_i : xi, qr = blk ? s.dc_i : xi;
+        double yr = blk ? -0.0 : s.dc_r;
+        double ui = blk ? b_r : xr, vi = blk ? s.dc_i : xi;
+        double wi = blk ? b_i : s.dc_i, xi2 = blk ? s.dc_r : 1.0;
+        s.xr = c * (pr_ * xr - pi_ * xi) + ur * vr - wr * qr + yr;
+        s.xi = c * (pr_ * xi + pi_ * xr + ui * vi) + wi * xi2;
+        s.m += blen;
+        s.n += blen;
+        s.blocks += blk ? 1u : 0u;
+        s.skipped += blk ? blen : 0u;
+        s.plain += blk ? 0u : 1u;
+    } else if (lvl >= 0) {
         uint64_t at = bla_level_offset(h.stride, lvl) + ((s.m - 1) >> (lvl + 1));
         double4 ab = h.ab[at];
         double a_r = ab.x, a_i = ab.y, b_r = ab.z, b_i = ab.w;
@@ -478,15 +530,12 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
             con_multiply(r.conr, r.coni, r.cone, st, a_r, a_i);
             s.flags |= kConStarted;
         }
-        if (!EXT && s.hold > 0) mu_multiply(r.mur, r.mui, r.mue, a_r, a_i);
-        if (EXT) {
-            normalise(s.xr, s.xi, s.delta_exp, 0);
-            if (D) {
-                normalise(s.dr, s.di, s.derivative_exp, INT64_MAX / 4);
-                s.inverse = exp2i(-(int64_t)s.derivative_exp);
-            }
-            refresh_dc<EXT>(s, h.oe);
+        normalise(s.xr, s.xi, s.delta_exp, 0);
+        if (D) {
+            normalise(s.dr, s.di, s.derivative_exp, INT64_MAX / 4);
+            s.inverse = exp2i(-(int64_t)s.derivative_exp);
         }
+        refresh_dc<EXT>(s, h.oe);
         if (D) {
             double t = a_r * s.dr - a_i * s.di + b_r * s.inverse;
             s.di = a_r * s.di + a_i * s.dr + b_i * s.inverse;
@@ -495,14 +544,12 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         double t = a_r * s.xr - a_i * s.xi + b_r * s.dc_r - b_i * s.dc_i;
         s.xi = a_r * s.xi + a_i * s.xr + b_r * s.dc_i + b_i * s.dc_r;
         s.xr = t;
-        if (EXT) {
-            normalise(s.xr, s.xi, s.delta_exp, 0);
-            if (D) {
-                normalise(s.dr, s.di, s.derivative_exp, INT64_MAX / 4);
-                s.inverse = exp2i(-(int64_t)s.derivative_exp);
-            }
-            refresh_dc<EXT>(s, h.oe);
+        normalise(s.xr, s.xi, s.delta_exp, 0);
+        if (D) {
+            normalise(s.dr, s.di, s.derivative_exp, INT64_MAX / 4);
+            s.inverse = exp2i(-(int64_t)s.derivative_exp);
         }
+        refresh_dc<EXT>(s, h.oe);
         s.m += blen;
         s.n += blen;
         s.blocks += 1;
@@ -514,12 +561,11 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
             con_multiply(r.conr, r.coni, r.cone, st, 2.0 * fr, 2.0 * fi);
             s.flags |= kConStarted;
         }
-        if (!EXT && s.hold > 0) mu_multiply(r.mur, r.mui, r.mue, 2.0 * fr, 2.0 * fi);
         if (D) {
             double t = 2.0 * (fr * s.dr - fi * s.di) + s.inverse;
             s.di = 2.0 * (fr * s.di + fi * s.dr);
             s.dr = t;
-            if (EXT && fmax(fabs(s.dr), fabs(s.di)) > kHi) {
+            if (fmax(fabs(s.dr), fabs(s.di)) > kHi) {
                 s.dr = s.dr * kLo;
                 s.di = s.di * kLo;
                 s.derivative_exp += 32;
@@ -527,13 +573,8 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
             }
         }
         double xr = s.xr, xi = s.xi;
-        if (EXT) {
-            s.xr = 2.0 * (Zmr * xr - Zmi * xi) + s.scale * (xr * xr - xi * xi) + s.dc_r;
-            s.xi = 2.0 * (Zmr * xi + Zmi * xr) + s.scale * (2.0 * xr * xi) + s.dc_i;
-        } else {
-            s.xr = 2.0 * (Zmr * xr - Zmi * xi) + xr * xr - xi * xi + s.dc_r;
-            s.xi = 2.0 * (Zmr * xi + Zmi * xr + xr * xi) + s.dc_i;
-        }
+        s.xr = 2.0 * (Zmr * xr - Zmi * xi) + s.scale * (xr * xr - xi * xi) + s.dc_r;
+        s.xi = 2.0 * (Zmr * xi + Zmi * xr) + s.scale * (2.0 * xr * xi) + s.dc_i;
         s.m += 1;
         s.n += 1;
         s.plain += 1;
@@ -666,7 +707,7 @@ __global__ void __launch_bounds__(kBlock) perturb_f64(const FrameDev* __restrict
     bool drained = false;
     uint32_t idx = 0;
     unsigned long long its = 0, eq = 0, blocks = 0, skipped = 0, fallback = 0, closed = 0,
-                       plain = 0, maxops = 0;
+                       plain = 0, maxops = 0, witers = 0, liters = 0;
     for (;;) {
         uint64_t idle = __ballot(!active);
         if (idle && !drained) {
@@ -712,6 +753,8 @@ __global__ void __launch_bounds__(kBlock) perturb_f64(const FrameDev* __restrict
         }
         // Step until some lane finishes (then refill) or, once drained, all are done.
         do {
+            witers += 1;
+            liters += __popcll(__ballot(active));
             if (active) {
                 double fr = 0.0, fi = 0.0;
                 int k = lane_advance<EXT, D, L>(s, r, h, fr, fi);
@@ -746,6 +789,10 @@ __global__ void __launch_bounds__(kBlock) perturb_f64(const FrameDev* __restrict
     }
     for (int off = 32; off > 0; off >>= 1) maxops = max(maxops, __shfl_xor(maxops, off));
     if (lane == 0 && maxops) atomicMax(&fp->stats[ST_MAXOPS], maxops);
+    if (lane == 0) {
+        atomicAdd(&fp->stats[ST_WAVE_ITERS], witers);
+        atomicAdd(&fp->stats[ST_LANE_ITERS], liters);
+    }
 }
 
 }  // namespace

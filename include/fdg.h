@@ -93,6 +93,8 @@ typedef struct {
     uint64_t blocks, skipped, plain_steps;
     uint64_t fallback_samples, closed_form_samples;
     uint64_t max_sample_ops;      /* most operations executed by one sample */
+    uint64_t wave_iterations;     /* kernel loop iterations summed over wavefronts */
+    uint64_t lane_iterations;     /* active lanes summed over those iterations */
     double reference_seconds;     /* host arbitrary-precision orbit */
     double bla_seconds;           /* GPU table build (device time) */
     double kernel_seconds;        /* GPU perturbation kernel (device time) */
