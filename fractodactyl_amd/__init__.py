@@ -20,7 +20,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libfdg.so")
+# FDG_LIB points at another build of the same library (e.g. an experimental variant).
+LIB_PATH = os.environ.get("FDG_LIB") or os.path.join(HERE, "libfdg.so")
 
 COL_CLASS, COL_NU, COL_DE, COL_NORMAL, COL_BOUND, COL_STRIPE = 1, 2, 4, 8, 16, 32
 TIER_AUTO, TIER_F64, TIER_FIXED, TIER_SCALED = -1, 0, 1, 2

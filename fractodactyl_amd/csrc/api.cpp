@@ -10,7 +10,7 @@
 #include <thread>
 #include <vector>
 
-#include "../../include/fdg.h"
+#include "fdg.h"
 #include "host.hpp"
 #include "kernels.hpp"
 

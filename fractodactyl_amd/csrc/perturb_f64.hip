@@ -291,7 +291,9 @@ struct Rare {
 // One lane's sample state (sample.rs:190-236) in registers.  (Zr, Zi) is Z_m, carried
 // from the previous escape test, so a step loads the orbit once (its landing point).
 struct Lane {
-    double xr, xi, dr, di, ar, ai, Zr, Zi, sr, si;
+    double xr, xi, dr, di, ar, ai, sr, si;
+    double Zr, Zi, Z1r, Z1i;            // Z_m and Z_{m+1}
+    double R0;                          // level-0 r2 of the block starting at m (when any)
     double scale, dc_r, dc_i, inverse;  // scaled tier only
     int32_t delta_exp, derivative_exp;  // scaled tier only
     uint32_t m, n, sn, hold, chk, nucleus, blocks, skipped, plain;
@@ -305,15 +307,31 @@ __device__ inline uint32_t np2_above(uint32_t n) {
     return 1u << (32 - __builtin_clz(n));
 }
 
+// Frame constants the loop reads, copied once into registers / SGPRs.
+struct Hot {
+    const double2* orbit;
+    const double* r2;
+    const double4* ab;
+    uint32_t last, max_iter, nlevels, steps;
+    uint64_t stride;
+    double esc2;
+    int32_t oe;
+    uint32_t len, count0;               // points; level-0 blocks
+    double Z1r, Z1i, Z2r, Z2i, R00;     // Z_1, Z_2 and the level-0 r2 at m = 1
+};
+
 template <bool EXT>
-__device__ inline void lane_start(Lane& s, Rare& r, uint32_t nucleus, double ar, double ai,
-                                  int32_t oe) {
+__device__ inline void lane_start(Lane& s, Rare& r, const Hot& h, uint32_t nucleus, double ar,
+                                  double ai, int32_t oe) {
     s.ar = ar;
     s.ai = ai;
     s.xr = 0.0;
     s.xi = 0.0;
     s.Zr = 0.0;  // Z_0 = 0
     s.Zi = 0.0;
+    s.Z1r = h.Z1r;
+    s.Z1i = h.Z1i;
+    s.R0 = 0.0;  // m = 0: no block starts here
     s.delta_exp = EXT ? oe : 0;
     s.scale = EXT ? exp2i(s.delta_exp) : 1.0;
     s.dc_r = ar;
@@ -354,17 +372,6 @@ __device__ inline void refresh_dc(Lane& s, int32_t oe) {
     }
 }
 
-// Frame constants the loop reads, copied once into registers / SGPRs.
-struct Hot {
-    const double2* orbit;
-    const double* r2;
-    const double4* ab;
-    uint32_t last, max_iter, nlevels, steps;
-    uint64_t stride;
-    double esc2;
-    int32_t oe;
-    uint32_t len;
-};
 
 enum : int { RUNNING = -1 };
 
@@ -417,6 +424,8 @@ __device__ void ext_rebase(Lane& s, const Hot& h, double fr, double fi, double f
     s.m = 0;
     s.Zr = 0.0;
     s.Zi = 0.0;
+    s.Z1r = h.Z1r;
+    s.Z1i = h.Z1i;
     if (s.scale < kTiny) {
         s.nucleus = 0;
         if (D) {
@@ -432,9 +441,11 @@ __device__ void ext_rebase(Lane& s, const Hot& h, double fr, double fi, double f
         s.scale = exp2i(s.delta_exp);
         s.m = 1;
         s.n = s.n + 1;
-        double2 z1 = h.orbit[1];
-        s.Zr = z1.x;
-        s.Zi = z1.y;
+        s.Zr = h.Z1r;
+        s.Zi = h.Z1i;
+        s.Z1r = h.Z2r;
+        s.Z1i = h.Z2i;
+        s.R0 = h.R00;
     }
     refresh_dc<true>(s, h.oe);
 }
@@ -446,7 +457,16 @@ template <bool EXT, bool D, bool L>
 __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, double& ofi) {
     // sample.rs:237 `while n < limit`: the budget ran out.
     if (s.n >= h.max_iter) return 2;
-    double2 Zn = h.orbit[s.m + 1];  // a plain step's landing point (s.m < last here)
+    // Gathers for the NEXT iteration, assuming this one is a plain step that does not
+    // rebase (the common case): Z_{m+2} and the level-0 r2 of a block starting at m+1.
+    // Their L2 latency then overlaps this whole iteration.  (s.m < last here, and the
+    // orbit buffer is padded past its end.)
+    double2 pz2 = h.orbit[s.m + 2];
+    double pr0 = 0.0;
+    if (L && (s.m & 1u) == 0 && (s.m >> 1) < h.count0) pr0 = h.r2[s.m >> 1];
+    double2 Zn = make_double2(s.Z1r, s.Z1i);  // a plain step's landing point
+    double2 Znext = pz2;
+    double r0next = pr0;
     int lvl = -1;
     if (L) {
         // Bla::find (bla.rs:234-249): a block of 2^(i+1) steps starts at k = m - 1 divisible
@@ -462,12 +482,16 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
             uint32_t nb = s.nucleus ? (s.n == 0 ? 1u : s.nucleus + 1) : 0xffffffffu;
             b = b < nb ? b : nb;
             b = b > s.n ? b - s.n : 0;
-            for (int lv = 0; lv < top; lv++) {
-                uint32_t idx = k >> (lv + 1);
-                if (!((2ull << lv) <= (uint64_t)b && idx < (h.steps >> (lv + 1)) &&
-                      d2 < h.r2[bla_level_offset(h.stride, lv) + idx]))
-                    break;
-                lvl = lv;
+            // level 0 from the carried radius (0 when no such block exists)
+            if (2u <= b && d2 < s.R0) {
+                lvl = 0;
+                for (int lv = 1; lv < top; lv++) {
+                    uint32_t idx = k >> (lv + 1);
+                    if (!((2ull << lv) <= (uint64_t)b && idx < (h.steps >> (lv + 1)) &&
+                          d2 < h.r2[bla_level_offset(h.stride, lv) + idx]))
+                        break;
+                    lvl = lv;
+                }
             }
         }
     }
@@ -489,6 +513,9 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
             b_i = ab.w;
             blen = 2u << lvl;
             Zn = h.orbit[s.m + blen];
+            Znext = h.orbit[s.m + blen + 1];
+            uint32_t i0 = (s.m + blen - 1) >> 1;
+            r0next = i0 < h.count0 ? h.r2[i0] : 0.0;
         }
         double xr = s.xr, xi = s.xi;
         double fr = Zmr + xr, fi = Zmi + xi;           // z = Z_m + delta (plain step)
@@ -525,6 +552,9 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         double a_r = ab.x, a_i = ab.y, b_r = ab.z, b_i = ab.w;
         uint32_t blen = 2u << lvl;
         Zn = h.orbit[s.m + blen];
+        Znext = h.orbit[s.m + blen + 1];
+        uint32_t i0 = (s.m + blen - 1) >> 1;
+        r0next = L && i0 < h.count0 ? h.r2[i0] : 0.0;
         if (s.nucleus && s.n >= 1) {
             bool st = s.flags & kConStarted;
             con_multiply(r.conr, r.coni, r.cone, st, a_r, a_i);
@@ -596,6 +626,9 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
     if (EXT) {
         s.Zr = Zn.x;
         s.Zi = Zn.y;
+        s.Z1r = Znext.x;
+        s.Z1i = Znext.y;
+        s.R0 = r0next;
         if (rebase) {
             ext_rebase<D>(s, h, fr, fi, f2, Zn.x, Zn.y);
         } else {
@@ -641,6 +674,9 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
     s.m = rebase ? 0u : s.m;
     s.Zr = rebase ? 0.0 : Zn.x;
     s.Zi = rebase ? 0.0 : Zn.y;
+    s.Z1r = rebase ? h.Z1r : Znext.x;
+    s.Z1i = rebase ? h.Z1i : Znext.y;
+    s.R0 = r0next;  // after a rebase m = 0 and no block is looked up
     bool res = s.xr * s.xr + s.xi * s.xi > kResolvable * kResolvable * f2;
     double dist = fabs(fr - s.sr) + fabs(fi - s.si);
     double size = fabs(s.sr) + fabs(s.si);
@@ -684,7 +720,7 @@ __device__ inline unsigned long long wave_sum(unsigned long long v) {
 // sample finished are refilled together from one atomic per wave, so the wave keeps
 // 64 samples in flight however uneven their iteration counts are.
 template <bool EXT, bool D, bool L>
-__global__ void __launch_bounds__(kBlock) perturb_f64(const FrameDev* __restrict__ fp) {
+__global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restrict__ fp) {
     __shared__ RareLds rare;
     const uint32_t lane = __lane_id();
     Rare r(rare, threadIdx.x);
@@ -700,6 +736,15 @@ __global__ void __launch_bounds__(kBlock) perturb_f64(const FrameDev* __restrict
     h.stride = fp->bla.stride;
     h.esc2 = fp->r2;
     h.oe = (int32_t)fp->h_e;
+    h.count0 = (uint32_t)(fp->bla.steps >> 1);
+    {
+        double2 z1 = h.orbit[1], z2 = h.orbit[2];
+        h.Z1r = z1.x;
+        h.Z1i = z1.y;
+        h.Z2r = z2.x;
+        h.Z2i = z2.y;
+        h.R00 = (L && h.count0 > 0) ? h.r2[0] : 0.0;
+    }
     const uint32_t count = fp->count, nx = fp->nx, nucleus = (uint32_t)fp->nucleus;
     uint32_t* next = fp->next;
     Lane s;
@@ -734,14 +779,14 @@ __global__ void __launch_bounds__(kBlock) perturb_f64(const FrameDev* __restrict
                             fp->n[idx] = 0;
                             closed += L ? 1 : 0;
                         } else {
-                            lane_start<EXT>(s, r, nucleus, ar, ai, 0);
+                            lane_start<EXT>(s, r, h, nucleus, ar, ai, 0);
                             active = true;
                         }
                     } else if (fp->tier == 1) {
-                        lane_start<EXT>(s, r, nucleus, ux * fp->h, uy * fp->h, 0);
+                        lane_start<EXT>(s, r, h, nucleus, ux * fp->h, uy * fp->h, 0);
                         active = true;
                     } else {
-                        lane_start<EXT>(s, r, nucleus, ux * fp->h_m, uy * fp->h_m, h.oe);
+                        lane_start<EXT>(s, r, h, nucleus, ux * fp->h_m, uy * fp->h_m, h.oe);
                         active = true;
                     }
                 }
