@@ -116,6 +116,8 @@ SIGNATURES = {
     "fdg_frame_read": (C.c_int, [C.c_void_p, C.POINTER(_Columns), C.POINTER(_Outcomes),
                                  C.POINTER(_Stats), C.c_char_p, C.c_size_t]),
     "fdg_frame_close": (None, [C.c_void_p]),
+    "fdg_jacobian": (C.c_int, [C.POINTER(_View), C.POINTER(_Params), C.POINTER(_Outcomes),
+                               C.c_void_p, C.c_void_p]),
     "fdg_write_fds": (C.c_int64, [C.POINTER(_View), C.POINTER(_Params), C.c_char_p,
                                   C.POINTER(_Columns), C.c_void_p, C.c_uint64]),
 }
@@ -332,6 +334,21 @@ def build_bla(re, im, eps, dc_max, dc_exp=0, ctx=None):
         levels.append(out[off:off + c].copy())
         off += c
     return levels
+
+
+def jacobian(header, samples):
+    """Screen-space 2x2 Jacobian of z at escape (fdg_jacobian): (j[n, 2, 2], exponent[n]),
+    J = j * 2^exponent, rows d(Re z, Im z), columns d/d(x, y) in samples."""
+    raw = samples["raw"]
+    n = len(raw["kind"])
+    oc = _Outcomes(*[_p(raw[k]) for k in ("kind", "n", "zr", "zi", "dr", "di", "dexp", "ops")])
+    j = np.zeros((n, 2, 2))
+    e = np.zeros(n, np.int64)
+    rc = lib().fdg_jacobian(C.byref(_view(header["view"])), C.byref(_params(header["params"])),
+                            C.byref(oc), _p(j), _p(e))
+    if rc != 0:
+        raise FdgError(rc, "jacobian")
+    return j, e
 
 
 def write(header, samples, minor=1):

@@ -629,6 +629,35 @@ int fdg_render(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int route, 
     return rc;
 }
 
+int fdg_jacobian(const fdg_view* v, const fdg_params* p, const fdg_outcomes* raw, double* j,
+                 int64_t* jexp) {
+    if (!v || !p || !raw || !raw->kind || !raw->dr || !raw->di || !raw->dexp || !j || !jexp)
+        return FDG_EINVAL;
+    Plane pl;
+    std::string e;
+    if (!Plane::make(v->center_re, v->center_im, v->width, v->rotation, p->nx, p->ny, &pl, &e))
+        return FDG_EINVAL;
+    size_t n = (size_t)p->nx * p->ny;
+    for (size_t i = 0; i < n; i++) {
+        double* o = j + 4 * i;
+        if (raw->kind[i] != FDG_ESCAPED) {
+            o[0] = o[1] = o[2] = o[3] = 0.0;
+            jexp[i] = 0;
+            continue;
+        }
+        // dz/dc h_m, then the grid's basis: d c / d x = h (cos, sin), d c / d y = h (sin, -cos)
+        double ar = raw->dr[i] * pl.h_m, ai = raw->di[i] * pl.h_m;
+        double xr = ar * pl.cos_r - ai * pl.sin_r, xi = ar * pl.sin_r + ai * pl.cos_r;
+        double yr = ar * pl.sin_r + ai * pl.cos_r, yi = ai * pl.sin_r - ar * pl.cos_r;
+        o[0] = xr;  // d Re z / d x
+        o[1] = yr;  // d Re z / d y
+        o[2] = xi;  // d Im z / d x
+        o[3] = yi;  // d Im z / d y
+        jexp[i] = raw->dexp[i] + pl.h_e;
+    }
+    return FDG_OK;
+}
+
 int64_t fdg_write_fds(const fdg_view* v, const fdg_params* p, const char* kernel_id,
                       const fdg_columns* c, uint8_t* buf, uint64_t cap) {
     if (!v || !p || !c || !c->class_ || !kernel_id) return FDG_EINVAL;

@@ -180,7 +180,11 @@ def _offset_centre(nuc, shift, width, seed, bits):
 
 
 def scene(k, generate=False):
-    """The benchmark frame k: dict with view strings, grid, iterations and provenance."""
+    """The benchmark frame k: dict with view strings, grid, iterations and provenance.
+
+    Frame 3 (1e-1000) uses the reference's own ladder nucleus of period 16116 (its minibrot
+    is ~1e-1000 wide), stored in scenes.json: the bisection search above only reaches
+    near-parabolic boundary points, whose minibrots are far larger than 1e-1000."""
     cache = json.load(open(CACHE)) if os.path.exists(CACHE) else {}
     key = str(k)
     if key not in cache:

@@ -156,6 +156,16 @@ int fdg_build_bla(fdg_ctx *ctx, const double *re, const double *im, uint64_t len
                   double dc_max, int64_t dc_exp, double *blocks, uint64_t cap_blocks,
                   uint32_t *levels);
 
+/* The screen-space Jacobian of z at escape for jet-AA: d(z.re, z.im)/d(x, y) where (x, y)
+ * is the sample's grid position (view.rs:117-122 unit_offset times h).  z_n(c) is
+ * holomorphic, so it is the rotation-scaling matrix of the carried dz/dc composed with the
+ * grid's rotation: J = [[dr, -di], [di, dr]] h [[cos, sin], [sin, -cos]].  Written per
+ * sample as j[4*i..4*i+3] = J00, J01, J10, J11 times 2^jexp[i] (escaped samples only; 0
+ * elsewhere).  Host-only.  No counterpart in the reference, whose jet-AA reconstructs
+ * derivatives from neighbour stencils (crates/fd-cli/src/jetaa.rs:1-28). */
+int fdg_jacobian(const fdg_view *view, const fdg_params *params, const fdg_outcomes *raw,
+                 double *j, int64_t *jexp);
+
 /* fd_samples::write (crates/fd-samples/src/write.rs:8-57): canonical .fds bytes of the
  * columns.  Returns the byte count (the size needed when cap is too small), or < 0. */
 int64_t fdg_write_fds(const fdg_view *view, const fdg_params *params, const char *kernel_id,
