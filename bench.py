@@ -29,8 +29,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(view, params, route_bla, target_seconds, threads):
-    """The oracle (C port of the reference CPU BLA renderer) on a row sample of the frame."""
+def cpu_baseline(view, params, route_bla, target_seconds, threads, orbit=None):
+    """The oracle (C port of the reference CPU BLA renderer) on a row sample of the frame
+    (orbit: a supplied (re, im, bits) reference, as the film's frames use)."""
     import numpy as np
     import oracle
     nx, ny = params.nx, params.ny
@@ -42,7 +43,8 @@ def cpu_baseline(view, params, route_bla, target_seconds, threads):
             mask[j * nx:(j + 1) * nx] = 1
         f = oracle.render(view.center_re, view.center_im, view.width, view.rotation, nx, ny,
                           params.max_iter, params.escape_radius, columns=params.columns,
-                          bla=route_bla, threads=threads, mask=mask, outcomes=False)
+                          bla=route_bla, threads=threads, mask=mask, outcomes=False,
+                          orbit=orbit)
         return f["stats"], len(rows)
 
     st, rows = run(ny // 4, ny // 8)   # probe: 4 rows
@@ -107,6 +109,82 @@ def measure(fa, view, params, route, steps, warmup, label):
                           "flops_model": f"{fp}/plain step + {fb}/applied block"})
 
 
+def measure_film(fa, sc, cols, steps, warmup, threads, cpu_seconds):
+    """The 64-keyframe film: one shared orbit, per-frame or shared BLA tables, rendered by
+    one graph of persistent batch launches or by one launch per frame."""
+    import numpy as np
+    from fractodactyl_amd.scenes import film_widths
+    cfg = sc["config"]
+    widths = film_widths(sc)
+    views = [fa.View(sc["center_re"], sc["center_im"], w, 0.0) for w in widths]
+    params = fa.Params(cfg["nx"], cfg["ny"], 1, cfg["max_iter"], cfg["escape_radius"], cols)
+    out = {"workload": f"film: {len(views)} keyframes {cfg['nx']}x{cfg['ny']}, widths "
+                       f"{widths[0]} .. {widths[-1]} (geometric) toward the period-"
+                       f"{sc['nucleus']['period']} nucleus, one shared orbit, max_iter "
+                       f"{cfg['max_iter']}, BLA eps 2^-50, FP64 deltas, nu/de/normal"}
+    for tables, tname in ((fa.TABLES_PER_FRAME, "per_frame_tables"),
+                          (fa.TABLES_SHARED, "shared_table")):
+        t = time.perf_counter()
+        film = fa.Film(views, params, tables=tables)
+        open_s = time.perf_counter() - t
+        res = {"open_seconds": open_s}
+        for launch, lname in ((fa.LAUNCH_BATCH, "one_graph_batch_launch"),
+                              (fa.LAUNCH_PER_FRAME, "launch_per_frame")):
+            for i in range(warmup):
+                st = film.run(launch)
+            walls, kerns = [], []
+            for i in range(steps):
+                t0 = time.perf_counter()
+                st = film.run(launch)
+                walls.append(time.perf_counter() - t0)
+                kerns.append(st["kernel_seconds"])
+                log(f"film {tname} {lname} step {i}: kernel {kerns[-1] * 1e3:.2f} ms")
+            w = sum(walls) / len(walls)
+            res[lname] = dict(ms_per_film=w * 1e3, kernel_ms=sum(kerns) / len(kerns) * 1e3,
+                              value=st["equivalent"] / w, equivalent_iterations=st["equivalent"],
+                              plain_steps=st["plain_steps"], blocks=st["blocks"],
+                              max_sample_ops=st["max_sample_ops"],
+                              lane_utilisation=st["lane_iterations"] /
+                              max(1, 64 * st["wave_iterations"]))
+        res["reference_seconds"] = st["reference_seconds"]
+        res["reference_bits"] = st["reference_bits"]
+        res["reference_len"] = st["reference_len"]
+        res["tables_seconds"] = st["bla_seconds"]
+        res["batch_speedup_over_per_frame_launches"] = (
+            res["launch_per_frame"]["ms_per_film"] / res["one_graph_batch_launch"]["ms_per_film"])
+        if tables == fa.TABLES_PER_FRAME:
+            eq, esc = [], []
+            for k in range(len(views)):
+                _, c, fs = film.read(k)
+                eq.append(fs["equivalent"])
+                esc.append(float(((c["class"] & 3) == 0).mean()))
+            res["per_frame_equivalent_min_median_max"] = [min(eq), float(np.median(eq)),
+                                                          max(eq)]
+            res["per_frame_escaped_fraction_min_max"] = [min(esc), max(esc)]
+        film.close()
+        out[tname] = res
+    if cpu_seconds > 0:
+        # the oracle on rows of every 8th keyframe, from the same shared orbit
+        import oracle
+        deepest = views[-1]
+        zr, zi, bits = oracle.reference(deepest.center_re, deepest.center_im, deepest.width,
+                                        0.0, params.nx, params.ny, params.max_iter,
+                                        params.escape_radius)
+        vals = [cpu_baseline(views[k], params, True, cpu_seconds / 8, threads,
+                             orbit=(zr, zi, bits)) for k in range(0, len(views), 8)]
+        out["cpu_baseline"] = dict(value=float(np.mean([v["value"] for v in vals])),
+                                   unit="pixel-iter/s", cores=threads, kind="port",
+                                   sample="mean over keyframes 0, 8, .., 56 of: " +
+                                          vals[0]["sample"].split(",")[0] + " (first frame)")
+    return out
+
+
+def deep_view(fa, s3, cols):
+    c3 = s3["config"]
+    return (fa.View(s3["center_re"], s3["center_im"], c3["width"], 0.0),
+            fa.Params(c3["nx"], c3["ny"], 1, c3["max_iter"], c3["escape_radius"], cols))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,8 +193,10 @@ def main():
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--frames", default="config,escape",
-                    help="which frames to run: config (the JSON line), escape (companion)")
+    ap.add_argument("--frames", default="config,escape,deep,film",
+                    help="which frames to run: config (the JSON line); companions escape "
+                         "(reference zoom path at 2e-49), deep (1e-1000, 1M iterations), "
+                         "film (64 keyframes sharing one orbit)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this path runs on one GPU; --gpus {a.gpus} is not supported "
@@ -133,20 +213,37 @@ def main():
     route = fa.ROUTE_BLA if cfg["bla"] else fa.ROUTE_PLAIN
     fa.default_context()
     frames = a.frames.split(",")
-    if "config" not in frames:
-        # profiling aid: the escape companion alone, no JSON line
-        ka = json.load(open(os.path.join(ROOT, "tests", "golden", "known_answers.json")))
-        v0 = fa.View(ka["v0_centre"]["re"], ka["v0_centre"]["im"], "2e-49", 0.0)
-        p0 = fa.Params(1920, 1080, 1, 100_000, 1e10, cols)
-        m0 = measure(fa, v0, p0, fa.ROUTE_BLA, a.steps, a.warmup, "v0-2e-49")
-        log(json.dumps(m0))
-        return
-    m = measure(fa, view, params, route, a.steps, a.warmup, f"config{a.config}")
+    ka = json.load(open(os.path.join(ROOT, "tests", "golden", "known_answers.json")))
     # Escape-dominated companion: the reference's own deep-zoom path (bench/path-atlas-v0.txt,
     # the period-764 minibrot's neighbourhood) at its frame-749 width, at 1080p.
-    ka = json.load(open(os.path.join(ROOT, "tests", "golden", "known_answers.json")))
     v0 = fa.View(ka["v0_centre"]["re"], ka["v0_centre"]["im"], "2e-49", 0.0)
     p0 = fa.Params(1920, 1080, 1, 100_000, 1e10, cols)
+    if "config" not in frames:
+        # profiling aid: companions alone, each printed as JSON on stderr, no bench line
+        if "escape" in frames:
+            log(json.dumps(measure(fa, v0, p0, fa.ROUTE_BLA, a.steps, a.warmup, "v0-2e-49")))
+        if "deep" in frames:
+            v3, p3 = deep_view(fa, get_scene(3), cols)
+            log(json.dumps(measure(fa, v3, p3, fa.ROUTE_BLA, a.steps, a.warmup,
+                                   "deep-1e-1000")))
+        if "film" in frames:
+            log(json.dumps(measure_film(fa, get_scene(4), cols, a.steps, a.warmup, 1, 0.0)))
+        return
+    m = measure(fa, view, params, route, a.steps, a.warmup, f"config{a.config}")
+    threads = min(16, os.cpu_count() or 1)
+    cpu_s = 0.0 if a.no_cpu_baseline else a.cpu_seconds
+    deep = {}
+    if "deep" in frames:
+        v3, p3 = deep_view(fa, get_scene(3), cols)
+        deep = measure(fa, v3, p3, fa.ROUTE_BLA, a.steps, a.warmup, "deep-1e-1000")
+        deep["workload"] = (f"1920x1080 width 1e-1000 at the period-"
+                            f"{get_scene(3)['nucleus']['period']} ladder nucleus, max_iter 1e6, "
+                            f"BLA eps 2^-50, scaled tier (f64 mantissa x 2^e deltas), dz/dc "
+                            f"carried")
+        if cpu_s > 0:
+            deep["cpu_baseline"] = cpu_baseline(v3, p3, True, cpu_s, threads)
+    film = measure_film(fa, get_scene(4), cols, a.steps, a.warmup, threads, cpu_s) \
+        if "film" in frames else {}
     m0 = (measure(fa, v0, p0, fa.ROUTE_BLA, a.steps, a.warmup, "v0-2e-49")
           if "escape" in frames else {})
     out = {
@@ -171,11 +268,12 @@ def main():
             "escape_frame": {"workload": "reference deep-zoom path (v0 centre, period-764 "
                                          "minibrot) at width 2e-49, 1920x1080, max_iter 1e5, BLA",
                              **{k: v for k, v in m0.items()}},
+            "deep_frame": deep,
+            "film": film,
         },
         "roofline": m["roofline"],
     }
     if not a.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
         t = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline(view, params, route == fa.ROUTE_BLA, a.cpu_seconds,
                                            threads)

@@ -116,6 +116,13 @@ SIGNATURES = {
     "fdg_frame_read": (C.c_int, [C.c_void_p, C.POINTER(_Columns), C.POINTER(_Outcomes),
                                  C.POINTER(_Stats), C.c_char_p, C.c_size_t]),
     "fdg_frame_close": (None, [C.c_void_p]),
+    "fdg_film_open": (C.c_int, [C.c_void_p, C.POINTER(_View), C.c_uint32, C.POINTER(_Params),
+                                C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_void_p)]),
+    "fdg_film_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(_Stats)]),
+    "fdg_film_read": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(_Columns),
+                                C.POINTER(_Outcomes), C.POINTER(_Stats), C.c_char_p,
+                                C.c_size_t]),
+    "fdg_film_close": (None, [C.c_void_p]),
     "fdg_jacobian": (C.c_int, [C.POINTER(_View), C.POINTER(_Params), C.POINTER(_Outcomes),
                                C.c_void_p, C.c_void_p]),
     "fdg_write_fds": (C.c_int64, [C.POINTER(_View), C.POINTER(_Params), C.c_char_p,
@@ -238,6 +245,29 @@ def render(view, params, route=ROUTE_PLAIN, form=FORM_F64, bla_eps=EPS, ctx=None
     return header, cols, {k: getattr(st, k) for k, _ in _Stats._fields_}
 
 
+def _out_buffers(p, raw):
+    """Caller-owned column (and raw outcome) arrays for one frame, and their C views."""
+    n = p.nx * p.ny
+    cols = {"class": np.zeros(n, np.uint8)}
+    if p.columns & COL_NU:
+        cols["nu"] = np.zeros(n, np.float64)
+    if p.columns & COL_DE:
+        cols["de"] = np.zeros(n, np.float32)
+    if p.columns & COL_NORMAL:
+        cols["normal"] = np.zeros(n, np.uint16)
+    cs = _Columns(_p(cols["class"]), _p(cols.get("nu")), _p(cols.get("de")),
+                  _p(cols.get("normal")))
+    oc = None
+    if raw:
+        rw = {"kind": np.zeros(n, np.uint8), "n": np.zeros(n, np.uint64),
+              "zr": np.zeros(n), "zi": np.zeros(n), "dr": np.zeros(n), "di": np.zeros(n),
+              "dexp": np.zeros(n, np.int64), "ops": np.zeros(n, np.uint32)}
+        oc = _Outcomes(*[_p(rw[k]) for k in ("kind", "n", "zr", "zi", "dr", "di", "dexp",
+                                             "ops")])
+        cols["raw"] = rw
+    return cols, cs, oc
+
+
 class ResidentFrame:
     """A frame whose orbit, table and outcomes stay in HBM between passes
     (fdg_frame_open / run / read / close)."""
@@ -258,37 +288,72 @@ class ResidentFrame:
         return {k: getattr(st, k) for k, _ in _Stats._fields_}
 
     def read(self, raw=False):
-        n = self.params.nx * self.params.ny
-        p = self.params
-        cols = {"class": np.zeros(n, np.uint8)}
-        if p.columns & COL_NU:
-            cols["nu"] = np.zeros(n, np.float64)
-        if p.columns & COL_DE:
-            cols["de"] = np.zeros(n, np.float32)
-        if p.columns & COL_NORMAL:
-            cols["normal"] = np.zeros(n, np.uint16)
-        cs = _Columns(_p(cols["class"]), _p(cols.get("nu")), _p(cols.get("de")),
-                      _p(cols.get("normal")))
-        oc = None
-        if raw:
-            rw = {"kind": np.zeros(n, np.uint8), "n": np.zeros(n, np.uint64),
-                  "zr": np.zeros(n), "zi": np.zeros(n), "dr": np.zeros(n), "di": np.zeros(n),
-                  "dexp": np.zeros(n, np.int64), "ops": np.zeros(n, np.uint32)}
-            oc = _Outcomes(*[_p(rw[k]) for k in ("kind", "n", "zr", "zi", "dr", "di", "dexp",
-                                                 "ops")])
-            cols["raw"] = rw
+        cols, cs, oc = _out_buffers(self.params, raw)
         st = _Stats()
         kid = C.create_string_buffer(256)
         self.ctx._check(lib().fdg_frame_read(self._h, C.byref(cs),
                                              C.byref(oc) if oc is not None else None,
                                              C.byref(st), kid, 256))
-        header = dict(view=self.view, params=p, kernel=kid.value.decode(),
-                      columns=p.columns | COL_CLASS)
+        header = dict(view=self.view, params=self.params, kernel=kid.value.decode(),
+                      columns=self.params.columns | COL_CLASS)
         return header, cols, {k: getattr(st, k) for k, _ in _Stats._fields_}
 
     def close(self):
         if getattr(self, "_h", None):
             lib().fdg_frame_close(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+TABLES_PER_FRAME, TABLES_SHARED = 0, 1
+LAUNCH_BATCH, LAUNCH_PER_FRAME = 0, 1
+
+
+class Film:
+    """Frames at one centre (a zoom's keyframes) rendered against one reference orbit at
+    the deepest frame's precision, as fd_kernel::render_with / render_bla with a supplied
+    orbit (crates/fd-kernel/src/grid.rs:95-109,190-210) renders each of them
+    (fdg_film_open / run / read / close)."""
+
+    def __init__(self, views, params, route=ROUTE_BLA, form=FORM_F64, bla_eps=EPS,
+                 tables=TABLES_PER_FRAME, ctx=None):
+        self.ctx = ctx or default_context()
+        self.views, self.params = list(views), params
+        arr = (_View * len(self.views))(*[_view(v) for v in self.views])
+        self._keep = arr
+        self._p = _params(params)
+        h = C.c_void_p()
+        self.ctx._check(lib().fdg_film_open(self.ctx._h, arr, len(self.views),
+                                            C.byref(self._p), route, form, bla_eps, tables,
+                                            C.byref(h)))
+        self._h = h
+
+    def run(self, launch=LAUNCH_BATCH, grid=0):
+        """Render every frame (one graph of persistent batch launches, or one launch per
+        frame); returns the film's aggregate stats."""
+        st = _Stats()
+        self.ctx._check(lib().fdg_film_run(self._h, launch, grid, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in _Stats._fields_}
+
+    def read(self, k, raw=False):
+        cols, cs, oc = _out_buffers(self.params, raw)
+        st = _Stats()
+        kid = C.create_string_buffer(256)
+        self.ctx._check(lib().fdg_film_read(self._h, k, C.byref(cs),
+                                            C.byref(oc) if oc is not None else None,
+                                            C.byref(st), kid, 256))
+        header = dict(view=self.views[k], params=self.params, kernel=kid.value.decode(),
+                      columns=self.params.columns | COL_CLASS)
+        return header, cols, {k: getattr(st, k) for k, _ in _Stats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fdg_film_close(self._h)
             self._h = None
 
     def __enter__(self):

@@ -1,114 +1,17 @@
-// C ABI (include/fdg.h): frame setup on the host, resident device buffers, kernel
-// launches on one HIP stream, and the column encoding.
-#include <hip/hip_runtime.h>
-
+// C ABI (include/fdg.h): contexts, single frames (setup on the host, resident device
+// buffers, kernel launches on one HIP stream, the column encoding), host-only helpers.
 #include <algorithm>
-#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <string>
 #include <thread>
 #include <vector>
 
-#include "fdg.h"
-#include "host.hpp"
-#include "kernels.hpp"
+#include "internal.hpp"
 
 namespace fdg {
-hipError_t launch_bla_build(const double* orbit, double* r, double* r2, double* ab, double* alpha,
-                            double* beta, uint32_t* valid, uint64_t steps, uint64_t stride,
-                            double eps, double dc_max, int64_t dc_exp, uint32_t max_levels,
-                            hipStream_t stream);
-hipError_t launch_perturb_f64(const FrameDev* f, bool ext, bool deriv, bool bla, int grid,
-                              hipStream_t stream);
-}  // namespace fdg
-
-using namespace fdg;
-
-namespace {
-
-constexpr uint32_t kMaxLevels = 48;
-constexpr double kEpsLimit = 1.0 / 16777216.0;  // bla.rs:288-292 EPS_LIMIT = 2^-24
 
 std::string g_create_error;
-
-// A grow-only device allocation.
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    hipError_t ensure(size_t n) {
-        if (n <= bytes) return hipSuccess;
-        if (p) hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        hipError_t e = hipMalloc(&p, n);
-        if (e == hipSuccess) bytes = n;
-        return e;
-    }
-    template <class T>
-    T* as() const {
-        return static_cast<T*>(p);
-    }
-    void release() {
-        if (p) hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-};
-
-double now_s() {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
-        .count();
-}
-
-}  // namespace
-
-struct fdg_ctx {
-    int device = 0;
-    int cus = 256;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    std::string err;
-
-    int fail(int code, const std::string& m) {
-        err = m;
-        return code;
-    }
-    int hip(hipError_t e, const char* what) {
-        if (e == hipSuccess) return FDG_OK;
-        err = std::string(what) + ": " + hipGetErrorString(e);
-        return e == hipErrorOutOfMemory ? FDG_ENOMEM : FDG_EDEVICE;
-    }
-};
-
-namespace {
-// Device residency of one frame: the orbit, its BLA table and the per-sample outcomes.
-struct Buffers {
-    DevBuf orbit, bla_r, bla_r2, bla_ab, bla_alpha, bla_beta, bla_valid;
-    DevBuf o_kind, o_n, o_zr, o_zi, o_dr, o_di, o_dexp, o_ops, counters, fdev;
-    uint64_t bla_stride = 0, bla_steps = 0;
-    void release() {
-        for (DevBuf* b : {&orbit, &bla_r, &bla_r2, &bla_ab, &bla_alpha, &bla_beta, &bla_valid,
-                          &o_kind, &o_n, &o_zr, &o_zi, &o_dr, &o_di, &o_dexp, &o_ops,
-                          &counters, &fdev})
-            b->release();
-    }
-    int outputs(fdg_ctx* ctx, size_t n);
-};
-}  // namespace
-
-#define FDG_TRY(ctx, call, what)                       \
-    do {                                               \
-        int rc_ = (ctx)->hip((call), (what));          \
-        if (rc_ != FDG_OK) return rc_;                 \
-    } while (0)
-
-namespace {
-
-struct Frame {
-    Plane pl;
-    Tier tier;
-};
 
 int Buffers::outputs(fdg_ctx* ctx, size_t n) {
     FDG_TRY(ctx, o_kind.ensure(n), "hipMalloc");
@@ -160,8 +63,113 @@ uint64_t next_pow2_host(uint64_t x) {
     return s;
 }
 
-// Upload the orbit interleaved (re, im) and build the BLA table on the GPU.  Returns the
-// level count (0 when no level has a valid block) in *levels.
+int upload_orbit(fdg_ctx* ctx, DevBuf* dst, const Orbit& o, double* seconds) {
+    double t0 = now_s();
+    std::vector<double> inter(2 * o.len());
+    for (uint64_t k = 0; k < o.len(); k++) {
+        inter[2 * k] = o.re[k];
+        inter[2 * k + 1] = o.im[k];
+    }
+    FDG_TRY(ctx, dst->ensure(inter.size() * 8 + 64), "hipMalloc orbit");
+    FDG_TRY(ctx,
+            hipMemcpyAsync(dst->p, inter.data(), inter.size() * 8, hipMemcpyHostToDevice,
+                           ctx->stream),
+            "orbit upload");
+    FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "orbit upload");
+    *seconds += now_s() - t0;
+    return FDG_OK;
+}
+
+// Leading and trailing zero digits are dropped before comparing (orbit.rs:235-247).
+bool same_decimal(const std::string& a, const std::string& b) {
+    Decimal x, y;
+    if (!Decimal::parse(a, &x) || !Decimal::parse(b, &y)) return false;
+    auto canon = [](Decimal& d) {
+        size_t lead = 0;
+        while (lead < d.digits.size() && d.digits[lead] == 0) lead++;
+        d.digits.erase(d.digits.begin(), d.digits.begin() + lead);
+        while (!d.digits.empty() && d.digits.back() == 0) {
+            d.digits.pop_back();
+            d.exp10 += 1;
+        }
+        if (d.digits.empty()) d.exp10 = 0;
+    };
+    canon(x);
+    canon(y);
+    return x.neg == y.neg && x.digits == y.digits && x.exp10 == y.exp10;
+}
+
+int read_encode(fdg_ctx* ctx, const Buffers& b, size_t off, size_t n, const Plane& pl,
+                const fdg_params* p, fdg_columns* out, fdg_outcomes* raw, double* transfer_s,
+                double* encode_s, std::vector<uint64_t>* n_out, std::vector<uint8_t>* kind_out,
+                std::vector<uint32_t>* ops_out) {
+    double t1 = now_s();
+    std::vector<uint8_t> kind(n);
+    std::vector<uint64_t> nn(n);
+    std::vector<double> zr(n), zi(n), dr(n), di(n);
+    std::vector<int64_t> dexp(n);
+    std::vector<uint32_t> ops(n);
+    auto get = [&](void* dst, const DevBuf& src, size_t elem) {
+        return hipMemcpy(dst, static_cast<const uint8_t*>(src.p) + off * elem, n * elem,
+                         hipMemcpyDeviceToHost);
+    };
+    FDG_TRY(ctx, get(kind.data(), b.o_kind, 1), "readback");
+    FDG_TRY(ctx, get(nn.data(), b.o_n, 8), "readback");
+    FDG_TRY(ctx, get(zr.data(), b.o_zr, 8), "readback");
+    FDG_TRY(ctx, get(zi.data(), b.o_zi, 8), "readback");
+    FDG_TRY(ctx, get(dr.data(), b.o_dr, 8), "readback");
+    FDG_TRY(ctx, get(di.data(), b.o_di, 8), "readback");
+    FDG_TRY(ctx, get(dexp.data(), b.o_dexp, 8), "readback");
+    FDG_TRY(ctx, get(ops.data(), b.o_ops, 4), "readback");
+    *transfer_s += now_s() - t1;
+    if (n_out) *n_out = nn;
+    if (kind_out) *kind_out = kind;
+    if (ops_out) *ops_out = ops;
+    // The device holds the stop iterate of Unresolved samples too; Outcome::Unresolved
+    // carries none (sample.rs:21-40).
+    for (size_t i = 0; i < n; i++)
+        if (kind[i] == KIND_UNRESOLVED) nn[i] = 0;
+    double t2 = now_s();
+    Encoder enc(pl, p->ss);
+    double* nu = (p->columns & FDG_COL_NU) ? out->nu : nullptr;
+    float* de = (p->columns & FDG_COL_DE) ? out->de : nullptr;
+    uint16_t* nm = (p->columns & FDG_COL_NORMAL) ? out->normal : nullptr;
+    unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned w = 0; w < nt; w++)
+        th.emplace_back([&, w] {
+            size_t lo = n * w / nt, hi = n * (w + 1) / nt;
+            for (size_t i = lo; i < hi; i++) {
+                if (nu) nu[i] = 0.0;
+                if (de) de[i] = 0.0f;
+                if (nm) nm[i] = 0;
+                enc.put(i, kind[i], nn[i], zr[i], zi[i], dr[i], di[i], dexp[i], out->class_, nu,
+                        de, nm);
+            }
+        });
+    for (auto& t : th) t.join();
+    *encode_s = now_s() - t2;
+    if (raw) {
+        if (raw->kind) std::memcpy(raw->kind, kind.data(), n);
+        if (raw->n) std::memcpy(raw->n, nn.data(), n * 8);
+        if (raw->zr) std::memcpy(raw->zr, zr.data(), n * 8);
+        if (raw->zi) std::memcpy(raw->zi, zi.data(), n * 8);
+        if (raw->dr) std::memcpy(raw->dr, dr.data(), n * 8);
+        if (raw->di) std::memcpy(raw->di, di.data(), n * 8);
+        if (raw->dexp) std::memcpy(raw->dexp, dexp.data(), n * 8);
+        if (raw->ops) std::memcpy(raw->ops, ops.data(), n * 4);
+    }
+    return FDG_OK;
+}
+
+}  // namespace fdg
+
+using namespace fdg;
+
+namespace {
+
+// Build the BLA table on the GPU over the uploaded orbit.  Returns the level count (0 when
+// no level has a valid block) in *levels.
 int build_table(fdg_ctx* ctx, Buffers* bf, uint64_t len, double eps, double dc_max,
                 int64_t dc_exp, uint32_t* levels, double* seconds) {
     Buffers& B = *bf;
@@ -203,23 +211,6 @@ int build_table(fdg_ctx* ctx, Buffers* bf, uint64_t len, double eps, double dc_m
     uint32_t n = 0;
     while (n < kMaxLevels && (steps >> (n + 1)) > 0 && valid[n]) n++;
     *levels = n;
-    return FDG_OK;
-}
-
-int upload_orbit(fdg_ctx* ctx, Buffers* B, const Orbit& o, double* seconds) {
-    double t0 = now_s();
-    std::vector<double> inter(2 * o.len());
-    for (uint64_t k = 0; k < o.len(); k++) {
-        inter[2 * k] = o.re[k];
-        inter[2 * k + 1] = o.im[k];
-    }
-    FDG_TRY(ctx, B->orbit.ensure(inter.size() * 8 + 64), "hipMalloc orbit");
-    FDG_TRY(ctx,
-            hipMemcpyAsync(B->orbit.p, inter.data(), inter.size() * 8, hipMemcpyHostToDevice,
-                           ctx->stream),
-            "orbit upload");
-    FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "orbit upload");
-    *seconds += now_s() - t0;
     return FDG_OK;
 }
 
@@ -343,7 +334,7 @@ int fdg_build_bla(fdg_ctx* ctx, const double* re, const double* im, uint64_t len
         Buffers* b;
         ~Free() { b->release(); }
     } guard{&B};
-    int rc = upload_orbit(ctx, &B, o, &t);
+    int rc = upload_orbit(ctx, &B.orbit, o, &t);
     if (rc) return rc;
     uint32_t nl = 0;
     rc = build_table(ctx, &B, len, eps, dc_max, dc_exp, &nl, &t);
@@ -393,7 +384,9 @@ int fdg_frame_open(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int rou
     if (route != FDG_ROUTE_PLAIN && route != FDG_ROUTE_BLA)
         return ctx->fail(FDG_EINVAL, "unknown route");
     uint64_t count64 = (uint64_t)p->nx * p->ny;
-    if (count64 >= (1ull << 32)) return ctx->fail(FDG_EINVAL, "grid too large");
+    // Sample indices and the work counter are 32-bit; the counter may run past the count
+    // by one chunk per wave.
+    if (count64 >= (1ull << 31)) return ctx->fail(FDG_EINVAL, "grid too large");
     // The kernels count iterates in 32 bits.
     if (p->max_iter >= (1ull << 31)) return ctx->fail(FDG_EINVAL, "max_iter must be < 2^31");
     if (route == FDG_ROUTE_BLA) {
@@ -429,7 +422,7 @@ int fdg_frame_open(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int rou
     f->len = o.len();
     f->bits = o.bits;
     f->nucleus = o.nucleus_period();
-    rc = upload_orbit(ctx, &f->buf, o, &f->transfer_seconds);
+    rc = upload_orbit(ctx, &f->buf.orbit, o, &f->transfer_seconds);
     if (!rc) rc = f->buf.outputs(ctx, f->count);
     if (rc) {
         f->buf.release();
@@ -461,11 +454,9 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
     s.bla_levels = levels;
     f->use_bla = levels > 0;
     Buffers& b = f->buf;
-    FDG_TRY(ctx, hipMemsetAsync(b.counters.p, 0, 256, ctx->stream), "memset");
-    size_t n = f->count;
-    for (DevBuf* d : {&b.o_zr, &b.o_zi, &b.o_dr, &b.o_di, &b.o_dexp})
-        FDG_TRY(ctx, hipMemsetAsync(d->p, 0, n * 8, ctx->stream), "memset");
-    FDG_TRY(ctx, hipMemsetAsync(b.o_ops.p, 0, n * 4, ctx->stream), "memset");
+    // The kernel writes every output field of every sample; only the counters start at 0.
+    FDG_TRY(ctx, launch_zero_words(b.counters.as<unsigned long long>(), 32, ctx->stream),
+            "counter clear");
     FrameDev d;
     std::memset(&d, 0, sizeof(d));
     const Plane& pl = f->fr.pl;
@@ -512,7 +503,7 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
     FDG_TRY(ctx, hipEventRecord(ctx->ev[2], ctx->stream), "event");
     FDG_TRY(ctx,
             launch_perturb_f64(b.fdev.as<FrameDev>(), f->fr.tier == TIER_SCALED, deriv,
-                               f->use_bla, grid, ctx->stream),
+                               f->use_bla, false, grid, ctx->stream),
             "perturbation launch");
     FDG_TRY(ctx, hipEventRecord(ctx->ev[3], ctx->stream), "event");
     unsigned long long cnt[ST_COUNT];
@@ -551,52 +542,9 @@ int fdg_frame_read(fdg_frame* f, fdg_columns* out, fdg_outcomes* raw, fdg_stats*
         return ctx->fail(FDG_EINVAL, "a requested column has no buffer");
     FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
     fdg_stats s = f->last;
-    size_t n = f->count;
-    Buffers& b = f->buf;
-    double t1 = now_s();
-    std::vector<uint8_t> kind(n);
-    std::vector<uint64_t> nn(n);
-    std::vector<double> zr(n), zi(n), dr(n), di(n);
-    std::vector<int64_t> dexp(n);
-    FDG_TRY(ctx, hipMemcpy(kind.data(), b.o_kind.p, n, hipMemcpyDeviceToHost), "readback");
-    FDG_TRY(ctx, hipMemcpy(nn.data(), b.o_n.p, n * 8, hipMemcpyDeviceToHost), "readback");
-    FDG_TRY(ctx, hipMemcpy(zr.data(), b.o_zr.p, n * 8, hipMemcpyDeviceToHost), "readback");
-    FDG_TRY(ctx, hipMemcpy(zi.data(), b.o_zi.p, n * 8, hipMemcpyDeviceToHost), "readback");
-    FDG_TRY(ctx, hipMemcpy(dr.data(), b.o_dr.p, n * 8, hipMemcpyDeviceToHost), "readback");
-    FDG_TRY(ctx, hipMemcpy(di.data(), b.o_di.p, n * 8, hipMemcpyDeviceToHost), "readback");
-    FDG_TRY(ctx, hipMemcpy(dexp.data(), b.o_dexp.p, n * 8, hipMemcpyDeviceToHost), "readback");
-    if (raw && raw->ops)
-        FDG_TRY(ctx, hipMemcpy(raw->ops, b.o_ops.p, n * 4, hipMemcpyDeviceToHost), "readback");
-    s.transfer_seconds += now_s() - t1;
-    double t2 = now_s();
-    Encoder enc(f->fr.pl, p->ss);
-    double* nu = (p->columns & FDG_COL_NU) ? out->nu : nullptr;
-    float* de = (p->columns & FDG_COL_DE) ? out->de : nullptr;
-    uint16_t* nm = (p->columns & FDG_COL_NORMAL) ? out->normal : nullptr;
-    unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
-    std::vector<std::thread> th;
-    for (unsigned w = 0; w < nt; w++)
-        th.emplace_back([&, w] {
-            size_t lo = n * w / nt, hi = n * (w + 1) / nt;
-            for (size_t i = lo; i < hi; i++) {
-                if (nu) nu[i] = 0.0;
-                if (de) de[i] = 0.0f;
-                if (nm) nm[i] = 0;
-                enc.put(i, kind[i], nn[i], zr[i], zi[i], dr[i], di[i], dexp[i], out->class_, nu,
-                        de, nm);
-            }
-        });
-    for (auto& t : th) t.join();
-    s.encode_seconds = now_s() - t2;
-    if (raw) {
-        if (raw->kind) std::memcpy(raw->kind, kind.data(), n);
-        if (raw->n) std::memcpy(raw->n, nn.data(), n * 8);
-        if (raw->zr) std::memcpy(raw->zr, zr.data(), n * 8);
-        if (raw->zi) std::memcpy(raw->zi, zi.data(), n * 8);
-        if (raw->dr) std::memcpy(raw->dr, dr.data(), n * 8);
-        if (raw->di) std::memcpy(raw->di, di.data(), n * 8);
-        if (raw->dexp) std::memcpy(raw->dexp, dexp.data(), n * 8);
-    }
+    int rc = read_encode(ctx, f->buf, 0, f->count, f->fr.pl, p, out, raw, &s.transfer_seconds,
+                         &s.encode_seconds, nullptr, nullptr, nullptr);
+    if (rc) return rc;
     if (st) *st = s;
     if (kernel_id && kernel_cap) {
         std::string k = f->fr.pl.kernel(f->fr.tier);

@@ -10,12 +10,12 @@ namespace fdg {
 namespace dm {
 
 // 2^k: exact where representable, 0 below the subnormals, inf above
-// (crates/fd-fixed/src/lib.rs:30-42).
+// (crates/fd-fixed/src/lib.rs:30-42).  ldexp(1, k) is exactly that: it rounds 2^k to
+// nearest, which is 2^k itself down to 2^-1074, 0 from 2^-1075 (a tie, to even) down,
+// and inf above 2^1023; one v_ldexp_f64 once k is clamped into int range.
 __device__ inline double exp2i(int64_t k) {
-    if (k > 1023) return __longlong_as_double(0x7ff0000000000000ll);
-    if (k >= -1022) return __longlong_as_double((long long)((uint64_t)(k + 1023) << 52));
-    if (k >= -1074) return __longlong_as_double((long long)(1ull << (k + 1074)));
-    return 0.0;
+    int32_t c = k > 4096 ? 4096 : (k < -4096 ? -4096 : (int32_t)k);
+    return ldexp(1.0, c);
 }
 
 // floor(log2(a)) for finite a > 0, as the reference computes it (a.log2().floor()).

@@ -1,0 +1,125 @@
+// Host-side internals shared by the C ABI's translation units (api.cpp: contexts, single
+// frames; film.cpp: batches of frames sharing an orbit).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "fdg.h"
+#include "host.hpp"
+#include "kernels.hpp"
+
+namespace fdg {
+
+hipError_t launch_bla_build(const double* orbit, double* r, double* r2, double* ab, double* alpha,
+                            double* beta, uint32_t* valid, uint64_t steps, uint64_t stride,
+                            double eps, double dc_max, int64_t dc_exp, uint32_t max_levels,
+                            hipStream_t stream);
+hipError_t launch_perturb_f64(const FrameDev* f, bool ext, bool deriv, bool bla, bool batch,
+                              int grid, hipStream_t stream);
+hipError_t launch_zero_words(unsigned long long* p, uint32_t n, hipStream_t stream);
+
+constexpr uint32_t kMaxLevels = 48;
+constexpr double kEpsLimit = 1.0 / 16777216.0;  // bla.rs:288-292 EPS_LIMIT = 2^-24
+
+extern std::string g_create_error;
+
+// A grow-only device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= bytes) return hipSuccess;
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+inline double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+}  // namespace fdg
+
+struct fdg_ctx {
+    int device = 0;
+    int cus = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::string err;
+
+    int fail(int code, const std::string& m) {
+        err = m;
+        return code;
+    }
+    int hip(hipError_t e, const char* what) {
+        if (e == hipSuccess) return FDG_OK;
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? FDG_ENOMEM : FDG_EDEVICE;
+    }
+};
+
+#define FDG_TRY(ctx, call, what)                       \
+    do {                                               \
+        int rc_ = (ctx)->hip((call), (what));          \
+        if (rc_ != FDG_OK) return rc_;                 \
+    } while (0)
+
+namespace fdg {
+
+// Device residency of one frame: the orbit, its BLA table and the per-sample outcomes.
+struct Buffers {
+    DevBuf orbit, bla_r, bla_r2, bla_ab, bla_alpha, bla_beta, bla_valid;
+    DevBuf o_kind, o_n, o_zr, o_zi, o_dr, o_di, o_dexp, o_ops, counters, fdev;
+    uint64_t bla_stride = 0, bla_steps = 0;
+    void release() {
+        for (DevBuf* b : {&orbit, &bla_r, &bla_r2, &bla_ab, &bla_alpha, &bla_beta, &bla_valid,
+                          &o_kind, &o_n, &o_zr, &o_zi, &o_dr, &o_di, &o_dexp, &o_ops,
+                          &counters, &fdev})
+            b->release();
+    }
+    // Per-sample outcome arrays for n samples, the counters and one FrameDev.
+    int outputs(fdg_ctx* ctx, size_t n);
+};
+
+struct Frame {
+    Plane pl;
+    Tier tier;
+};
+
+// grid.rs:779-799 setup: plane, tier (and its validity at the depth).
+int setup(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, Frame* fr);
+// grid.rs:802-815 compute: the frame's own reference orbit.
+int compute_orbit(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, const Frame& fr,
+                  Orbit* o);
+uint64_t next_pow2_host(uint64_t x);
+// Upload an orbit interleaved (re, im) with a 64-byte pad into *dst.
+int upload_orbit(fdg_ctx* ctx, DevBuf* dst, const Orbit& o, double* seconds);
+// orbit.rs:228-247 `same`: two decimal strings name the same number.
+bool same_decimal(const std::string& a, const std::string& b);
+// Copy n outcomes from offset off of B's arrays, encode the requested columns (Store::put)
+// and fill raw (Unresolved samples report n = 0).  ops_out (nullable) receives the
+// per-sample operation counts and n_out (nullable) the stop iterates, before zeroing.
+int read_encode(fdg_ctx* ctx, const Buffers& B, size_t off, size_t n, const Plane& pl,
+                const fdg_params* p, fdg_columns* out, fdg_outcomes* raw, double* transfer_s,
+                double* encode_s, std::vector<uint64_t>* n_out, std::vector<uint8_t>* kind_out,
+                std::vector<uint32_t>* ops_out);
+
+}  // namespace fdg

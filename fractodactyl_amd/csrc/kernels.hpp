@@ -36,7 +36,19 @@ enum StatSlot : int {
     ST_COUNT = 16
 };
 
-// One frame for the perturbation kernels.
+// Geometry of one frame of a batch whose frames share the reference orbit (the keyframes
+// of a zoom film at one centre; grid.rs:95-101 "one deep orbit serves every shallower
+// frame at its centre").  Frames of a batch have the same grid, so sample g of the batch
+// is sample g % per_frame of the batch's frame g / per_frame.
+struct BatchFrameDev {
+    double h, h_m;
+    int64_t h_e;
+    double cos_r, sin_r, half_x, half_y;
+    uint32_t out0;          // output index of the frame's first sample
+    uint32_t tb;            // first block of the frame's BLA table (0: the shared table)
+};
+
+// One frame (or one batch of frames) for the perturbation kernels.
 struct FrameDev {
     const double* orbit;    // 2 doubles per point: Z_m re, im
     uint64_t len;           // reference points
@@ -53,7 +65,7 @@ struct FrameDev {
     const uint8_t* mask;    // nullable: compute only mask[i] != 0
     // outputs (structure of arrays, count each)
     uint8_t* kind;
-    uint64_t* n;
+    uint64_t* n;                    // iterate reached (also for Unresolved: the stop point)
     double* zr;
     double* zi;
     double* dr;
@@ -62,6 +74,9 @@ struct FrameDev {
     uint32_t* ops;                  // executed plain steps + applied blocks per sample
     uint32_t* next;                 // work-stealing counter (zeroed by the host)
     unsigned long long* stats;      // ST_COUNT counters (zeroed by the host)
+    // batches only (the single-frame kernels read the geometry above)
+    const BatchFrameDev* frames;    // nframes entries
+    uint32_t nframes, per_frame;    // count = nframes * per_frame
 };
 
 }  // namespace fdg

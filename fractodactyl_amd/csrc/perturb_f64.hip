@@ -294,10 +294,12 @@ struct Lane {
     double xr, xi, dr, di, ar, ai, sr, si;
     double Zr, Zi, Z1r, Z1i;            // Z_m and Z_{m+1}
     double R0;                          // level-0 r2 of the block starting at m (when any)
-    double scale, dc_r, dc_i, inverse;  // scaled tier only
+    double dc_r, dc_i;                  // dc in the delta's units (scaled tier: ar 2^(oe - e))
     int32_t delta_exp, derivative_exp;  // scaled tier only
     uint32_t m, n, sn, hold, chk, nucleus, blocks, skipped, plain;
     uint32_t flags;                     // kPending | kPredicted | kConStarted | returns << 4
+    uint32_t tb;                        // batches: first block of this frame's BLA table
+    int32_t oe;                         // batches: this frame's offset exponent h_e
 };
 
 constexpr uint32_t kPending = 1, kPredicted = 2, kConStarted = 4;
@@ -305,6 +307,18 @@ constexpr uint32_t kPending = 1, kPredicted = 2, kConStarted = 4;
 __device__ inline uint32_t np2_above(uint32_t n) {
     // next_power_of_two(n + 1) for 1 <= n < 2^31
     return 1u << (32 - __builtin_clz(n));
+}
+
+// 2^delta_exp and 2^-derivative_exp, the scaled tier's unit of the delta and inverse unit
+// of the derivative (1 on the ordinary tiers, whose exponents stay 0): recomputed where
+// used (one v_ldexp each) rather than carried in registers.
+template <bool EXT>
+__device__ inline double scale_of(const Lane& s) {
+    return EXT ? exp2i(s.delta_exp) : 1.0;
+}
+template <bool EXT>
+__device__ inline double inverse_of(const Lane& s) {
+    return EXT ? exp2i(-(int64_t)s.derivative_exp) : 1.0;
 }
 
 // Frame constants the loop reads, copied once into registers / SGPRs.
@@ -333,13 +347,11 @@ __device__ inline void lane_start(Lane& s, Rare& r, const Hot& h, uint32_t nucle
     s.Z1i = h.Z1i;
     s.R0 = 0.0;  // m = 0: no block starts here
     s.delta_exp = EXT ? oe : 0;
-    s.scale = EXT ? exp2i(s.delta_exp) : 1.0;
     s.dc_r = ar;
     s.dc_i = ai;
     s.dr = 0.0;
     s.di = 0.0;
     s.derivative_exp = 0;
-    s.inverse = 1.0;
     s.m = 0;
     s.n = 0;
     s.sr = 0.0;
@@ -365,7 +377,6 @@ __device__ inline void lane_start(Lane& s, Rare& r, const Hot& h, uint32_t nucle
 template <bool EXT>
 __device__ inline void refresh_dc(Lane& s, int32_t oe) {
     if (EXT) {
-        s.scale = exp2i(s.delta_exp);
         double factor = exp2i((int64_t)oe - s.delta_exp);
         s.dc_r = s.ar * factor;
         s.dc_i = s.ai * factor;
@@ -411,8 +422,8 @@ __device__ bool near_return(Lane& s, Rare& r, double fr, double fi, double size)
 // The scaled tier's rebase (sample.rs:422-449): exact two-term sums below 1e-290 and the
 // exponent-safe step from Z_0 when the delta is below 1e-270.
 template <bool D>
-__device__ void ext_rebase(Lane& s, const Hot& h, double fr, double fi, double f2,
-                           double Zmr, double Zmi) {
+__device__ void ext_rebase(Lane& s, const Hot& h, int32_t oe, double R00, double fr, double fi,
+                           double f2, double Zmr, double Zmi) {
     if (f2 > kSquareOk) {
         s.xr = fr;
         s.xi = fi;
@@ -420,50 +431,50 @@ __device__ void ext_rebase(Lane& s, const Hot& h, double fr, double fi, double f
     } else {
         sum2(Zmr, Zmi, 0, s.xr, s.xi, s.delta_exp, s.delta_exp, s.xr, s.xi, s.delta_exp);
     }
-    s.scale = exp2i(s.delta_exp);
     s.m = 0;
     s.Zr = 0.0;
     s.Zi = 0.0;
     s.Z1r = h.Z1r;
     s.Z1i = h.Z1i;
-    if (s.scale < kTiny) {
+    if (scale_of<true>(s) < kTiny) {
         s.nucleus = 0;
         if (D) {
             double qr = 2.0 * (s.xr * s.dr - s.xi * s.di);
             double qi = 2.0 * (s.xr * s.di + s.xi * s.dr);
             sum2(qr, qi, (int64_t)s.delta_exp + s.derivative_exp, 1.0, 0.0, 0, s.derivative_exp,
                  s.dr, s.di, s.derivative_exp);
-            s.inverse = exp2i(-(int64_t)s.derivative_exp);
         }
         double wr = s.xr, wi = s.xi;
-        sum2(wr * wr - wi * wi, 2.0 * wr * wi, 2 * (int64_t)s.delta_exp, s.ar, s.ai, h.oe,
+        sum2(wr * wr - wi * wi, 2.0 * wr * wi, 2 * (int64_t)s.delta_exp, s.ar, s.ai, oe,
              s.delta_exp, s.xr, s.xi, s.delta_exp);
-        s.scale = exp2i(s.delta_exp);
         s.m = 1;
         s.n = s.n + 1;
         s.Zr = h.Z1r;
         s.Zi = h.Z1i;
         s.Z1r = h.Z2r;
         s.Z1i = h.Z2i;
-        s.R0 = h.R00;
+        s.R0 = R00;
     }
-    refresh_dc<true>(s, h.oe);
+    refresh_dc<true>(s, oe);
 }
 
 // One block or one plain step plus the checks after it (sample.rs:237-575).  Returns
 // RUNNING, or the finished outcome kind (0 escaped with z in ofr/ofi, 1 interior,
 // 2 unresolved).
-template <bool EXT, bool D, bool L>
+template <bool EXT, bool D, bool L, bool B>
 __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, double& ofi) {
     // sample.rs:237 `while n < limit`: the budget ran out.
     if (s.n >= h.max_iter) return 2;
+    // A batch's frames each have their own table (at block tb) and offset exponent.
+    const uint32_t tb = B ? s.tb : 0u;
+    const int32_t oe = B ? s.oe : h.oe;
     // Gathers for the NEXT iteration, assuming this one is a plain step that does not
     // rebase (the common case): Z_{m+2} and the level-0 r2 of a block starting at m+1.
     // Their L2 latency then overlaps this whole iteration.  (s.m < last here, and the
     // orbit buffer is padded past its end.)
     double2 pz2 = h.orbit[s.m + 2];
     double pr0 = 0.0;
-    if (L && (s.m & 1u) == 0 && (s.m >> 1) < h.count0) pr0 = h.r2[s.m >> 1];
+    if (L && (s.m & 1u) == 0 && (s.m >> 1) < h.count0) pr0 = h.r2[tb + (s.m >> 1)];
     double2 Zn = make_double2(s.Z1r, s.Z1i);  // a plain step's landing point
     double2 Znext = pz2;
     double r0next = pr0;
@@ -475,9 +486,12 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         if ((k & 1u) == 0 && (s.nucleus == 0 || s.m != 0)) {
             int top = k == 0 ? (int)h.nlevels : __builtin_ctz(k);
             top = top < (int)h.nlevels ? top : (int)h.nlevels;
-            double d2 = EXT ? (s.xr * s.xr + s.xi * s.xi) * s.scale * s.scale
+            double d2 = EXT ? (s.xr * s.xr + s.xi * s.xi) * scale_of<EXT>(s) * scale_of<EXT>(s)
                             : s.xr * s.xr + s.xi * s.xi;
-            uint32_t b = s.hold > 0 ? s.sn + s.hold : ((s.flags & kPending) ? 0xffffffffu : s.chk);
+            // holds and deferred saves exist on the ordinary tiers only
+            uint32_t b = EXT ? s.chk
+                             : (s.hold > 0 ? s.sn + s.hold
+                                           : ((s.flags & kPending) ? 0xffffffffu : s.chk));
             b = b < h.max_iter ? b : h.max_iter;
             uint32_t nb = s.nucleus ? (s.n == 0 ? 1u : s.nucleus + 1) : 0xffffffffu;
             b = b < nb ? b : nb;
@@ -488,7 +502,7 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
                 for (int lv = 1; lv < top; lv++) {
                     uint32_t idx = k >> (lv + 1);
                     if (!((2ull << lv) <= (uint64_t)b && idx < (h.steps >> (lv + 1)) &&
-                          d2 < h.r2[bla_level_offset(h.stride, lv) + idx]))
+                          d2 < h.r2[tb + bla_level_offset(h.stride, lv) + idx]))
                         break;
                     lvl = lv;
                 }
@@ -505,7 +519,7 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         double a_r = 0.0, a_i = 0.0, b_r = 1.0, b_i = 0.0;
         uint32_t blen = 1;
         if (blk) {
-            uint64_t at = bla_level_offset(h.stride, lvl) + ((s.m - 1) >> (lvl + 1));
+            uint64_t at = tb + bla_level_offset(h.stride, lvl) + ((s.m - 1) >> (lvl + 1));
             double4 ab = h.ab[at];
             a_r = ab.x;
             a_i = ab.y;
@@ -515,7 +529,7 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
             Zn = h.orbit[s.m + blen];
             Znext = h.orbit[s.m + blen + 1];
             uint32_t i0 = (s.m + blen - 1) >> 1;
-            r0next = i0 < h.count0 ? h.r2[i0] : 0.0;
+            r0next = i0 < h.count0 ? h.r2[tb + i0] : 0.0;
         }
         double xr = s.xr, xi = s.xi;
         double fr = Zmr + xr, fi = Zmi + xi;           // z = Z_m + delta (plain step)
@@ -530,8 +544,8 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         if (D) {
             double p2r = blk ? a_r : fr, p2i = blk ? a_i : fi;
             double dr = s.dr, di = s.di;
-            s.dr = c * (p2r * dr - p2i * di) + b_r * s.inverse;
-            s.di = c * (p2r * di + p2i * dr) + (blk ? b_i : -0.0) * s.inverse;
+            s.dr = c * (p2r * dr - p2i * di) + b_r * inverse_of<EXT>(s);
+            s.di = c * (p2r * di + p2i * dr) + (blk ? b_i : -0.0) * inverse_of<EXT>(s);
         }
         double pr_ = blk ? a_r : Zmr, pi_ = blk ? a_i : Zmi;
         double ur = blk ? b_r : xr, vr = blk ? s.dc_r : xr;
@@ -547,14 +561,14 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         s.skipped += blk ? blen : 0u;
         s.plain += blk ? 0u : 1u;
     } else if (lvl >= 0) {
-        uint64_t at = bla_level_offset(h.stride, lvl) + ((s.m - 1) >> (lvl + 1));
+        uint64_t at = tb + bla_level_offset(h.stride, lvl) + ((s.m - 1) >> (lvl + 1));
         double4 ab = h.ab[at];
         double a_r = ab.x, a_i = ab.y, b_r = ab.z, b_i = ab.w;
         uint32_t blen = 2u << lvl;
         Zn = h.orbit[s.m + blen];
         Znext = h.orbit[s.m + blen + 1];
         uint32_t i0 = (s.m + blen - 1) >> 1;
-        r0next = L && i0 < h.count0 ? h.r2[i0] : 0.0;
+        r0next = L && i0 < h.count0 ? h.r2[tb + i0] : 0.0;
         if (s.nucleus && s.n >= 1) {
             bool st = s.flags & kConStarted;
             con_multiply(r.conr, r.coni, r.cone, st, a_r, a_i);
@@ -563,12 +577,12 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         normalise(s.xr, s.xi, s.delta_exp, 0);
         if (D) {
             normalise(s.dr, s.di, s.derivative_exp, INT64_MAX / 4);
-            s.inverse = exp2i(-(int64_t)s.derivative_exp);
         }
-        refresh_dc<EXT>(s, h.oe);
+        refresh_dc<EXT>(s, oe);
         if (D) {
-            double t = a_r * s.dr - a_i * s.di + b_r * s.inverse;
-            s.di = a_r * s.di + a_i * s.dr + b_i * s.inverse;
+            double inv = inverse_of<EXT>(s);
+            double t = a_r * s.dr - a_i * s.di + b_r * inv;
+            s.di = a_r * s.di + a_i * s.dr + b_i * inv;
             s.dr = t;
         }
         double t = a_r * s.xr - a_i * s.xi + b_r * s.dc_r - b_i * s.dc_i;
@@ -577,39 +591,38 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         normalise(s.xr, s.xi, s.delta_exp, 0);
         if (D) {
             normalise(s.dr, s.di, s.derivative_exp, INT64_MAX / 4);
-            s.inverse = exp2i(-(int64_t)s.derivative_exp);
         }
-        refresh_dc<EXT>(s, h.oe);
+        refresh_dc<EXT>(s, oe);
         s.m += blen;
         s.n += blen;
         s.blocks += 1;
         s.skipped += blen;
     } else {
-        double fr = Zmr + s.scale * s.xr, fi = Zmi + s.scale * s.xi;
+        const double scale = scale_of<EXT>(s);
+        double fr = Zmr + scale * s.xr, fi = Zmi + scale * s.xi;
         if (s.nucleus && s.n >= 1) {
             bool st = s.flags & kConStarted;
             con_multiply(r.conr, r.coni, r.cone, st, 2.0 * fr, 2.0 * fi);
             s.flags |= kConStarted;
         }
         if (D) {
-            double t = 2.0 * (fr * s.dr - fi * s.di) + s.inverse;
+            double t = 2.0 * (fr * s.dr - fi * s.di) + inverse_of<EXT>(s);
             s.di = 2.0 * (fr * s.di + fi * s.dr);
             s.dr = t;
             if (fmax(fabs(s.dr), fabs(s.di)) > kHi) {
                 s.dr = s.dr * kLo;
                 s.di = s.di * kLo;
                 s.derivative_exp += 32;
-                s.inverse = exp2i(-(int64_t)s.derivative_exp);
             }
         }
         double xr = s.xr, xi = s.xi;
-        s.xr = 2.0 * (Zmr * xr - Zmi * xi) + s.scale * (xr * xr - xi * xi) + s.dc_r;
-        s.xi = 2.0 * (Zmr * xi + Zmi * xr) + s.scale * (2.0 * xr * xi) + s.dc_i;
+        s.xr = 2.0 * (Zmr * xr - Zmi * xi) + scale * (xr * xr - xi * xi) + s.dc_r;
+        s.xi = 2.0 * (Zmr * xi + Zmi * xr) + scale * (2.0 * xr * xi) + s.dc_i;
         s.m += 1;
         s.n += 1;
         s.plain += 1;
     }
-    double pr = s.scale * s.xr, pi = s.scale * s.xi;
+    double pr = scale_of<EXT>(s) * s.xr, pi = scale_of<EXT>(s) * s.xi;
     double fr = Zn.x + pr, fi = Zn.y + pi;
     double f2 = fr * fr + fi * fi;
     if (f2 > h.esc2) {
@@ -630,7 +643,8 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         s.Z1i = Znext.y;
         s.R0 = r0next;
         if (rebase) {
-            ext_rebase<D>(s, h, fr, fi, f2, Zn.x, Zn.y);
+            double R00 = B ? (L && h.count0 > 0 ? h.r2[tb] : 0.0) : h.R00;
+            ext_rebase<D>(s, h, oe, R00, fr, fi, f2, Zn.x, Zn.y);
         } else {
             double magnitude = fmax(fabs(s.xr), fabs(s.xi));
             int32_t k = 0;
@@ -643,7 +657,7 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
                 s.xr = s.xr * factor;
                 s.xi = s.xi * factor;
                 s.delta_exp = s.delta_exp + k;
-                refresh_dc<EXT>(s, h.oe);
+                refresh_dc<EXT>(s, oe);
             }
         }
         // extended tier: judged on the physical delta before rebase/rescale
@@ -654,7 +668,7 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
             double size = fabs(s.sr) + fabs(s.si);
             if (dist < 1e-13 * size + 1e-300) {
                 if (contracting(reinterpret_cast<const double*>(h.orbit), h.len, s.m, pr, pi,
-                                s.ar * exp2i(h.oe), s.ai * exp2i(h.oe), s.n - s.sn))
+                                s.ar * exp2i(oe), s.ai * exp2i(oe), s.n - s.sn))
                     return 2;  // Unresolved
                 s.flags += 16;
             }
@@ -711,17 +725,20 @@ __device__ inline uint32_t lanes_below(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-__device__ inline unsigned long long wave_sum(unsigned long long v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    return v;
-}
-
 // Persistent work-stealing kernel: every lane owns one sample at a time; lanes whose
 // sample finished are refilled together from one atomic per wave, so the wave keeps
-// 64 samples in flight however uneven their iteration counts are.
-template <bool EXT, bool D, bool L>
+// 64 samples in flight however uneven their iteration counts are.  With B, the samples
+// are those of a batch of frames sharing the orbit (frame-major), each frame with its own
+// geometry and BLA table, so one launch drains every frame and no frame's tail idles
+// the GPU.
+template <bool EXT, bool D, bool L, bool B>
 __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restrict__ fp) {
     __shared__ RareLds rare;
+    // Statistics accumulate in LDS at each sample's end (rare next to the steps), not in
+    // per-lane registers held across the loop.
+    __shared__ unsigned long long stat[ST_COUNT];
+    if (threadIdx.x < ST_COUNT) stat[threadIdx.x] = 0;
+    __syncthreads();
     const uint32_t lane = __lane_id();
     Rare r(rare, threadIdx.x);
     Hot h;
@@ -751,20 +768,67 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
     bool active = false;
     bool drained = false;
     uint32_t idx = 0;
-    unsigned long long its = 0, eq = 0, blocks = 0, skipped = 0, fallback = 0, closed = 0,
-                       plain = 0, maxops = 0, witers = 0, liters = 0;
+    unsigned long long witers = 0, liters = 0;
+    // Each wave claims samples in chunks, one global atomic per chunk rather than per
+    // refill; the chunk shrinks with the work left (guided self-scheduling) so the last
+    // chunks are small and the tail short.  [cb, ce) is the wave's unclaimed range.
+    uint32_t cb = 0, ce = 0, seen = 0;
+    const uint32_t waves = gridDim.x * (kBlock / 64);
     for (;;) {
         uint64_t idle = __ballot(!active);
         if (idle && !drained) {
             uint32_t want = __popcll(idle);
-            uint32_t leader = __ffsll((unsigned long long)idle) - 1;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(next, want);
-            base = __shfl(base, leader);
-            if (base + want >= count) drained = true;
-            if (!active) {
-                uint32_t my = base + lanes_below(idle);
-                if (my < count && (!fp->mask || fp->mask[my])) {
+            uint32_t avail = ce - cb;
+            uint32_t nb = 0, ne = 0;
+            if (avail < want) {
+                uint32_t left = count > seen ? count - seen : 0;
+                uint32_t size = max(min(left / (4 * waves), 1024u), want - avail);
+                uint32_t leader = __ffsll((unsigned long long)idle) - 1;
+                if (lane == leader) nb = atomicAdd(next, size);
+                // readlane keeps the range wave-uniform (scalar registers)
+                nb = __builtin_amdgcn_readlane(nb, leader);
+                seen = nb + size;
+                ne = nb < count ? min(nb + size, count) : nb;
+            }
+            uint32_t rank = lanes_below(idle);
+            uint32_t my = rank < avail ? cb + rank : nb + (rank - avail);
+            bool have = rank < avail || my < ne;
+            if (avail >= want) {
+                cb += want;
+            } else {
+                cb = min(nb + (want - avail), ne);
+                ce = ne;
+                if (nb >= count) drained = true;  // the global range is exhausted
+            }
+            if (!active && have) {
+                if (B) {
+                    uint32_t f = my / fp->per_frame, loc = my - f * fp->per_frame;
+                    const BatchFrameDev& g = fp->frames[f];
+                    idx = g.out0 + loc;
+                    uint32_t i = loc % nx, j = loc / nx;
+                    double x = (double)i + 0.5 - g.half_x;
+                    double y = (double)j + 0.5 - g.half_y;
+                    double ux = g.cos_r * x + g.sin_r * y;
+                    double uy = g.sin_r * x - g.cos_r * y;
+                    fp->ops[idx] = 0;
+                    double ar = EXT ? ux * g.h_m : ux * g.h, ai = EXT ? uy * g.h_m : uy * g.h;
+                    if (!EXT && fp->tier == 0 && in_main_components(fp->c_re + ar, fp->c_im + ai)) {
+                        // f64 tier: closed-form interior (grid.rs Job::one, interior.rs:6-11)
+                        fp->kind[idx] = 1;
+                        fp->n[idx] = 0;
+                        fp->zr[idx] = 0.0;
+                        fp->zi[idx] = 0.0;
+                        fp->dr[idx] = 0.0;
+                        fp->di[idx] = 0.0;
+                        fp->dexp[idx] = 0;
+                        if (L) atomicAdd(&stat[ST_CLOSED], 1ull);
+                    } else {
+                        lane_start<EXT>(s, r, h, nucleus, ar, ai, EXT ? (int32_t)g.h_e : 0);
+                        s.tb = g.tb;
+                        s.oe = (int32_t)g.h_e;
+                        active = true;
+                    }
+                } else if (!B && (!fp->mask || fp->mask[my])) {
                     idx = my;
                     uint32_t i = idx % nx, j = idx / nx;
                     double x = (double)i + 0.5 - fp->half_x;
@@ -777,7 +841,12 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
                         if (in_main_components(fp->c_re + ar, fp->c_im + ai)) {
                             fp->kind[idx] = 1;
                             fp->n[idx] = 0;
-                            closed += L ? 1 : 0;
+                            fp->zr[idx] = 0.0;
+                            fp->zi[idx] = 0.0;
+                            fp->dr[idx] = 0.0;
+                            fp->di[idx] = 0.0;
+                            fp->dexp[idx] = 0;
+                            if (L) atomicAdd(&stat[ST_CLOSED], 1ull);
                         } else {
                             lane_start<EXT>(s, r, h, nucleus, ar, ai, 0);
                             active = true;
@@ -802,41 +871,45 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
             liters += __popcll(__ballot(active));
             if (active) {
                 double fr = 0.0, fi = 0.0;
-                int k = lane_advance<EXT, D, L>(s, r, h, fr, fi);
+                int k = lane_advance<EXT, D, L, B>(s, r, h, fr, fi);
                 if (k != RUNNING) {
+                    // Every field is written (zeros where the outcome has none), so the
+                    // output buffers need no clearing between passes.
+                    bool esc = k == 0;
                     fp->kind[idx] = (uint8_t)k;
-                    fp->n[idx] = k == 2 ? 0 : s.n;
+                    fp->n[idx] = s.n;
                     fp->ops[idx] = s.plain + s.blocks;
-                    if (k == 0) {
-                        fp->zr[idx] = fr;
-                        fp->zi[idx] = fi;
-                        fp->dr[idx] = s.dr;
-                        fp->di[idx] = s.di;
-                        fp->dexp[idx] = s.derivative_exp;
+                    fp->zr[idx] = esc ? fr : 0.0;
+                    fp->zi[idx] = esc ? fi : 0.0;
+                    fp->dr[idx] = esc ? s.dr : 0.0;
+                    fp->di[idx] = esc ? s.di : 0.0;
+                    fp->dexp[idx] = esc ? s.derivative_exp : 0;
+                    unsigned long long spent = k == 2 ? h.max_iter : s.n;
+                    atomicAdd(&stat[ST_EQUIVALENT], spent);
+                    atomicAdd(&stat[ST_ITERATIONS], L ? spent - s.skipped + s.blocks : spent);
+                    if (L) {
+                        atomicAdd(&stat[ST_BLOCKS], (unsigned long long)s.blocks);
+                        atomicAdd(&stat[ST_SKIPPED], (unsigned long long)s.skipped);
+                        if (s.blocks == 0) atomicAdd(&stat[ST_FALLBACK], 1ull);
                     }
-                    uint64_t spent = k == 2 ? h.max_iter : s.n;
-                    eq += spent;
-                    its += L ? spent - s.skipped + s.blocks : spent;
-                    blocks += s.blocks;
-                    skipped += s.skipped;
-                    plain += s.plain;
-                    maxops = max(maxops, (unsigned long long)(s.plain + s.blocks));
-                    fallback += (L && s.blocks == 0) ? 1 : 0;
+                    atomicAdd(&stat[ST_PLAIN], (unsigned long long)s.plain);
+                    atomicMax(&stat[ST_MAXOPS], (unsigned long long)(s.plain + s.blocks));
                     active = false;
                 }
             }
         } while (!__ballot(!active) || (drained && __ballot(active)));
     }
-    unsigned long long v[7] = {its, eq, blocks, skipped, fallback, closed, plain};
-    for (int q = 0; q < 7; q++) {
-        unsigned long long t = wave_sum(v[q]);
-        if (lane == 0 && t) atomicAdd(&fp->stats[q], t);
-    }
-    for (int off = 32; off > 0; off >>= 1) maxops = max(maxops, __shfl_xor(maxops, off));
-    if (lane == 0 && maxops) atomicMax(&fp->stats[ST_MAXOPS], maxops);
     if (lane == 0) {
-        atomicAdd(&fp->stats[ST_WAVE_ITERS], witers);
-        atomicAdd(&fp->stats[ST_LANE_ITERS], liters);
+        atomicAdd(&stat[ST_WAVE_ITERS], witers);
+        atomicAdd(&stat[ST_LANE_ITERS], liters);
+    }
+    __syncthreads();
+    if (threadIdx.x < ST_COUNT) {
+        unsigned long long v = stat[threadIdx.x];
+        if (v && threadIdx.x == ST_MAXOPS)
+            atomicMax(&fp->stats[ST_MAXOPS], v);
+        else if (v)
+            atomicAdd(&fp->stats[threadIdx.x], v);
     }
 }
 
@@ -861,20 +934,38 @@ hipError_t launch_bla_build(const double* orbit, double* r, double* r2, double* 
     return hipGetLastError();
 }
 
-hipError_t launch_perturb_f64(const FrameDev* f, bool ext, bool deriv, bool bla, int grid,
-                              hipStream_t stream) {
-#define FDG_LAUNCH(E, D, L) \
-    hipLaunchKernelGGL((perturb_f64<E, D, L>), dim3(grid), dim3(kBlock), 0, stream, f)
-    int key = (ext ? 4 : 0) | (deriv ? 2 : 0) | (bla ? 1 : 0);
+__global__ void zero_words(unsigned long long* p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+hipError_t launch_zero_words(unsigned long long* p, uint32_t n, hipStream_t stream) {
+    hipLaunchKernelGGL(zero_words, dim3((n + 255) / 256), dim3(256), 0, stream, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_perturb_f64(const FrameDev* f, bool ext, bool deriv, bool bla, bool batch,
+                              int grid, hipStream_t stream) {
+#define FDG_LAUNCH(E, D, L, B) \
+    hipLaunchKernelGGL((perturb_f64<E, D, L, B>), dim3(grid), dim3(kBlock), 0, stream, f)
+    int key = (batch ? 8 : 0) | (ext ? 4 : 0) | (deriv ? 2 : 0) | (bla ? 1 : 0);
     switch (key) {
-        case 0: FDG_LAUNCH(false, false, false); break;
-        case 1: FDG_LAUNCH(false, false, true); break;
-        case 2: FDG_LAUNCH(false, true, false); break;
-        case 3: FDG_LAUNCH(false, true, true); break;
-        case 4: FDG_LAUNCH(true, false, false); break;
-        case 5: FDG_LAUNCH(true, false, true); break;
-        case 6: FDG_LAUNCH(true, true, false); break;
-        default: FDG_LAUNCH(true, true, true); break;
+        case 0: FDG_LAUNCH(false, false, false, false); break;
+        case 1: FDG_LAUNCH(false, false, true, false); break;
+        case 2: FDG_LAUNCH(false, true, false, false); break;
+        case 3: FDG_LAUNCH(false, true, true, false); break;
+        case 4: FDG_LAUNCH(true, false, false, false); break;
+        case 5: FDG_LAUNCH(true, false, true, false); break;
+        case 6: FDG_LAUNCH(true, true, false, false); break;
+        case 7: FDG_LAUNCH(true, true, true, false); break;
+        case 8: FDG_LAUNCH(false, false, false, true); break;
+        case 9: FDG_LAUNCH(false, false, true, true); break;
+        case 10: FDG_LAUNCH(false, true, false, true); break;
+        case 11: FDG_LAUNCH(false, true, true, true); break;
+        case 12: FDG_LAUNCH(true, false, false, true); break;
+        case 13: FDG_LAUNCH(true, false, true, true); break;
+        case 14: FDG_LAUNCH(true, true, false, true); break;
+        default: FDG_LAUNCH(true, true, true, true); break;
     }
 #undef FDG_LAUNCH
     return hipGetLastError();

@@ -184,7 +184,9 @@ def scene(k, generate=False):
 
     Frame 3 (1e-1000) uses the reference's own ladder nucleus of period 16116 (its minibrot
     is ~1e-1000 wide), stored in scenes.json: the bisection search above only reaches
-    near-parabolic boundary points, whose minibrots are far larger than 1e-1000."""
+    near-parabolic boundary points, whose minibrots are far larger than 1e-1000.  Scene 4
+    (the 64-keyframe film, widths 1e-10 .. 1e-300) zooms toward the same nucleus; see
+    film_widths."""
     cache = json.load(open(CACHE)) if os.path.exists(CACHE) else {}
     key = str(k)
     if key not in cache:
@@ -197,6 +199,20 @@ def scene(k, generate=False):
         cache[key] = dict(config=cfg, nucleus=nuc, center_re=re_, center_im=im_, theta=th)
         json.dump(cache, open(CACHE, "w"), indent=1)
     return cache[key]
+
+
+def film_widths(sc):
+    """Keyframe widths w_k = w_first (w_last / w_first)^(k / (frames - 1)), as decimals
+    (mantissa and exponent apart, so widths below 1e-308 print too)."""
+    cfg = sc["config"]
+    a, b = math.log10(float(cfg["width_first"])), math.log10(float(cfg["width_last"]))
+    n = cfg["frames"]
+    out = []
+    for k in range(n):
+        x = a + (b - a) * k / (n - 1)
+        e = math.floor(x)
+        out.append(f"{10.0 ** (x - e):.6f}e{e}")
+    return out
 
 
 if __name__ == "__main__":

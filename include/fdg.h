@@ -143,6 +143,37 @@ int fdg_frame_read(fdg_frame *frame, fdg_columns *out, fdg_outcomes *raw, fdg_st
                    char *kernel_id, size_t kernel_cap);
 void fdg_frame_close(fdg_frame *frame);
 
+/* Films: a batch of frames at one centre (the keyframes of a zoom) rendered against ONE
+ * reference orbit, as the reference renders a frame with a supplied deeper orbit:
+ * fd_kernel::render_with / render_bla with `Some((orbit, bits))` (crates/fd-kernel/src/
+ * grid.rs:95-109,190-210; "one deep orbit serves every shallower frame at its centre",
+ * and the header records the precision used).  Each frame equals that render bit for bit.
+ *
+ * fdg_film_open: `views` are nframes views whose centres name the same decimal number
+ * (orbit.rs:228-247 `same`); `params` is shared (grid, max_iter, escape radius, columns).
+ * The orbit is computed once at the deepest frame's precision and stays in HBM.  Route BLA
+ * builds the tables on the GPU at open: FDG_TABLES_PER_FRAME one table per frame at that
+ * frame's dc bound (as `fd control --bla per-frame`), FDG_TABLES_SHARED one table at the
+ * largest dc bound over the frames (grid.rs:378-413 bla_checks accepts it for every frame).
+ * An f64-tier frame is refused (its orbit must be the 53-bit one at the rounded centre,
+ * grid.rs:461-465).
+ * fdg_film_run: FDG_LAUNCH_BATCH renders every frame in one persistent work-stealing launch
+ * per delta arithmetic (the Binary64 frames and the Binary64Exponent frames, on two
+ * streams), captured once into a HIP graph and replayed; FDG_LAUNCH_PER_FRAME launches the
+ * single-frame kernel once per frame, in order.  grid = persistent workgroups per launch,
+ * 0 for the default.  stats aggregates the film.
+ * fdg_film_read: frame k's columns / raw outcomes / stats, as fdg_frame_read. */
+typedef struct fdg_film fdg_film;
+enum { FDG_TABLES_PER_FRAME = 0, FDG_TABLES_SHARED = 1 };
+enum { FDG_LAUNCH_BATCH = 0, FDG_LAUNCH_PER_FRAME = 1 };
+int fdg_film_open(fdg_ctx *ctx, const fdg_view *views, uint32_t nframes,
+                  const fdg_params *params, int route, int form, double bla_eps, int tables,
+                  fdg_film **out);
+int fdg_film_run(fdg_film *film, int launch, int grid, fdg_stats *stats);
+int fdg_film_read(fdg_film *film, uint32_t frame, fdg_columns *out, fdg_outcomes *raw,
+                  fdg_stats *stats, char *kernel_id, size_t kernel_cap);
+void fdg_film_close(fdg_film *film);
+
 /* fd_kernel::reference (grid.rs:84-87): the orbit the renderer uses and its precision in
  * bits.  With cap too small, only *len and *bits are written.  Host-only: ctx may be NULL. */
 int fdg_reference(fdg_ctx *ctx, const fdg_view *view, const fdg_params *params, double *re,

@@ -83,6 +83,14 @@ def lib():
                                  C.c_int, C.c_int, C.c_double, C.c_int, C.c_void_p,
                                  C.POINTER(_Outcomes), C.POINTER(_Columns), C.POINTER(_Stats),
                                  C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t]
+        L.fdo_render_with.restype = C.c_int
+        L.fdo_render_with.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_double,
+                                      C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_double,
+                                      C.c_uint32, C.c_int, C.c_int, C.c_double, C.c_void_p,
+                                      C.c_void_p, C.c_uint64, C.c_uint32, C.c_double, C.c_int64,
+                                      C.c_int, C.c_void_p, C.POINTER(_Outcomes),
+                                      C.POINTER(_Columns), C.POINTER(_Stats), C.c_char_p,
+                                      C.c_size_t, C.c_char_p, C.c_size_t]
         L.fdo_sample.restype = C.c_int
         L.fdo_sample.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int, C.c_int,
                                  C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
@@ -196,10 +204,13 @@ def sample(zr, zi, ar, ai, max_iter, r2, c=None, extended=False, offset_exp=0,
 
 def render(re, im, width, rotation, nx, ny, max_iter, escape_radius=1e10,
            columns=COL_NU | COL_DE | COL_NORMAL, ss=1, tier=TIER_AUTO, bla=False,
-           bla_eps=EPS_EXACT, threads=None, mask=None, outcomes=True):
+           bla_eps=EPS_EXACT, threads=None, mask=None, outcomes=True, orbit=None,
+           table_dc=None):
     """grid.rs render_with / render_bla (table built per frame as `fd control --bla
     per-frame` does, control.rs:356-366).  Returns a dict of numpy columns, outcomes,
-    stats and the header kernel id."""
+    stats and the header kernel id.  orbit = (re, im, bits) supplies the reference orbit
+    (grid.rs:95-109); table_dc = (dc_max, dc_exp) builds the table for that |dc| bound
+    instead of the frame's own (after grid.rs:378-413 bla_checks)."""
     n = nx * ny
     threads = threads or os.cpu_count() or 1
     cols = {"class": np.zeros(n, np.uint8)}
@@ -222,10 +233,18 @@ def render(re, im, width, rotation, nx, ny, max_iter, escape_radius=1e10,
     kid = C.create_string_buffer(128)
     err = C.create_string_buffer(256)
     mk = None if mask is None else np.ascontiguousarray(mask, np.uint8)
-    rc = lib().fdo_render(re.encode(), im.encode(), width.encode(), rotation, nx, ny, ss,
-                          max_iter, escape_radius, columns, tier, int(bla), bla_eps, threads,
-                          _p(mk), C.byref(oc) if oc is not None else None, C.byref(cs),
-                          C.byref(st), kid, 128, err, 256)
+    ore = oim = None
+    olen = obits = 0
+    if orbit is not None:
+        ore = np.ascontiguousarray(orbit[0], np.float64)
+        oim = np.ascontiguousarray(orbit[1], np.float64)
+        olen, obits = len(ore), int(orbit[2])
+    tdc, tde = table_dc if table_dc is not None else (0.0, 0)
+    rc = lib().fdo_render_with(re.encode(), im.encode(), width.encode(), rotation, nx, ny, ss,
+                               max_iter, escape_radius, columns, tier, int(bla), bla_eps,
+                               _p(ore), _p(oim), olen, obits, tdc, tde, threads, _p(mk),
+                               C.byref(oc) if oc is not None else None, C.byref(cs),
+                               C.byref(st), kid, 128, err, 256)
     if rc != 0:
         raise ValueError(err.value.decode())
     return dict(columns=cols, outcomes=res, kernel=kid.value.decode(),

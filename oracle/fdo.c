@@ -1437,11 +1437,48 @@ int64_t fdo_reference(const char *re, const char *im, const char *width, double 
 }
 
 /* grid.rs:446-535 run (+ control.rs:356-366 per-frame table for bla = 1) */
+/* grid.rs:446-477 run: the supplied-orbit checks and plane.bits = supplied bits. */
+static int supplied_orbit(fdo_plane *pl, int tier, const double *re, const double *im,
+                          uint64_t len, uint32_t bits, orbit_t *o, char *err, size_t errcap) {
+    uint64_t need = tier == FDO_TIER_F64 ? 53 : pl->bits;
+    if (bits != need && (tier == FDO_TIER_F64 || bits < need)) {
+        char m[160];
+        snprintf(m, sizeof(m), "supplied reference has %u bits; this view needs %s%llu", bits,
+                 tier == FDO_TIER_F64 ? "" : "at least ", (unsigned long long)need);
+        set_err(err, errcap, m);
+        return -1;
+    }
+    if (len == 0 || re[0] != 0.0 || im[0] != 0.0) {
+        set_err(err, errcap,
+                "supplied reference orbit must start at Z_0 = 0 with equal re/im lengths");
+        return -1;
+    }
+    o->re = (double *)malloc(len * sizeof(double));
+    o->im = (double *)malloc(len * sizeof(double));
+    memcpy(o->re, re, len * sizeof(double));
+    memcpy(o->im, im, len * sizeof(double));
+    o->len = o->cap = len;
+    pl->bits = bits;
+    return 0;
+}
+
 int fdo_render(const char *re, const char *im, const char *width, double rotation,
                uint32_t nx, uint32_t ny, uint32_t ss, uint64_t max_iter, double escape_radius,
                uint32_t columns, int tier_req, int bla, double bla_eps, int threads,
                const uint8_t *mask, fdo_outcomes *oc, fdo_columns *cols, fdo_stats *stats,
                char *kernel_id, size_t kcap, char *err, size_t errcap) {
+    return fdo_render_with(re, im, width, rotation, nx, ny, ss, max_iter, escape_radius, columns,
+                           tier_req, bla, bla_eps, NULL, NULL, 0, 0, 0.0, 0, threads, mask, oc,
+                           cols, stats, kernel_id, kcap, err, errcap);
+}
+
+int fdo_render_with(const char *re, const char *im, const char *width, double rotation,
+                    uint32_t nx, uint32_t ny, uint32_t ss, uint64_t max_iter,
+                    double escape_radius, uint32_t columns, int tier_req, int bla, double bla_eps,
+                    const double *orbit_re, const double *orbit_im, uint64_t orbit_len,
+                    uint32_t orbit_bits, double table_dc, int64_t table_dc_exp, int threads,
+                    const uint8_t *mask, fdo_outcomes *oc, fdo_columns *cols, fdo_stats *stats,
+                    char *kernel_id, size_t kcap, char *err, size_t errcap) {
     fdo_plane pl;
     int tier;
     if (setup(re, im, width, rotation, nx, ny, ss, tier_req, &pl, &tier, err, errcap) != 0)
@@ -1452,7 +1489,11 @@ int fdo_render(const char *re, const char *im, const char *width, double rotatio
     }
     double t0 = now_s();
     orbit_t o;
-    if (compute_reference(re, im, &pl, tier, max_iter, escape_radius, &o) != 0) {
+    if (orbit_re) {
+        if (supplied_orbit(&pl, tier, orbit_re, orbit_im, orbit_len, orbit_bits, &o, err,
+                           errcap) != 0)
+            return -1;
+    } else if (compute_reference(re, im, &pl, tier, max_iter, escape_radius, &o) != 0) {
         set_err(err, errcap, "bad centre");
         return -1;
     }
@@ -1464,6 +1505,20 @@ int fdo_render(const char *re, const char *im, const char *width, double rotatio
         double dc;
         int64_t de;
         dc_bound(&pl, tier, nx, ny, &dc, &de);
+        if (table_dc > 0.0) {
+            /* grid.rs:399-411 bla_checks: the view's |dc| bound against the table's */
+            int over = de == table_dc_exp
+                           ? dc > table_dc
+                           : log2(dc) + (double)de > log2(table_dc) + (double)table_dc_exp;
+            if (isnan(dc) || over) {
+                free(o.re);
+                free(o.im);
+                set_err(err, errcap, "view reaches |dc| beyond the BLA table's dc_max");
+                return -1;
+            }
+            dc = table_dc;
+            de = table_dc_exp;
+        }
         if (bla_build(&table, o.re, o.im, o.len, bla_eps, dc, de) != 0) {
             free(o.re);
             free(o.im);

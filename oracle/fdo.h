@@ -94,6 +94,18 @@ int fdo_render(const char *re, const char *im, const char *width, double rotatio
                const uint8_t *mask, fdo_outcomes *oc, fdo_columns *cols, fdo_stats *stats,
                char *kernel_id, size_t kcap, char *err, size_t errcap);
 
+/* fdo_render with a supplied reference orbit (grid.rs:95-109 render_with, :190-210
+ * render_bla): orbit_re/orbit_im/orbit_len of precision orbit_bits (NULL: computed as
+ * fdo_render does), and for bla a table built at table_dc 2^table_dc_exp (table_dc <= 0:
+ * the frame's own bound).  Refusals as grid.rs:461-471 and bla_checks (:378-413). */
+int fdo_render_with(const char *re, const char *im, const char *width, double rotation,
+                    uint32_t nx, uint32_t ny, uint32_t ss, uint64_t max_iter,
+                    double escape_radius, uint32_t columns, int tier, int bla, double bla_eps,
+                    const double *orbit_re, const double *orbit_im, uint64_t orbit_len,
+                    uint32_t orbit_bits, double table_dc, int64_t table_dc_exp, int threads,
+                    const uint8_t *mask, fdo_outcomes *oc, fdo_columns *cols, fdo_stats *stats,
+                    char *kernel_id, size_t kcap, char *err, size_t errcap);
+
 /* One sample through the kernel (sample.rs iterate) on a given orbit, for unit tests.
  * extended selects Binary64Exponent; offset_exp is its exponent.  Returns kind. */
 int fdo_sample(const double *re, const double *im, uint64_t len, const double *bla_blocks,
