@@ -303,6 +303,7 @@ struct Lane {
 };
 
 constexpr uint32_t kPending = 1, kPredicted = 2, kConStarted = 4;
+constexpr int kWalk = 4;  // BLA levels whose radii one lookup step loads together
 
 __device__ inline uint32_t np2_above(uint32_t n) {
     // next_power_of_two(n + 1) for 1 <= n < 2^31
@@ -321,11 +322,32 @@ __device__ inline double inverse_of(const Lane& s) {
     return EXT ? exp2i(-(int64_t)s.derivative_exp) : 1.0;
 }
 
+// Pointers into device (global) memory.  Pointers read from a struct in memory are
+// generic to the compiler, which then emits flat accesses (slower to issue, and counted
+// against the LDS wait counter too); these casts keep the hot accesses global.
+#define FDG_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ inline FDG_GLOBAL T* global_ptr(T* p) {
+    return (FDG_GLOBAL T*)p;
+}
+// Native vector types load from global memory as one 16- / 32-byte access (HIP's double2
+// and double4 classes cannot be copied out of an address-space-qualified lvalue).
+typedef double v2d __attribute__((ext_vector_type(2)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+__device__ inline double2 ld2(const FDG_GLOBAL v2d* p, uint64_t i) {
+    v2d t = p[i];
+    return make_double2(t.x, t.y);
+}
+__device__ inline double4 ld4(const FDG_GLOBAL v4d* p, uint64_t i) {
+    v4d t = p[i];
+    return make_double4(t.x, t.y, t.z, t.w);
+}
+
 // Frame constants the loop reads, copied once into registers / SGPRs.
 struct Hot {
-    const double2* orbit;
-    const double* r2;
-    const double4* ab;
+    const FDG_GLOBAL v2d* orbit;
+    const FDG_GLOBAL double* r2;
+    const FDG_GLOBAL v4d* ab;
     uint32_t last, max_iter, nlevels, steps;
     uint64_t stride;
     double esc2;
@@ -472,7 +494,7 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
     // rebase (the common case): Z_{m+2} and the level-0 r2 of a block starting at m+1.
     // Their L2 latency then overlaps this whole iteration.  (s.m < last here, and the
     // orbit buffer is padded past its end.)
-    double2 pz2 = h.orbit[s.m + 2];
+    double2 pz2 = ld2(h.orbit, s.m + 2);
     double pr0 = 0.0;
     if (L && (s.m & 1u) == 0 && (s.m >> 1) < h.count0) pr0 = h.r2[tb + (s.m >> 1)];
     double2 Zn = make_double2(s.Z1r, s.Z1i);  // a plain step's landing point
@@ -499,12 +521,25 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
             // level 0 from the carried radius (0 when no such block exists)
             if (2u <= b && d2 < s.R0) {
                 lvl = 0;
-                for (int lv = 1; lv < top; lv++) {
-                    uint32_t idx = k >> (lv + 1);
-                    if (!((2ull << lv) <= (uint64_t)b && idx < (h.steps >> (lv + 1)) &&
-                          d2 < h.r2[tb + bla_level_offset(h.stride, lv) + idx]))
-                        break;
-                    lvl = lv;
+                // The radii of the candidate levels are independent loads: issue a group
+                // of kWalk together (one L2 latency instead of a chain of them, which the
+                // wave's deepest-walking lane would set), then take the valid prefix.
+                bool go = true;
+                for (int l0 = 1; go && l0 < top; l0 += kWalk) {
+                    double rr[kWalk];
+#pragma unroll
+                    for (int q = 0; q < kWalk; q++) {
+                        int lv = l0 + q;
+                        uint32_t idx = k >> (lv + 1);
+                        bool in = lv < top && (2ull << lv) <= (uint64_t)b &&
+                                  idx < (h.steps >> (lv + 1));
+                        rr[q] = in ? h.r2[tb + bla_level_offset(h.stride, lv) + idx] : -1.0;
+                    }
+#pragma unroll
+                    for (int q = 0; q < kWalk; q++) {
+                        go = go && d2 < rr[q];
+                        lvl = go ? l0 + q : lvl;
+                    }
                 }
             }
         }
@@ -520,14 +555,14 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         uint32_t blen = 1;
         if (blk) {
             uint64_t at = tb + bla_level_offset(h.stride, lvl) + ((s.m - 1) >> (lvl + 1));
-            double4 ab = h.ab[at];
+            double4 ab = ld4(h.ab, at);
             a_r = ab.x;
             a_i = ab.y;
             b_r = ab.z;
             b_i = ab.w;
             blen = 2u << lvl;
-            Zn = h.orbit[s.m + blen];
-            Znext = h.orbit[s.m + blen + 1];
+            Zn = ld2(h.orbit, s.m + blen);
+            Znext = ld2(h.orbit, s.m + blen + 1);
             uint32_t i0 = (s.m + blen - 1) >> 1;
             r0next = i0 < h.count0 ? h.r2[tb + i0] : 0.0;
         }
@@ -562,11 +597,11 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         s.plain += blk ? 0u : 1u;
     } else if (lvl >= 0) {
         uint64_t at = tb + bla_level_offset(h.stride, lvl) + ((s.m - 1) >> (lvl + 1));
-        double4 ab = h.ab[at];
+        double4 ab = ld4(h.ab, at);
         double a_r = ab.x, a_i = ab.y, b_r = ab.z, b_i = ab.w;
         uint32_t blen = 2u << lvl;
-        Zn = h.orbit[s.m + blen];
-        Znext = h.orbit[s.m + blen + 1];
+        Zn = ld2(h.orbit, s.m + blen);
+        Znext = ld2(h.orbit, s.m + blen + 1);
         uint32_t i0 = (s.m + blen - 1) >> 1;
         r0next = L && i0 < h.count0 ? h.r2[tb + i0] : 0.0;
         if (s.nucleus && s.n >= 1) {
@@ -667,7 +702,7 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
             double dist = fabs(fr - s.sr) + fabs(fi - s.si);
             double size = fabs(s.sr) + fabs(s.si);
             if (dist < 1e-13 * size + 1e-300) {
-                if (contracting(reinterpret_cast<const double*>(h.orbit), h.len, s.m, pr, pi,
+                if (contracting(reinterpret_cast<const double*>((const v2d*)h.orbit), h.len, s.m, pr, pi,
                                 s.ar * exp2i(oe), s.ai * exp2i(oe), s.n - s.sn))
                     return 2;  // Unresolved
                 s.flags += 16;
@@ -741,10 +776,18 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
     __syncthreads();
     const uint32_t lane = __lane_id();
     Rare r(rare, threadIdx.x);
+    FDG_GLOBAL uint8_t* o_kind = global_ptr(fp->kind);
+    FDG_GLOBAL uint64_t* o_n = global_ptr(fp->n);
+    FDG_GLOBAL double* o_zr = global_ptr(fp->zr);
+    FDG_GLOBAL double* o_zi = global_ptr(fp->zi);
+    FDG_GLOBAL double* o_dr = global_ptr(fp->dr);
+    FDG_GLOBAL double* o_di = global_ptr(fp->di);
+    FDG_GLOBAL int64_t* o_dexp = global_ptr(fp->dexp);
+    FDG_GLOBAL uint32_t* o_ops = global_ptr(fp->ops);
     Hot h;
-    h.orbit = reinterpret_cast<const double2*>(fp->orbit);
-    h.r2 = fp->bla.r2;
-    h.ab = reinterpret_cast<const double4*>(fp->bla.ab);
+    h.orbit = global_ptr(reinterpret_cast<const v2d*>(fp->orbit));
+    h.r2 = global_ptr(fp->bla.r2);
+    h.ab = global_ptr(reinterpret_cast<const v4d*>(fp->bla.ab));
     h.last = (uint32_t)fp->len - 1;
     h.len = (uint32_t)fp->len;
     h.max_iter = (uint32_t)fp->max_iter;
@@ -755,7 +798,7 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
     h.oe = (int32_t)fp->h_e;
     h.count0 = (uint32_t)(fp->bla.steps >> 1);
     {
-        double2 z1 = h.orbit[1], z2 = h.orbit[2];
+        double2 z1 = ld2(h.orbit, 1), z2 = ld2(h.orbit, 2);
         h.Z1r = z1.x;
         h.Z1i = z1.y;
         h.Z2r = z2.x;
@@ -810,17 +853,17 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
                     double y = (double)j + 0.5 - g.half_y;
                     double ux = g.cos_r * x + g.sin_r * y;
                     double uy = g.sin_r * x - g.cos_r * y;
-                    fp->ops[idx] = 0;
+                    o_ops[idx] = 0;
                     double ar = EXT ? ux * g.h_m : ux * g.h, ai = EXT ? uy * g.h_m : uy * g.h;
                     if (!EXT && fp->tier == 0 && in_main_components(fp->c_re + ar, fp->c_im + ai)) {
                         // f64 tier: closed-form interior (grid.rs Job::one, interior.rs:6-11)
-                        fp->kind[idx] = 1;
-                        fp->n[idx] = 0;
-                        fp->zr[idx] = 0.0;
-                        fp->zi[idx] = 0.0;
-                        fp->dr[idx] = 0.0;
-                        fp->di[idx] = 0.0;
-                        fp->dexp[idx] = 0;
+                        o_kind[idx] = 1;
+                        o_n[idx] = 0;
+                        o_zr[idx] = 0.0;
+                        o_zi[idx] = 0.0;
+                        o_dr[idx] = 0.0;
+                        o_di[idx] = 0.0;
+                        o_dexp[idx] = 0;
                         if (L) atomicAdd(&stat[ST_CLOSED], 1ull);
                     } else {
                         lane_start<EXT>(s, r, h, nucleus, ar, ai, EXT ? (int32_t)g.h_e : 0);
@@ -835,17 +878,17 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
                     double y = (double)j + 0.5 - fp->half_y;
                     double ux = fp->cos_r * x + fp->sin_r * y;
                     double uy = fp->sin_r * x - fp->cos_r * y;
-                    fp->ops[idx] = 0;
+                    o_ops[idx] = 0;
                     if (fp->tier == 0) {
                         double ar = ux * fp->h, ai = uy * fp->h;
                         if (in_main_components(fp->c_re + ar, fp->c_im + ai)) {
-                            fp->kind[idx] = 1;
-                            fp->n[idx] = 0;
-                            fp->zr[idx] = 0.0;
-                            fp->zi[idx] = 0.0;
-                            fp->dr[idx] = 0.0;
-                            fp->di[idx] = 0.0;
-                            fp->dexp[idx] = 0;
+                            o_kind[idx] = 1;
+                            o_n[idx] = 0;
+                            o_zr[idx] = 0.0;
+                            o_zi[idx] = 0.0;
+                            o_dr[idx] = 0.0;
+                            o_di[idx] = 0.0;
+                            o_dexp[idx] = 0;
                             if (L) atomicAdd(&stat[ST_CLOSED], 1ull);
                         } else {
                             lane_start<EXT>(s, r, h, nucleus, ar, ai, 0);
@@ -876,14 +919,14 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
                     // Every field is written (zeros where the outcome has none), so the
                     // output buffers need no clearing between passes.
                     bool esc = k == 0;
-                    fp->kind[idx] = (uint8_t)k;
-                    fp->n[idx] = s.n;
-                    fp->ops[idx] = s.plain + s.blocks;
-                    fp->zr[idx] = esc ? fr : 0.0;
-                    fp->zi[idx] = esc ? fi : 0.0;
-                    fp->dr[idx] = esc ? s.dr : 0.0;
-                    fp->di[idx] = esc ? s.di : 0.0;
-                    fp->dexp[idx] = esc ? s.derivative_exp : 0;
+                    o_kind[idx] = (uint8_t)k;
+                    o_n[idx] = s.n;
+                    o_ops[idx] = s.plain + s.blocks;
+                    o_zr[idx] = esc ? fr : 0.0;
+                    o_zi[idx] = esc ? fi : 0.0;
+                    o_dr[idx] = esc ? s.dr : 0.0;
+                    o_di[idx] = esc ? s.di : 0.0;
+                    o_dexp[idx] = esc ? s.derivative_exp : 0;
                     unsigned long long spent = k == 2 ? h.max_iter : s.n;
                     atomicAdd(&stat[ST_EQUIVALENT], spent);
                     atomicAdd(&stat[ST_ITERATIONS], L ? spent - s.skipped + s.blocks : spent);
