@@ -61,10 +61,12 @@ def cpu_baseline(view, params, route_bla, target_seconds, threads, orbit=None):
                        f"reported apart)")
 
 
-def measure(fa, view, params, route, steps, warmup, label):
-    """Open the frame (orbit resident in HBM), warm up, time `steps` passes."""
+def measure(fa, view, params, route, steps, warmup, label, form=None, keep=None):
+    """Open the frame (orbit resident in HBM), warm up, time `steps` passes.  keep: a
+    list that receives the frame's columns and raw outcomes."""
+    form = fa.FORM_F64 if form is None else form
     t = time.perf_counter()
-    frame = fa.ResidentFrame(view, params, route, fa.FORM_F64, fa.EPS)
+    frame = fa.ResidentFrame(view, params, route, form, fa.EPS)
     log(f"{label}: frame open (host reference orbit + upload) {time.perf_counter() - t:.2f}s")
     for i in range(warmup):
         st = frame.run()
@@ -78,6 +80,8 @@ def measure(fa, view, params, route, steps, warmup, label):
     t1 = time.perf_counter()
     h, cols, _ = frame.read(raw=True)
     frame.close()
+    if keep is not None:
+        keep.append(cols)
     import numpy as np
     ops = cols["raw"]["ops"].astype(np.float64)
     ops_pct = {f"p{q}": float(np.percentile(ops, q)) for q in (50, 90, 99, 99.9, 100)}
@@ -107,6 +111,39 @@ def measure(fa, view, params, route, steps, warmup, label):
                           "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
                           "traffic": None, "flops_per_launch": flops,
                           "flops_model": f"{fp}/plain step + {fb}/applied block"})
+
+
+def agreement(ref, other):
+    """Per-sample agreement of another delta form with the FP64 form's outcomes."""
+    import numpy as np
+    ka, kb = ref["raw"]["kind"], other["raw"]["kind"]
+    esc = (ka == 0) & (kb == 0)
+    out = {"samples": int(ka.size), "class_agreement": float((ka == kb).mean()),
+           "escaped_in_both": int(esc.sum())}
+    if esc.any():
+        dn = other["raw"]["n"][esc].astype(np.int64) - ref["raw"]["n"][esc].astype(np.int64)
+        dnu = np.abs(other["nu"][esc] - ref["nu"][esc])
+        out.update(escape_count_equal=float((dn == 0).mean()),
+                   escape_count_within_1=float((np.abs(dn) <= 1).mean()),
+                   smooth_count_err_median=float(np.median(dnu)),
+                   smooth_count_err_p99=float(np.percentile(dnu, 99)),
+                   smooth_count_within_1e_3=float((dnu <= 1e-3).mean()))
+    return out
+
+
+def measure_forms(fa, view, params, route, steps, warmup, label, ref_metrics, ref_cols):
+    """The FP32 floatexp form and the mixed form (FP32 pass + FP64 recompute of its escaped
+    samples) on the same frame: speed against the FP64 form and per-sample agreement with
+    it (the FP64 form equals the oracle bit for bit)."""
+    out = {}
+    for form, name in ((fa.FORM_FE32, "fe32"), (fa.FORM_MIXED, "mixed")):
+        keep = []
+        m = measure(fa, view, params, route, steps, warmup, f"{label}-{name}", form, keep)
+        out[name] = {"kernel_ms": m["kernel_ms"], "ms_per_step": m["ms_per_step"],
+                     "value": m["value"], "kernel_id": m["kernel_id"],
+                     "speedup_over_fp64_kernel": ref_metrics["kernel_ms"] / m["kernel_ms"],
+                     "agreement_with_fp64": agreement(ref_cols, keep[0])}
+    return out
 
 
 def measure_film(fa, sc, cols, steps, warmup, threads, cpu_seconds):
@@ -229,13 +266,19 @@ def main():
         if "film" in frames:
             log(json.dumps(measure_film(fa, get_scene(4), cols, a.steps, a.warmup, 1, 0.0)))
         return
-    m = measure(fa, view, params, route, a.steps, a.warmup, f"config{a.config}")
+    keep = []
+    m = measure(fa, view, params, route, a.steps, a.warmup, f"config{a.config}", keep=keep)
+    forms = measure_forms(fa, view, params, route, a.steps, a.warmup, f"config{a.config}", m,
+                          keep[0])
     threads = min(16, os.cpu_count() or 1)
     cpu_s = 0.0 if a.no_cpu_baseline else a.cpu_seconds
     deep = {}
     if "deep" in frames:
         v3, p3 = deep_view(fa, get_scene(3), cols)
-        deep = measure(fa, v3, p3, fa.ROUTE_BLA, a.steps, a.warmup, "deep-1e-1000")
+        keep3 = []
+        deep = measure(fa, v3, p3, fa.ROUTE_BLA, a.steps, a.warmup, "deep-1e-1000", keep=keep3)
+        deep["precision_forms"] = measure_forms(fa, v3, p3, fa.ROUTE_BLA, a.steps, a.warmup,
+                                                "deep-1e-1000", deep, keep3[0])
         deep["workload"] = (f"1920x1080 width 1e-1000 at the period-"
                             f"{get_scene(3)['nucleus']['period']} ladder nucleus, max_iter 1e6, "
                             f"BLA eps 2^-50, scaled tier (f64 mantissa x 2^e deltas), dz/dc "
@@ -265,6 +308,7 @@ def main():
                         f"~1e{sc['nucleus']['log10_size']:.0f}), max_iter {params.max_iter}, BLA "
                         f"eps 2^-50, FP64 deltas, nu/de/normal",
             **{k: v for k, v in m.items() if k not in ("value", "ms_per_step", "roofline")},
+            "precision_forms": forms,
             "escape_frame": {"workload": "reference deep-zoom path (v0 centre, period-764 "
                                          "minibrot) at width 2e-49, 1920x1080, max_iter 1e5, BLA",
                              **{k: v for k, v in m0.items()}},

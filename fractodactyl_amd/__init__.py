@@ -27,6 +27,8 @@ COL_CLASS, COL_NU, COL_DE, COL_NORMAL, COL_BOUND, COL_STRIPE = 1, 2, 4, 8, 16, 3
 TIER_AUTO, TIER_F64, TIER_FIXED, TIER_SCALED = -1, 0, 1, 2
 ROUTE_PLAIN, ROUTE_BLA = 0, 1
 FORM_F64 = 0
+FORM_FE32 = 1  # FP32 mantissas + int32 exponents: close to, not equal to, FORM_F64
+FORM_MIXED = 2  # FE32 pass, then its escaped samples recomputed in FP64
 ESCAPED, INTERIOR, UNRESOLVED = 0, 1, 2
 EPS = 1.0 / 1125899906842624.0          # 2^-50, crates/fd-cli/src/orbit.rs:16
 FILM_EPS = 1.0 / (1 << 26)              # orbit.rs:22

@@ -380,7 +380,8 @@ int fdg_frame_open(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int rou
     if (p->columns & (FDG_COL_BOUND | FDG_COL_STRIPE))
         return ctx->fail(FDG_EDECLINED,
                          "bound/stripe columns are CPU-kernel columns (frame.rs:133-144)");
-    if (form != FDG_FORM_F64) return ctx->fail(FDG_EINVAL, "unknown delta form");
+    if (form != FDG_FORM_F64 && form != FDG_FORM_FE32 && form != FDG_FORM_MIXED)
+        return ctx->fail(FDG_EINVAL, "unknown delta form");
     if (route != FDG_ROUTE_PLAIN && route != FDG_ROUTE_BLA)
         return ctx->fail(FDG_EINVAL, "unknown route");
     uint64_t count64 = (uint64_t)p->nx * p->ny;
@@ -424,6 +425,17 @@ int fdg_frame_open(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int rou
     f->nucleus = o.nucleus_period();
     rc = upload_orbit(ctx, &f->buf.orbit, o, &f->transfer_seconds);
     if (!rc) rc = f->buf.outputs(ctx, f->count);
+    if (!rc && form != FDG_FORM_F64) {
+        // FP32 floatexp orbit, 16 bytes per point, with two zero points of pad past the end
+        uint64_t bytes = 16 * (f->len + 2);
+        rc = ctx->hip(f->buf.orbit32.ensure(bytes), "hipMalloc");
+        if (!rc) rc = ctx->hip(hipMemsetAsync(f->buf.orbit32.p, 0, bytes, ctx->stream), "memset");
+        if (!rc)
+            rc = ctx->hip(launch_orbit_to_f32(f->buf.orbit.as<double>(),
+                                              f->buf.orbit32.as<float>(), f->len, ctx->stream),
+                          "orbit conversion");
+        if (!rc) rc = ctx->hip(hipStreamSynchronize(ctx->stream), "orbit conversion");
+    }
     if (rc) {
         f->buf.release();
         delete f;
@@ -500,17 +512,78 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
     if (grid < 1) grid = 1;
     FDG_TRY(ctx, hipMemcpyAsync(b.fdev.p, &d, sizeof(d), hipMemcpyHostToDevice, ctx->stream),
             "frame upload");
+    bool mixed = f->form == FDG_FORM_MIXED;
+    bool fe32 = f->form == FDG_FORM_FE32 || mixed;
+    if (fe32) {
+        // FP32 table coefficients (A, B), same padded layout
+        if (f->use_bla) {
+            FDG_TRY(ctx, b.ab32.ensure(b.bla_stride * 16), "hipMalloc");
+            FDG_TRY(ctx, b.abexp32.ensure(b.bla_stride * 8), "hipMalloc");
+            FDG_TRY(ctx,
+                    launch_table_to_f32(b.bla_ab.as<double>(), b.ab32.as<float>(),
+                                        b.abexp32.as<int32_t>(), b.bla_stride, ctx->stream),
+                    "table conversion");
+        }
+        d.orbit32 = b.orbit32.as<float>();
+        d.ab32 = b.ab32.as<float>();
+        d.abexp32 = b.abexp32.as<int32_t>();
+        d.defer_escaped = mixed ? 1u : 0u;
+        FDG_TRY(ctx,
+                hipMemcpyAsync(b.fdev.p, &d, sizeof(d), hipMemcpyHostToDevice, ctx->stream),
+                "frame upload");
+    }
     FDG_TRY(ctx, hipEventRecord(ctx->ev[2], ctx->stream), "event");
-    FDG_TRY(ctx,
-            launch_perturb_f64(b.fdev.as<FrameDev>(), f->fr.tier == TIER_SCALED, deriv,
-                               f->use_bla, false, grid, ctx->stream),
-            "perturbation launch");
+    if (fe32)
+        FDG_TRY(ctx,
+                launch_perturb_fe32(b.fdev.as<FrameDev>(), deriv, f->use_bla, grid,
+                                    ctx->stream),
+                "perturbation launch");
+    else
+        FDG_TRY(ctx,
+                launch_perturb_f64(b.fdev.as<FrameDev>(), f->fr.tier == TIER_SCALED, deriv,
+                                   f->use_bla, false, grid, ctx->stream),
+                "perturbation launch");
+    unsigned long long cnt2[ST_COUNT];
+    std::memset(cnt2, 0, sizeof(cnt2));
+    if (mixed) {
+        // Second pass: the samples the FP32 pass found escaped, recomputed in FP64 (the
+        // outcomes of every other sample are the FP32 pass's).
+        FDG_TRY(ctx, b.list.ensure((size_t)f->count * 4), "hipMalloc");
+        FDG_TRY(ctx, b.counters2.ensure(256), "hipMalloc");
+        FDG_TRY(ctx, b.fdev2.ensure(sizeof(FrameDev)), "hipMalloc");
+        // words 0 (work index) and 1 (list length) and the statistics start at 0
+        FDG_TRY(ctx, launch_zero_words(b.counters2.as<unsigned long long>(), 32, ctx->stream),
+                "counter clear");
+        uint32_t* listed = b.counters2.as<uint32_t>() + 1;
+        FDG_TRY(ctx,
+                launch_escaped_list(b.o_kind.as<uint8_t>(), b.list.as<uint32_t>(), listed,
+                                    f->count, ctx->stream),
+                "escaped list");
+        FrameDev d2 = d;
+        d2.list = b.list.as<uint32_t>();
+        d2.list_count = listed;
+        d2.defer_escaped = 0;
+        d2.next = b.counters2.as<uint32_t>();
+        d2.stats = reinterpret_cast<unsigned long long*>(b.counters2.as<uint8_t>() + 64);
+        FDG_TRY(ctx,
+                hipMemcpyAsync(b.fdev2.p, &d2, sizeof(d2), hipMemcpyHostToDevice, ctx->stream),
+                "frame upload");
+        FDG_TRY(ctx,
+                launch_perturb_f64(b.fdev2.as<FrameDev>(), f->fr.tier == TIER_SCALED, deriv,
+                                   f->use_bla, false, grid, ctx->stream, true),
+                "refinement launch");
+        FDG_TRY(ctx,
+                hipMemcpyAsync(cnt2, d2.stats, sizeof(cnt2), hipMemcpyDeviceToHost, ctx->stream),
+                "readback");
+    }
     FDG_TRY(ctx, hipEventRecord(ctx->ev[3], ctx->stream), "event");
     unsigned long long cnt[ST_COUNT];
     FDG_TRY(ctx,
             hipMemcpyAsync(cnt, d.stats, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream),
             "readback");
     FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "perturbation kernel");
+    for (int q = 0; q < ST_COUNT; q++)
+        cnt[q] = q == ST_MAXOPS ? std::max(cnt[q], cnt2[q]) : cnt[q] + cnt2[q];
     float ms = 0;
     hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
     s.kernel_seconds = ms * 1e-3;
@@ -549,7 +622,8 @@ int fdg_frame_read(fdg_frame* f, fdg_columns* out, fdg_outcomes* raw, fdg_stats*
     if (kernel_id && kernel_cap) {
         std::string k = f->fr.pl.kernel(f->fr.tier);
         if (f->use_bla) k += " bla/1";
-        k += " gpu-hip-f64/1";
+        k += f->form == FDG_FORM_FE32 ? " gpu-hip-fe32/1"
+             : f->form == FDG_FORM_MIXED ? " gpu-hip-mixed/1" : " gpu-hip-f64/1";
         std::snprintf(kernel_id, kernel_cap, "%s", k.c_str());
     }
     return FDG_OK;

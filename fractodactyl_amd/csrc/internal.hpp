@@ -19,8 +19,16 @@ hipError_t launch_bla_build(const double* orbit, double* r, double* r2, double* 
                             double eps, double dc_max, int64_t dc_exp, uint32_t max_levels,
                             hipStream_t stream);
 hipError_t launch_perturb_f64(const FrameDev* f, bool ext, bool deriv, bool bla, bool batch,
-                              int grid, hipStream_t stream);
+                              int grid, hipStream_t stream, bool list = false);
 hipError_t launch_zero_words(unsigned long long* p, uint32_t n, hipStream_t stream);
+hipError_t launch_orbit_to_f32(const double* orbit, float* o32, uint64_t points,
+                               hipStream_t stream);
+hipError_t launch_table_to_f32(const double* ab, float* ab32, int32_t* abexp, uint64_t blocks,
+                               hipStream_t stream);
+hipError_t launch_escaped_list(const uint8_t* kind, uint32_t* list, uint32_t* count, uint32_t n,
+                               hipStream_t stream);
+hipError_t launch_perturb_fe32(const FrameDev* f, bool deriv, bool bla, int grid,
+                               hipStream_t stream);
 
 constexpr uint32_t kMaxLevels = 48;
 constexpr double kEpsLimit = 1.0 / 16777216.0;  // bla.rs:288-292 EPS_LIMIT = 2^-24
@@ -88,11 +96,13 @@ namespace fdg {
 struct Buffers {
     DevBuf orbit, bla_r, bla_r2, bla_ab, bla_alpha, bla_beta, bla_valid;
     DevBuf o_kind, o_n, o_zr, o_zi, o_dr, o_di, o_dexp, o_ops, counters, fdev;
+    DevBuf orbit32, ab32, abexp32;  // FP32 copies for the FE32 and mixed forms
+    DevBuf list, counters2, fdev2;  // the mixed form's FP64 refinement pass
     uint64_t bla_stride = 0, bla_steps = 0;
     void release() {
         for (DevBuf* b : {&orbit, &bla_r, &bla_r2, &bla_ab, &bla_alpha, &bla_beta, &bla_valid,
                           &o_kind, &o_n, &o_zr, &o_zi, &o_dr, &o_di, &o_dexp, &o_ops,
-                          &counters, &fdev})
+                          &counters, &fdev, &orbit32, &ab32, &abexp32, &list, &counters2, &fdev2})
             b->release();
     }
     // Per-sample outcome arrays for n samples, the counters and one FrameDev.

@@ -74,6 +74,15 @@ struct FrameDev {
     uint32_t* ops;                  // executed plain steps + applied blocks per sample
     uint32_t* next;                 // work-stealing counter (zeroed by the host)
     unsigned long long* stats;      // ST_COUNT counters (zeroed by the host)
+    // FP32 delta form only: the orbit as float pairs and the table coefficients
+    // (A, B) as float quadruples, in the same layouts as orbit / bla.ab
+    const float* orbit32;
+    const float* ab32;
+    const int32_t* abexp32;         // per block: the exponents of A and B
+    uint32_t defer_escaped;         // FP32 pass of the mixed form: escaped samples are
+                                    // recomputed in FP64, so they add no statistics here
+    const uint32_t* list;           // nullable: compute only samples list[0 .. *list_count)
+    const uint32_t* list_count;
     // batches only (the single-frame kernels read the geometry above)
     const BatchFrameDev* frames;    // nframes entries
     uint32_t nframes, per_frame;    // count = nframes * per_frame

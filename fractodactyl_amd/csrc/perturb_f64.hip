@@ -766,7 +766,7 @@ __device__ inline uint32_t lanes_below(uint64_t mask) {
 // are those of a batch of frames sharing the orbit (frame-major), each frame with its own
 // geometry and BLA table, so one launch drains every frame and no frame's tail idles
 // the GPU.
-template <bool EXT, bool D, bool L, bool B>
+template <bool EXT, bool D, bool L, bool B, bool LIST = false>
 __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restrict__ fp) {
     __shared__ RareLds rare;
     // Statistics accumulate in LDS at each sample's end (rare next to the steps), not in
@@ -805,7 +805,9 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
         h.Z2i = z2.y;
         h.R00 = (L && h.count0 > 0) ? h.r2[0] : 0.0;
     }
-    const uint32_t count = fp->count, nx = fp->nx, nucleus = (uint32_t)fp->nucleus;
+    // LIST: an index list (the mixed form's refinement pass) replaces 0 .. count
+    const uint32_t count = LIST ? *fp->list_count : fp->count, nx = fp->nx,
+                   nucleus = (uint32_t)fp->nucleus;
     uint32_t* next = fp->next;
     Lane s;
     bool active = false;
@@ -872,7 +874,7 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
                         active = true;
                     }
                 } else if (!B && (!fp->mask || fp->mask[my])) {
-                    idx = my;
+                    idx = LIST ? fp->list[my] : my;
                     uint32_t i = idx % nx, j = idx / nx;
                     double x = (double)i + 0.5 - fp->half_x;
                     double y = (double)j + 0.5 - fp->half_y;
@@ -988,7 +990,25 @@ hipError_t launch_zero_words(unsigned long long* p, uint32_t n, hipStream_t stre
 }
 
 hipError_t launch_perturb_f64(const FrameDev* f, bool ext, bool deriv, bool bla, bool batch,
-                              int grid, hipStream_t stream) {
+                              int grid, hipStream_t stream, bool list) {
+    if (list) {
+        // the mixed form's refinement pass: an index list, never a batch
+#define FDG_LAUNCH_LIST(E, D, L) \
+    hipLaunchKernelGGL((perturb_f64<E, D, L, false, true>), dim3(grid), dim3(kBlock), 0, stream, f)
+        int k = (ext ? 4 : 0) | (deriv ? 2 : 0) | (bla ? 1 : 0);
+        switch (k) {
+            case 0: FDG_LAUNCH_LIST(false, false, false); break;
+            case 1: FDG_LAUNCH_LIST(false, false, true); break;
+            case 2: FDG_LAUNCH_LIST(false, true, false); break;
+            case 3: FDG_LAUNCH_LIST(false, true, true); break;
+            case 4: FDG_LAUNCH_LIST(true, false, false); break;
+            case 5: FDG_LAUNCH_LIST(true, false, true); break;
+            case 6: FDG_LAUNCH_LIST(true, true, false); break;
+            default: FDG_LAUNCH_LIST(true, true, true); break;
+        }
+#undef FDG_LAUNCH_LIST
+        return hipGetLastError();
+    }
 #define FDG_LAUNCH(E, D, L, B) \
     hipLaunchKernelGGL((perturb_f64<E, D, L, B>), dim3(grid), dim3(kBlock), 0, stream, f)
     int key = (batch ? 8 : 0) | (ext ? 4 : 0) | (deriv ? 2 : 0) | (bla ? 1 : 0);

@@ -37,8 +37,16 @@ enum { FDG_TIER_AUTO = -1, FDG_TIER_F64 = 0, FDG_TIER_FIXED = 1, FDG_TIER_SCALED
 /* Routes: `--gpu-perturb on|bla` (crates/fd-cli/src/perturb_gpu/frame.rs:62-84). */
 enum { FDG_ROUTE_PLAIN = 0, FDG_ROUTE_BLA = 1 };
 /* Delta arithmetic.  F64 reproduces the reference CPU kernel bit for bit (Binary64 on
- * the f64/fx tiers, Binary64Exponent on the scaled tier). */
-enum { FDG_FORM_F64 = 0 };
+ * the f64/fx tiers, Binary64Exponent on the scaled tier).  FE32 ("floatexp") iterates
+ * FP32 mantissas with carried int32 exponents on every tier, with FP64 only in the BLA
+ * validity test and the host encoding: close to, not equal to, the reference's results;
+ * it has no Brent/Newton periodicity confirmation, so samples the reference proves
+ * Interior that way end Unresolved (the counterpart of the reference GPU route's
+ * float-float CE words, crates/fd-cli/src/perturb_gpu/frame.rs:15-117).  MIXED runs the
+ * FE32 pass, then recomputes in F64 every sample it found escaped: escaped samples equal
+ * the F64 form's, the others are the FE32 pass's (the reverse of the reference route's
+ * admit-escaped / restart-the-rest split, frame.rs:338-341). */
+enum { FDG_FORM_F64 = 0, FDG_FORM_FE32 = 1, FDG_FORM_MIXED = 2 };
 /* Outcome kinds (crates/fd-kernel/src/sample.rs:21-40). */
 enum { FDG_ESCAPED = 0, FDG_INTERIOR = 1, FDG_UNRESOLVED = 2 };
 
@@ -123,7 +131,8 @@ int fdg_enabled(const char *value, int *enabled, int *route);
  * frame's |dc| bound (as `fd control --bla per-frame` does, control.rs:356-366) and
  * applies its blocks.  Writes the requested columns, exactly fd_kernel::render_with /
  * render_bla's values for form FDG_FORM_F64, optionally the raw outcomes, and the header
- * kernel id (plane kernel, " bla/1" when blocks were used, then " gpu-hip-f64/1"). */
+ * kernel id (plane kernel, " bla/1" when blocks were used, then " gpu-hip-f64/1", or
+ * " gpu-hip-fe32/1" / " gpu-hip-mixed/1" for forms FE32 / MIXED). */
 int fdg_render(fdg_ctx *ctx, const fdg_view *view, const fdg_params *params, int route, int form,
                double bla_eps, fdg_columns *out, fdg_outcomes *raw, fdg_stats *stats,
                char *kernel_id, size_t kernel_cap);
