@@ -113,6 +113,18 @@ def measure(fa, view, params, route, steps, warmup, label, form=None, keep=None)
                           "flops_model": f"{fp}/plain step + {fb}/applied block"})
 
 
+def measured_traffic(config):
+    """HBM bytes per launch of the dominant kernel for this workload, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs committed under profiles/ (each counter scaled by what it
+    reads for a plain copy of known size), or None when no profile covers the workload."""
+    path = os.path.join(ROOT, "profiles", f"traffic_config{config}.json")
+    if not os.path.exists(path):
+        return None
+    t = json.load(open(path))
+    kib = t["fetch_kib_per_launch"] * t["fetch_scale"] + t["write_kib_per_launch"] * t["write_scale"]
+    return kib * 1024.0
+
+
 def agreement(ref, other):
     """Per-sample agreement of another delta form with the FP64 form's outcomes."""
     import numpy as np
@@ -216,6 +228,19 @@ def measure_film(fa, sc, cols, steps, warmup, threads, cpu_seconds):
     return out
 
 
+def copy_probe(nbytes=1 << 28, reps=3):
+    """A plain device-to-device copy of known size (torch's copy kernel): profiling its
+    FETCH_SIZE / WRITE_SIZE counters gives the scale that turns those counters into HBM
+    bytes for the perturbation kernel's traffic figure."""
+    import torch
+    x = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
+
+
 def deep_view(fa, s3, cols):
     c3 = s3["config"]
     return (fa.View(s3["center_re"], s3["center_im"], c3["width"], 0.0),
@@ -234,7 +259,14 @@ def main():
                     help="which frames to run: config (the JSON line); companions escape "
                          "(reference zoom path at 2e-49), deep (1e-1000, 1M iterations), "
                          "film (64 keyframes sharing one orbit)")
+    ap.add_argument("--forms", action="store_true",
+                    help="with companions only: also run the FE32 and mixed forms")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 256 MiB device copy (the traffic counters' calibration)")
     a = ap.parse_args()
+    if a.copy_probe:
+        copy_probe()
+        return
     if a.gpus != 1:
         print(f"bench.py: this path runs on one GPU; --gpus {a.gpus} is not supported "
               "(the frame shards by tiles, see DESIGN.md)", file=sys.stderr)
@@ -257,12 +289,15 @@ def main():
     p0 = fa.Params(1920, 1080, 1, 100_000, 1e10, cols)
     if "config" not in frames:
         # profiling aid: companions alone, each printed as JSON on stderr, no bench line
-        if "escape" in frames:
-            log(json.dumps(measure(fa, v0, p0, fa.ROUTE_BLA, a.steps, a.warmup, "v0-2e-49")))
-        if "deep" in frames:
-            v3, p3 = deep_view(fa, get_scene(3), cols)
-            log(json.dumps(measure(fa, v3, p3, fa.ROUTE_BLA, a.steps, a.warmup,
-                                   "deep-1e-1000")))
+        for name, (v, p) in (("escape", (v0, p0)), ("deep", deep_view(fa, get_scene(3), cols))):
+            if name not in frames:
+                continue
+            keep = []
+            m = measure(fa, v, p, fa.ROUTE_BLA, a.steps, a.warmup, name, keep=keep)
+            if a.forms:
+                m["precision_forms"] = measure_forms(fa, v, p, fa.ROUTE_BLA, a.steps, a.warmup,
+                                                     name, m, keep[0])
+            log(json.dumps(m))
         if "film" in frames:
             log(json.dumps(measure_film(fa, get_scene(4), cols, a.steps, a.warmup, 1, 0.0)))
         return
@@ -287,8 +322,12 @@ def main():
             deep["cpu_baseline"] = cpu_baseline(v3, p3, True, cpu_s, threads)
     film = measure_film(fa, get_scene(4), cols, a.steps, a.warmup, threads, cpu_s) \
         if "film" in frames else {}
-    m0 = (measure(fa, v0, p0, fa.ROUTE_BLA, a.steps, a.warmup, "v0-2e-49")
-          if "escape" in frames else {})
+    m0 = {}
+    if "escape" in frames:
+        keep0 = []
+        m0 = measure(fa, v0, p0, fa.ROUTE_BLA, a.steps, a.warmup, "v0-2e-49", keep=keep0)
+        m0["precision_forms"] = measure_forms(fa, v0, p0, fa.ROUTE_BLA, a.steps, a.warmup,
+                                              "v0-2e-49", m0, keep0[0])
     out = {
         "metric": "effective pixel-iter/s at 1080p deep zoom (BLA-skipped steps included)",
         "value": m["value"],
@@ -315,7 +354,7 @@ def main():
             "deep_frame": deep,
             "film": film,
         },
-        "roofline": m["roofline"],
+        "roofline": dict(m["roofline"], traffic=measured_traffic(a.config)),
     }
     if not a.no_cpu_baseline:
         t = time.perf_counter()

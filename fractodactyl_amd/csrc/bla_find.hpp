@@ -1,0 +1,65 @@
+// Bla::find on the device (crates/fd-kernel/src/bla.rs:234-249), shared by the FP64 and
+// FP32 perturbation kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace fdg {
+
+// Levels whose radii one lookup step loads together.
+constexpr int kFindGroup = 4;
+
+// The highest candidate level of a lookup at k = m - 1 (k even): a level-i block starts
+// where 2^(i+1) divides k (level_top = min(trailing zeros of k, stored levels); k = 0
+// allows every level), is at most `budget` steps long (2 << i <= budget, budget >= 2), and
+// exists below the table's end (checked by the loads).  -1 when no level qualifies.
+__device__ inline int find_cap(uint32_t k, uint32_t nlevels, uint32_t budget) {
+    int top = k == 0 ? (int)nlevels : __builtin_ctz(k);
+    top = top < (int)nlevels ? top : (int)nlevels;
+    int by_budget = budget >= 2u ? 30 - __builtin_clz(budget) : -1;
+    int cap = top - 1;
+    return cap < by_budget ? cap : by_budget;
+}
+
+// Bla::find's level once level 0 is known valid (d2 < its r2): the highest level i <= cap
+// whose block exists and admits d2 < r2.  The reference walks up from level 1 and stops at
+// the first failure; validity only shrinks going up (a level-i block's radius is at most
+// its first half's, the level i-1 block at the same start, bla.rs:91-111, and blocks exist
+// for a prefix of levels), so the first failure bounds the same prefix a top-down search
+// finds.  Searching down from cap in groups of kFindGroup radii loaded together decides
+// most lookups with one memory latency: the level taken is usually within three of cap
+// (the alignment and budget allow long blocks, the radius trims the last one or two).
+//   r2: the table's squared radii in the level-major layout of BlaDev (level i at
+//   tb + bla_level_offset(stride, i)), whose padding past each level's last block holds
+//   0 (the builder writes it), so a block past the table's end simply fails the test.  A
+//   lookup happens at m <= len - 2 (k <= steps - 1), so k >> (i + 1) stays inside level
+//   i's slot of stride >> (i + 1) entries.  Indices are 32-bit (tables below 2^32 blocks).
+template <class P>
+__device__ inline int find_level_down(P r2, uint32_t tb, uint32_t stride, uint32_t k, int cap,
+                                      double d2) {
+    int lvl = 0;
+    for (int hi = cap; hi >= 1; hi -= kFindGroup) {
+        double rr[kFindGroup];
+#pragma unroll
+        for (int q = 0; q < kFindGroup; q++) {
+            int lv = hi - q;
+            int lc = lv >= 1 ? lv : 1;  // (levels below 1 load nothing)
+            uint32_t at = tb + (stride - (stride >> lc)) + (k >> (lc + 1));
+            rr[q] = lv >= 1 ? r2[at] : -1.0;
+        }
+        int got = -1;
+#pragma unroll
+        for (int q = kFindGroup - 1; q >= 0; q--)
+            got = d2 < rr[q] ? hi - q : got;
+        if (got >= 1) {
+            lvl = got;
+            break;
+        }
+    }
+    return lvl;
+}
+
+}  // namespace fdg

@@ -23,6 +23,16 @@ __device__ inline double exp2i(int64_t k) {
 // adjacent exponents is chosen.
 __device__ inline int64_t floor_log2(double a) { return (int64_t)floor(log2(a)); }
 
+// The same value without the libm call on all but a sliver of inputs.  a = f 2^e with f in
+// [0.5, 1) (subnormals included) has floor(log2(a)) = e - 1 exactly; the rounded log2 can
+// only differ where log2(a) = e + log2(f) lies within an ulp of the integer e, i.e. for f
+// within about 2^-43 of 1 (|log2(a)| <= 1075).  Those inputs (f > 1 - 2^-40) take the
+// reference's expression itself.
+__device__ inline int64_t floor_log2_fast(double a) {
+    if (__builtin_amdgcn_frexp_mant(a) > 1.0 - 0x1p-40) return floor_log2(a);
+    return (int64_t)__builtin_amdgcn_frexp_exp(a) - 1;
+}
+
 __device__ inline uint64_t next_pow2(uint64_t x) {
     if (x <= 1) return 1;
     return 1ull << (64 - __builtin_clzll(x - 1));

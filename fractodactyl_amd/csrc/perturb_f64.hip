@@ -10,6 +10,7 @@
 
 #include <cstdint>
 
+#include "bla_find.hpp"
 #include "device_math.hpp"
 #include "kernels.hpp"
 
@@ -97,28 +98,35 @@ __device__ inline void flag_valid(const BlaBuild& t, uint32_t level, bool v) {
     if (__ballot(v) && __lane_id() == 0) atomicOr(&t.valid[level], 1u);
 }
 
-// Level 0 (2-step blocks) straight from the orbit: singles at k = 1 + 2t and 2 + 2t.
+// Level 0 (2-step blocks) straight from the orbit: singles at k = 1 + 2t and 2 + 2t.  Each
+// level's slot holds stride >> (level + 1) entries; the ones past its last block are
+// written as zero blocks, so a lookup that reads past the table's end finds no valid block.
 __global__ void __launch_bounds__(256) bla_level0(const double* orbit, BlaBuild t) {
-    uint64_t cnt = t.steps >> 1;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < cnt;
+    uint64_t cnt = t.steps >> 1, slot = t.stride >> 1;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < slot;
          i += (uint64_t)gridDim.x * blockDim.x) {
-        uint64_t k = 1 + 2 * i;
-        Block x = block_single(orbit[2 * k], orbit[2 * k + 1], t.eps);
-        Block y = block_single(orbit[2 * k + 2], orbit[2 * k + 3], t.eps);
-        Block m = block_merge(x, y, t.dc_max, t.dc_exp);
+        Block m = {0, 0, 0, 0, 0, 0, 0};
+        if (i < cnt) {
+            uint64_t k = 1 + 2 * i;
+            Block x = block_single(orbit[2 * k], orbit[2 * k + 1], t.eps);
+            Block y = block_single(orbit[2 * k + 2], orbit[2 * k + 3], t.eps);
+            m = block_merge(x, y, t.dc_max, t.dc_exp);
+        }
         store_block(t, bla_level_offset(t.stride, 0) + i, m);
         flag_valid(t, 0, m.r > 0.0);
     }
 }
 
-// Level `lv` >= 1: pairs of level lv-1 blocks.
+// Level `lv` >= 1: pairs of level lv-1 blocks (zero blocks in the padding, as level 0).
 __global__ void __launch_bounds__(256) bla_level(BlaBuild t, uint32_t lv) {
-    uint64_t cnt = t.steps >> (lv + 1);
+    uint64_t cnt = t.steps >> (lv + 1), slot = t.stride >> (lv + 1);
     uint64_t src = bla_level_offset(t.stride, lv - 1), dst = bla_level_offset(t.stride, lv);
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < cnt;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < slot;
          i += (uint64_t)gridDim.x * blockDim.x) {
-        Block m = block_merge(load_block(t, src + 2 * i), load_block(t, src + 2 * i + 1),
-                              t.dc_max, t.dc_exp);
+        Block m = {0, 0, 0, 0, 0, 0, 0};
+        if (i < cnt)
+            m = block_merge(load_block(t, src + 2 * i), load_block(t, src + 2 * i + 1), t.dc_max,
+                            t.dc_exp);
         store_block(t, dst + i, m);
         flag_valid(t, lv, m.r > 0.0);
     }
@@ -227,7 +235,7 @@ __device__ inline void con_multiply(double& re_, double& im_, int32_t& ex, bool&
 __device__ inline void normalise(double& re, double& im, int32_t& e, int64_t max_e) {
     double a = fmax(fabs(re), fabs(im));
     if (a == 0.0 || !isfinite(a)) return;
-    int64_t k = floor_log2(a);
+    int64_t k = floor_log2_fast(a);
     int64_t lim = max_e - e;
     if (lim < k) k = lim;
     double h = exp2i(-(k / 2)), l = exp2i(-(k - k / 2));
@@ -303,7 +311,6 @@ struct Lane {
 };
 
 constexpr uint32_t kPending = 1, kPredicted = 2, kConStarted = 4;
-constexpr int kWalk = 4;  // BLA levels whose radii one lookup step loads together
 
 __device__ inline uint32_t np2_above(uint32_t n) {
     // next_power_of_two(n + 1) for 1 <= n < 2^31
@@ -506,8 +513,6 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         // by 2^(i+1), so an odd k (m = 0 included: k wraps) has none.
         uint32_t k = s.m - 1;
         if ((k & 1u) == 0 && (s.nucleus == 0 || s.m != 0)) {
-            int top = k == 0 ? (int)h.nlevels : __builtin_ctz(k);
-            top = top < (int)h.nlevels ? top : (int)h.nlevels;
             double d2 = EXT ? (s.xr * s.xr + s.xi * s.xi) * scale_of<EXT>(s) * scale_of<EXT>(s)
                             : s.xr * s.xr + s.xi * s.xi;
             // holds and deferred saves exist on the ordinary tiers only
@@ -518,30 +523,10 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
             uint32_t nb = s.nucleus ? (s.n == 0 ? 1u : s.nucleus + 1) : 0xffffffffu;
             b = b < nb ? b : nb;
             b = b > s.n ? b - s.n : 0;
-            // level 0 from the carried radius (0 when no such block exists)
-            if (2u <= b && d2 < s.R0) {
-                lvl = 0;
-                // The radii of the candidate levels are independent loads: issue a group
-                // of kWalk together (one L2 latency instead of a chain of them, which the
-                // wave's deepest-walking lane would set), then take the valid prefix.
-                bool go = true;
-                for (int l0 = 1; go && l0 < top; l0 += kWalk) {
-                    double rr[kWalk];
-#pragma unroll
-                    for (int q = 0; q < kWalk; q++) {
-                        int lv = l0 + q;
-                        uint32_t idx = k >> (lv + 1);
-                        bool in = lv < top && (2ull << lv) <= (uint64_t)b &&
-                                  idx < (h.steps >> (lv + 1));
-                        rr[q] = in ? h.r2[tb + bla_level_offset(h.stride, lv) + idx] : -1.0;
-                    }
-#pragma unroll
-                    for (int q = 0; q < kWalk; q++) {
-                        go = go && d2 < rr[q];
-                        lvl = go ? l0 + q : lvl;
-                    }
-                }
-            }
+            int cap = find_cap(k, h.nlevels, b);
+            // level 0 from the carried radius (0 when no such block exists), then the rest
+            if (cap >= 0 && d2 < s.R0)
+                lvl = find_level_down(h.r2, tb, (uint32_t)h.stride, k, cap, d2);
         }
     }
     double Zmr = s.Zr, Zmi = s.Zi;
@@ -808,6 +793,9 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
     // LIST: an index list (the mixed form's refinement pass) replaces 0 .. count
     const uint32_t count = LIST ? *fp->list_count : fp->count, nx = fp->nx,
                    nucleus = (uint32_t)fp->nucleus;
+    // An empty refinement list (a frame with no escaped sample): nothing to claim, and no
+    // statistics to add (count is uniform over the launch, so every thread leaves here).
+    if (LIST && count == 0) return;
     uint32_t* next = fp->next;
     Lane s;
     bool active = false;
@@ -970,11 +958,12 @@ hipError_t launch_bla_build(const double* orbit, double* r, double* r2, double* 
     uint64_t cnt = steps >> 1;
     if (cnt == 0) return hipSuccess;
     auto grid = [](uint64_t c) { return (unsigned)((c + 255) / 256 < 65536 ? (c + 255) / 256 : 65536); };
-    hipLaunchKernelGGL(bla_level0, dim3(grid(cnt)), dim3(256), 0, stream, orbit, t);
+    hipLaunchKernelGGL(bla_level0, dim3(grid(stride >> 1)), dim3(256), 0, stream, orbit, t);
     for (uint32_t lv = 1; lv < max_levels; lv++) {
         uint64_t c = steps >> (lv + 1);
         if (c == 0) break;
-        hipLaunchKernelGGL(bla_level, dim3(grid(c)), dim3(256), 0, stream, t, lv);
+        hipLaunchKernelGGL(bla_level, dim3(grid(stride >> (lv + 1))), dim3(256), 0, stream, t,
+                           lv);
     }
     return hipGetLastError();
 }

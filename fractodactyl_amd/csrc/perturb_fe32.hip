@@ -14,6 +14,7 @@
 
 #include <cstdint>
 
+#include "bla_find.hpp"
 #include "device_math.hpp"
 #include "kernels.hpp"
 
@@ -154,35 +155,14 @@ __device__ inline int advance32(Lane32& s, const Hot32& h, int32_t aexp, float& 
     if (L) {
         uint32_t k = s.m - 1;
         if ((k & 1u) == 0 && (s.nucleus == 0 || s.m != 0)) {
-            int top = k == 0 ? (int)h.nlevels : __builtin_ctz(k);
-            top = top < (int)h.nlevels ? top : (int)h.nlevels;
-            // |delta|^2 in FP64 for the validity test against the FP64 radii
             double xr = (double)s.xr, xi = (double)s.xi;
             double d2 = (xr * xr + xi * xi) * exp2i(2 * (int64_t)s.e);
             uint32_t b = h.max_iter;
             uint32_t nb = s.nucleus ? (s.n == 0 ? 1u : s.nucleus + 1) : 0xffffffffu;
             b = b < nb ? b : nb;
             b = b > s.n ? b - s.n : 0;
-            if (2u <= b && d2 < s.R0) {
-                lvl = 0;
-                bool go = true;
-                for (int l0 = 1; go && l0 < top; l0 += kWalk32) {
-                    double rr[kWalk32];
-#pragma unroll
-                    for (int q = 0; q < kWalk32; q++) {
-                        int lv = l0 + q;
-                        uint32_t idx = k >> (lv + 1);
-                        bool in = lv < top && (2ull << lv) <= (uint64_t)b &&
-                                  idx < (h.steps >> (lv + 1));
-                        rr[q] = in ? h.r2[bla_level_offset(h.stride, lv) + idx] : -1.0;
-                    }
-#pragma unroll
-                    for (int q = 0; q < kWalk32; q++) {
-                        go = go && d2 < rr[q];
-                        lvl = go ? l0 + q : lvl;
-                    }
-                }
-            }
+            int cap = find_cap(k, h.nlevels, b);
+            if (cap >= 0 && d2 < s.R0) lvl = find_level_down(h.r2, 0u, (uint32_t)h.stride, k, cap, d2);
         }
     }
     float mr, mi;  // this step's multiplier (for the contraction product), times 2^mexp
