@@ -18,6 +18,11 @@ __device__ inline double exp2i(int64_t k) {
     return ldexp(1.0, c);
 }
 
+// The same for an int32 exponent: ldexp's exponent operand is 32-bit and it rounds 2^k
+// to nearest like exp2i for every k (0 below the subnormals, inf above the range), so
+// no clamp is needed; one v_ldexp_f64.
+__device__ inline double exp2i32(int32_t k) { return ldexp(1.0, k); }
+
 // floor(log2(a)) for finite a > 0, as the reference computes it (a.log2().floor()).
 // Only ever used to pick a power-of-two rescaling, which is exact whichever of two
 // adjacent exponents is chosen.
@@ -28,9 +33,9 @@ __device__ inline int64_t floor_log2(double a) { return (int64_t)floor(log2(a));
 // only differ where log2(a) = e + log2(f) lies within an ulp of the integer e, i.e. for f
 // within about 2^-43 of 1 (|log2(a)| <= 1075).  Those inputs (f > 1 - 2^-40) take the
 // reference's expression itself.
-__device__ inline int64_t floor_log2_fast(double a) {
-    if (__builtin_amdgcn_frexp_mant(a) > 1.0 - 0x1p-40) return floor_log2(a);
-    return (int64_t)__builtin_amdgcn_frexp_exp(a) - 1;
+__device__ inline int32_t floor_log2_fast(double a) {
+    if (__builtin_amdgcn_frexp_mant(a) > 1.0 - 0x1p-40) return (int32_t)floor_log2(a);
+    return __builtin_amdgcn_frexp_exp(a) - 1;
 }
 
 __device__ inline uint64_t next_pow2(uint64_t x) {

@@ -190,7 +190,7 @@ __device__ inline void mu_multiply(double& re_, double& im_, int32_t& ex, double
     double a = fmax(fabs(nr), fabs(ni));
     if (!(a > 1e-120 && a < 1e120) && a > 0.0 && isfinite(a)) {
         int32_t e = exponent_of(a);
-        double sc = exp2i(-e);
+        double sc = exp2i32(-e);
         re_ *= sc;
         im_ *= sc;
         ex += e;
@@ -199,7 +199,7 @@ __device__ inline void mu_multiply(double& re_, double& im_, int32_t& ex, double
 
 __device__ inline bool scaled_below(double re, double im, int32_t ex, double bound) {
     return isfinite(re) && isfinite(im) &&
-           (ex < -64 || (ex <= 64 && (re * re + im * im) * exp2i(2 * (int64_t)ex) < bound));
+           (ex < -64 || (ex <= 64 && (re * re + im * im) * exp2i32(2 * ex) < bound));
 }
 
 __device__ inline void con_multiply(double& re_, double& im_, int32_t& ex, bool& started,
@@ -212,7 +212,7 @@ __device__ inline void con_multiply(double& re_, double& im_, int32_t& ex, bool&
     double a = fmax(fabs(re), fabs(im));
     if (a != 0.0 && !(kLo <= a && a <= kHi) && isfinite(a)) {
         int32_t e = exponent_of(a);
-        double sc = exp2i(-e);
+        double sc = exp2i32(-e);
         re *= sc;
         im *= sc;
         exp += e;
@@ -224,24 +224,27 @@ __device__ inline void con_multiply(double& re_, double& im_, int32_t& ex, bool&
     a = fmax(fabs(re_), fabs(im_));
     if (a != 0.0 && !(kLo <= a && a <= kHi) && isfinite(a)) {
         int32_t e = exponent_of(a);
-        double sc = exp2i(-e);
+        double sc = exp2i32(-e);
         re_ *= sc;
         im_ *= sc;
         ex += e;
     }
 }
 
-// sample.rs:586-595
-__device__ inline void normalise(double& re, double& im, int32_t& e, int64_t max_e) {
+// sample.rs:586-595, in 32-bit exponent arithmetic (exponents here stay far inside int32).
+// CAP: the delta's form, max_e = 0, so k <= -e; the derivative's max_e (i64::MAX / 4)
+// never binds.  The two half-power factors are the reference's (a single 2^-k can
+// overflow for subnormal mantissas).
+template <bool CAP>
+__device__ inline void normalise(double& re, double& im, int32_t& e) {
     double a = fmax(fabs(re), fabs(im));
     if (a == 0.0 || !isfinite(a)) return;
-    int64_t k = floor_log2_fast(a);
-    int64_t lim = max_e - e;
-    if (lim < k) k = lim;
-    double h = exp2i(-(k / 2)), l = exp2i(-(k - k / 2));
+    int32_t k = floor_log2_fast(a);
+    if (CAP && -e < k) k = -e;
+    double h = exp2i32(-(k / 2)), l = exp2i32(-(k - k / 2));
     re = re * h * l;
     im = im * h * l;
-    e = (int32_t)(e + k);
+    e = e + k;
 }
 
 // sample.rs:1094-1114 sum2 over two terms
@@ -322,11 +325,11 @@ __device__ inline uint32_t np2_above(uint32_t n) {
 // used (one v_ldexp each) rather than carried in registers.
 template <bool EXT>
 __device__ inline double scale_of(const Lane& s) {
-    return EXT ? exp2i(s.delta_exp) : 1.0;
+    return EXT ? exp2i32(s.delta_exp) : 1.0;
 }
 template <bool EXT>
 __device__ inline double inverse_of(const Lane& s) {
-    return EXT ? exp2i(-(int64_t)s.derivative_exp) : 1.0;
+    return EXT ? exp2i32(-s.derivative_exp) : 1.0;
 }
 
 // Pointers into device (global) memory.  Pointers read from a struct in memory are
@@ -406,7 +409,7 @@ __device__ inline void lane_start(Lane& s, Rare& r, const Hot& h, uint32_t nucle
 template <bool EXT>
 __device__ inline void refresh_dc(Lane& s, int32_t oe) {
     if (EXT) {
-        double factor = exp2i((int64_t)oe - s.delta_exp);
+        double factor = exp2i32(oe - s.delta_exp);
         s.dc_r = s.ar * factor;
         s.dc_i = s.ai * factor;
     }
@@ -594,9 +597,9 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
             con_multiply(r.conr, r.coni, r.cone, st, a_r, a_i);
             s.flags |= kConStarted;
         }
-        normalise(s.xr, s.xi, s.delta_exp, 0);
+        normalise<true>(s.xr, s.xi, s.delta_exp);
         if (D) {
-            normalise(s.dr, s.di, s.derivative_exp, INT64_MAX / 4);
+            normalise<false>(s.dr, s.di, s.derivative_exp);
         }
         refresh_dc<EXT>(s, oe);
         if (D) {
@@ -608,9 +611,9 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         double t = a_r * s.xr - a_i * s.xi + b_r * s.dc_r - b_i * s.dc_i;
         s.xi = a_r * s.xi + a_i * s.xr + b_r * s.dc_i + b_i * s.dc_r;
         s.xr = t;
-        normalise(s.xr, s.xi, s.delta_exp, 0);
+        normalise<true>(s.xr, s.xi, s.delta_exp);
         if (D) {
-            normalise(s.dr, s.di, s.derivative_exp, INT64_MAX / 4);
+            normalise<false>(s.dr, s.di, s.derivative_exp);
         }
         refresh_dc<EXT>(s, oe);
         s.m += blen;
@@ -673,7 +676,7 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
             else if (magnitude < kLo && magnitude != 0.0)
                 k = -32;
             if (k != 0) {
-                double factor = exp2i(-k);
+                double factor = exp2i32(-k);
                 s.xr = s.xr * factor;
                 s.xi = s.xi * factor;
                 s.delta_exp = s.delta_exp + k;
