@@ -125,6 +125,23 @@ def measured_traffic(config):
     return kib * 1024.0
 
 
+def end_to_end(fa, view, params, route, reps=2):
+    """The drop-in call as the route's caller makes it (fdg_render: host reference orbit,
+    upload, table, kernel, readback of the outcomes, host column encoding), wall time per
+    frame over `reps` calls after one warm call, with its parts from the call's stats."""
+    fa.render(view, params, route)
+    walls, parts = [], None
+    for _ in range(reps):
+        t = time.perf_counter()
+        _, _, st = fa.render(view, params, route)
+        walls.append(time.perf_counter() - t)
+        parts = {k: st[k] * 1e3 for k in ("reference_seconds", "bla_seconds", "kernel_seconds",
+                                          "transfer_seconds", "encode_seconds")}
+    w = sum(walls) / len(walls)
+    return dict(ms_per_frame=w * 1e3, value=st["equivalent"] / w,
+                parts_ms={k.replace("_seconds", ""): v for k, v in parts.items()})
+
+
 def agreement(ref, other):
     """Per-sample agreement of another delta form with the FP64 form's outcomes."""
     import numpy as np
@@ -356,6 +373,8 @@ def main():
         },
         "roofline": dict(m["roofline"], traffic=measured_traffic(a.config)),
     }
+    # PCIe- and host-inclusive: the whole fdg_render call on the headline frame
+    out["config"]["end_to_end"] = end_to_end(fa, view, params, route)
     if not a.no_cpu_baseline:
         t = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline(view, params, route == fa.ROUTE_BLA, a.cpu_seconds,

@@ -360,6 +360,7 @@ struct Hot {
     const FDG_GLOBAL v4d* ab;
     uint32_t last, max_iter, nlevels, steps;
     uint64_t stride;
+    uint32_t stride32;                  // the same (tables stay below 2^32 blocks)
     double esc2;
     int32_t oe;
     uint32_t len, count0;               // points; level-0 blocks
@@ -529,7 +530,7 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
             int cap = find_cap(k, h.nlevels, b);
             // level 0 from the carried radius (0 when no such block exists), then the rest
             if (cap >= 0 && d2 < s.R0)
-                lvl = find_level_down(h.r2, tb, (uint32_t)h.stride, k, cap, d2);
+                lvl = find_level_down(h.r2, tb, h.stride32, k, cap, d2);
         }
     }
     double Zmr = s.Zr, Zmi = s.Zi;
@@ -542,7 +543,7 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         double a_r = 0.0, a_i = 0.0, b_r = 1.0, b_i = 0.0;
         uint32_t blen = 1;
         if (blk) {
-            uint64_t at = tb + bla_level_offset(h.stride, lvl) + ((s.m - 1) >> (lvl + 1));
+            uint32_t at = tb + (h.stride32 - (h.stride32 >> lvl)) + ((s.m - 1) >> (lvl + 1));
             double4 ab = ld4(h.ab, at);
             a_r = ab.x;
             a_i = ab.y;
@@ -584,7 +585,7 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         s.skipped += blk ? blen : 0u;
         s.plain += blk ? 0u : 1u;
     } else if (lvl >= 0) {
-        uint64_t at = tb + bla_level_offset(h.stride, lvl) + ((s.m - 1) >> (lvl + 1));
+        uint32_t at = tb + (h.stride32 - (h.stride32 >> lvl)) + ((s.m - 1) >> (lvl + 1));
         double4 ab = ld4(h.ab, at);
         double a_r = ab.x, a_i = ab.y, b_r = ab.z, b_i = ab.w;
         uint32_t blen = 2u << lvl;
@@ -714,7 +715,8 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
     s.Z1r = rebase ? h.Z1r : Znext.x;
     s.Z1i = rebase ? h.Z1i : Znext.y;
     s.R0 = r0next;  // after a rebase m = 0 and no block is looked up
-    bool res = s.xr * s.xr + s.xi * s.xi > kResolvable * kResolvable * f2;
+    // |delta|^2 after the rebase select: f2 when rebased, else delta2 (same expressions)
+    bool res = (rebase ? f2 : delta2) > kResolvable * kResolvable * f2;
     double dist = fabs(fr - s.sr) + fabs(fi - s.si);
     double size = fabs(s.sr) + fabs(s.si);
     if (res && dist < 1e-3 * size) {
@@ -782,6 +784,7 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
     h.nlevels = fp->bla.nlevels;
     h.steps = (uint32_t)fp->bla.steps;
     h.stride = fp->bla.stride;
+    h.stride32 = (uint32_t)fp->bla.stride;
     h.esc2 = fp->r2;
     h.oe = (int32_t)fp->h_e;
     h.count0 = (uint32_t)(fp->bla.steps >> 1);

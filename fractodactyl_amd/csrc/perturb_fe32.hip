@@ -168,7 +168,7 @@ __device__ inline int advance32(Lane32& s, const Hot32& h, int32_t aexp, float& 
     float mr, mi;  // this step's multiplier (for the contraction product), times 2^mexp
     int32_t mexp = 0;
     if (lvl >= 0) {
-        uint64_t at = bla_level_offset(h.stride, lvl) + ((s.m - 1) >> (lvl + 1));
+        uint32_t at = ((uint32_t)h.stride - ((uint32_t)h.stride >> lvl)) + ((s.m - 1) >> (lvl + 1));
         v4f ab = h.ab[at];
         v2i ex = h.abexp[at];
         uint32_t blen = 2u << lvl;
