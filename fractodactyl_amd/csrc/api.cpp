@@ -205,7 +205,7 @@ int build_table(fdg_ctx* ctx, Buffers* bf, uint64_t len, double eps, double dc_m
             "readback");
     FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "BLA build");
     float ms = 0;
-    hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+    FDG_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]), "event timing");
     *seconds = ms * 1e-3;
     // bla.rs:442-445: levels end before the first level with no valid block.
     uint32_t n = 0;
@@ -258,7 +258,16 @@ int fdg_create(fdg_ctx** out, int device) {
         delete c;
         return FDG_EDEVICE;
     }
-    for (auto& e : c->ev) hipEventCreate(&e);
+    for (auto& e : c->ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            g_create_error = "cannot create HIP events";
+            for (auto& x : c->ev)
+                if (x) (void)hipEventDestroy(x);
+            (void)hipStreamDestroy(c->stream);
+            delete c;
+            return FDG_EDEVICE;
+        }
+    }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
     *out = c;
@@ -267,11 +276,12 @@ int fdg_create(fdg_ctx** out, int device) {
 
 void fdg_destroy(fdg_ctx* c) {
     if (!c) return;
-    hipSetDevice(c->device);
-    hipStreamSynchronize(c->stream);
+    // Teardown has no caller to report to: each step runs whatever the previous returned.
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
     for (auto& e : c->ev)
-        if (e) hipEventDestroy(e);
-    hipStreamDestroy(c->stream);
+        if (e) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -585,7 +595,7 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
     for (int q = 0; q < ST_COUNT; q++)
         cnt[q] = q == ST_MAXOPS ? std::max(cnt[q], cnt2[q]) : cnt[q] + cnt2[q];
     float ms = 0;
-    hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
+    FDG_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]), "event timing");
     s.kernel_seconds = ms * 1e-3;
     s.iterations = cnt[ST_ITERATIONS];
     s.equivalent = cnt[ST_EQUIVALENT];
@@ -631,8 +641,8 @@ int fdg_frame_read(fdg_frame* f, fdg_columns* out, fdg_outcomes* raw, fdg_stats*
 
 void fdg_frame_close(fdg_frame* f) {
     if (!f) return;
-    hipSetDevice(f->ctx->device);
-    hipStreamSynchronize(f->ctx->stream);
+    (void)hipSetDevice(f->ctx->device);  // teardown: nothing to report to
+    (void)hipStreamSynchronize(f->ctx->stream);
     f->buf.release();
     delete f;
 }

@@ -75,14 +75,14 @@ struct fdg_film {
     unsigned long long* frame_counters(uint32_t k) const {
         return counters.as<unsigned long long>() + 32 * (NGROUPS + k);
     }
-    void release() {
-        if (exec) hipGraphExecDestroy(exec);
-        if (graph) hipGraphDestroy(graph);
+    void release() {  // teardown: each step runs whatever the previous returned
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
         exec = nullptr;
         graph = nullptr;
-        if (fork) hipEventDestroy(fork);
-        if (join) hipEventDestroy(join);
-        if (side) hipStreamDestroy(side);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (side) (void)hipStreamDestroy(side);
         side = nullptr;
         fork = join = nullptr;
         buf.release();
@@ -197,7 +197,7 @@ int film_tables(fdg_film* f) {
                 "readback");
     FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "BLA build");
     float ms = 0;
-    hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+    FDG_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]), "event timing");
     f->bla_seconds = ms * 1e-3;
     for (size_t t = 0; t < valid_base.size(); t++) {
         FilmOrbit& ob = f->orb[orbit_of_table[t]];
@@ -259,8 +259,9 @@ bool deriv_of(const fdg_film* f) {
 // side stream, joined back.
 int film_capture(fdg_film* f, int grid) {
     fdg_ctx* ctx = f->ctx;
-    if (f->exec) hipGraphExecDestroy(f->exec);
-    if (f->graph) hipGraphDestroy(f->graph);
+    // a previous capture (another grid) is replaced; its teardown has nothing to report
+    if (f->exec) (void)hipGraphExecDestroy(f->exec);
+    if (f->graph) (void)hipGraphDestroy(f->graph);
     f->exec = nullptr;
     f->graph = nullptr;
     bool deriv = deriv_of(f);
@@ -282,7 +283,7 @@ int film_capture(fdg_film* f, int grid) {
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
     if (e != hipSuccess || e2 != hipSuccess) {
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g);  // the capture's own error is the one reported
         return ctx->hip(e != hipSuccess ? e : e2, "graph capture");
     }
     f->graph = g;
@@ -526,7 +527,7 @@ int fdg_film_run(fdg_film* f, int launch, int grid, fdg_stats* st) {
         for (uint32_t k = 0; k < f->nframes; k++) add_counters(cnt.data() + 32 * k + 8, &s);
     }
     float ms = 0;
-    hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
+    FDG_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]), "event timing");
     s.kernel_seconds = ms * 1e-3;
     f->last = s;
     f->ran = true;
@@ -588,9 +589,9 @@ int fdg_film_read(fdg_film* f, uint32_t k, fdg_columns* out, fdg_outcomes* raw, 
 
 void fdg_film_close(fdg_film* f) {
     if (!f) return;
-    hipSetDevice(f->ctx->device);
-    hipStreamSynchronize(f->ctx->stream);
-    if (f->side) hipStreamSynchronize(f->side);
+    (void)hipSetDevice(f->ctx->device);  // teardown: nothing to report to
+    (void)hipStreamSynchronize(f->ctx->stream);
+    if (f->side) (void)hipStreamSynchronize(f->side);
     f->release();
     delete f;
 }

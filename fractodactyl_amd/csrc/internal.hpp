@@ -41,7 +41,7 @@ struct DevBuf {
     size_t bytes = 0;
     hipError_t ensure(size_t n) {
         if (n <= bytes) return hipSuccess;
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);  // the allocation below reports any device failure
         p = nullptr;
         bytes = 0;
         hipError_t e = hipMalloc(&p, n);
@@ -53,7 +53,7 @@ struct DevBuf {
         return static_cast<T*>(p);
     }
     void release() {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);  // teardown: nothing to report to
         p = nullptr;
         bytes = 0;
     }
