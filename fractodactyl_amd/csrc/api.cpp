@@ -170,8 +170,11 @@ namespace {
 
 // Build the BLA table on the GPU over the uploaded orbit.  Returns the level count (0 when
 // no level has a valid block) in *levels.
-int build_table(fdg_ctx* ctx, Buffers* bf, uint64_t len, double eps, double dc_max,
-                int64_t dc_exp, uint32_t* levels, double* seconds) {
+// Launch the table build on the context's stream without waiting for it (*built: a
+// table exists, i.e. the orbit has at least 4 points); the level count is read once the
+// stream has been synchronised, by finish_table.
+int launch_table(fdg_ctx* ctx, Buffers* bf, uint64_t len, double eps, double dc_max,
+                 int64_t dc_exp, bool* built) {
     Buffers& B = *bf;
     if (!(eps > 0.0 && eps <= kEpsLimit && dc_max > 0.0 && std::isfinite(dc_max)))
         return ctx->fail(FDG_EINVAL,
@@ -180,7 +183,7 @@ int build_table(fdg_ctx* ctx, Buffers* bf, uint64_t len, double eps, double dc_m
     uint64_t stride = next_pow2_host(std::max<uint64_t>(steps, 2));
     B.bla_stride = stride;
     B.bla_steps = steps;
-    *levels = 0;
+    *built = false;
     if (steps < 2) return FDG_OK;
     FDG_TRY(ctx, B.bla_r.ensure(stride * 8), "hipMalloc");
     FDG_TRY(ctx, B.bla_r2.ensure(stride * 8), "hipMalloc");
@@ -198,20 +201,38 @@ int build_table(fdg_ctx* ctx, Buffers* bf, uint64_t len, double eps, double dc_m
                              kMaxLevels, ctx->stream),
             "BLA build launch");
     FDG_TRY(ctx, hipEventRecord(ctx->ev[1], ctx->stream), "event");
-    uint32_t valid[kMaxLevels];
-    FDG_TRY(ctx,
-            hipMemcpyAsync(valid, B.bla_valid.p, sizeof(valid), hipMemcpyDeviceToHost,
-                           ctx->stream),
-            "readback");
-    FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "BLA build");
+    *built = true;
+    return FDG_OK;
+}
+
+// After the stream has been synchronised: the table's level count from the per-level
+// "any valid block" flags copied to the host, and the build's device time.
+int finish_table(fdg_ctx* ctx, const Buffers& B, const uint32_t* valid, uint32_t* levels,
+                 double* seconds) {
     float ms = 0;
     FDG_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]), "event timing");
     *seconds = ms * 1e-3;
     // bla.rs:442-445: levels end before the first level with no valid block.
     uint32_t n = 0;
-    while (n < kMaxLevels && (steps >> (n + 1)) > 0 && valid[n]) n++;
+    while (n < kMaxLevels && (B.bla_steps >> (n + 1)) > 0 && valid[n]) n++;
     *levels = n;
     return FDG_OK;
+}
+
+// The table built and its level count read back (a synchronous build).
+int build_table(fdg_ctx* ctx, Buffers* bf, uint64_t len, double eps, double dc_max,
+                int64_t dc_exp, uint32_t* levels, double* seconds) {
+    bool built = false;
+    *levels = 0;
+    int rc = launch_table(ctx, bf, len, eps, dc_max, dc_exp, &built);
+    if (rc || !built) return rc;
+    uint32_t valid[kMaxLevels];
+    FDG_TRY(ctx,
+            hipMemcpyAsync(valid, bf->bla_valid.p, sizeof(valid), hipMemcpyDeviceToHost,
+                           ctx->stream),
+            "readback");
+    FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "BLA build");
+    return finish_table(ctx, *bf, valid, levels, seconds);
 }
 
 }  // namespace
@@ -467,14 +488,15 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
     s.reference_bits = f->bits;
     s.nucleus_period = f->nucleus;
     s.transfer_seconds = f->transfer_seconds;
-    uint32_t levels = 0;
+    // The table is built on the stream and the kernel reads its level count from the
+    // frame descriptor, where a one-thread kernel writes it from the build's flags: no host
+    // round trip between the build and the perturbation kernel.  The host learns the count
+    // with the statistics at the end.
+    bool built = false;
     if (f->route == FDG_ROUTE_BLA) {
-        int rc = build_table(ctx, &f->buf, f->len, f->eps, f->dc, f->dc_exp, &levels,
-                             &s.bla_seconds);
+        int rc = launch_table(ctx, &f->buf, f->len, f->eps, f->dc, f->dc_exp, &built);
         if (rc) return rc;
     }
-    s.bla_levels = levels;
-    f->use_bla = levels > 0;
     Buffers& b = f->buf;
     // The kernel writes every output field of every sample; only the counters start at 0.
     FDG_TRY(ctx, launch_zero_words(b.counters.as<unsigned long long>(), 32, ctx->stream),
@@ -490,7 +512,7 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
     d.bla.ab = b.bla_ab.as<double>();
     d.bla.steps = b.bla_steps;
     d.bla.stride = b.bla_stride;
-    d.bla.nlevels = f->use_bla ? levels : 0;
+    d.bla.nlevels = 0;  // written on the device from the build's flags (built tables)
     d.c_re = pl.c_re;
     d.c_im = pl.c_im;
     d.h = pl.h();
@@ -522,11 +544,16 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
     if (grid < 1) grid = 1;
     FDG_TRY(ctx, hipMemcpyAsync(b.fdev.p, &d, sizeof(d), hipMemcpyHostToDevice, ctx->stream),
             "frame upload");
+    auto count_levels = [&](DevBuf& fdev) {
+        return launch_count_levels(b.bla_valid.as<uint32_t>(), b.bla_steps, kMaxLevels,
+                                   fdev.as<FrameDev>(), ctx->stream);
+    };
+    if (built) FDG_TRY(ctx, count_levels(b.fdev), "level count");
     bool mixed = f->form == FDG_FORM_MIXED;
     bool fe32 = f->form == FDG_FORM_FE32 || mixed;
     if (fe32) {
         // FP32 table coefficients (A, B), same padded layout
-        if (f->use_bla) {
+        if (built) {
             FDG_TRY(ctx, b.ab32.ensure(b.bla_stride * 16), "hipMalloc");
             FDG_TRY(ctx, b.abexp32.ensure(b.bla_stride * 8), "hipMalloc");
             FDG_TRY(ctx,
@@ -541,17 +568,18 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
         FDG_TRY(ctx,
                 hipMemcpyAsync(b.fdev.p, &d, sizeof(d), hipMemcpyHostToDevice, ctx->stream),
                 "frame upload");
+        if (built) FDG_TRY(ctx, count_levels(b.fdev), "level count");
     }
     FDG_TRY(ctx, hipEventRecord(ctx->ev[2], ctx->stream), "event");
     if (fe32)
         FDG_TRY(ctx,
-                launch_perturb_fe32(b.fdev.as<FrameDev>(), deriv, f->use_bla, grid,
+                launch_perturb_fe32(b.fdev.as<FrameDev>(), deriv, built, grid,
                                     ctx->stream),
                 "perturbation launch");
     else
         FDG_TRY(ctx,
                 launch_perturb_f64(b.fdev.as<FrameDev>(), f->fr.tier == TIER_SCALED, deriv,
-                                   f->use_bla, false, grid, ctx->stream),
+                                   built, false, grid, ctx->stream),
                 "perturbation launch");
     unsigned long long cnt2[ST_COUNT];
     std::memset(cnt2, 0, sizeof(cnt2));
@@ -578,9 +606,10 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
         FDG_TRY(ctx,
                 hipMemcpyAsync(b.fdev2.p, &d2, sizeof(d2), hipMemcpyHostToDevice, ctx->stream),
                 "frame upload");
+        if (built) FDG_TRY(ctx, count_levels(b.fdev2), "level count");
         FDG_TRY(ctx,
                 launch_perturb_f64(b.fdev2.as<FrameDev>(), f->fr.tier == TIER_SCALED, deriv,
-                                   f->use_bla, false, grid, ctx->stream, true),
+                                   built, false, grid, ctx->stream, true),
                 "refinement launch");
         FDG_TRY(ctx,
                 hipMemcpyAsync(cnt2, d2.stats, sizeof(cnt2), hipMemcpyDeviceToHost, ctx->stream),
@@ -591,7 +620,27 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
     FDG_TRY(ctx,
             hipMemcpyAsync(cnt, d.stats, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream),
             "readback");
+    uint32_t valid[kMaxLevels];
+    if (built)
+        FDG_TRY(ctx,
+                hipMemcpyAsync(valid, b.bla_valid.p, sizeof(valid), hipMemcpyDeviceToHost,
+                               ctx->stream),
+                "readback");
     FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "perturbation kernel");
+    uint32_t levels = 0;
+    if (built) {
+        int rc = finish_table(ctx, b, valid, &levels, &s.bla_seconds);
+        if (rc) return rc;
+    }
+    s.bla_levels = levels;
+    f->use_bla = levels > 0;
+    if (built && levels == 0) {
+        // A table without a valid block: the kernel ran its BLA form with no level to
+        // search, which counts every sample as a fallback and the closed-form ones apart;
+        // report what the plain kernel (fd_kernel::render_with) reports.
+        cnt[ST_FALLBACK] = cnt2[ST_FALLBACK] = 0;
+        cnt[ST_CLOSED] = cnt2[ST_CLOSED] = 0;
+    }
     for (int q = 0; q < ST_COUNT; q++)
         cnt[q] = q == ST_MAXOPS ? std::max(cnt[q], cnt2[q]) : cnt[q] + cnt2[q];
     float ms = 0;

@@ -21,6 +21,9 @@ hipError_t launch_bla_build(const double* orbit, double* r, double* r2, double* 
 hipError_t launch_perturb_f64(const FrameDev* f, bool ext, bool deriv, bool bla, bool batch,
                               int grid, hipStream_t stream, bool list = false);
 hipError_t launch_zero_words(unsigned long long* p, uint32_t n, hipStream_t stream);
+// f->bla.nlevels from a table build's per-level "any valid block" flags (bla.rs:442-445).
+hipError_t launch_count_levels(const uint32_t* valid, uint64_t steps, uint32_t max_levels,
+                               FrameDev* f, hipStream_t stream);
 hipError_t launch_orbit_to_f32(const double* orbit, float* o32, uint64_t points,
                                hipStream_t stream);
 hipError_t launch_table_to_f32(const double* ab, float* ab32, int32_t* abexp, uint64_t blocks,

@@ -132,6 +132,71 @@ __global__ void __launch_bounds__(256) bla_level(BlaBuild t, uint32_t lv) {
     }
 }
 
+// Ten levels per launch: a workgroup builds one span of 512 blocks of level lv0 (two per
+// thread: singles merged straight from the orbit for level 0, pairs of the stored level
+// lv0-1 blocks above it) and merges them up through LDS to the single level lv0+9 block
+// over the span, storing every level on the way.  Each merge takes the same two blocks
+// as the level-by-level build, so the table is the same bit for bit, from one launch per
+// ten levels instead of one per level.  Blocks past a level's last one are stored as
+// zero blocks (as in bla_level0 / bla_level), also where both halves exist.
+constexpr int kSpanThreads = 256;
+constexpr int kSpan = 2 * kSpanThreads;  // blocks of level lv0 per workgroup
+constexpr int kSpanLevels = 10;          // levels lv0 .. lv0 + 9 per launch
+
+__global__ void __launch_bounds__(kSpanThreads) bla_span(const double* orbit, BlaBuild t,
+                                                          uint32_t lv0, uint32_t lv_end) {
+    __shared__ Block lds[kSpan];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t slot0 = t.stride >> (lv0 + 1), nspans = (slot0 + kSpan - 1) / kSpan;
+    for (uint64_t span = blockIdx.x; span < nspans; span += gridDim.x) {
+        // level lv0: blocks 2 tid and 2 tid + 1 of the span
+        const uint64_t cnt0 = t.steps >> (lv0 + 1);
+        bool any = false;
+        for (int h = 0; h < 2; h++) {
+            uint64_t i = span * kSpan + 2 * tid + h;
+            Block m = {0, 0, 0, 0, 0, 0, 0};
+            if (i < cnt0) {
+                if (lv0 == 0) {
+                    uint64_t k = 1 + 2 * i;
+                    Block x = block_single(orbit[2 * k], orbit[2 * k + 1], t.eps);
+                    Block y = block_single(orbit[2 * k + 2], orbit[2 * k + 3], t.eps);
+                    m = block_merge(x, y, t.dc_max, t.dc_exp);
+                } else {
+                    uint64_t src = bla_level_offset(t.stride, lv0 - 1);
+                    m = block_merge(load_block(t, src + 2 * i), load_block(t, src + 2 * i + 1),
+                                    t.dc_max, t.dc_exp);
+                }
+            }
+            if (i < slot0) store_block(t, bla_level_offset(t.stride, lv0) + i, m);
+            any = any || m.r > 0.0;
+            lds[2 * tid + h] = m;
+        }
+        flag_valid(t, lv0, any);
+        // levels lv0 + 1 .. : pairs of the level below, through LDS
+        for (uint32_t lv = lv0 + 1; lv < lv_end; lv++) {
+            const uint32_t j = lv - lv0;            // 1 .. 9
+            const uint32_t n = kSpan >> j;          // blocks of this level in the span
+            const uint64_t cnt = t.steps >> (lv + 1), slot = t.stride >> (lv + 1);
+            __syncthreads();
+            Block m = {0, 0, 0, 0, 0, 0, 0};
+            Block x = m, y = m;
+            if (tid < n) {
+                x = lds[2 * tid];
+                y = lds[2 * tid + 1];
+            }
+            __syncthreads();
+            uint64_t i = (span * kSpan >> j) + tid;
+            if (tid < n) {
+                if (i < cnt) m = block_merge(x, y, t.dc_max, t.dc_exp);
+                if (i < slot) store_block(t, bla_level_offset(t.stride, lv) + i, m);
+                lds[tid] = m;
+            }
+            flag_valid(t, lv, tid < n && m.r > 0.0);
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------------ perturbation
 
 constexpr double kHi = 4294967296.0;
@@ -963,13 +1028,15 @@ hipError_t launch_bla_build(const double* orbit, double* r, double* r2, double* 
     BlaBuild t{r, r2, ab, alpha, beta, steps, stride, eps, dc_max, dc_exp, valid};
     uint64_t cnt = steps >> 1;
     if (cnt == 0) return hipSuccess;
-    auto grid = [](uint64_t c) { return (unsigned)((c + 255) / 256 < 65536 ? (c + 255) / 256 : 65536); };
-    hipLaunchKernelGGL(bla_level0, dim3(grid(stride >> 1)), dim3(256), 0, stream, orbit, t);
-    for (uint32_t lv = 1; lv < max_levels; lv++) {
-        uint64_t c = steps >> (lv + 1);
-        if (c == 0) break;
-        hipLaunchKernelGGL(bla_level, dim3(grid(stride >> (lv + 1))), dim3(256), 0, stream, t,
-                           lv);
+    // Levels exist while steps >> (lv + 1) > 0; each launch builds up to ten of them.
+    uint32_t top = 0;
+    while (top < max_levels && (steps >> (top + 1)) > 0) top++;
+    for (uint32_t lv0 = 0; lv0 < top; lv0 += kSpanLevels) {
+        uint32_t lv_end = lv0 + kSpanLevels < top ? lv0 + kSpanLevels : top;
+        uint64_t spans = ((stride >> (lv0 + 1)) + kSpan - 1) / kSpan;
+        unsigned grid = (unsigned)(spans < 65536 ? spans : 65536);
+        hipLaunchKernelGGL(bla_span, dim3(grid), dim3(kSpanThreads), 0, stream, orbit, t, lv0,
+                           lv_end);
     }
     return hipGetLastError();
 }
@@ -977,6 +1044,21 @@ hipError_t launch_bla_build(const double* orbit, double* r, double* r2, double* 
 __global__ void zero_words(unsigned long long* p, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
+}
+
+// Levels end before the first level without a valid block (bla.rs:442-445).
+__global__ void count_levels(const uint32_t* valid, uint64_t steps, uint32_t max_levels,
+                             FrameDev* f) {
+    if (threadIdx.x != 0) return;
+    uint32_t n = 0;
+    while (n < max_levels && (steps >> (n + 1)) > 0 && valid[n]) n++;
+    f->bla.nlevels = n;
+}
+
+hipError_t launch_count_levels(const uint32_t* valid, uint64_t steps, uint32_t max_levels,
+                               FrameDev* f, hipStream_t stream) {
+    hipLaunchKernelGGL(count_levels, dim3(1), dim3(64), 0, stream, valid, steps, max_levels, f);
+    return hipGetLastError();
 }
 
 hipError_t launch_zero_words(unsigned long long* p, uint32_t n, hipStream_t stream) {
