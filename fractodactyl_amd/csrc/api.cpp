@@ -104,27 +104,40 @@ int read_encode(fdg_ctx* ctx, const Buffers& b, size_t off, size_t n, const Plan
                 double* encode_s, std::vector<uint64_t>* n_out, std::vector<uint8_t>* kind_out,
                 std::vector<uint32_t>* ops_out) {
     double t1 = now_s();
-    std::vector<uint8_t> kind(n);
-    std::vector<uint64_t> nn(n);
-    std::vector<double> zr(n), zi(n), dr(n), di(n);
-    std::vector<int64_t> dexp(n);
-    std::vector<uint32_t> ops(n);
+    // Outcomes land in the context's pinned staging area (page-locked host memory: the
+    // copies run at full PCIe rate and the encoder reads them in place), one
+    // asynchronous copy per array on the context's stream and one synchronisation.
+    bool want_ops = (raw && raw->ops) || ops_out;
+    size_t kbytes = (n + 7) & ~(size_t)7;
+    size_t need = kbytes + 6 * 8 * n + (want_ops ? 4 * n : 0);
+    int rc = ctx->staging(need);
+    if (rc) return rc;
+    uint8_t* base = static_cast<uint8_t*>(ctx->pinned);
+    uint8_t* kind = base;
+    uint64_t* nn = reinterpret_cast<uint64_t*>(base + kbytes);
+    double* zr = reinterpret_cast<double*>(nn + n);
+    double* zi = zr + n;
+    double* dr = zi + n;
+    double* di = dr + n;
+    int64_t* dexp = reinterpret_cast<int64_t*>(di + n);
+    uint32_t* ops = want_ops ? reinterpret_cast<uint32_t*>(dexp + n) : nullptr;
     auto get = [&](void* dst, const DevBuf& src, size_t elem) {
-        return hipMemcpy(dst, static_cast<const uint8_t*>(src.p) + off * elem, n * elem,
-                         hipMemcpyDeviceToHost);
+        return hipMemcpyAsync(dst, static_cast<const uint8_t*>(src.p) + off * elem, n * elem,
+                              hipMemcpyDeviceToHost, ctx->stream);
     };
-    FDG_TRY(ctx, get(kind.data(), b.o_kind, 1), "readback");
-    FDG_TRY(ctx, get(nn.data(), b.o_n, 8), "readback");
-    FDG_TRY(ctx, get(zr.data(), b.o_zr, 8), "readback");
-    FDG_TRY(ctx, get(zi.data(), b.o_zi, 8), "readback");
-    FDG_TRY(ctx, get(dr.data(), b.o_dr, 8), "readback");
-    FDG_TRY(ctx, get(di.data(), b.o_di, 8), "readback");
-    FDG_TRY(ctx, get(dexp.data(), b.o_dexp, 8), "readback");
-    FDG_TRY(ctx, get(ops.data(), b.o_ops, 4), "readback");
+    FDG_TRY(ctx, get(kind, b.o_kind, 1), "readback");
+    FDG_TRY(ctx, get(nn, b.o_n, 8), "readback");
+    FDG_TRY(ctx, get(zr, b.o_zr, 8), "readback");
+    FDG_TRY(ctx, get(zi, b.o_zi, 8), "readback");
+    FDG_TRY(ctx, get(dr, b.o_dr, 8), "readback");
+    FDG_TRY(ctx, get(di, b.o_di, 8), "readback");
+    FDG_TRY(ctx, get(dexp, b.o_dexp, 8), "readback");
+    if (want_ops) FDG_TRY(ctx, get(ops, b.o_ops, 4), "readback");
+    FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "readback");
     *transfer_s += now_s() - t1;
-    if (n_out) *n_out = nn;
-    if (kind_out) *kind_out = kind;
-    if (ops_out) *ops_out = ops;
+    if (n_out) n_out->assign(nn, nn + n);
+    if (kind_out) kind_out->assign(kind, kind + n);
+    if (ops_out) ops_out->assign(ops, ops + n);
     // The device holds the stop iterate of Unresolved samples too; Outcome::Unresolved
     // carries none (sample.rs:21-40).
     for (size_t i = 0; i < n; i++)
@@ -150,14 +163,14 @@ int read_encode(fdg_ctx* ctx, const Buffers& b, size_t off, size_t n, const Plan
     for (auto& t : th) t.join();
     *encode_s = now_s() - t2;
     if (raw) {
-        if (raw->kind) std::memcpy(raw->kind, kind.data(), n);
-        if (raw->n) std::memcpy(raw->n, nn.data(), n * 8);
-        if (raw->zr) std::memcpy(raw->zr, zr.data(), n * 8);
-        if (raw->zi) std::memcpy(raw->zi, zi.data(), n * 8);
-        if (raw->dr) std::memcpy(raw->dr, dr.data(), n * 8);
-        if (raw->di) std::memcpy(raw->di, di.data(), n * 8);
-        if (raw->dexp) std::memcpy(raw->dexp, dexp.data(), n * 8);
-        if (raw->ops) std::memcpy(raw->ops, ops.data(), n * 4);
+        if (raw->kind) std::memcpy(raw->kind, kind, n);
+        if (raw->n) std::memcpy(raw->n, nn, n * 8);
+        if (raw->zr) std::memcpy(raw->zr, zr, n * 8);
+        if (raw->zi) std::memcpy(raw->zi, zi, n * 8);
+        if (raw->dr) std::memcpy(raw->dr, dr, n * 8);
+        if (raw->di) std::memcpy(raw->di, di, n * 8);
+        if (raw->dexp) std::memcpy(raw->dexp, dexp, n * 8);
+        if (raw->ops) std::memcpy(raw->ops, ops, n * 4);
     }
     return FDG_OK;
 }
@@ -302,6 +315,7 @@ void fdg_destroy(fdg_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    if (c->pinned) (void)hipHostFree(c->pinned);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }

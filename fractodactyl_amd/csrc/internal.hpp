@@ -75,10 +75,23 @@ struct fdg_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::string err;
+    void* pinned = nullptr;  // grow-only page-locked staging for outcome readback
+    size_t pinned_bytes = 0;
 
     int fail(int code, const std::string& m) {
         err = m;
         return code;
+    }
+    // At least n bytes of pinned staging (hipHostMalloc'd: page-locked from the start).
+    int staging(size_t n) {
+        if (n <= pinned_bytes) return 0;
+        if (pinned) (void)hipHostFree(pinned);  // the allocation below reports failures
+        pinned = nullptr;
+        pinned_bytes = 0;
+        hipError_t e = hipHostMalloc(&pinned, n, hipHostMallocDefault);
+        if (e != hipSuccess) return hip(e, "hipHostMalloc staging");
+        pinned_bytes = n;
+        return 0;
     }
     int hip(hipError_t e, const char* what) {
         if (e == hipSuccess) return FDG_OK;
