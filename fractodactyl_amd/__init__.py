@@ -118,6 +118,11 @@ SIGNATURES = {
     "fdg_frame_read": (C.c_int, [C.c_void_p, C.POINTER(_Columns), C.POINTER(_Outcomes),
                                  C.POINTER(_Stats), C.c_char_p, C.c_size_t]),
     "fdg_frame_close": (None, [C.c_void_p]),
+    "fdg_frame_mask": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fdg_render_masked": (C.c_int, [C.c_void_p, C.POINTER(_View), C.POINTER(_Params), C.c_int,
+                                    C.c_int, C.c_double, C.c_void_p, C.POINTER(_Columns),
+                                    C.POINTER(_Outcomes), C.POINTER(_Stats), C.c_char_p,
+                                    C.c_size_t]),
     "fdg_film_open": (C.c_int, [C.c_void_p, C.POINTER(_View), C.c_uint32, C.POINTER(_Params),
                                 C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_void_p)]),
     "fdg_film_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(_Stats)]),
@@ -289,6 +294,14 @@ class ResidentFrame:
         self.ctx._check(lib().fdg_frame_run(self._h, C.byref(st)))
         return {k: getattr(st, k) for k, _ in _Stats._fields_}
 
+    def mask(self, mask):
+        """Compute only the samples whose entry is set in the following runs (None: all):
+        fd_kernel::render_with_masked's mask (crates/fd-kernel/src/grid.rs:173-181)."""
+        self._mask = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+        if self._mask is not None and self._mask.size != self.params.nx * self.params.ny:
+            raise FdgError(-1, "sample mask must cover the sample grid")
+        self.ctx._check(lib().fdg_frame_mask(self._h, _p(self._mask)))
+
     def read(self, raw=False):
         cols, cs, oc = _out_buffers(self.params, raw)
         st = _Stats()
@@ -363,6 +376,27 @@ class Film:
 
     def __exit__(self, *a):
         self.close()
+
+
+def render_masked(view, params, mask, route=ROUTE_PLAIN, form=FORM_F64, bla_eps=EPS, ctx=None,
+                  raw=False):
+    """fd_kernel::render_with_masked / render_bla_masked (crates/fd-kernel/src/grid.rs:
+    173-209): only the samples whose mask entry is set are computed; the others are left
+    unset (zero columns; raw kind 255)."""
+    ctx = ctx or default_context()
+    m = np.ascontiguousarray(mask, np.uint8)
+    if m.size != params.nx * params.ny:
+        raise FdgError(-1, "sample mask must cover the sample grid")
+    cols, cs, oc = _out_buffers(params, raw)
+    st = _Stats()
+    kid = C.create_string_buffer(256)
+    ctx._check(lib().fdg_render_masked(ctx._h, C.byref(_view(view)), C.byref(_params(params)),
+                                       route, form, bla_eps, _p(m), C.byref(cs),
+                                       C.byref(oc) if oc is not None else None, C.byref(st),
+                                       kid, 256))
+    header = dict(view=view, params=params, kernel=kid.value.decode(),
+                  columns=params.columns | COL_CLASS)
+    return header, cols, {k: getattr(st, k) for k, _ in _Stats._fields_}
 
 
 def render_bla(view, params, form=FORM_F64, bla_eps=EPS, ctx=None, raw=False):

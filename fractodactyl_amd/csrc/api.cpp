@@ -140,8 +140,16 @@ int read_encode(fdg_ctx* ctx, const Buffers& b, size_t off, size_t n, const Plan
     if (ops_out) ops_out->assign(ops, ops + n);
     // The device holds the stop iterate of Unresolved samples too; Outcome::Unresolved
     // carries none (sample.rs:21-40).
-    for (size_t i = 0; i < n; i++)
+    for (size_t i = 0; i < n; i++) {
         if (kind[i] == KIND_UNRESOLVED) nn[i] = 0;
+        if (kind[i] == kKindUnset) {
+            // outside a sample mask: left unset, as render_with_masked leaves it
+            nn[i] = 0;
+            zr[i] = zi[i] = dr[i] = di[i] = 0.0;
+            dexp[i] = 0;
+            if (ops) ops[i] = 0;
+        }
+    }
     double t2 = now_s();
     Encoder enc(pl, p->ss);
     double* nu = (p->columns & FDG_COL_NU) ? out->nu : nullptr;
@@ -156,6 +164,10 @@ int read_encode(fdg_ctx* ctx, const Buffers& b, size_t off, size_t n, const Plan
                 if (nu) nu[i] = 0.0;
                 if (de) de[i] = 0.0f;
                 if (nm) nm[i] = 0;
+                if (kind[i] == kKindUnset) {
+                    out->class_[i] = 0;
+                    continue;
+                }
                 enc.put(i, kind[i], nn[i], zr[i], zi[i], dr[i], di[i], dexp[i], out->class_, nu,
                         de, nm);
             }
@@ -265,6 +277,8 @@ struct fdg_frame {
     int64_t dc_exp = 0;
     double reference_seconds = 0, transfer_seconds = 0;
     bool use_bla = false, ran = false;
+    bool masked = false;  // a sample mask is set (fdg_frame_mask)
+    DevBuf mask;          // 1 byte per sample on the device
     fdg_stats last;
     Buffers buf;
 };
@@ -542,7 +556,11 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
     d.tier = f->fr.tier;
     d.max_iter = p->max_iter;
     d.r2 = p->escape_radius * p->escape_radius;
-    d.mask = nullptr;
+    d.mask = f->masked ? f->mask.as<uint8_t>() : nullptr;
+    if (f->masked)
+        // samples outside the mask keep this "not computed" kind through the pass, which
+        // the mixed form's escaped list and the encoder both skip
+        FDG_TRY(ctx, hipMemsetAsync(b.o_kind.p, kKindUnset, f->count, ctx->stream), "fill");
     d.kind = b.o_kind.as<uint8_t>();
     d.n = b.o_n.as<uint64_t>();
     d.zr = b.o_zr.as<double>();
@@ -612,6 +630,7 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
                                     f->count, ctx->stream),
                 "escaped list");
         FrameDev d2 = d;
+        d2.mask = nullptr;  // the list holds only computed samples; it is indexed by position
         d2.list = b.list.as<uint32_t>();
         d2.list_count = listed;
         d2.defer_escaped = 0;
@@ -702,11 +721,41 @@ int fdg_frame_read(fdg_frame* f, fdg_columns* out, fdg_outcomes* raw, fdg_stats*
     return FDG_OK;
 }
 
+int fdg_frame_mask(fdg_frame* f, const uint8_t* mask) {
+    if (!f) return FDG_EINVAL;
+    fdg_ctx* ctx = f->ctx;
+    FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    f->masked = mask != nullptr;
+    if (!mask) return FDG_OK;
+    FDG_TRY(ctx, f->mask.ensure(f->count), "hipMalloc");
+    FDG_TRY(ctx, hipMemcpyAsync(f->mask.p, mask, f->count, hipMemcpyHostToDevice, ctx->stream),
+            "mask upload");
+    FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "mask upload");
+    return FDG_OK;
+}
+
+int fdg_render_masked(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int route, int form,
+                      double bla_eps, const uint8_t* mask, fdg_columns* out, fdg_outcomes* raw,
+                      fdg_stats* st, char* kernel_id, size_t kernel_cap) {
+    if (!ctx) return FDG_EINVAL;
+    if (!out || !out->class_) return ctx->fail(FDG_EINVAL, "the class column is required");
+    if (!mask) return ctx->fail(FDG_EINVAL, "sample mask must cover the sample grid");
+    fdg_frame* f = nullptr;
+    int rc = fdg_frame_open(ctx, v, p, route, form, bla_eps, &f);
+    if (rc) return rc;
+    rc = fdg_frame_mask(f, mask);
+    if (!rc) rc = fdg_frame_run(f, nullptr);
+    if (!rc) rc = fdg_frame_read(f, out, raw, st, kernel_id, kernel_cap);
+    fdg_frame_close(f);
+    return rc;
+}
+
 void fdg_frame_close(fdg_frame* f) {
     if (!f) return;
     (void)hipSetDevice(f->ctx->device);  // teardown: nothing to report to
     (void)hipStreamSynchronize(f->ctx->stream);
     f->buf.release();
+    f->mask.release();
     delete f;
 }
 

@@ -34,6 +34,8 @@ hipError_t launch_perturb_fe32(const FrameDev* f, bool deriv, bool bla, int grid
                                hipStream_t stream);
 
 constexpr uint32_t kMaxLevels = 48;
+// Outcome kind of a sample outside a sample mask (not computed; its columns stay unset).
+constexpr uint8_t kKindUnset = 0xff;
 constexpr double kEpsLimit = 1.0 / 16777216.0;  // bla.rs:288-292 EPS_LIMIT = 2^-24
 
 extern std::string g_create_error;

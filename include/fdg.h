@@ -152,6 +152,19 @@ int fdg_frame_read(fdg_frame *frame, fdg_columns *out, fdg_outcomes *raw, fdg_st
                    char *kernel_id, size_t kernel_cap);
 void fdg_frame_close(fdg_frame *frame);
 
+/* fd_kernel::render_with_masked / render_bla_masked (crates/fd-kernel/src/grid.rs:173-209):
+ * compute only the samples whose mask entry (one byte per sample, row-major over the
+ * sample grid, non-zero = compute) is set; the others are left unset — class 0, nu 0, de 0,
+ * normal 0 — and their raw outcome reads kind 255 with zeros.  The statistics count the
+ * computed samples only.  fdg_frame_mask sets the mask for the following runs of a
+ * resident frame (NULL clears it), so the two passes of a jet-AA frame
+ * (crates/fd-cli/src/jetaa.rs:1-45) share one orbit and table upload; fdg_render_masked is
+ * the one-call form. */
+int fdg_frame_mask(fdg_frame *frame, const uint8_t *mask);
+int fdg_render_masked(fdg_ctx *ctx, const fdg_view *view, const fdg_params *params, int route,
+                      int form, double bla_eps, const uint8_t *mask, fdg_columns *out,
+                      fdg_outcomes *raw, fdg_stats *stats, char *kernel_id, size_t kernel_cap);
+
 /* Films: a batch of frames at one centre (the keyframes of a zoom) rendered against ONE
  * reference orbit, as the reference renders a frame with a supplied deeper orbit:
  * fd_kernel::render_with / render_bla with `Some((orbit, bits))` (crates/fd-kernel/src/
