@@ -245,6 +245,68 @@ def measure_film(fa, sc, cols, steps, warmup, threads, cpu_seconds):
     return out
 
 
+# The reference's series-initializer frames (docs/2026-10-20/series-frames/results/
+# frames.txt): one centre on the PROB-22 zoom path, 24 widths from 5e-15 to 2.4e-29.
+SERIES_CENTRE = ("-0.8296874999999996981089858669738646984522311568651036619209",
+                 "0.1859374999999989417838556112859064221737945709381783514212")
+SERIES_WIDTHS = ["5.0118723e-15", "1.2589254e-15", "3.1622777e-16", "7.9432823e-17",
+                 "1.9952623e-17", "5.0118723e-18", "1.2589254e-18", "3.1622777e-19",
+                 "7.9432823e-20", "1.9952623e-20", "5.0118723e-21", "1.2589254e-21",
+                 "3.1622777e-22", "7.9432823e-23", "1.9952623e-23", "5.0118723e-24",
+                 "1.2589254e-24", "3.1622777e-25", "7.9432823e-26", "1.9952623e-26",
+                 "4.216965e-27", "7.4989421e-28", "1.3335214e-28", "2.3713737e-29"]
+
+
+def measure_series(fa, cols, reps=3):
+    """The four arms of the reference's series study (plain, bla, series, series-bla) on
+    its 24 frames at 480x270, max_iter 1e5, on the GPU: per arm the median device time
+    (table build + kernel) per frame, the host series build, and the series arms'
+    agreement with the plain render (class mismatches, worst nu shift in output pixels
+    where de > 1e-3, fd compare's contract)."""
+    import numpy as np
+    rows, tot = [], {k: 0.0 for k in ("plain", "bla", "series", "series_bla")}
+    for w in SERIES_WIDTHS:
+        v = fa.View(SERIES_CENTRE[0], SERIES_CENTRE[1], w, 0.0)
+        p = fa.Params(480, 270, 1, 100_000, 1e10, cols)
+        re, im, _ = fa.reference(v, p)
+        t = time.perf_counter()
+        sr = fa.series_build(re, im, fa.bla_dc_max(v, p), float(w) / p.nx, p.max_iter)
+        build_ms = (time.perf_counter() - t) * 1e3
+        row = dict(width=w, series_n=sr.n, orders=len(sr.b), stop=sr.stop,
+                   series_build_ms=build_ms)
+        outs = {}
+        for arm, route, use in (("plain", fa.ROUTE_PLAIN, False), ("bla", fa.ROUTE_BLA, False),
+                                ("series", fa.ROUTE_PLAIN, True),
+                                ("series_bla", fa.ROUTE_BLA, True)):
+            with fa.ResidentFrame(v, p, route) as f:
+                if use and sr.n > 0:
+                    f.series(sr)
+                times = []
+                for _ in range(reps):
+                    st = f.run()
+                    times.append(st["kernel_seconds"] + st["bla_seconds"])
+                _, c, _ = f.read()
+            outs[arm] = c
+            ms = float(np.median(times)) * 1e3
+            row[arm + "_ms"] = ms
+            tot[arm] += ms
+        a, de = outs["plain"]["nu"], outs["plain"]["de"].astype(np.float64)
+        sel = de > 1e-3
+        for arm in ("series", "series_bla"):
+            b = outs[arm]["nu"]
+            row[arm + "_class_mismatch"] = int((outs[arm]["class"] != outs["plain"]["class"])
+                                              .sum())
+            shift = np.abs(a - b) * de * math.log(2) / 2.0
+            row[arm + "_worst_nu_px"] = float(shift[sel].max()) if sel.any() else 0.0
+        log(json.dumps(row))
+        rows.append(row)
+    tot["series_build"] = sum(r["series_build_ms"] for r in rows)
+    return dict(workload="24 frames of the reference's series study (frames.txt), 480x270, "
+                         "max_iter 1e5, nu/de/normal; device ms = table build + kernel, "
+                         f"median of {reps}",
+                total_ms=tot, frames=rows)
+
+
 def copy_probe(nbytes=1 << 28, reps=3):
     """A plain device-to-device copy of known size (torch's copy kernel): profiling its
     FETCH_SIZE / WRITE_SIZE counters gives the scale that turns those counters into HBM
@@ -275,7 +337,8 @@ def main():
     ap.add_argument("--frames", default="config,escape,deep,film",
                     help="which frames to run: config (the JSON line); companions escape "
                          "(reference zoom path at 2e-49), deep (1e-1000, 1M iterations), "
-                         "film (64 keyframes sharing one orbit)")
+                         "film (64 keyframes sharing one orbit); series (the reference's "
+                         "24 series-study frames in four arms, companion mode only)")
     ap.add_argument("--forms", action="store_true",
                     help="with companions only: also run the FE32 and mixed forms")
     ap.add_argument("--copy-probe", action="store_true",
@@ -317,6 +380,8 @@ def main():
             log(json.dumps(m))
         if "film" in frames:
             log(json.dumps(measure_film(fa, get_scene(4), cols, a.steps, a.warmup, 1, 0.0)))
+        if "series" in frames:
+            log(json.dumps(measure_series(fa, cols)))
         return
     keep = []
     m = measure(fa, view, params, route, a.steps, a.warmup, f"config{a.config}", keep=keep)

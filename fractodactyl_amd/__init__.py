@@ -96,6 +96,23 @@ class _Stats(C.Structure):
                 ("encode_seconds", C.c_double)]
 
 
+class _SeriesConfig(C.Structure):
+    _fields_ = [("start_order", C.c_uint32), ("max_order", C.c_uint32),
+                ("grow_at", C.c_double), ("shift_px", C.c_double),
+                ("derivative_rel", C.c_double)]
+
+
+class _Series(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("rho", C.c_double), ("points", C.c_uint64),
+                ("b", C.c_void_p), ("b_cap", C.c_uint32), ("b_len", C.c_uint32),
+                ("state_bound", C.c_double), ("derivative_bound", C.c_double),
+                ("shift_px", C.c_double), ("built_steps", C.c_uint64),
+                ("coefficient_pairs", C.c_uint64), ("growth_steps", C.c_uint64),
+                ("stop", C.c_int32)]
+
+
+SERIES_STOPS = ("reference", "radius", "non-finite", "escape", "budget", "stall")
+
 # Every entry point of include/fdg.h with its ctypes signature.
 SIGNATURES = {
     "fdg_abi_version": (C.c_int, []),
@@ -123,6 +140,16 @@ SIGNATURES = {
                                     C.c_int, C.c_double, C.c_void_p, C.POINTER(_Columns),
                                     C.POINTER(_Outcomes), C.POINTER(_Stats), C.c_char_p,
                                     C.c_size_t]),
+    "fdg_series_config_default": (None, [C.POINTER(_SeriesConfig)]),
+    "fdg_series_build": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_double, C.c_double,
+                                   C.c_uint64, C.POINTER(_SeriesConfig), C.POINTER(_Series)]),
+    "fdg_bla_dc_max": (C.c_int, [C.POINTER(_View), C.POINTER(_Params),
+                                 C.POINTER(C.c_double)]),
+    "fdg_frame_series": (C.c_int, [C.c_void_p, C.POINTER(_Series)]),
+    "fdg_render_series": (C.c_int, [C.c_void_p, C.POINTER(_View), C.POINTER(_Params), C.c_int,
+                                    C.c_double, C.POINTER(_Series), C.c_void_p,
+                                    C.POINTER(_Columns), C.POINTER(_Outcomes),
+                                    C.POINTER(_Stats), C.c_char_p, C.c_size_t]),
     "fdg_film_open": (C.c_int, [C.c_void_p, C.POINTER(_View), C.c_uint32, C.POINTER(_Params),
                                 C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_void_p)]),
     "fdg_film_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(_Stats)]),
@@ -302,6 +329,16 @@ class ResidentFrame:
             raise FdgError(-1, "sample mask must cover the sample grid")
         self.ctx._check(lib().fdg_frame_mask(self._h, _p(self._mask)))
 
+    def series(self, series):
+        """Start every sample of the following runs from `series` (a Series; None clears
+        it): Series::render_masked (crates/fd-kernel/src/series.rs:342-374)."""
+        self._series = series
+        if series is None:
+            self.ctx._check(lib().fdg_frame_series(self._h, None))
+        else:
+            st, self._series_b = series._struct()
+            self.ctx._check(lib().fdg_frame_series(self._h, C.byref(st)))
+
     def read(self, raw=False):
         cols, cs, oc = _out_buffers(self.params, raw)
         st = _Stats()
@@ -392,6 +429,92 @@ def render_masked(view, params, mask, route=ROUTE_PLAIN, form=FORM_F64, bla_eps=
     kid = C.create_string_buffer(256)
     ctx._check(lib().fdg_render_masked(ctx._h, C.byref(_view(view)), C.byref(_params(params)),
                                        route, form, bla_eps, _p(m), C.byref(cs),
+                                       C.byref(oc) if oc is not None else None, C.byref(st),
+                                       kid, 256))
+    header = dict(view=view, params=params, kernel=kid.value.decode(),
+                  columns=params.columns | COL_CLASS)
+    return header, cols, {k: getattr(st, k) for k, _ in _Stats._fields_}
+
+
+class Series:
+    """An admitted parameter-series start (fd_kernel::Series, crates/fd-kernel/src/
+    series.rs:57-82): every sample with |dc| <= rho starts at iterate n with delta
+    sum b[k] (dc / rho)^k.  Built on the host by series_build."""
+
+    def __init__(self, n, rho, points, b, state_bound=0.0, derivative_bound=0.0,
+                 shift_px=0.0, built_steps=0, coefficient_pairs=0, growth_steps=0,
+                 stop="reference"):
+        self.n, self.rho, self.points = int(n), float(rho), int(points)
+        self.b = np.asarray(b, complex)
+        self.state_bound, self.derivative_bound = state_bound, derivative_bound
+        self.shift_px, self.built_steps = shift_px, built_steps
+        self.coefficient_pairs, self.growth_steps, self.stop = (coefficient_pairs,
+                                                                growth_steps, stop)
+
+    def _struct(self):
+        b = np.ascontiguousarray(np.stack([self.b.real, self.b.imag], axis=1).reshape(-1))
+        st = _Series(n=self.n, rho=self.rho, points=self.points, b=_p(b), b_cap=len(self.b),
+                     b_len=len(self.b), state_bound=self.state_bound,
+                     derivative_bound=self.derivative_bound, shift_px=self.shift_px)
+        return st, b
+
+    def replace(self, **kw):
+        d = dict(self.__dict__)
+        d.update(kw)
+        return Series(**d)
+
+
+def series_config_default():
+    """SeriesConfig::default (series.rs:43-55) as a dict."""
+    c = _SeriesConfig()
+    lib().fdg_series_config_default(C.byref(c))
+    return {k: getattr(c, k) for k, _ in _SeriesConfig._fields_}
+
+
+def series_build(re, im, rho, pixel, max_iter, config=None):
+    """Series::build (series.rs:88-234) over an orbit, on the host (fdg_series_build)."""
+    re = np.ascontiguousarray(re, np.float64)
+    im = np.ascontiguousarray(im, np.float64)
+    cfg = dict(series_config_default(), **(config or {}))
+    c = _SeriesConfig(**cfg)
+    cap = max(int(cfg["max_order"]), 1)
+    buf = np.zeros(2 * cap)
+    out = _Series(b=_p(buf), b_cap=cap)
+    rc = lib().fdg_series_build(_p(re), _p(im), len(re), rho, pixel, max_iter, C.byref(c),
+                                C.byref(out))
+    if rc != 0:
+        raise FdgError(rc, "series build: bad arguments")
+    b = buf[:2 * out.b_len].reshape(-1, 2)
+    return Series(out.n, out.rho, out.points, b[:, 0] + 1j * b[:, 1], out.state_bound,
+                  out.derivative_bound, out.shift_px, out.built_steps, out.coefficient_pairs,
+                  out.growth_steps, SERIES_STOPS[out.stop])
+
+
+def bla_dc_max(view, params):
+    """fd_kernel::bla_dc_max (grid.rs:432-440): the frame's |dc| bound (not on the scaled
+    tier)."""
+    d = C.c_double()
+    rc = lib().fdg_bla_dc_max(C.byref(_view(view)), C.byref(_params(params)), C.byref(d))
+    if rc != 0:
+        raise FdgError(rc, "bla_dc_max: invalid view, or the scaled tier")
+    return d.value
+
+
+def render_series(view, params, series, route=ROUTE_PLAIN, bla_eps=EPS, mask=None, ctx=None,
+                  raw=False):
+    """fd_kernel::render_series / Series::render_masked (grid.rs:218-226, series.rs:342-374)
+    on the GPU: every sample starts from the series state; unresolved ones retry from
+    zero."""
+    ctx = ctx or default_context()
+    m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+    if m is not None and m.size != params.nx * params.ny:
+        raise FdgError(-1, "sample mask must cover the sample grid")
+    cols, cs, oc = _out_buffers(params, raw)
+    st = _Stats()
+    kid = C.create_string_buffer(256)
+    sst, keep = series._struct()
+    ctx._check(lib().fdg_render_series(ctx._h, C.byref(_view(view)), C.byref(_params(params)),
+                                       route, bla_eps, C.byref(sst), _p(m), C.byref(cs),
                                        C.byref(oc) if oc is not None else None, C.byref(st),
                                        kid, 256))
     header = dict(view=view, params=params, kernel=kid.value.decode(),

@@ -279,6 +279,10 @@ struct fdg_frame {
     bool use_bla = false, ran = false;
     bool masked = false;  // a sample mask is set (fdg_frame_mask)
     DevBuf mask;          // 1 byte per sample on the device
+    bool has_series = false;  // a series start is set (fdg_frame_series), maybe with n = 0
+    uint32_t series_n = 0, series_len = 0;
+    double series_inv_rho = 0;
+    DevBuf series_b;      // its coefficients, (re, im) pairs
     fdg_stats last;
     Buffers buf;
 };
@@ -571,6 +575,13 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
     d.ops = b.o_ops.as<uint32_t>();
     d.next = b.counters.as<uint32_t>();
     d.stats = reinterpret_cast<unsigned long long*>(b.counters.as<uint8_t>() + 64);
+    bool series = f->has_series && f->series_n > 0;
+    if (series) {
+        d.series_b = f->series_b.as<double>();
+        d.series_n = f->series_n;
+        d.series_len = f->series_len;
+        d.series_inv_rho = f->series_inv_rho;
+    }
     bool deriv = (p->columns & (FDG_COL_DE | FDG_COL_NORMAL)) != 0;
     int grid = (int)std::min<uint64_t>((f->count + 255) / 256, (uint64_t)ctx->cus * 8);
     if (grid < 1) grid = 1;
@@ -607,6 +618,10 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
         FDG_TRY(ctx,
                 launch_perturb_fe32(b.fdev.as<FrameDev>(), deriv, built, grid,
                                     ctx->stream),
+                "perturbation launch");
+    else if (series)
+        FDG_TRY(ctx, launch_perturb_f64_series(b.fdev.as<FrameDev>(), deriv, built, grid,
+                                               ctx->stream),
                 "perturbation launch");
     else
         FDG_TRY(ctx,
@@ -714,6 +729,7 @@ int fdg_frame_read(fdg_frame* f, fdg_columns* out, fdg_outcomes* raw, fdg_stats*
     if (kernel_id && kernel_cap) {
         std::string k = f->fr.pl.kernel(f->fr.tier);
         if (f->use_bla) k += " bla/1";
+        if (f->has_series) k += " series/0";  // series.rs:371-372
         k += f->form == FDG_FORM_FE32 ? " gpu-hip-fe32/1"
              : f->form == FDG_FORM_MIXED ? " gpu-hip-mixed/1" : " gpu-hip-f64/1";
         std::snprintf(kernel_id, kernel_cap, "%s", k.c_str());
@@ -756,7 +772,115 @@ void fdg_frame_close(fdg_frame* f) {
     (void)hipStreamSynchronize(f->ctx->stream);
     f->buf.release();
     f->mask.release();
+    f->series_b.release();
     delete f;
+}
+
+void fdg_series_config_default(fdg_series_config* cfg) {
+    if (!cfg) return;
+    SeriesConfig c;
+    cfg->start_order = c.start_order;
+    cfg->max_order = c.max_order;
+    cfg->grow_at = c.grow_at;
+    cfg->shift_px = c.shift_px;
+    cfg->derivative_rel = c.derivative_rel;
+}
+
+int fdg_series_build(const double* re, const double* im, uint64_t len, double rho, double pixel,
+                     uint64_t max_iter, const fdg_series_config* cfg, fdg_series* out) {
+    if (!re || !im || !out || len < 1) return FDG_EINVAL;
+    SeriesConfig c;
+    if (cfg) {
+        c.start_order = cfg->start_order;
+        c.max_order = cfg->max_order;
+        c.grow_at = cfg->grow_at;
+        c.shift_px = cfg->shift_px;
+        c.derivative_rel = cfg->derivative_rel;
+    }
+    if (!out->b || out->b_cap < std::max<uint32_t>(c.max_order, 1)) return FDG_EINVAL;
+    Series sr = Series::build(re, im, len, rho, pixel, max_iter, c);
+    out->n = sr.n;
+    out->rho = sr.rho;
+    out->points = sr.points;
+    out->b_len = (uint32_t)(sr.b.size() / 2);
+    if (!sr.b.empty()) std::memcpy(out->b, sr.b.data(), sr.b.size() * sizeof(double));
+    out->state_bound = sr.state_bound;
+    out->derivative_bound = sr.derivative_bound;
+    out->shift_px = sr.shift_px;
+    out->built_steps = sr.built_steps;
+    out->coefficient_pairs = sr.coefficient_pairs;
+    out->growth_steps = sr.growth_steps;
+    out->stop = sr.stop;
+    return FDG_OK;
+}
+
+int fdg_bla_dc_max(const fdg_view* v, const fdg_params* p, double* dc_max) {
+    if (!v || !p || !dc_max) return FDG_EINVAL;
+    if (p->ss == 0 || p->nx % p->ss || p->ny % p->ss) return FDG_EINVAL;
+    Plane pl;
+    std::string e;
+    if (!Plane::make(v->center_re, v->center_im, v->width, v->rotation, p->nx, p->ny, &pl, &e))
+        return FDG_EINVAL;
+    Tier t = p->tier < 0 ? pl.tier : (Tier)p->tier;
+    if (t == TIER_SCALED) return FDG_EINVAL;  // BLA tables need f64 deltas
+    int64_t de;
+    dc_bound(pl, t, p->nx, p->ny, dc_max, &de);
+    return FDG_OK;
+}
+
+int fdg_frame_series(fdg_frame* f, const fdg_series* sr) {
+    if (!f) return FDG_EINVAL;
+    fdg_ctx* ctx = f->ctx;
+    if (!sr) {
+        f->has_series = false;
+        return FDG_OK;
+    }
+    // series.rs:342-366 render_masked's refusals
+    if (f->fr.tier == TIER_SCALED)
+        return ctx->fail(FDG_EINVAL, "a series start needs f64 deltas and no Bound column");
+    if (f->form != FDG_FORM_F64)
+        return ctx->fail(FDG_EDECLINED, "a series start is served by the FP64 form");
+    if (sr->points != f->len) return ctx->fail(FDG_EINVAL, "series built over another orbit");
+    if (std::isnan(f->dc) || f->dc > sr->rho) {
+        char m[160];
+        std::snprintf(m, sizeof(m), "view reaches |dc| = %e, beyond the series radius %e",
+                      f->dc, sr->rho);
+        return ctx->fail(FDG_EINVAL, m);
+    }
+    if (sr->n > 0 && (sr->n + 1 >= f->len || sr->b_len == 0 || !sr->b))
+        return ctx->fail(FDG_EINVAL, "series start outside its orbit");
+    FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    if (sr->n > 0) {
+        size_t bytes = (size_t)sr->b_len * 2 * sizeof(double);
+        FDG_TRY(ctx, f->series_b.ensure(bytes), "hipMalloc");
+        FDG_TRY(ctx,
+                hipMemcpyAsync(f->series_b.p, sr->b, bytes, hipMemcpyHostToDevice, ctx->stream),
+                "series upload");
+        FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "series upload");
+    }
+    f->has_series = true;
+    f->series_n = (uint32_t)sr->n;
+    f->series_len = sr->b_len;
+    f->series_inv_rho = 1.0 / sr->rho;
+    return FDG_OK;
+}
+
+int fdg_render_series(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int route,
+                      double bla_eps, const fdg_series* sr, const uint8_t* mask,
+                      fdg_columns* out, fdg_outcomes* raw, fdg_stats* st, char* kernel_id,
+                      size_t kernel_cap) {
+    if (!ctx) return FDG_EINVAL;
+    if (!out || !out->class_) return ctx->fail(FDG_EINVAL, "the class column is required");
+    if (!sr) return ctx->fail(FDG_EINVAL, "no series");
+    fdg_frame* f = nullptr;
+    int rc = fdg_frame_open(ctx, v, p, route, FDG_FORM_F64, bla_eps, &f);
+    if (rc) return rc;
+    rc = fdg_frame_series(f, sr);
+    if (!rc && mask) rc = fdg_frame_mask(f, mask);
+    if (!rc) rc = fdg_frame_run(f, nullptr);
+    if (!rc) rc = fdg_frame_read(f, out, raw, st, kernel_id, kernel_cap);
+    fdg_frame_close(f);
+    return rc;
 }
 
 int fdg_render(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int route, int form,

@@ -2,6 +2,7 @@
 // floating-point expression keeps the reference's operation order.
 #include "host.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -462,6 +463,142 @@ void dc_bound(const Plane& pl, Tier tier, uint32_t nx, uint32_t ny, double* dc, 
         *dc = pl.h() * half;
         *e = 0;
     }
+}
+
+// ------------------------------------------------------------------ series
+
+// The series recurrence on the coefficients' own arrays: with dc = rho x the delta after
+// n steps is S_n(x) = sum b[k] x^k, b[1]' = 2 Z b[1] + rho, b[k]' = 2 Z b[k] + (S^2)[k],
+// keeping `active` orders; E and F bound the state and d/dx error of what is dropped
+// (series.rs:8-24).  Each sum runs in the reference's order, so the coefficients and
+// bounds are its bits.
+Series Series::build(const double* re, const double* im, uint64_t len, double rho,
+                     double pixel, uint64_t max_iter, const SeriesConfig& cfg,
+                     uint64_t stalled_steps) {
+    const uint32_t k_max = std::max<uint32_t>(cfg.max_order, 1);
+    uint32_t active = std::min(std::max<uint32_t>(cfg.start_order, 1), k_max);
+    std::vector<double> br(k_max + 1, 0.0), bi(k_max + 1, 0.0);
+    std::vector<double> sq_r(2 * k_max + 1, 0.0), sq_i(2 * k_max + 1, 0.0);
+    Series out;
+    out.rho = rho;
+    out.points = len;
+    const uint64_t limit = std::min(len >= 2 ? len - 2 : 0, max_iter >= 1 ? max_iter - 1 : 0);
+    // the last admitted step: n, coefficients, E, F, their budgets and Dmin
+    bool have = false;
+    uint64_t best_n = 0;
+    std::vector<double> best;
+    double best_e = 0, best_f = 0, budget_e = 0, budget_f = 0, best_dmin = 0;
+    if (!(rho > 0.0 && std::isfinite(rho) && pixel > 0.0)) {
+        out.stop = STOP_RADIUS;
+        return out;
+    }
+    const double cap_e = cfg.shift_px * pixel * 2.0 / rho, cap_f = cfg.derivative_rel * 2.0;
+    double e = 0.0, f = 0.0;
+    for (uint64_t n = 0; n < limit; n++) {
+        double zr = re[n], zi = im[n];
+        const double az = hypot_ref(zr, zi);
+        std::fill(sq_r.begin(), sq_r.begin() + 2 * active + 1, 0.0);
+        std::fill(sq_i.begin(), sq_i.begin() + 2 * active + 1, 0.0);
+        for (uint32_t i = 1; i <= active; i++) {
+            const double pr = br[i], pi = bi[i];
+            if (pr == 0.0 && pi == 0.0) continue;
+            sq_r[2 * i] += pr * pr - pi * pi;
+            sq_i[2 * i] += 2.0 * pr * pi;
+            const double tr = 2.0 * pr, ti = 2.0 * pi;
+            for (uint32_t j = i + 1; j <= active; j++) {
+                sq_r[i + j] += tr * br[j] - ti * bi[j];
+                sq_i[i + j] += tr * bi[j] + ti * br[j];
+            }
+        }
+        out.coefficient_pairs += (uint64_t)active * (active + 1) / 2;
+        double m = 0.0, d = 0.0, t = 0.0, td = 0.0;
+        for (uint32_t k = 1; k <= active; k++) {
+            const double a = hypot_ref(br[k], bi[k]);
+            m += a;
+            d += (double)k * a;
+        }
+        for (uint32_t k = active + 1; k <= 2 * active; k++) {
+            const double a = hypot_ref(sq_r[k], sq_i[k]);
+            t += a;
+            td += (double)k * a;
+        }
+        const double e_next = 2.0 * (az + m) * e + e * e + t;
+        f = 2.0 * (az + m + e) * f + 2.0 * e * d + td;
+        e = e_next;
+        for (uint32_t k = 1; k <= active; k++) {
+            const double xr = br[k], xi = bi[k];
+            br[k] = 2.0 * (zr * xr - zi * xi) + sq_r[k];
+            bi[k] = 2.0 * (zr * xi + zi * xr) + sq_i[k];
+        }
+        br[1] += rho;
+        const uint64_t step = n + 1;
+        out.built_steps = step;
+        zr = re[step];
+        zi = im[step];
+        double tail_d = 0.0;
+        m = 0.0;
+        for (uint32_t k = 1; k <= active; k++) {
+            const double a = hypot_ref(br[k], bi[k]);
+            m += a;
+            if (k >= 2) tail_d += (double)k * a;
+        }
+        if (!(std::isfinite(e) && std::isfinite(f))) {
+            out.stop = STOP_NONFINITE;
+            break;
+        }
+        if (hypot_ref(zr, zi) + m + e >= 2.0) {  // no sample may leave the escape disk
+            out.stop = STOP_ESCAPE;
+            break;
+        }
+        const double dmin = hypot_ref(br[1], bi[1]) - tail_d - f;
+        const double budget = dmin > 0.0 ? cfg.shift_px * pixel * dmin / rho : 0.0;
+        if (dmin > 0.0 && e <= budget && f <= cfg.derivative_rel * dmin) {
+            have = true;
+            best_n = step;
+            best.resize(2 * active);
+            for (uint32_t k = 1; k <= active; k++) {
+                best[2 * (k - 1)] = br[k];
+                best[2 * (k - 1) + 1] = bi[k];
+            }
+            best_e = e;
+            best_f = f;
+            budget_e = budget;
+            budget_f = cfg.derivative_rel * dmin;
+            best_dmin = dmin;
+        } else if (e > cap_e || f > cap_f) {
+            // |b1| <= M < 2 caps every later budget: nothing past here can pass
+            out.stop = STOP_BUDGET;
+            break;
+        }
+        if (active < k_max && e > cfg.grow_at * budget) {
+            active = std::min(2 * active, k_max);
+            out.growth_steps++;
+        }
+        if (stalled_steps > 0 && have && step - best_n >= stalled_steps) {
+            out.stop = STOP_STALL;
+            break;
+        }
+    }
+    if (have) {
+        // drop high orders whose norms fit in half the remaining slack of each bound
+        const double slack_e = budget_e - best_e, slack_f = budget_f - best_f;
+        double te = 0.0, tf = 0.0;
+        size_t kept = best.size() / 2;
+        while (kept > 1) {
+            const double a = hypot_ref(best[2 * (kept - 1)], best[2 * (kept - 1) + 1]);
+            if (te + a > 0.5 * slack_e || tf + (double)kept * a > 0.5 * slack_f) break;
+            te += a;
+            tf += (double)kept * a;
+            kept--;
+        }
+        best.resize(2 * kept);
+        out.n = best_n;
+        out.b = std::move(best);
+        out.state_bound = best_e + te;
+        out.derivative_bound = best_f + tf;
+        out.shift_px = out.state_bound * rho / (best_dmin * pixel);
+    }
+    return out;
 }
 
 // ------------------------------------------------------------------ encoder

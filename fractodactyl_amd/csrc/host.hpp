@@ -78,6 +78,34 @@ struct Orbit {
 // Upper bound m 2^e on |dc| over a view's samples (grid.rs:423-444).
 void dc_bound(const Plane& pl, Tier tier, uint32_t nx, uint32_t ny, double* dc, int64_t* e);
 
+// Guard thresholds and order schedule of the parameter series (series.rs:28-55 SeriesConfig,
+// its Default).
+struct SeriesConfig {
+    uint32_t start_order = 8, max_order = 64;
+    double grow_at = 1e-4, shift_px = 1e-8, derivative_rel = 1e-6;
+};
+
+// Why a series build stopped (series.rs:57-82 `stop`).
+enum SeriesStop : int32_t { STOP_REFERENCE = 0, STOP_RADIUS, STOP_NONFINITE, STOP_ESCAPE,
+                            STOP_BUDGET, STOP_STALL };
+
+// An admitted series start (series.rs:57-82 Series): every sample with |dc| <= rho starts
+// at iterate n with delta S_n(dc / rho) = sum b[k] (dc / rho)^k.
+struct Series {
+    uint64_t n = 0;
+    double rho = 0;
+    uint64_t points = 0;
+    std::vector<double> b;  // b[1..] as (re, im) pairs
+    double state_bound = 0, derivative_bound = 0, shift_px = 0;
+    uint64_t built_steps = 0, coefficient_pairs = 0, growth_steps = 0;
+    int32_t stop = STOP_REFERENCE;
+    // Series::build_stalled (series.rs:99-234) over an orbit of `len` points, for samples
+    // with |dc| <= rho and output pixel `pixel`; stalled_steps 32 is Series::build.
+    static Series build(const double* re, const double* im, uint64_t len, double rho,
+                        double pixel, uint64_t max_iter, const SeriesConfig& cfg,
+                        uint64_t stalled_steps = 32);
+};
+
 // Raw per-sample result of the kernel (sample.rs:21-40).
 enum Kind : uint8_t { KIND_ESCAPED = 0, KIND_INTERIOR = 1, KIND_UNRESOLVED = 2 };
 

@@ -30,6 +30,10 @@ hipError_t launch_table_to_f32(const double* ab, float* ab32, int32_t* abexp, ui
                                hipStream_t stream);
 hipError_t launch_escaped_list(const uint8_t* kind, uint32_t* list, uint32_t* count, uint32_t n,
                                hipStream_t stream);
+// The FP64 kernel on the ordinary tiers with every sample starting from a series state
+// (FrameDev::series_*); unresolved series starts retry from zero in the same launch.
+hipError_t launch_perturb_f64_series(const FrameDev* f, bool deriv, bool bla, int grid,
+                                     hipStream_t stream);
 hipError_t launch_perturb_fe32(const FrameDev* f, bool deriv, bool bla, int grid,
                                hipStream_t stream);
 

@@ -83,6 +83,12 @@ struct FrameDev {
                                     // recomputed in FP64, so they add no statistics here
     const uint32_t* list;           // nullable: compute only samples list[0 .. *list_count)
     const uint32_t* list_count;
+    // Series start (the series kernels only, ordinary tiers): every sample starts at
+    // iterate series_n with delta sum b[k] x^k, x = dc / rho (series.rs:236-257), b[1..]
+    // as series_len (re, im) pairs.
+    const double* series_b;
+    uint32_t series_n, series_len;
+    double series_inv_rho;          // 1 / rho, as Series::eval computes it
     // batches only (the single-frame kernels read the geometry above)
     const BatchFrameDev* frames;    // nframes entries
     uint32_t nframes, per_frame;    // count = nframes * per_frame

@@ -379,6 +379,8 @@ struct Lane {
 };
 
 constexpr uint32_t kPending = 1, kPredicted = 2, kConStarted = 4;
+// series_carry's mark of a sample running from the series start
+constexpr uint32_t kSeriesTry = 0xffffffffu;
 
 __device__ inline uint32_t np2_above(uint32_t n) {
     // next_power_of_two(n + 1) for 1 <= n < 2^31
@@ -481,6 +483,55 @@ __device__ inline void refresh_dc(Lane& s, int32_t oe) {
     }
 }
 
+
+// A supplied start on the ordinary tiers (sample.rs:186-228 with Start from grid.rs:663-673):
+// the series state at the sample's offset (Series::eval, series.rs:236-257: Horner on
+// x = dc / rho for p = sum b[k] x^(k-1) and q = p', then S = x p and dz/dc = (p + x q) / rho),
+// continued from iterate n with the state as the first Brent save, saves resuming at the
+// next power of two, and no nucleus contraction (it would miss the skipped factors).
+// Called after lane_start.
+template <bool D, bool L>
+__device__ inline void series_start(Lane& s, const Hot& h, const FrameDev* __restrict__ fp) {
+    const FDG_GLOBAL v2d* b = global_ptr(reinterpret_cast<const v2d*>(fp->series_b));
+    const double sc = fp->series_inv_rho;
+    const double xr = s.ar * sc, xi = s.ai * sc;
+    double pr = 0.0, pi = 0.0, qr = 0.0, qi = 0.0;
+    for (uint32_t k = fp->series_len; k-- > 0;) {
+        double2 c = ld2(b, k);
+        if (D) {
+            double tr = qr * xr - qi * xi + pr;
+            qi = qr * xi + qi * xr + pi;
+            qr = tr;
+        }
+        double tr = pr * xr - pi * xi + c.x;
+        pi = pr * xi + pi * xr + c.y;
+        pr = tr;
+    }
+    s.xr = pr * xr - pi * xi;
+    s.xi = pr * xi + pi * xr;
+    if (D) {
+        double dr = pr + qr * xr - qi * xi, di = pi + qr * xi + qi * xr;
+        s.dr = dr * sc;
+        s.di = di * sc;
+    }
+    const uint32_t m = fp->series_n;  // 1 <= m <= len - 2
+    double2 zm = ld2(h.orbit, m), z1 = ld2(h.orbit, m + 1);
+    s.m = m;
+    s.n = m;
+    s.Zr = zm.x;
+    s.Zi = zm.y;
+    s.Z1r = z1.x;
+    s.Z1i = z1.y;
+    // the level-0 radius of the block starting at m, when one does (k = m - 1 even)
+    uint32_t k0 = m - 1;
+    s.R0 = (L && (k0 & 1u) == 0 && (k0 >> 1) < h.count0) ? h.r2[k0 >> 1] : 0.0;
+    s.sr = zm.x + s.xr;
+    s.si = zm.y + s.xi;
+    s.sn = m;
+    uint32_t chk = np2_above(m);
+    s.chk = chk > 16u ? chk : 16u;
+    s.nucleus = 0;
+}
 
 enum : int { RUNNING = -1 };
 
@@ -821,9 +872,13 @@ __device__ inline uint32_t lanes_below(uint64_t mask) {
 // are those of a batch of frames sharing the orbit (frame-major), each frame with its own
 // geometry and BLA table, so one launch drains every frame and no frame's tail idles
 // the GPU.
-template <bool EXT, bool D, bool L, bool B, bool LIST = false>
+template <bool EXT, bool D, bool L, bool B, bool LIST = false, bool SER = false>
 __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restrict__ fp) {
     __shared__ RareLds rare;
+    // SER (ordinary tiers, single frames): per lane, kSeriesTry while the sample runs from
+    // the series start, else the iterates its unresolved series attempt spent (grid.rs:
+    // 623-649: that attempt's work counts, minus the skipped prefix, and it retries from 0)
+    __shared__ uint32_t series_carry[SER ? kBlock : 1];
     // Statistics accumulate in LDS at each sample's end (rare next to the steps), not in
     // per-lane registers held across the loop.
     __shared__ unsigned long long stat[ST_COUNT];
@@ -962,6 +1017,11 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
                         lane_start<EXT>(s, r, h, nucleus, ux * fp->h_m, uy * fp->h_m, h.oe);
                         active = true;
                     }
+                    if (SER && active) {
+                        // (the series kernels run the ordinary tiers only)
+                        series_start<D, L>(s, h, fp);
+                        series_carry[threadIdx.x] = kSeriesTry;
+                    }
                 }
             }
         }
@@ -976,6 +1036,26 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
             if (active) {
                 double fr = 0.0, fi = 0.0;
                 int k = lane_advance<EXT, D, L, B>(s, r, h, fr, fi);
+                uint32_t carried = 0, prefix = 0;
+                if (SER && k != RUNNING) {
+                    carried = series_carry[threadIdx.x];
+                    if (carried == kSeriesTry) {
+                        carried = 0;
+                        prefix = fp->series_n;
+                        if (k == 2) {
+                            // an unresolved series start retries from zero, where the
+                            // nucleus heuristic the start declined applies again; the
+                            // work counters run on (one Skip per sample)
+                            series_carry[threadIdx.x] = h.max_iter - prefix;
+                            uint32_t bl = s.blocks, sk = s.skipped, pl = s.plain;
+                            lane_start<EXT>(s, r, h, nucleus, s.ar, s.ai, 0);
+                            s.blocks = bl;
+                            s.skipped = sk;
+                            s.plain = pl;
+                            k = RUNNING;
+                        }
+                    }
+                }
                 if (k != RUNNING) {
                     // Every field is written (zeros where the outcome has none), so the
                     // output buffers need no clearing between passes.
@@ -988,7 +1068,7 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
                     o_dr[idx] = esc ? s.dr : 0.0;
                     o_di[idx] = esc ? s.di : 0.0;
                     o_dexp[idx] = esc ? s.derivative_exp : 0;
-                    unsigned long long spent = k == 2 ? h.max_iter : s.n;
+                    unsigned long long spent = (k == 2 ? h.max_iter : s.n) - prefix + carried;
                     atomicAdd(&stat[ST_EQUIVALENT], spent);
                     atomicAdd(&stat[ST_ITERATIONS], L ? spent - s.skipped + s.blocks : spent);
                     if (L) {
@@ -1063,6 +1143,21 @@ hipError_t launch_count_levels(const uint32_t* valid, uint64_t steps, uint32_t m
 
 hipError_t launch_zero_words(unsigned long long* p, uint32_t n, hipStream_t stream) {
     hipLaunchKernelGGL(zero_words, dim3((n + 255) / 256), dim3(256), 0, stream, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_perturb_f64_series(const FrameDev* f, bool deriv, bool bla, int grid,
+                                     hipStream_t stream) {
+#define FDG_LAUNCH_SER(D, L)                                                              \
+    hipLaunchKernelGGL((perturb_f64<false, D, L, false, false, true>), dim3(grid),          \
+                       dim3(kBlock), 0, stream, f)
+    switch ((deriv ? 2 : 0) | (bla ? 1 : 0)) {
+        case 0: FDG_LAUNCH_SER(false, false); break;
+        case 1: FDG_LAUNCH_SER(false, true); break;
+        case 2: FDG_LAUNCH_SER(true, false); break;
+        default: FDG_LAUNCH_SER(true, true); break;
+    }
+#undef FDG_LAUNCH_SER
     return hipGetLastError();
 }
 

@@ -165,6 +165,52 @@ int fdg_render_masked(fdg_ctx *ctx, const fdg_view *view, const fdg_params *para
                       int form, double bla_eps, const uint8_t *mask, fdg_columns *out,
                       fdg_outcomes *raw, fdg_stats *stats, char *kernel_id, size_t kernel_cap);
 
+/* The parameter-series initializer (crates/fd-kernel/src/series.rs; fd_kernel::Series,
+ * render_series at grid.rs:218-226).  With dc = rho x, |x| <= 1 over the frame, the delta
+ * after n steps is S_n(x) = sum b[k] x^k; fdg_series_build runs Series::build (the
+ * growing-order recurrence, its state/derivative majorants and admission guard, stopping
+ * 32 steps after the last admitted one) on the host over an orbit — the one fdg_reference
+ * returns for the frame — and writes the admitted start: n (0 = none admitted), the
+ * coefficients b[1..b_len] as (re, im) pairs into the caller's b (b_cap >= max_order pairs)
+ * and the bounds, bit for bit as the reference computes them.  rho is the frame's |dc|
+ * bound (fdg_bla_dc_max, fd_kernel::bla_dc_max at grid.rs:432-440; refused on the scaled
+ * tier) and pixel the output pixel size in c.  cfg NULL = SeriesConfig::default
+ * (series.rs:43-55).
+ *
+ * fdg_frame_series sets a series start for the following runs of a resident frame (NULL
+ * clears it); fdg_render_series is the one-call form (mask nullable), the counterpart of
+ * Series::render_masked (series.rs:342-374) with its refusals: the scaled tier, a series
+ * over another orbit (points), a view reaching beyond rho; forms other than FP64 are
+ * declined.  Every sample starts at iterate n from its series state; an Unresolved one
+ * retries from zero in the same launch (grid.rs:623-649), and the statistics count the
+ * work as the reference does (the skipped prefix is not work).  Kernel id: " series/0"
+ * after " bla/1". */
+typedef struct {
+    uint32_t start_order, max_order;
+    double grow_at, shift_px, derivative_rel;
+} fdg_series_config;
+enum { FDG_SERIES_STOP_REFERENCE = 0, FDG_SERIES_STOP_RADIUS, FDG_SERIES_STOP_NONFINITE,
+       FDG_SERIES_STOP_ESCAPE, FDG_SERIES_STOP_BUDGET, FDG_SERIES_STOP_STALL };
+typedef struct {
+    uint64_t n;            /* iterates every sample skips */
+    double rho;            /* normalising radius */
+    uint64_t points;       /* orbit points it was built over */
+    double *b;             /* caller-owned: b[1..b_len] as (re, im) pairs */
+    uint32_t b_cap, b_len;
+    double state_bound, derivative_bound, shift_px;
+    uint64_t built_steps, coefficient_pairs, growth_steps;
+    int32_t stop;          /* FDG_SERIES_STOP_* */
+} fdg_series;
+void fdg_series_config_default(fdg_series_config *cfg);
+int fdg_series_build(const double *re, const double *im, uint64_t len, double rho, double pixel,
+                     uint64_t max_iter, const fdg_series_config *cfg, fdg_series *out);
+int fdg_bla_dc_max(const fdg_view *view, const fdg_params *params, double *dc_max);
+int fdg_frame_series(fdg_frame *frame, const fdg_series *series);
+int fdg_render_series(fdg_ctx *ctx, const fdg_view *view, const fdg_params *params, int route,
+                      double bla_eps, const fdg_series *series, const uint8_t *mask,
+                      fdg_columns *out, fdg_outcomes *raw, fdg_stats *stats, char *kernel_id,
+                      size_t kernel_cap);
+
 /* Films: a batch of frames at one centre (the keyframes of a zoom) rendered against ONE
  * reference orbit, as the reference renders a frame with a supplied deeper orbit:
  * fd_kernel::render_with / render_bla with `Some((orbit, bits))` (crates/fd-kernel/src/

@@ -51,6 +51,26 @@ class _Columns(C.Structure):
                 ("normal", C.c_void_p)]
 
 
+class _SeriesConfig(C.Structure):
+    _fields_ = [("start_order", C.c_uint32), ("max_order", C.c_uint32),
+                ("grow_at", C.c_double), ("shift_px", C.c_double),
+                ("derivative_rel", C.c_double)]
+
+
+class _Series(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("rho", C.c_double), ("points", C.c_uint64),
+                ("b", C.c_void_p), ("b_cap", C.c_uint32), ("b_len", C.c_uint32),
+                ("state_bound", C.c_double), ("derivative_bound", C.c_double),
+                ("shift_px", C.c_double), ("built_steps", C.c_uint64),
+                ("coefficient_pairs", C.c_uint64), ("growth_steps", C.c_uint64),
+                ("stop", C.c_int32)]
+
+
+# series.rs:43-55 SeriesConfig::default
+SERIES_DEFAULT = dict(start_order=8, max_order=64, grow_at=1e-4, shift_px=1e-8,
+                      derivative_rel=1e-6)
+SERIES_STOPS = ("reference", "radius", "non-finite", "escape", "budget", "stall")
+
 _lib = None
 
 
@@ -91,6 +111,21 @@ def lib():
                                       C.c_int, C.c_void_p, C.POINTER(_Outcomes),
                                       C.POINTER(_Columns), C.POINTER(_Stats), C.c_char_p,
                                       C.c_size_t, C.c_char_p, C.c_size_t]
+        L.fdo_series_build.restype = C.c_int
+        L.fdo_series_build.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_double,
+                                       C.c_double, C.c_uint64, C.POINTER(_SeriesConfig),
+                                       C.c_uint64, C.POINTER(_Series)]
+        L.fdo_series_eval.restype = None
+        L.fdo_series_eval.argtypes = [C.POINTER(_Series), C.c_int, C.c_double, C.c_double,
+                                      C.c_void_p]
+        L.fdo_render_series.restype = C.c_int
+        L.fdo_render_series.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_double,
+                                        C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
+                                        C.c_double, C.c_uint32, C.c_int, C.c_int, C.c_double,
+                                        C.POINTER(_Series), C.c_int, C.c_void_p,
+                                        C.POINTER(_Outcomes), C.POINTER(_Columns),
+                                        C.POINTER(_Stats), C.c_char_p, C.c_size_t, C.c_char_p,
+                                        C.c_size_t]
         L.fdo_sample.restype = C.c_int
         L.fdo_sample.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int, C.c_int,
                                  C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
@@ -202,15 +237,55 @@ def sample(zr, zi, ar, ai, max_iter, r2, c=None, extended=False, offset_exp=0,
                 blocks=nb.value, skipped=ns.value)
 
 
+def series_build(zr, zi, rho, pixel, max_iter, config=None, stalled_steps=32):
+    """series.rs Series::build (stalled_steps 32; 0 = the unbounded control) over an orbit.
+    Returns a dict: n, rho, points, b (complex coefficients b[1..]), bounds, stop."""
+    zr = np.ascontiguousarray(zr, np.float64)
+    zi = np.ascontiguousarray(zi, np.float64)
+    cfg = dict(SERIES_DEFAULT, **(config or {}))
+    c = _SeriesConfig(**cfg)
+    cap = max(int(cfg["max_order"]), 1)
+    buf = np.zeros(2 * cap, np.float64)
+    out = _Series(b=_p(buf), b_cap=cap)
+    if lib().fdo_series_build(_p(zr), _p(zi), len(zr), rho, pixel, max_iter, C.byref(c),
+                              stalled_steps, C.byref(out)) != 0:
+        raise ValueError("coefficient buffer too small")
+    b = buf[:2 * out.b_len].reshape(-1, 2)
+    return dict(n=out.n, rho=out.rho, points=out.points, b=b[:, 0] + 1j * b[:, 1],
+                state_bound=out.state_bound, derivative_bound=out.derivative_bound,
+                shift_px=out.shift_px, built_steps=out.built_steps,
+                coefficient_pairs=out.coefficient_pairs, growth_steps=out.growth_steps,
+                stop=SERIES_STOPS[out.stop])
+
+
+def _series_struct(sr):
+    b = np.ascontiguousarray(np.stack([np.real(sr["b"]), np.imag(sr["b"])], axis=1)
+                             .reshape(-1), np.float64)
+    st = _Series(n=sr["n"], rho=sr["rho"], points=sr["points"], b=_p(b), b_cap=len(sr["b"]),
+                 b_len=len(sr["b"]), state_bound=sr["state_bound"],
+                 derivative_bound=sr["derivative_bound"], shift_px=sr["shift_px"])
+    return st, b
+
+
+def series_eval(sr, ar, ai, derivative=True):
+    """Series::eval: (delta, dz/dc) as complex numbers at the offset (ar, ai)."""
+    st, keep = _series_struct(sr)
+    out = np.zeros(4)
+    lib().fdo_series_eval(C.byref(st), int(derivative), ar, ai, _p(out))
+    return complex(out[0], out[1]), complex(out[2], out[3])
+
+
 def render(re, im, width, rotation, nx, ny, max_iter, escape_radius=1e10,
            columns=COL_NU | COL_DE | COL_NORMAL, ss=1, tier=TIER_AUTO, bla=False,
            bla_eps=EPS_EXACT, threads=None, mask=None, outcomes=True, orbit=None,
-           table_dc=None):
+           table_dc=None, series=None):
     """grid.rs render_with / render_bla (table built per frame as `fd control --bla
     per-frame` does, control.rs:356-366).  Returns a dict of numpy columns, outcomes,
     stats and the header kernel id.  orbit = (re, im, bits) supplies the reference orbit
     (grid.rs:95-109); table_dc = (dc_max, dc_exp) builds the table for that |dc| bound
-    instead of the frame's own (after grid.rs:378-413 bla_checks)."""
+    instead of the frame's own (after grid.rs:378-413 bla_checks).  series = a
+    series_build() result renders with its start (grid.rs:218-226 render_series, over the
+    frame's own orbit)."""
     n = nx * ny
     threads = threads or os.cpu_count() or 1
     cols = {"class": np.zeros(n, np.uint8)}
@@ -240,11 +315,19 @@ def render(re, im, width, rotation, nx, ny, max_iter, escape_radius=1e10,
         oim = np.ascontiguousarray(orbit[1], np.float64)
         olen, obits = len(ore), int(orbit[2])
     tdc, tde = table_dc if table_dc is not None else (0.0, 0)
-    rc = lib().fdo_render_with(re.encode(), im.encode(), width.encode(), rotation, nx, ny, ss,
-                               max_iter, escape_radius, columns, tier, int(bla), bla_eps,
-                               _p(ore), _p(oim), olen, obits, tdc, tde, threads, _p(mk),
-                               C.byref(oc) if oc is not None else None, C.byref(cs),
-                               C.byref(st), kid, 128, err, 256)
+    if series is not None:
+        sst, keep = _series_struct(series)
+        rc = lib().fdo_render_series(re.encode(), im.encode(), width.encode(), rotation, nx, ny,
+                                     ss, max_iter, escape_radius, columns, tier, int(bla),
+                                     bla_eps, C.byref(sst), threads, _p(mk),
+                                     C.byref(oc) if oc is not None else None, C.byref(cs),
+                                     C.byref(st), kid, 128, err, 256)
+    else:
+        rc = lib().fdo_render_with(re.encode(), im.encode(), width.encode(), rotation, nx, ny,
+                                   ss, max_iter, escape_radius, columns, tier, int(bla), bla_eps,
+                                   _p(ore), _p(oim), olen, obits, tdc, tde, threads, _p(mk),
+                                   C.byref(oc) if oc is not None else None, C.byref(cs),
+                                   C.byref(st), kid, 128, err, 256)
     if rc != 0:
         raise ValueError(err.value.decode())
     return dict(columns=cols, outcomes=res, kernel=kid.value.decode(),

@@ -924,14 +924,21 @@ static inline uint64_t next_pow2(uint64_t x) {
     return 1ull << (64 - __builtin_clzll(x - 1));
 }
 
+/* sample.rs:69-80 Start: a sample's state after n iterates (n = 0: the ordinary start). */
+typedef struct {
+    uint64_t n;
+    double xr, xi, dr, di;
+} start_t;
+
 /*
- * sample.rs:170-581 iterate<A, D, B=false, L> (no stripe trace, no series start).
+ * sample.rs:170-581 iterate<A, D, B=false, L> (no stripe trace), continuing from `st`
+ * (NULL: the ordinary start; a series start only on the ordinary tiers).
  * EXT selects Binary64Exponent (number.rs:112-124) over Binary64 (number.rs:95-107).
  */
 static inline __attribute__((always_inline)) outcome_t
 iterate(const int EXT, const int D, const int L, const double *zr, const double *zi, uint64_t len,
         const bla_t *bla, skip_t *skip, int have_c, double c_re, double c_im, double ar, double ai,
-        int64_t offset_exp, uint64_t max_iter, double r2, uint64_t nucleus_in) {
+        int64_t offset_exp, uint64_t max_iter, double r2, uint64_t nucleus_in, const start_t *st) {
     outcome_t out = {FDO_UNRESOLVED, 0, 0, 0, 0, 0, 0};
     if (have_c && in_main_components(c_re, c_im)) {
         out.kind = FDO_INTERIOR;
@@ -939,16 +946,25 @@ iterate(const int EXT, const int D, const int L, const double *zr, const double 
         return out;
     }
     const uint64_t last = len - 1;
-    double xr = 0.0, xi = 0.0;
+    double xr = st ? st->xr : 0.0, xi = st ? st->xi : 0.0;
     int64_t delta_exp = EXT ? offset_exp : 0;
     double scale = EXT ? exp2i(delta_exp) : 1.0;
     double dc_r = ar, dc_i = ai;
-    double dr = 0.0, di = 0.0;
+    double dr = st ? st->dr : 0.0, di = st ? st->di : 0.0;
     int64_t derivative_exp = 0;
     double inverse = 1.0;
-    uint64_t m = 0, n = 0;
+    uint64_t m = st ? st->n : 0, n = m;
     double sr = 0.0, si = 0.0;
     uint64_t chk = 16, sn = EXT ? 0 : 8;
+    if (n > 0) {
+        /* sample.rs:203-211: a supplied state is the first save; saves resume at the next
+         * power of two */
+        sr = zr[m] + xr;
+        si = zi[m] + xi;
+        sn = n;
+        chk = next_pow2(n + 1);
+        if (chk < 16) chk = 16;
+    }
     int returns = 0, pending = 0;
     multiplier mu = {1.0, 0.0, 0};
     uint64_t hold = 0;
@@ -956,7 +972,8 @@ iterate(const int EXT, const int D, const int L, const double *zr, const double 
     double qr = 0.0, qi = 0.0, qstep = 0.0;
     double adc = L ? sqrt(ar * ar + ai * ai) : 0.0;
     (void)adc;
-    uint64_t nucleus = nucleus_in; /* 0 = None */
+    /* sample.rs:226-228: a supplied state skipped the contraction's first factors */
+    uint64_t nucleus = n > 0 ? 0 : nucleus_in; /* 0 = None */
     contraction con = {0.0, 0.0, 0, 0};
     while (n < max_iter) {
         int lvl = -1;
@@ -1184,9 +1201,10 @@ iterate(const int EXT, const int D, const int L, const double *zr, const double 
 #define FDO_INST(NAME, EXT, D, L)                                                              \
     static outcome_t NAME(const double *zr, const double *zi, uint64_t len, const bla_t *bla,   \
                           skip_t *skip, int have_c, double c_re, double c_im, double ar,       \
-                          double ai, int64_t oe, uint64_t mi, double r2, uint64_t nuc) {       \
+                          double ai, int64_t oe, uint64_t mi, double r2, uint64_t nuc,         \
+                          const start_t *st) {                                                 \
         return iterate(EXT, D, L, zr, zi, len, bla, skip, have_c, c_re, c_im, ar, ai, oe, mi,  \
-                       r2, nuc);                                                               \
+                       r2, nuc, st);                                                           \
     }
 FDO_INST(it_o00, 0, 0, 0)
 FDO_INST(it_o10, 0, 1, 0)
@@ -1199,7 +1217,7 @@ FDO_INST(it_e11, 1, 1, 1)
 
 typedef outcome_t (*iter_fn)(const double *, const double *, uint64_t, const bla_t *, skip_t *,
                              int, double, double, double, double, int64_t, uint64_t, double,
-                             uint64_t);
+                             uint64_t, const start_t *);
 
 static iter_fn pick_iter(int ext, int d, int l) {
     static const iter_fn tab[8] = {it_o00, it_o01, it_o10, it_o11, it_e00, it_e01, it_e10, it_e11};
@@ -1232,7 +1250,7 @@ int fdo_sample(const double *re, const double *im, uint64_t len, const double *b
     }
     skip_t sk = {0, 0};
     outcome_t o = pick_iter(extended, derivative, L)(re, im, len, L ? &t : NULL, &sk, have_c, c_re,
-                                                    c_im, ar, ai, offset_exp, max_iter, r2, nucleus);
+                                                    c_im, ar, ai, offset_exp, max_iter, r2, nucleus, NULL);
     if (L) bla_free(&t);
     if (n_out) *n_out = o.n;
     if (z_out) z_out[0] = o.zr, z_out[1] = o.zi;
@@ -1241,6 +1259,183 @@ int fdo_sample(const double *re, const double *im, uint64_t len, const double *b
     if (blocks_out) *blocks_out = sk.blocks;
     if (skipped_out) *skipped_out = sk.skipped;
     return o.kind;
+}
+
+/* ---------------------------------------------------------------- series.rs */
+
+/* series.rs:99-234 Series::build_stalled: the growing-order parameter series
+ * S_n(x) = sum b[k] x^k with dc = rho x, b[1]' = 2 Z b[1] + rho, b[k]' = 2 Z b[k] + sum(i+j=k)
+ * b[i] b[j], its state / derivative majorants E, F and the admission guard. */
+int fdo_series_build(const double *re, const double *im, uint64_t len, double rho, double pixel,
+                     uint64_t max_iter, const fdo_series_config *cfg, uint64_t stalled_steps,
+                     fdo_series *out) {
+    uint32_t k_max = cfg->max_order < 1 ? 1 : cfg->max_order;
+    uint32_t active = cfg->start_order < 1 ? 1 : cfg->start_order;
+    if (active > k_max) active = k_max;
+    if (out->b_cap < k_max) return -1;
+    double *br = (double *)calloc(k_max + 1, sizeof(double));
+    double *bi = (double *)calloc(k_max + 1, sizeof(double));
+    double *sr = (double *)calloc(2 * k_max + 1, sizeof(double));
+    double *si = (double *)calloc(2 * k_max + 1, sizeof(double));
+    double *best_b = (double *)calloc(2 * k_max, sizeof(double));
+    double e = 0.0, f = 0.0;
+    out->n = 0;
+    out->rho = rho;
+    out->points = len;
+    out->b_len = 0;
+    out->state_bound = out->derivative_bound = out->shift_px = 0.0;
+    out->built_steps = 0;
+    out->stop = FDO_STOP_REFERENCE;
+    out->coefficient_pairs = 0;
+    out->growth_steps = 0;
+    uint64_t a = len >= 2 ? len - 2 : 0, b = max_iter >= 1 ? max_iter - 1 : 0;
+    uint64_t limit = a < b ? a : b;
+    int have_best = 0;
+    uint64_t best_n = 0;
+    uint32_t best_len = 0;
+    double best_e = 0, best_f = 0, budget_e = 0, budget_f = 0, best_dmin = 0;
+    if (!(rho > 0.0 && isfinite(rho) && pixel > 0.0)) {
+        out->stop = FDO_STOP_RADIUS;
+        goto done;
+    }
+    double cap_e = cfg->shift_px * pixel * 2.0 / rho, cap_f = cfg->derivative_rel * 2.0;
+    for (uint64_t n = 0; n < limit; n++) {
+        double zr = re[n], zi = im[n];
+        double az = hypot(zr, zi);
+        for (uint32_t k = 0; k <= 2 * active; k++) sr[k] = si[k] = 0.0;
+        for (uint32_t i = 1; i <= active; i++) {
+            double ar = br[i], ai = bi[i];
+            if (ar == 0.0 && ai == 0.0) continue;
+            sr[2 * i] += ar * ar - ai * ai;
+            si[2 * i] += 2.0 * ar * ai;
+            ar = 2.0 * ar;
+            ai = 2.0 * ai;
+            for (uint32_t j = i + 1; j <= active; j++) {
+                sr[i + j] += ar * br[j] - ai * bi[j];
+                si[i + j] += ar * bi[j] + ai * br[j];
+            }
+        }
+        out->coefficient_pairs += (uint64_t)active * (active + 1) / 2;
+        double m = 0.0, d = 0.0, t = 0.0, td = 0.0;
+        for (uint32_t k = 1; k <= active; k++) {
+            double v = hypot(br[k], bi[k]);
+            m += v;
+            d += (double)k * v;
+        }
+        for (uint32_t k = active + 1; k <= 2 * active; k++) {
+            double v = hypot(sr[k], si[k]);
+            t += v;
+            td += (double)k * v;
+        }
+        double e_next = 2.0 * (az + m) * e + e * e + t;
+        f = 2.0 * (az + m + e) * f + 2.0 * e * d + td;
+        e = e_next;
+        for (uint32_t k = 1; k <= active; k++) {
+            double xr = br[k], xi = bi[k];
+            br[k] = 2.0 * (zr * xr - zi * xi) + sr[k];
+            bi[k] = 2.0 * (zr * xi + zi * xr) + si[k];
+        }
+        br[1] += rho;
+        uint64_t step = n + 1;
+        out->built_steps = step;
+        zr = re[step];
+        zi = im[step];
+        double tail_d = 0.0;
+        m = 0.0;
+        for (uint32_t k = 1; k <= active; k++) {
+            double v = hypot(br[k], bi[k]);
+            m += v;
+            if (k >= 2) tail_d += (double)k * v;
+        }
+        if (!(isfinite(e) && isfinite(f))) {
+            out->stop = FDO_STOP_NONFINITE;
+            break;
+        }
+        if (hypot(zr, zi) + m + e >= 2.0) {
+            out->stop = FDO_STOP_ESCAPE;
+            break;
+        }
+        double dmin = hypot(br[1], bi[1]) - tail_d - f;
+        double budget = dmin > 0.0 ? cfg->shift_px * pixel * dmin / rho : 0.0;
+        if (dmin > 0.0 && e <= budget && f <= cfg->derivative_rel * dmin) {
+            have_best = 1;
+            best_n = step;
+            best_len = active;
+            for (uint32_t k = 1; k <= active; k++) {
+                best_b[2 * (k - 1)] = br[k];
+                best_b[2 * (k - 1) + 1] = bi[k];
+            }
+            best_e = e;
+            best_f = f;
+            budget_e = budget;
+            budget_f = cfg->derivative_rel * dmin;
+            best_dmin = dmin;
+        } else if (e > cap_e || f > cap_f) {
+            out->stop = FDO_STOP_BUDGET;
+            break;
+        }
+        if (active < k_max && e > cfg->grow_at * budget) {
+            active = 2 * active < k_max ? 2 * active : k_max;
+            out->growth_steps++;
+        }
+        if (stalled_steps > 0 && have_best && step - best_n >= stalled_steps) {
+            out->stop = FDO_STOP_STALL;
+            break;
+        }
+    }
+    if (have_best) {
+        /* series.rs:215-229: trim high orders that fit in half the remaining slack */
+        double slack_e = budget_e - best_e, slack_f = budget_f - best_f;
+        double te = 0.0, tf = 0.0;
+        while (best_len > 1) {
+            uint32_t k = best_len;
+            double v = hypot(best_b[2 * (k - 1)], best_b[2 * (k - 1) + 1]);
+            if (te + v > 0.5 * slack_e || tf + (double)k * v > 0.5 * slack_f) break;
+            te += v;
+            tf += (double)k * v;
+            best_len--;
+        }
+        out->n = best_n;
+        out->b_len = best_len;
+        memcpy(out->b, best_b, 2 * best_len * sizeof(double));
+        out->state_bound = best_e + te;
+        out->derivative_bound = best_f + tf;
+        out->shift_px = out->state_bound * rho / (best_dmin * pixel);
+    }
+done:
+    free(br);
+    free(bi);
+    free(sr);
+    free(si);
+    free(best_b);
+    return 0;
+}
+
+/* series.rs:236-257 Series::eval: Horner on x = dc / rho for S = x p(x) and S' = p + x p'. */
+void fdo_series_eval(const fdo_series *s, int derivative, double ar, double ai, double out[4]) {
+    double sc = 1.0 / s->rho;
+    double xr = ar * sc, xi = ai * sc;
+    double pr = 0.0, pi = 0.0, qr = 0.0, qi = 0.0;
+    for (uint32_t k = s->b_len; k-- > 0;) {
+        double cr = s->b[2 * k], ci = s->b[2 * k + 1];
+        if (derivative) {
+            double tr = qr * xr - qi * xi + pr;
+            qi = qr * xi + qi * xr + pi;
+            qr = tr;
+        }
+        double tr = pr * xr - pi * xi + cr;
+        pi = pr * xi + pi * xr + ci;
+        pr = tr;
+    }
+    out[0] = pr * xr - pi * xi;
+    out[1] = pr * xi + pi * xr;
+    if (!derivative) {
+        out[2] = out[3] = 0.0;
+        return;
+    }
+    double dr = pr + qr * xr - qi * xi, di = pi + qr * xi + qi * xr;
+    out[2] = dr * sc;
+    out[3] = di * sc;
 }
 
 /* ---------------------------------------------------------------- store.rs */
@@ -1296,6 +1491,7 @@ typedef struct {
     double r2, h;
     uint32_t nx, ny, ss;
     const uint8_t *mask;
+    const fdo_series *series; /* admitted series start (n > 0), or NULL */
     fdo_outcomes *oc;
     const fdo_columns *cols;
     pthread_mutex_t lock;
@@ -1303,7 +1499,23 @@ typedef struct {
     uint64_t iterations, equivalent, blocks, skipped, fallback, closed;
 } job_t;
 
-/* grid.rs:652-725 one_from + grid.rs:623-649 one + grid.rs:748-775 fill */
+/* grid.rs:652-725 one_from: the sample's kernel call from start `st` */
+static outcome_t one_from(const job_t *job, iter_fn f, double ux, double uy, skip_t *sk,
+                          const start_t *st) {
+    const fdo_plane *pl = job->pl;
+    if (job->tier == FDO_TIER_F64) {
+        double ar = ux * job->h, ai = uy * job->h;
+        return f(job->zr, job->zi, job->len, job->bla, sk, 1, pl->c_re + ar, pl->c_im + ai, ar,
+                 ai, 0, job->max_iter, job->r2, job->nucleus, st);
+    }
+    if (job->tier == FDO_TIER_FIXED)
+        return f(job->zr, job->zi, job->len, job->bla, sk, 0, 0, 0, ux * job->h, uy * job->h, 0,
+                 job->max_iter, job->r2, job->nucleus, st);
+    return f(job->zr, job->zi, job->len, job->bla, sk, 0, 0, 0, ux * pl->h_m, uy * pl->h_m,
+             pl->h_e, job->max_iter, job->r2, job->nucleus, st);
+}
+
+/* grid.rs:623-649 one + grid.rs:748-775 fill */
 static void fill_row(job_t *job, uint32_t j, uint64_t acc[6]) {
     iter_fn f = pick_iter(job->ext, job->deriv, job->L);
     const fdo_plane *pl = job->pl;
@@ -1313,19 +1525,26 @@ static void fill_row(job_t *job, uint32_t j, uint64_t acc[6]) {
         double ux, uy;
         unit_offset(pl, i, j, &ux, &uy);
         skip_t sk = {0, 0};
-        outcome_t o;
-        if (job->tier == FDO_TIER_F64) {
-            double ar = ux * job->h, ai = uy * job->h;
-            o = f(job->zr, job->zi, job->len, job->bla, &sk, 1, pl->c_re + ar, pl->c_im + ai, ar,
-                  ai, 0, job->max_iter, job->r2, job->nucleus);
-        } else if (job->tier == FDO_TIER_FIXED) {
-            o = f(job->zr, job->zi, job->len, job->bla, &sk, 0, 0, 0, ux * job->h, uy * job->h, 0,
-                  job->max_iter, job->r2, job->nucleus);
-        } else {
-            o = f(job->zr, job->zi, job->len, job->bla, &sk, 0, 0, 0, ux * pl->h_m, uy * pl->h_m,
-                  pl->h_e, job->max_iter, job->r2, job->nucleus);
+        start_t st;
+        if (job->series) {
+            /* grid.rs:663-673: the series state at the sample's offset (ordinary tiers) */
+            double v[4];
+            fdo_series_eval(job->series, job->deriv, ux * job->h, uy * job->h, v);
+            st.n = job->series->n;
+            st.xr = v[0];
+            st.xi = v[1];
+            st.dr = v[2];
+            st.di = v[3];
         }
-        uint64_t its = o.kind == FDO_UNRESOLVED ? job->max_iter : o.n;
+        outcome_t o = one_from(job, f, ux, uy, &sk, job->series ? &st : NULL);
+        /* grid.rs:630-642: the prefix is not counted as work; an Unresolved series start
+         * retries from zero (keeping the nucleus heuristic the start declined) */
+        uint64_t prefix = job->series && !(o.kind == FDO_INTERIOR && o.n == 0) ? job->series->n : 0;
+        uint64_t its = (o.kind == FDO_UNRESOLVED ? job->max_iter : o.n) - prefix;
+        if (prefix > 0 && o.kind == FDO_UNRESOLVED) {
+            o = one_from(job, f, ux, uy, &sk, NULL);
+            its += o.kind == FDO_UNRESOLVED ? job->max_iter : o.n;
+        }
         acc[1] += its;
         if (job->L) {
             its = its - sk.skipped + sk.blocks;
@@ -1472,19 +1691,26 @@ int fdo_render(const char *re, const char *im, const char *width, double rotatio
                            cols, stats, kernel_id, kcap, err, errcap);
 }
 
-int fdo_render_with(const char *re, const char *im, const char *width, double rotation,
-                    uint32_t nx, uint32_t ny, uint32_t ss, uint64_t max_iter,
-                    double escape_radius, uint32_t columns, int tier_req, int bla, double bla_eps,
-                    const double *orbit_re, const double *orbit_im, uint64_t orbit_len,
-                    uint32_t orbit_bits, double table_dc, int64_t table_dc_exp, int threads,
-                    const uint8_t *mask, fdo_outcomes *oc, fdo_columns *cols, fdo_stats *stats,
-                    char *kernel_id, size_t kcap, char *err, size_t errcap) {
+static int render_impl(const char *re, const char *im, const char *width, double rotation,
+                       uint32_t nx, uint32_t ny, uint32_t ss, uint64_t max_iter,
+                       double escape_radius, uint32_t columns, int tier_req, int bla,
+                       double bla_eps, const double *orbit_re, const double *orbit_im,
+                       uint64_t orbit_len, uint32_t orbit_bits, double table_dc,
+                       int64_t table_dc_exp, const fdo_series *series, int threads,
+                       const uint8_t *mask, fdo_outcomes *oc, fdo_columns *cols,
+                       fdo_stats *stats, char *kernel_id, size_t kcap, char *err,
+                       size_t errcap) {
     fdo_plane pl;
     int tier;
     if (setup(re, im, width, rotation, nx, ny, ss, tier_req, &pl, &tier, err, errcap) != 0)
         return -1;
     if (columns & FDO_COL_BOUND) {
         set_err(err, errcap, "the Bound column is not restated by this oracle");
+        return -1;
+    }
+    if (series && tier == FDO_TIER_SCALED) {
+        /* series.rs:353-356 */
+        set_err(err, errcap, "a series start needs f64 deltas and no Bound column");
         return -1;
     }
     double t0 = now_s();
@@ -1496,6 +1722,25 @@ int fdo_render_with(const char *re, const char *im, const char *width, double ro
     } else if (compute_reference(re, im, &pl, tier, max_iter, escape_radius, &o) != 0) {
         set_err(err, errcap, "bad centre");
         return -1;
+    }
+    if (series) {
+        /* series.rs:357-366: built over this orbit, and a radius covering the view */
+        double dc = plane_h(&pl) * hypot((double)nx / 2.0, (double)ny / 2.0);
+        const char *why = NULL;
+        char m[200];
+        if (series->points != o.len) {
+            why = "series built over another orbit";
+        } else if (isnan(dc) || dc > series->rho) {
+            snprintf(m, sizeof(m), "view reaches |dc| = %e, beyond the series radius %e", dc,
+                     series->rho);
+            why = m;
+        }
+        if (why) {
+            free(o.re);
+            free(o.im);
+            set_err(err, errcap, why);
+            return -1;
+        }
     }
     double t1 = now_s();
     bla_t table;
@@ -1554,6 +1799,7 @@ int fdo_render_with(const char *re, const char *im, const char *width, double ro
     job.ny = ny;
     job.ss = ss;
     job.mask = mask;
+    job.series = series && series->n > 0 ? series : NULL;
     job.oc = oc;
     job.cols = cols;
     pthread_mutex_init(&job.lock, NULL);
@@ -1582,7 +1828,9 @@ int fdo_render_with(const char *re, const char *im, const char *width, double ro
     }
     if (kernel_id && kcap) {
         /* view.rs:90-96 kernel, grid.rs:527-529 " bla/1" */
-        const char *sfx = use_bla ? " bla/1" : "";
+        /* series.rs:371-372 " series/0" after it */
+        const char *sfx = use_bla ? (series ? " bla/1 series/0" : " bla/1")
+                                  : (series ? " series/0" : "");
         if (tier == FDO_TIER_F64)
             snprintf(kernel_id, kcap, "pert-f64/1%s", sfx);
         else if (tier == FDO_TIER_FIXED)
@@ -1594,6 +1842,31 @@ int fdo_render_with(const char *re, const char *im, const char *width, double ro
     free(o.re);
     free(o.im);
     return 0;
+}
+
+int fdo_render_with(const char *re, const char *im, const char *width, double rotation,
+                    uint32_t nx, uint32_t ny, uint32_t ss, uint64_t max_iter,
+                    double escape_radius, uint32_t columns, int tier_req, int bla, double bla_eps,
+                    const double *orbit_re, const double *orbit_im, uint64_t orbit_len,
+                    uint32_t orbit_bits, double table_dc, int64_t table_dc_exp, int threads,
+                    const uint8_t *mask, fdo_outcomes *oc, fdo_columns *cols, fdo_stats *stats,
+                    char *kernel_id, size_t kcap, char *err, size_t errcap) {
+    return render_impl(re, im, width, rotation, nx, ny, ss, max_iter, escape_radius, columns,
+                       tier_req, bla, bla_eps, orbit_re, orbit_im, orbit_len, orbit_bits,
+                       table_dc, table_dc_exp, NULL, threads, mask, oc, cols, stats, kernel_id,
+                       kcap, err, errcap);
+}
+
+/* grid.rs:218-226 render_series / series.rs:342-374 Series::render_masked */
+int fdo_render_series(const char *re, const char *im, const char *width, double rotation,
+                      uint32_t nx, uint32_t ny, uint32_t ss, uint64_t max_iter,
+                      double escape_radius, uint32_t columns, int tier_req, int bla,
+                      double bla_eps, const fdo_series *series, int threads,
+                      const uint8_t *mask, fdo_outcomes *oc, fdo_columns *cols, fdo_stats *stats,
+                      char *kernel_id, size_t kcap, char *err, size_t errcap) {
+    return render_impl(re, im, width, rotation, nx, ny, ss, max_iter, escape_radius, columns,
+                       tier_req, bla, bla_eps, NULL, NULL, 0, 0, 0.0, 0, series, threads, mask,
+                       oc, cols, stats, kernel_id, kcap, err, errcap);
 }
 
 /* ---------------------------------------------------------------- fd-samples */

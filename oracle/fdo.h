@@ -121,6 +121,41 @@ uint64_t fdo_write_fds(uint8_t *buf, uint64_t cap, const char *re, const char *i
                        uint64_t max_iter, double escape_radius, uint32_t columns,
                        const char *kernel_id, const fdo_columns *cols);
 
+/* series.rs:28-55 SeriesConfig (defaults: 8, 64, 1e-4, 1e-8, 1e-6). */
+typedef struct {
+    uint32_t start_order, max_order;
+    double grow_at, shift_px, derivative_rel;
+} fdo_series_config;
+
+/* series.rs:57-82 Series; b (caller-owned, b_cap >= max_order pairs) receives b[1..b_len]
+ * as re, im pairs. */
+enum { FDO_STOP_REFERENCE = 0, FDO_STOP_RADIUS, FDO_STOP_NONFINITE, FDO_STOP_ESCAPE,
+       FDO_STOP_BUDGET, FDO_STOP_STALL };
+typedef struct {
+    uint64_t n;
+    double rho;
+    uint64_t points;
+    double *b;
+    uint32_t b_cap, b_len;
+    double state_bound, derivative_bound, shift_px;
+    uint64_t built_steps, coefficient_pairs, growth_steps;
+    int32_t stop;
+} fdo_series;
+
+/* Series::build (stalled_steps 32) / build_stalled over an orbit.  0 ok, -1 b_cap short. */
+int fdo_series_build(const double *re, const double *im, uint64_t len, double rho, double pixel,
+                     uint64_t max_iter, const fdo_series_config *cfg, uint64_t stalled_steps,
+                     fdo_series *out);
+/* Series::eval: (delta re, im, dz/dc re, im) at offset (ar, ai). */
+void fdo_series_eval(const fdo_series *s, int derivative, double ar, double ai, double out[4]);
+/* render_series over the frame's own orbit (fdo_render's with a series start). */
+int fdo_render_series(const char *re, const char *im, const char *width, double rotation,
+                      uint32_t nx, uint32_t ny, uint32_t ss, uint64_t max_iter,
+                      double escape_radius, uint32_t columns, int tier, int bla, double bla_eps,
+                      const fdo_series *series, int threads, const uint8_t *mask,
+                      fdo_outcomes *oc, fdo_columns *cols, fdo_stats *stats, char *kernel_id,
+                      size_t kcap, char *err, size_t errcap);
+
 int fdo_bla_find_many(const double *re, const double *im, uint64_t len, double eps, double dc_max,
                       int64_t dc_exp, const uint64_t *ms, uint64_t nm, double d2, uint64_t budget,
                       int32_t *out_level);
