@@ -264,7 +264,8 @@ def measure_series(fa, cols, reps=3):
     agreement with the plain render (class mismatches, worst nu shift in output pixels
     where de > 1e-3, fd compare's contract)."""
     import numpy as np
-    rows, tot = [], {k: 0.0 for k in ("plain", "bla", "series", "series_bla")}
+    arms = ("plain", "bla", "series", "series_bla", "auto", "auto_bla")
+    rows, tot = [], {k: 0.0 for k in arms + ("prepare",)}
     for w in SERIES_WIDTHS:
         v = fa.View(SERIES_CENTRE[0], SERIES_CENTRE[1], w, 0.0)
         p = fa.Params(480, 270, 1, 100_000, 1e10, cols)
@@ -277,9 +278,18 @@ def measure_series(fa, cols, reps=3):
         outs = {}
         for arm, route, use in (("plain", fa.ROUTE_PLAIN, False), ("bla", fa.ROUTE_BLA, False),
                                 ("series", fa.ROUTE_PLAIN, True),
-                                ("series_bla", fa.ROUTE_BLA, True)):
+                                ("series_bla", fa.ROUTE_BLA, True),
+                                ("auto", fa.ROUTE_PLAIN, "auto"),
+                                ("auto_bla", fa.ROUTE_BLA, "auto")):
             with fa.ResidentFrame(v, p, route) as f:
-                if use and sr.n > 0:
+                if use == "auto":
+                    t = time.perf_counter()
+                    _, ok = f.series_prepare()
+                    prep = (time.perf_counter() - t) * 1e3
+                    row[arm + "_admitted"] = ok
+                    row[arm + "_prepare_ms"] = prep
+                    tot["prepare"] += prep
+                elif use and sr.n > 0:
                     f.series(sr)
                 times = []
                 for _ in range(reps):
@@ -292,7 +302,7 @@ def measure_series(fa, cols, reps=3):
             tot[arm] += ms
         a, de = outs["plain"]["nu"], outs["plain"]["de"].astype(np.float64)
         sel = de > 1e-3
-        for arm in ("series", "series_bla"):
+        for arm in ("series", "series_bla", "auto", "auto_bla"):
             b = outs[arm]["nu"]
             row[arm + "_class_mismatch"] = int((outs[arm]["class"] != outs["plain"]["class"])
                                               .sum())
@@ -303,7 +313,8 @@ def measure_series(fa, cols, reps=3):
     tot["series_build"] = sum(r["series_build_ms"] for r in rows)
     return dict(workload="24 frames of the reference's series study (frames.txt), 480x270, "
                          "max_iter 1e5, nu/de/normal; device ms = table build + kernel, "
-                         f"median of {reps}",
+                         f"median of {reps}; auto = fdg_frame_series_prepare's choice "
+                         "(its probe runs reported apart as prepare_ms)",
                 total_ms=tot, frames=rows)
 
 

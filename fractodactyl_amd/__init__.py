@@ -146,6 +146,8 @@ SIGNATURES = {
     "fdg_bla_dc_max": (C.c_int, [C.POINTER(_View), C.POINTER(_Params),
                                  C.POINTER(C.c_double)]),
     "fdg_frame_series": (C.c_int, [C.c_void_p, C.POINTER(_Series)]),
+    "fdg_frame_series_prepare": (C.c_int, [C.c_void_p, C.POINTER(_SeriesConfig),
+                                           C.POINTER(_Series), C.POINTER(C.c_int)]),
     "fdg_render_series": (C.c_int, [C.c_void_p, C.POINTER(_View), C.POINTER(_Params), C.c_int,
                                     C.c_double, C.POINTER(_Series), C.c_void_p,
                                     C.POINTER(_Columns), C.POINTER(_Outcomes),
@@ -339,6 +341,21 @@ class ResidentFrame:
             st, self._series_b = series._struct()
             self.ctx._check(lib().fdg_frame_series(self._h, C.byref(st)))
 
+    def series_prepare(self, config=None):
+        """Series::prepare on this frame (fdg_frame_series_prepare): builds the series and
+        sets it when a probe of every 61st sample predicts a net win.  Returns (Series,
+        admitted)."""
+        cfg = dict(series_config_default(), **(config or {}))
+        cap = max(int(cfg["max_order"]), 1)
+        buf = np.zeros(2 * cap)
+        out = _Series(b=_p(buf), b_cap=cap)
+        ok = C.c_int()
+        self.ctx._check(lib().fdg_frame_series_prepare(self._h, C.byref(_SeriesConfig(**cfg)),
+                                                       C.byref(out), C.byref(ok)))
+        sr = _series_from(out, buf)
+        self._series = sr if ok.value else None
+        return sr, bool(ok.value)
+
     def read(self, raw=False):
         cols, cs, oc = _out_buffers(self.params, raw)
         st = _Stats()
@@ -484,6 +501,10 @@ def series_build(re, im, rho, pixel, max_iter, config=None):
                                 C.byref(out))
     if rc != 0:
         raise FdgError(rc, "series build: bad arguments")
+    return _series_from(out, buf)
+
+
+def _series_from(out, buf):
     b = buf[:2 * out.b_len].reshape(-1, 2)
     return Series(out.n, out.rho, out.points, b[:, 0] + 1j * b[:, 1], out.state_bound,
                   out.derivative_bound, out.shift_px, out.built_steps, out.coefficient_pairs,

@@ -260,6 +260,34 @@ int build_table(fdg_ctx* ctx, Buffers* bf, uint64_t len, double eps, double dc_m
     return finish_table(ctx, *bf, valid, levels, seconds);
 }
 
+void fill_series(const Series& sr, fdg_series* out) {
+    out->n = sr.n;
+    out->rho = sr.rho;
+    out->points = sr.points;
+    out->b_len = (uint32_t)(sr.b.size() / 2);
+    if (!sr.b.empty()) std::memcpy(out->b, sr.b.data(), sr.b.size() * sizeof(double));
+    out->state_bound = sr.state_bound;
+    out->derivative_bound = sr.derivative_bound;
+    out->shift_px = sr.shift_px;
+    out->built_steps = sr.built_steps;
+    out->coefficient_pairs = sr.coefficient_pairs;
+    out->growth_steps = sr.growth_steps;
+    out->stop = sr.stop;
+}
+
+// Predicted frame cost of a probe run, in units of one sample's step: the larger of the
+// throughput term (the frame's operations — plain steps, applied blocks and the series
+// evaluations at half a step per Horner term — over the steps the chip advances in
+// parallel, `parallel`) and the latency term (the longest chain one sample ran, which a
+// persistent launch cannot split).  Measured on this kernel: ~3e4 sample-steps advance
+// in the time one lane takes one step (128 per CU), and frames with interior samples run
+// for as long as their longest sample.
+double run_cost(const fdg_stats& st, uint64_t started, uint32_t b_len, double scale,
+                double parallel) {
+    double ops = (double)st.plain_steps + (double)st.blocks +
+                 0.5 * (double)b_len * (double)started;
+    return std::max(ops * scale / parallel, (double)st.max_sample_ops);
+}
 }  // namespace
 
 struct fdg_frame {
@@ -279,10 +307,13 @@ struct fdg_frame {
     bool use_bla = false, ran = false;
     bool masked = false;  // a sample mask is set (fdg_frame_mask)
     DevBuf mask;          // 1 byte per sample on the device
+    std::vector<double> orbit_re, orbit_im;  // host copy of the reference orbit
+    std::vector<uint8_t> mask_host;          // host copy of the sample mask, when set
     bool has_series = false;  // a series start is set (fdg_frame_series), maybe with n = 0
     uint32_t series_n = 0, series_len = 0;
     double series_inv_rho = 0;
     DevBuf series_b;      // its coefficients, (re, im) pairs
+    uint64_t run_max_iter = 0;  // the series probe's budget for a run (0: params.max_iter)
     fdg_stats last;
     Buffers buf;
 };
@@ -487,6 +518,8 @@ int fdg_frame_open(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int rou
     f->bits = o.bits;
     f->nucleus = o.nucleus_period();
     rc = upload_orbit(ctx, &f->buf.orbit, o, &f->transfer_seconds);
+    f->orbit_re = std::move(o.re);
+    f->orbit_im = std::move(o.im);
     if (!rc) rc = f->buf.outputs(ctx, f->count);
     if (!rc && form != FDG_FORM_F64) {
         // FP32 floatexp orbit, 16 bytes per point, with two zero points of pad past the end
@@ -558,7 +591,7 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
     d.ny = p->ny;
     d.count = f->count;
     d.tier = f->fr.tier;
-    d.max_iter = p->max_iter;
+    d.max_iter = f->run_max_iter ? f->run_max_iter : p->max_iter;
     d.r2 = p->escape_radius * p->escape_radius;
     d.mask = f->masked ? f->mask.as<uint8_t>() : nullptr;
     if (f->masked)
@@ -742,7 +775,11 @@ int fdg_frame_mask(fdg_frame* f, const uint8_t* mask) {
     fdg_ctx* ctx = f->ctx;
     FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
     f->masked = mask != nullptr;
-    if (!mask) return FDG_OK;
+    if (!mask) {
+        f->mask_host.clear();
+        return FDG_OK;
+    }
+    f->mask_host.assign(mask, mask + f->count);
     FDG_TRY(ctx, f->mask.ensure(f->count), "hipMalloc");
     FDG_TRY(ctx, hipMemcpyAsync(f->mask.p, mask, f->count, hipMemcpyHostToDevice, ctx->stream),
             "mask upload");
@@ -798,19 +835,7 @@ int fdg_series_build(const double* re, const double* im, uint64_t len, double rh
         c.derivative_rel = cfg->derivative_rel;
     }
     if (!out->b || out->b_cap < std::max<uint32_t>(c.max_order, 1)) return FDG_EINVAL;
-    Series sr = Series::build(re, im, len, rho, pixel, max_iter, c);
-    out->n = sr.n;
-    out->rho = sr.rho;
-    out->points = sr.points;
-    out->b_len = (uint32_t)(sr.b.size() / 2);
-    if (!sr.b.empty()) std::memcpy(out->b, sr.b.data(), sr.b.size() * sizeof(double));
-    out->state_bound = sr.state_bound;
-    out->derivative_bound = sr.derivative_bound;
-    out->shift_px = sr.shift_px;
-    out->built_steps = sr.built_steps;
-    out->coefficient_pairs = sr.coefficient_pairs;
-    out->growth_steps = sr.growth_steps;
-    out->stop = sr.stop;
+    fill_series(Series::build(re, im, len, rho, pixel, max_iter, c), out);
     return FDG_OK;
 }
 
@@ -849,6 +874,9 @@ int fdg_frame_series(fdg_frame* f, const fdg_series* sr) {
     }
     if (sr->n > 0 && (sr->n + 1 >= f->len || sr->b_len == 0 || !sr->b))
         return ctx->fail(FDG_EINVAL, "series start outside its orbit");
+    // (Series::build stops below max_iter; a later start would leave no iterate to run)
+    if (sr->n >= f->params.max_iter)
+        return ctx->fail(FDG_EINVAL, "series start at or beyond max_iter");
     FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
     if (sr->n > 0) {
         size_t bytes = (size_t)sr->b_len * 2 * sizeof(double);
@@ -863,6 +891,87 @@ int fdg_frame_series(fdg_frame* f, const fdg_series* sr) {
     f->series_len = sr->b_len;
     f->series_inv_rho = 1.0 / sr->rho;
     return FDG_OK;
+}
+
+
+int fdg_frame_series_prepare(fdg_frame* f, const fdg_series_config* cfg, fdg_series* out,
+                             int* admitted) {
+    if (!f || !out || !admitted) return FDG_EINVAL;
+    fdg_ctx* ctx = f->ctx;
+    *admitted = 0;
+    SeriesConfig c;
+    if (cfg) {
+        c.start_order = cfg->start_order;
+        c.max_order = cfg->max_order;
+        c.grow_at = cfg->grow_at;
+        c.shift_px = cfg->shift_px;
+        c.derivative_rel = cfg->derivative_rel;
+    }
+    if (!out->b || out->b_cap < std::max<uint32_t>(c.max_order, 1))
+        return ctx->fail(FDG_EINVAL, "series coefficient buffer too small");
+    out->n = 0;
+    out->b_len = 0;
+    const fdg_params* p = &f->params;
+    uint64_t active = f->count;
+    if (f->masked) {
+        active = 0;
+        for (uint8_t m : f->mask_host) active += m != 0;
+    }
+    // series.rs:248-261: small frames, the scaled tier and short budgets decline before
+    // building (the FP32 forms take no series start)
+    int rc = fdg_frame_series(f, nullptr);
+    if (rc) return rc;
+    if (active < 16384 || f->fr.tier == TIER_SCALED || p->max_iter < 256 ||
+        f->form != FDG_FORM_F64)
+        return FDG_OK;
+    // series.rs:275-284: the series of the probe's budget (max_iter capped at 8192)
+    Series sr = Series::build(f->orbit_re.data(), f->orbit_im.data(), f->len, f->dc,
+                              f->fr.pl.h() * (double)p->ss,
+                              std::min<uint64_t>(p->max_iter, 8192), c);
+    fill_series(sr, out);
+    if (sr.n == 0) return FDG_OK;
+    // The probe: every 61st active sample plus the 16 x 16 block at the centre (where a
+    // zoom's minibrot and its slow samples sit; the reference's nine probe points include
+    // the centre), rendered without and with the start on this frame's route, at the
+    // reference probe's budget: max_iter capped at 8192, so a long sample is censored and
+    // its series attempt retried, which makes the admission conservative (series.rs:
+    // 266-274).  Costs are predicted from counted work, so the decision is the same on
+    // every run (the reference times the nine samples on the CPU, series.rs:285-339).
+    std::vector<uint8_t> saved = f->mask_host;
+    bool was_masked = f->masked;
+    std::vector<uint8_t> probe(f->count, 0);
+    uint64_t started = 0, seen = 0;
+    const uint32_t nx = p->nx, ny = p->ny;
+    for (uint32_t i = 0; i < f->count; i++) {
+        if (was_masked && !saved[i]) continue;
+        uint32_t x = i % nx, y = i / nx;
+        bool centre = x + 8 >= nx / 2 && x < nx / 2 + 8 && y + 8 >= ny / 2 && y < ny / 2 + 8;
+        if (seen++ % 61 == 0 || centre) {
+            probe[i] = 1;
+            started++;
+        }
+    }
+    f->run_max_iter = std::min<uint64_t>(p->max_iter, 8192);
+    rc = fdg_frame_mask(f, probe.data());
+    fdg_stats base, with;
+    if (!rc) rc = fdg_frame_run(f, &base);
+    if (!rc) rc = fdg_frame_series(f, out);
+    if (!rc) rc = fdg_frame_run(f, &with);
+    f->run_max_iter = 0;
+    int rc2 = fdg_frame_mask(f, was_masked ? saved.data() : nullptr);
+    if (rc) return rc;
+    if (rc2) return rc2;
+    f->ran = false;  // the probe's outcomes are not the frame's
+    const double scale = (double)active / (double)std::max<uint64_t>(started, 1);
+    const double parallel = 128.0 * ctx->cus;
+    double cost0 = run_cost(base, 0, 0, scale, parallel);
+    double cost1 = run_cost(with, started, out->b_len, scale, parallel);
+    // series.rs:336-339: a predicted net win with a 20% margin
+    if (cost1 < 0.8 * cost0) {
+        *admitted = 1;
+        return FDG_OK;
+    }
+    return fdg_frame_series(f, nullptr);
 }
 
 int fdg_render_series(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int route,

@@ -206,6 +206,18 @@ int fdg_series_build(const double *re, const double *im, uint64_t len, double rh
                      uint64_t max_iter, const fdg_series_config *cfg, fdg_series *out);
 int fdg_bla_dc_max(const fdg_view *view, const fdg_params *params, double *dc_max);
 int fdg_frame_series(fdg_frame *frame, const fdg_series *series);
+/* Series::prepare (series.rs:229-340) on a resident FP64 frame: declines (no series, *admitted
+ * 0) below 16,384 active samples, on the scaled tier and below 256 iterations; otherwise
+ * builds the series over the frame's orbit for max_iter capped at 8192 into *out, renders
+ * a probe (every 61st active sample and the 16 x 16 centre block, max_iter capped at 8192
+ * as the reference's probe) without and with it on the frame's route, and sets it on the
+ * frame (*admitted 1) when the series arm's predicted cost is below 0.8x: the larger of
+ * the frame's operations (scaled from the probe) over the chip's parallel step rate and
+ * the longest sample's chain of operations.  The reference decides by timing nine samples
+ * on the CPU; predicting from counted work makes the decision reproducible.  The probe
+ * runs overwrite the frame's outcomes: run the frame afterwards. */
+int fdg_frame_series_prepare(fdg_frame *frame, const fdg_series_config *cfg, fdg_series *out,
+                             int *admitted);
 int fdg_render_series(fdg_ctx *ctx, const fdg_view *view, const fdg_params *params, int route,
                       double bla_eps, const fdg_series *series, const uint8_t *mask,
                       fdg_columns *out, fdg_outcomes *raw, fdg_stats *stats, char *kernel_id,

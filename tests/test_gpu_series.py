@@ -139,3 +139,39 @@ def test_series_refusals():
     with fa.ResidentFrame(v, p, fa.ROUTE_PLAIN, fa.FORM_FE32) as f:
         with pytest.raises(fa.FdgError, match="FP64 form"):
             f.series(sr)
+
+
+SERIES_CENTRE = ("-0.8296874999999996981089858669738646984522311568651036619209",
+                 "0.1859374999999989417838556112859064221737945709381783514212")
+
+
+@pytest.mark.parametrize("width,route", [("5.0118723e-15", fa.ROUTE_PLAIN),
+                                         ("7.9432823e-26", fa.ROUTE_BLA),
+                                         ("1.9952623e-20", fa.ROUTE_BLA)])
+def test_prepare_renders_what_it_chose(width, route):
+    # Series::prepare's counterpart: whatever it decides, the frame then renders exactly
+    # the oracle's render_series (admitted) or plain/BLA render (declined)
+    v = fa.View(SERIES_CENTRE[0], SERIES_CENTRE[1], width, 0.0)
+    p = fa.Params(192, 108, 1, 20_000, 1e10, fa.COL_NU | fa.COL_DE | fa.COL_NORMAL)
+    with fa.ResidentFrame(v, p, route) as f:
+        sr, ok = f.series_prepare()
+        print(f"prepare {width}: n={sr.n} admitted={ok}", flush=True)
+        f.run()
+        g = f.read(raw=True)
+    c = oracle.render(v.center_re, v.center_im, width, 0.0, p.nx, p.ny, p.max_iter,
+                      columns=p.columns, bla=route == fa.ROUTE_BLA,
+                      series=to_oracle(sr) if ok else None)
+    if ok:
+        check(g, c)
+    else:
+        assert " series/0" not in g[0]["kernel"]
+        for k in ("class", "nu", "de", "normal"):
+            assert_same(g[1][k], c["columns"][k], k)
+
+
+def test_prepare_declines_small_frames():
+    v = fa.View(SERIES_CENTRE[0], SERIES_CENTRE[1], "5.0118723e-15", 0.0)
+    p = fa.Params(96, 54, 1, 20_000, 1e10, fa.COL_NU)  # 5,184 < 16,384 samples
+    with fa.ResidentFrame(v, p, fa.ROUTE_PLAIN) as f:
+        sr, ok = f.series_prepare()
+        assert not ok and sr.n == 0
