@@ -3,8 +3,10 @@
 #include "host.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 namespace fdg {
 
@@ -111,20 +113,56 @@ static bool add_signed(bool an, const uint64_t* a, bool bn, const uint64_t* b, u
     sub(a, b, o, l);
     return an && nonzero(o, l);
 }
-// Exact square into a 2l-limb product, then truncation to the top l limbs below the
-// integer limb: o = floor(a^2) at 64(l-1) fraction bits.
-static void sqr(const uint64_t* a, uint64_t* prod, uint64_t* o, size_t l) {
-    std::memset(prod, 0, 2 * l * sizeof(uint64_t));
-    for (size_t i = 0; i < l; i++) {
-        uint64_t ai = a[i];
-        if (!ai) continue;
-        uint64_t carry = 0;
-        for (size_t j = i + 1; j < l; j++) {
-            u128 t = (u128)ai * a[j] + prod[i + j] + carry;
-            prod[i + j] = (uint64_t)t;
-            carry = (uint64_t)(t >> 64);
+// rp[0 .. n) += up[0 .. n) * v, returning the carry limb.  Four limbs per block with two
+// independent carry chains (adcx: the previous product's high limb, adox: rp) and mulx,
+// which leaves the flags alone (BMI2 + ADX, every x86-64 server CPU since 2014 and the
+// ones this library runs on); the plain loop finishes the tail and serves CPUs without.
+static bool g_adx = __builtin_cpu_supports("bmi2") && __builtin_cpu_supports("adx");
+
+static uint64_t addmul_1(uint64_t* rp, const uint64_t* up, size_t n, uint64_t v) {
+    uint64_t carry = 0;
+    size_t i = 0;
+    if (g_adx) {
+        for (; i + 4 <= n; i += 4) {
+            asm("xorl %%r10d, %%r10d\n\t"
+                "mulx 0(%[up]), %%r8, %%r9\n\t"
+                "adcx %[c], %%r8\n\t"
+                "adox 0(%[rp]), %%r8\n\t"
+                "movq %%r8, 0(%[rp])\n\t"
+                "mulx 8(%[up]), %%r8, %[c]\n\t"
+                "adcx %%r9, %%r8\n\t"
+                "adox 8(%[rp]), %%r8\n\t"
+                "movq %%r8, 8(%[rp])\n\t"
+                "mulx 16(%[up]), %%r8, %%r9\n\t"
+                "adcx %[c], %%r8\n\t"
+                "adox 16(%[rp]), %%r8\n\t"
+                "movq %%r8, 16(%[rp])\n\t"
+                "mulx 24(%[up]), %%r8, %[c]\n\t"
+                "adcx %%r9, %%r8\n\t"
+                "adox 24(%[rp]), %%r8\n\t"
+                "movq %%r8, 24(%[rp])\n\t"
+                "adcx %%r10, %[c]\n\t"
+                "adox %%r10, %[c]"
+                : [c] "+&r"(carry)
+                : [up] "r"(up + i), [rp] "r"(rp + i), "d"(v)
+                : "r8", "r9", "r10", "cc", "memory");
         }
-        prod[i + l] = carry;
+    }
+    for (; i < n; i++) {
+        u128 t = (u128)up[i] * v + rp[i] + carry;
+        rp[i] = (uint64_t)t;
+        carry = (uint64_t)(t >> 64);
+    }
+    return carry;
+}
+
+// Exact schoolbook square into prod[0 .. 2l): the off-diagonal products once (a row per
+// limb), doubled, then the diagonal.
+static void sqr_base(const uint64_t* a, size_t l, uint64_t* prod) {
+    std::memset(prod, 0, 2 * l * sizeof(uint64_t));
+    for (size_t i = 0; i + 1 < l; i++) {
+        if (!a[i]) continue;
+        prod[i + l] = addmul_1(prod + 2 * i + 1, a + i + 1, l - i - 1, a[i]);
     }
     uint64_t top = 0;
     for (size_t i = 0; i < 2 * l; i++) {
@@ -141,6 +179,88 @@ static void sqr(const uint64_t* a, uint64_t* prod, uint64_t* o, size_t l) {
         prod[2 * i + 1] = (uint64_t)t2;
         carry = (uint64_t)(t2 >> 64);
     }
+}
+
+// Below this many limbs the schoolbook square is faster than splitting.
+constexpr size_t kKaratsuba = 40;
+
+// Orbits of at least this many limbs (about 900 bits) square on three threads.
+constexpr size_t kOrbitThreadLimbs = 16;
+
+// Scratch limbs sqr_full needs for an l-limb square.
+static size_t sqr_scratch(size_t l) {
+    if (l < kKaratsuba) return 0;
+    size_t hh = l - l / 2;
+    return 5 * hh + 2 + sqr_scratch(hh);
+}
+
+// Exact square into prod[0 .. 2l) by Karatsuba: with a = a1 B^h + a0,
+// a^2 = a1^2 B^2h + (a0^2 + a1^2 - (a1 - a0)^2) B^h + a0^2 — three half-size squares.
+// The full product is exact, so its truncation is the schoolbook one bit for bit.
+static void sqr_full(const uint64_t* a, size_t l, uint64_t* prod, uint64_t* scratch) {
+    if (l < kKaratsuba) {
+        sqr_base(a, l, prod);
+        return;
+    }
+    const size_t h = l / 2, hh = l - h;  // a0: h limbs, a1: hh limbs (hh >= h)
+    uint64_t* d = scratch;               // |a1 - a0|, hh limbs
+    uint64_t* t = d + hh;                // d^2, 2 hh limbs
+    uint64_t* m = t + 2 * hh;            // a0^2 + a1^2 - d^2, 2 hh + 1 limbs
+    uint64_t* deeper = m + 2 * hh + 1;
+    sqr_full(a, h, prod, deeper);            // a0^2 -> prod[0 .. 2h)
+    sqr_full(a + h, hh, prod + 2 * h, deeper);  // a1^2 -> prod[2h .. 2l)
+    // d = |a1 - a0| with a0 zero-extended to hh limbs
+    int c = 0;
+    for (size_t i = hh; i-- > 0;) {
+        uint64_t x = a[h + i], y = i < h ? a[i] : 0;
+        if (x != y) {
+            c = x > y ? 1 : -1;
+            break;
+        }
+    }
+    const uint64_t* big = c >= 0 ? a + h : a;
+    const uint64_t* small = c >= 0 ? a : a + h;
+    size_t big_n = c >= 0 ? hh : h, small_n = c >= 0 ? h : hh;
+    uint64_t borrow = 0;
+    for (size_t i = 0; i < hh; i++) {
+        uint64_t x = i < big_n ? big[i] : 0, y = i < small_n ? small[i] : 0;
+        u128 r = (u128)x - y - borrow;
+        d[i] = (uint64_t)r;
+        borrow = (uint64_t)(r >> 64) & 1;
+    }
+    sqr_full(d, hh, t, deeper);
+    // m = a0^2 + a1^2 - d^2 (= 2 a0 a1 >= 0)
+    uint64_t carry = 0;
+    for (size_t i = 0; i < 2 * hh; i++) {
+        uint64_t x = i < 2 * h ? prod[i] : 0;
+        u128 r = (u128)x + prod[2 * h + i] + carry;
+        m[i] = (uint64_t)r;
+        carry = (uint64_t)(r >> 64);
+    }
+    m[2 * hh] = carry;
+    borrow = 0;
+    for (size_t i = 0; i <= 2 * hh; i++) {
+        uint64_t y = i < 2 * hh ? t[i] : 0;
+        u128 r = (u128)m[i] - y - borrow;
+        m[i] = (uint64_t)r;
+        borrow = (uint64_t)(r >> 64) & 1;
+    }
+    // prod += m B^h
+    carry = 0;
+    for (size_t i = 0; h + i < 2 * l; i++) {
+        uint64_t y = i <= 2 * hh ? m[i] : 0;
+        if (i > 2 * hh && !carry) break;
+        u128 r = (u128)prod[h + i] + y + carry;
+        prod[h + i] = (uint64_t)r;
+        carry = (uint64_t)(r >> 64);
+    }
+}
+
+// o = floor(a^2) at 64(l-1) fraction bits: the exact 2l-limb square truncated to the l
+// limbs from l-1 up (crates/fd-fixed/src/limbs.rs:66-101).  prod: 2l limbs; scratch:
+// sqr_scratch(l) limbs.
+static void sqr(const uint64_t* a, uint64_t* prod, uint64_t* o, size_t l, uint64_t* scratch) {
+    sqr_full(a, l, prod, scratch);
     std::memcpy(o, prod + (l - 1), l * sizeof(uint64_t));
 }
 }  // namespace limbs
@@ -401,6 +521,80 @@ Orbit Orbit::f64(double c_re, double c_im, uint64_t max_iter, double radius, uin
     return o;
 }
 
+namespace {
+
+// A reusable barrier for the orbit's three threads: the last to arrive advances the
+// phase, the others spin on it (a step is a few microseconds, far below a futex wake).
+struct SpinBarrier {
+    std::atomic<uint32_t> arrived{0}, phase{0};
+    const uint32_t n;
+    explicit SpinBarrier(uint32_t count) : n(count) {}
+    void wait() {
+        uint32_t ph = phase.load(std::memory_order_acquire);
+        if (arrived.fetch_add(1, std::memory_order_acq_rel) + 1 == n) {
+            arrived.store(0, std::memory_order_relaxed);
+            phase.store(ph + 1, std::memory_order_release);
+            return;
+        }
+        while (phase.load(std::memory_order_acquire) == ph) __builtin_ia32_pause();
+    }
+};
+
+// One thread's share of the fixed-point orbit (orbit.rs:14-43: x' = x^2 - y^2 + cr,
+// y' = (x + y)^2 - (x^2 + y^2) + ci).  Thread `role` squares x (0), y (1) or x + y (2)
+// into its slot of the step's buffer; after the barrier every thread combines the three
+// squares into its own copy of (x, y), so one barrier per step suffices (the squares are
+// double-buffered by step parity, and a thread can only write a buffer again after all
+// have passed the next barrier, by which time they have read it).  With roles {0, 1, 2}
+// on one thread this is the serial loop.
+struct OrbitShared {
+    size_t l;
+    uint64_t steps;
+    const Fixed *cr, *ci;
+    std::vector<uint64_t> sq;  // [parity][role] squares, l limbs each
+    SpinBarrier bar;
+    Orbit* out;
+    OrbitShared(size_t limbs, uint32_t threads) : l(limbs), sq(6 * limbs), bar(threads) {}
+};
+
+void orbit_worker(OrbitShared* sh, int first_role, int last_role, bool emit) {
+    const size_t l = sh->l;
+    std::vector<uint64_t> buf(6 * l + limbs::sqr_scratch(l), 0);
+    uint64_t *x = &buf[0], *y = &buf[l], *s = &buf[2 * l], *t = &buf[3 * l],
+             *prod = &buf[4 * l], *scratch = &buf[6 * l];
+    bool xn = false, yn = false;
+    const bool threaded = !(first_role == 0 && last_role == 2);
+    for (uint64_t k = 0; k < sh->steps; k++) {
+        uint64_t* sq = &sh->sq[(k & 1) * 3 * l];
+        uint64_t *x2 = sq, *y2 = sq + l, *s2 = sq + 2 * l;
+        for (int role = first_role; role <= last_role; role++) {
+            if (role == 0) {
+                limbs::sqr(x, prod, x2, l, scratch);
+            } else if (role == 1) {
+                limbs::sqr(y, prod, y2, l, scratch);
+            } else {
+                limbs::add_signed(xn, x, yn, y, s, l);  // its sign is squared away
+                limbs::sqr(s, prod, s2, l, scratch);
+            }
+        }
+        if (threaded) sh->bar.wait();
+        bool tn = limbs::add_signed(false, x2, true, y2, t, l);
+        xn = limbs::add_signed(tn, t, sh->cr->neg, sh->cr->mag.data(), x, l);
+        limbs::add(x2, y2, s, l);
+        tn = limbs::add_signed(false, s2, true, s, t, l);
+        yn = limbs::add_signed(tn, t, sh->ci->neg, sh->ci->mag.data(), y, l);
+        double fx = to_f64_limbs(xn, x, l), fy = to_f64_limbs(yn, y, l);
+        if (emit) {
+            sh->out->re.push_back(fx);
+            sh->out->im.push_back(fy);
+        }
+        // every thread holds the same (x, y), so all stop at the same step
+        if (fx * fx + fy * fy > 4294967296.0) break;
+    }
+}
+
+}  // namespace
+
 bool Orbit::fixed(const std::string& re, const std::string& im, uint64_t bits, uint64_t max_iter,
                   Orbit* out, uint64_t max_len) {
     size_t l = Fixed::limbs_for(bits);
@@ -412,25 +606,20 @@ bool Orbit::fixed(const std::string& re, const std::string& im, uint64_t bits, u
     o.im.reserve(steps + 1);
     o.re.push_back(0.0);
     o.im.push_back(0.0);
-    std::vector<uint64_t> buf(9 * l, 0);
-    uint64_t *x = &buf[0], *y = &buf[l], *x2 = &buf[2 * l], *y2 = &buf[3 * l], *s = &buf[4 * l],
-             *s2 = &buf[5 * l], *t = &buf[6 * l], *prod = &buf[7 * l];
-    bool xn = false, yn = false;
-    for (uint64_t k = 0; k < steps; k++) {
-        // x' = x^2 - y^2 + cr;  y' = (x + y)^2 - (x^2 + y^2) + ci
-        limbs::sqr(x, prod, x2, l);
-        limbs::sqr(y, prod, y2, l);
-        limbs::add_signed(xn, x, yn, y, s, l);
-        limbs::sqr(s, prod, s2, l);
-        bool tn = limbs::add_signed(false, x2, true, y2, t, l);
-        xn = limbs::add_signed(tn, t, cr.neg, cr.mag.data(), x, l);
-        limbs::add(x2, y2, s, l);
-        tn = limbs::add_signed(false, s2, true, s, t, l);
-        yn = limbs::add_signed(tn, t, ci.neg, ci.mag.data(), y, l);
-        double fx = to_f64_limbs(xn, x, l), fy = to_f64_limbs(yn, y, l);
-        o.re.push_back(fx);
-        o.im.push_back(fy);
-        if (fx * fx + fy * fy > 4294967296.0) break;
+    // Three threads from kOrbitThreadLimbs limbs up, where a square outweighs a barrier.
+    const bool threaded = l >= limbs::kOrbitThreadLimbs && std::thread::hardware_concurrency() >= 3;
+    OrbitShared sh(l, threaded ? 3 : 1);
+    sh.steps = steps;
+    sh.cr = &cr;
+    sh.ci = &ci;
+    sh.out = &o;
+    if (threaded) {
+        std::thread t1(orbit_worker, &sh, 1, 1, false), t2(orbit_worker, &sh, 2, 2, false);
+        orbit_worker(&sh, 0, 0, true);
+        t1.join();
+        t2.join();
+    } else {
+        orbit_worker(&sh, 0, 2, true);
     }
     o.bits = (uint32_t)bits;
     *out = std::move(o);

@@ -79,6 +79,24 @@ def test_host_reference_orbit_matches_oracle(re_, im_, width, iters, tier):
     assert np.array_equal(a_im.view(np.int64), b_im.view(np.int64))
 
 
+@pytest.mark.parametrize("width,iters", [("1e-300", 30000), ("1e-700", 20000),
+                                         ("1e-1000", 20000), ("1e-2000", 6000)])
+def test_threaded_karatsuba_orbit_matches_oracle(width, iters):
+    # the fixed-point orbit from 16 limbs up squares on three threads, and from 40 limbs
+    # up by Karatsuba (one or two levels at these widths: 18 to 108 limbs); both are
+    # exact, so the orbit is the oracle's single-threaded schoolbook one bit for bit
+    from fractodactyl_amd.scenes import scene
+    s3 = scene(3)
+    re_, im_ = s3["center_re"], s3["center_im"]
+    v = fa.View(re_, im_, width, 0.0)
+    p = fa.Params(64, 36, max_iter=iters)
+    a_re, a_im, a_bits = _host_reference(v, p)
+    b_re, b_im, b_bits = oracle.reference(re_, im_, width, 0.0, 64, 36, iters, 1e10)
+    assert a_bits == b_bits and len(a_re) == len(b_re) > 1000
+    assert np.array_equal(a_re.view(np.int64), b_re.view(np.int64))
+    assert np.array_equal(a_im.view(np.int64), b_im.view(np.int64))
+
+
 def test_host_reference_rejects_bad_requests():
     v, p = fa._view(fa.View("abc", "0", "1e-3")), fa._params(fa.Params(8, 8))
     ln, bits = C.c_uint64(), C.c_uint32()
