@@ -184,8 +184,10 @@ static void sqr_base(const uint64_t* a, size_t l, uint64_t* prod) {
 // Below this many limbs the schoolbook square is faster than splitting.
 constexpr size_t kKaratsuba = 40;
 
-// Orbits of at least this many limbs (about 900 bits) square on three threads.
-constexpr size_t kOrbitThreadLimbs = 16;
+// Orbits of at least this many limbs (about 5,000 bits) square on three threads: below,
+// a square (0.4 us at 56 limbs on the EPYC hosts of this GPU) costs less than the
+// barrier and the exchange of the squares between cores.
+constexpr size_t kOrbitThreadLimbs = 80;
 
 // Scratch limbs sqr_full needs for an l-limb square.
 static size_t sqr_scratch(size_t l) {
@@ -256,11 +258,46 @@ static void sqr_full(const uint64_t* a, size_t l, uint64_t* prod, uint64_t* scra
     }
 }
 
+// The high half of a square, exactly: floor(a^2 / B^(l-1)) (B = 2^64) from the products
+// a[i] a[j] with i + j >= m = l - 3 only — about half of them.  The omitted products sum
+// to L < 2m B^(m+1) (column k holds k + 1 ordered products, each below B^(k+2)), so with
+// H the sum of the kept ones, floor((H + L) / B^(l-1)) = floor(H / B^(l-1)) whenever
+// (H mod B^(l-1)) + 2m B^(m+1) < B^(l-1), i.e. H's limb l-2 is below B - 2m - 1 (its
+// limbs below m are zero).  Returns false, having computed nothing usable, in the other
+// case (probability ~2m / 2^64), where the caller takes the full square.
+static bool sqr_high(const uint64_t* a, size_t l, uint64_t* prod) {
+    if (l < 8) return false;
+    const size_t m = l - 3;
+    std::memset(prod + m, 0, (2 * l - m) * sizeof(uint64_t));
+    for (size_t i = 0; i + 1 < l; i++) {
+        size_t j0 = std::max(i + 1, m > i ? m - i : (size_t)0);
+        if (j0 >= l || !a[i]) continue;
+        prod[i + l] = addmul_1(prod + i + j0, a + j0, l - j0, a[i]);
+    }
+    // double the off-diagonal sum (zero below column m), then add a[i]^2 for 2i >= m
+    uint64_t top = 0;
+    for (size_t k = m; k < 2 * l; k++) {
+        uint64_t nt = prod[k] >> 63;
+        prod[k] = (prod[k] << 1) | top;
+        top = nt;
+    }
+    uint64_t carry = 0;
+    for (size_t i = (m + 1) / 2; i < l; i++) {
+        u128 sq = (u128)a[i] * a[i];
+        u128 t = (u128)prod[2 * i] + (uint64_t)sq + carry;
+        prod[2 * i] = (uint64_t)t;
+        u128 t2 = (u128)prod[2 * i + 1] + (uint64_t)(sq >> 64) + (uint64_t)(t >> 64);
+        prod[2 * i + 1] = (uint64_t)t2;
+        carry = (uint64_t)(t2 >> 64);
+    }
+    return prod[l - 2] < ~(uint64_t)0 - 2 * (uint64_t)m - 1;
+}
+
 // o = floor(a^2) at 64(l-1) fraction bits: the exact 2l-limb square truncated to the l
-// limbs from l-1 up (crates/fd-fixed/src/limbs.rs:66-101).  prod: 2l limbs; scratch:
-// sqr_scratch(l) limbs.
+// limbs from l-1 up (crates/fd-fixed/src/limbs.rs:66-101), from its high half when that
+// is provably the same.  prod: 2l limbs; scratch: sqr_scratch(l) limbs.
 static void sqr(const uint64_t* a, uint64_t* prod, uint64_t* o, size_t l, uint64_t* scratch) {
-    sqr_full(a, l, prod, scratch);
+    if (!sqr_high(a, l, prod)) sqr_full(a, l, prod, scratch);
     std::memcpy(o, prod + (l - 1), l * sizeof(uint64_t));
 }
 }  // namespace limbs
