@@ -152,6 +152,12 @@ SIGNATURES = {
                                     C.c_double, C.POINTER(_Series), C.c_void_p,
                                     C.POINTER(_Columns), C.POINTER(_Outcomes),
                                     C.POINTER(_Stats), C.c_char_p, C.c_size_t]),
+    "fdg_jetaa_first": (C.c_int, [C.c_uint32, C.c_uint32, C.c_void_p]),
+    "fdg_jetaa_rebuild": (C.c_int, [C.POINTER(_Columns), C.c_uint32, C.c_uint32, C.c_void_p]),
+    "fdg_jetaa_merge": (C.c_int, [C.POINTER(_Columns), C.POINTER(_Columns), C.c_void_p,
+                                  C.c_uint64]),
+    "fdg_frame_jet": (C.c_int, [C.c_void_p, C.POINTER(_Columns), C.POINTER(_Stats), C.c_char_p,
+                                C.c_size_t, C.POINTER(C.c_uint64)]),
     "fdg_film_open": (C.c_int, [C.c_void_p, C.POINTER(_View), C.c_uint32, C.POINTER(_Params),
                                 C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_void_p)]),
     "fdg_film_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(_Stats)]),
@@ -356,6 +362,20 @@ class ResidentFrame:
         self._series = sr if ok.value else None
         return sr, bool(ok.value)
 
+    def jet(self):
+        """A jet-AA frame (fdg_frame_jet; crates/fd-cli/src/film.rs:1359-1457): pass 1 on
+        each pixel's top-left sample, the host rebuild, pass 2 on the rejected samples.
+        Returns (header, columns, stats, samples computed)."""
+        cols, cs, _ = _out_buffers(self.params, False)
+        st = _Stats()
+        kid = C.create_string_buffer(256)
+        done = C.c_uint64()
+        self.ctx._check(lib().fdg_frame_jet(self._h, C.byref(cs), C.byref(st), kid, 256,
+                                            C.byref(done)))
+        header = dict(view=self.view, params=self.params, kernel=kid.value.decode(),
+                      columns=self.params.columns | COL_CLASS)
+        return header, cols, {k: getattr(st, k) for k, _ in _Stats._fields_}, done.value
+
     def read(self, raw=False):
         cols, cs, oc = _out_buffers(self.params, raw)
         st = _Stats()
@@ -541,6 +561,43 @@ def render_series(view, params, series, route=ROUTE_PLAIN, bla_eps=EPS, mask=Non
     header = dict(view=view, params=params, kernel=kid.value.decode(),
                   columns=params.columns | COL_CLASS)
     return header, cols, {k: getattr(st, k) for k, _ in _Stats._fields_}
+
+
+def jetaa_first(w, h):
+    """jetaa.rs `first` (fdg_jetaa_first): the pass-1 mask of a w x h pixel ss 2 grid."""
+    m = np.zeros(4 * w * h, np.uint8)
+    rc = lib().fdg_jetaa_first(w, h, _p(m))
+    if rc != 0:
+        raise FdgError(rc, "jetaa_first")
+    return m
+
+
+def _cols_struct(cols):
+    return _Columns(_p(cols["class"]), _p(cols.get("nu")), _p(cols.get("de")),
+                    _p(cols.get("normal")))
+
+
+def jetaa_rebuild(cols, w, h):
+    """jetaa.rs `rebuild` (fdg_jetaa_rebuild) in place on ss 2 columns (class, nu, de,
+    normal); returns the pass-2 mask."""
+    for k, t in (("class", np.uint8), ("nu", np.float64), ("de", np.float32),
+                 ("normal", np.uint16)):
+        if cols[k].dtype != t or not cols[k].flags.c_contiguous or cols[k].size != 4 * w * h:
+            raise FdgError(-1, f"column {k} must be {t.__name__}[{4 * w * h}]")
+    todo = np.zeros(4 * w * h, np.uint8)
+    rc = lib().fdg_jetaa_rebuild(C.byref(_cols_struct(cols)), w, h, _p(todo))
+    if rc != 0:
+        raise FdgError(rc, "jetaa_rebuild")
+    return todo
+
+
+def jetaa_merge(to, frm, mask):
+    """jetaa.rs `merge` (fdg_jetaa_merge): copy the masked samples of frm into to."""
+    m = np.ascontiguousarray(mask, np.uint8)
+    rc = lib().fdg_jetaa_merge(C.byref(_cols_struct(to)), C.byref(_cols_struct(frm)), _p(m),
+                               m.size)
+    if rc != 0:
+        raise FdgError(rc, "jetaa_merge")
 
 
 def render_bla(view, params, form=FORM_F64, bla_eps=EPS, ctx=None, raw=False):

@@ -992,6 +992,95 @@ int fdg_render_series(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int 
     return rc;
 }
 
+int fdg_jetaa_first(uint32_t w, uint32_t h, uint8_t* mask) {
+    if (!mask) return FDG_EINVAL;
+    jetaa_first(w, h, mask);
+    return FDG_OK;
+}
+
+int fdg_jetaa_rebuild(fdg_columns* c, uint32_t w, uint32_t h, uint8_t* todo) {
+    if (!c || !c->class_ || !c->nu || !c->de || !c->normal || !todo) return FDG_EINVAL;
+    jetaa_rebuild(c->class_, c->nu, c->de, c->normal, w, h, todo);
+    return FDG_OK;
+}
+
+int fdg_jetaa_merge(fdg_columns* to, const fdg_columns* from, const uint8_t* mask, uint64_t n) {
+    if (!to || !from || !mask || !to->class_ || !from->class_) return FDG_EINVAL;
+    jetaa_merge(to->class_, to->nu, to->de, to->normal, from->class_, from->nu, from->de,
+                from->normal, mask, n);
+    return FDG_OK;
+}
+
+int fdg_frame_jet(fdg_frame* f, fdg_columns* out, fdg_stats* st, char* kernel_id,
+                  size_t kernel_cap, uint64_t* computed) {
+    if (!f) return FDG_EINVAL;
+    fdg_ctx* ctx = f->ctx;
+    const fdg_params* p = &f->params;
+    if (p->ss != 2) return ctx->fail(FDG_EINVAL, "jet-AA rebuilds ss 2 frames");
+    const uint32_t need = FDG_COL_NU | FDG_COL_DE | FDG_COL_NORMAL;
+    if ((p->columns & need) != need || !out || !out->class_ || !out->nu || !out->de ||
+        !out->normal)
+        return ctx->fail(FDG_EINVAL, "jet-AA needs the nu, de and normal columns");
+    const size_t w = p->nx / 2, h = p->ny / 2, n = (size_t)f->count;
+    std::vector<uint8_t> first(n), todo(n);
+    jetaa_first(w, h, first.data());
+    fdg_stats s1, s2;
+    std::memset(&s2, 0, sizeof(s2));
+    // pass 1: each pixel's top-left sample
+    int rc = fdg_frame_mask(f, first.data());
+    if (!rc) rc = fdg_frame_run(f, nullptr);
+    if (!rc) rc = fdg_frame_read(f, out, nullptr, &s1, kernel_id, kernel_cap);
+    if (rc) {
+        (void)fdg_frame_mask(f, nullptr);
+        return rc;
+    }
+    double t = now_s();
+    jetaa_rebuild(out->class_, out->nu, out->de, out->normal, w, h, todo.data());
+    double rebuild_s = now_s() - t;
+    uint64_t rest = 0;
+    for (uint8_t v : todo) rest += v;
+    if (rest > 0) {
+        // pass 2: the rejected predictions, then merged over the rebuilt frame
+        std::vector<uint8_t> c2(n);
+        std::vector<double> nu2(n);
+        std::vector<float> de2(n);
+        std::vector<uint16_t> nm2(n);
+        fdg_columns second{c2.data(), nu2.data(), de2.data(), nm2.data()};
+        rc = fdg_frame_mask(f, todo.data());
+        if (!rc) rc = fdg_frame_run(f, nullptr);
+        if (!rc) rc = fdg_frame_read(f, &second, nullptr, &s2, nullptr, 0);
+        if (rc) {
+            (void)fdg_frame_mask(f, nullptr);
+            return rc;
+        }
+        jetaa_merge(out->class_, out->nu, out->de, out->normal, second.class_, second.nu,
+                    second.de, second.normal, todo.data(), n);
+    }
+    rc = fdg_frame_mask(f, nullptr);
+    if (rc) return rc;
+    if (computed) *computed = w * h + rest;
+    if (st) {
+        fdg_stats s = s1;
+        s.iterations += s2.iterations;
+        s.equivalent += s2.equivalent;
+        s.blocks += s2.blocks;
+        s.skipped += s2.skipped;
+        s.plain_steps += s2.plain_steps;
+        s.fallback_samples += s2.fallback_samples;
+        s.closed_form_samples += s2.closed_form_samples;
+        s.max_sample_ops = std::max(s.max_sample_ops, s2.max_sample_ops);
+        s.wave_iterations += s2.wave_iterations;
+        s.lane_iterations += s2.lane_iterations;
+        s.bla_seconds += s2.bla_seconds;
+        s.kernel_seconds += s2.kernel_seconds;
+        // (each pass's transfers include the frame's orbit upload once)
+        if (rest > 0) s.transfer_seconds += s2.transfer_seconds - f->transfer_seconds;
+        s.encode_seconds += s2.encode_seconds + rebuild_s;
+        *st = s;
+    }
+    return FDG_OK;
+}
+
 int fdg_render(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int route, int form,
                double bla_eps, fdg_columns* out, fdg_outcomes* raw, fdg_stats* st,
                char* kernel_id, size_t kernel_cap) {

@@ -124,6 +124,19 @@ struct Encoder {
              int64_t dexp, uint8_t* cls, double* nu, float* de, uint16_t* normal) const;
 };
 
+// Jet-AA (crates/fd-cli/src/jetaa.rs) on an ss 2 sample grid of w x h pixels (2w x 2h
+// samples, row-major).  jetaa_first: the pass-1 mask, each pixel's top-left sample
+// (jetaa.rs:38-44).  jetaa_rebuild: predicts the other three samples of every pixel from
+// the pass-1 columns, writing the accepted ones into the columns, and sets todo (4wh
+// bytes) for the rejected ones (jetaa.rs:123-245).  jetaa_merge: copies the samples
+// selected by mask from the second pass's columns (jetaa.rs:248-261).
+void jetaa_first(size_t w, size_t h, uint8_t* mask);
+void jetaa_rebuild(uint8_t* cls, double* nu, float* de, uint16_t* normal, size_t w, size_t h,
+                   uint8_t* todo);
+void jetaa_merge(uint8_t* cls, double* nu, float* de, uint16_t* normal, const uint8_t* fcls,
+                 const double* fnu, const float* fde, const uint16_t* fnormal,
+                 const uint8_t* mask, size_t n);
+
 // fd_samples::write (write.rs:8-57, header.rs:311-330): the canonical .fds bytes.
 std::vector<uint8_t> write_fds(const std::string& re, const std::string& im,
                                const std::string& width, double rotation, uint32_t nx,

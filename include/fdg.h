@@ -223,6 +223,24 @@ int fdg_render_series(fdg_ctx *ctx, const fdg_view *view, const fdg_params *para
                       fdg_columns *out, fdg_outcomes *raw, fdg_stats *stats, char *kernel_id,
                       size_t kernel_cap);
 
+/* Jet-AA for ss 2 film frames (crates/fd-cli/src/jetaa.rs; film.rs:1359-1457
+ * render_frame_jet).  On a 2w x 2h sample grid: fdg_jetaa_first writes the pass-1 mask
+ * (each pixel's top-left sample, jetaa.rs:38-44); fdg_jetaa_rebuild predicts every pixel's
+ * other three samples from the pass-1 columns (nu, de and normal required), writes the
+ * accepted predictions into the columns and sets todo (one byte per sample) where pass 2
+ * must compute (jetaa.rs:123-245); fdg_jetaa_merge copies the masked samples of a second
+ * pass (jetaa.rs:248-261).  All three are host-only and equal the reference's bytes.
+ * fdg_frame_jet runs the whole jet-AA frame on a resident ss 2 frame: pass 1 on the GPU,
+ * the rebuild on the host, pass 2 on the GPU for the rejected samples only, merged into
+ * out; *computed receives the samples the GPU computed (w h + rejected), stats the sum of
+ * both passes (encode_seconds includes the rebuild).  The frame's mask is cleared after;
+ * a series start set on the frame applies to both passes. */
+int fdg_jetaa_first(uint32_t w, uint32_t h, uint8_t *mask);
+int fdg_jetaa_rebuild(fdg_columns *columns, uint32_t w, uint32_t h, uint8_t *todo);
+int fdg_jetaa_merge(fdg_columns *to, const fdg_columns *from, const uint8_t *mask, uint64_t n);
+int fdg_frame_jet(fdg_frame *frame, fdg_columns *out, fdg_stats *stats, char *kernel_id,
+                  size_t kernel_cap, uint64_t *computed);
+
 /* Films: a batch of frames at one centre (the keyframes of a zoom) rendered against ONE
  * reference orbit, as the reference renders a frame with a supplied deeper orbit:
  * fd_kernel::render_with / render_bla with `Some((orbit, bits))` (crates/fd-kernel/src/

@@ -126,6 +126,12 @@ def lib():
                                         C.POINTER(_Outcomes), C.POINTER(_Columns),
                                         C.POINTER(_Stats), C.c_char_p, C.c_size_t, C.c_char_p,
                                         C.c_size_t]
+        L.fdo_jetaa_first.restype = None
+        L.fdo_jetaa_first.argtypes = [C.c_uint64, C.c_uint64, C.c_void_p]
+        L.fdo_jetaa_rebuild.restype = None
+        L.fdo_jetaa_rebuild.argtypes = [C.c_void_p] * 4 + [C.c_uint64, C.c_uint64, C.c_void_p]
+        L.fdo_jetaa_merge.restype = None
+        L.fdo_jetaa_merge.argtypes = [C.c_void_p] * 9 + [C.c_uint64]
         L.fdo_sample.restype = C.c_int
         L.fdo_sample.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int, C.c_int,
                                  C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
@@ -372,3 +378,29 @@ def fixed_parse(s, limbs):
     if neg < 0:
         raise ValueError("not a decimal number / too large: %r" % s)
     return bool(neg), [int(v) for v in out]
+
+
+def jetaa_first(w, h):
+    """jetaa.rs `first`: the pass-1 mask of a w x h pixel ss 2 grid (uint8)."""
+    m = np.zeros(4 * w * h, np.uint8)
+    lib().fdo_jetaa_first(w, h, _p(m))
+    return m
+
+
+def jetaa_rebuild(cols, w, h):
+    """jetaa.rs `rebuild` in place on a columns dict (class, nu, de, normal); returns the
+    pass-2 mask (uint8)."""
+    for k, t in (("class", np.uint8), ("nu", np.float64), ("de", np.float32),
+                 ("normal", np.uint16)):
+        assert cols[k].dtype == t and cols[k].flags.c_contiguous and cols[k].size == 4 * w * h
+    todo = np.zeros(4 * w * h, np.uint8)
+    lib().fdo_jetaa_rebuild(_p(cols["class"]), _p(cols["nu"]), _p(cols["de"]),
+                            _p(cols["normal"]), w, h, _p(todo))
+    return todo
+
+
+def jetaa_merge(to, frm, mask):
+    """jetaa.rs `merge`: copy the masked samples of `frm` into `to` (columns dicts)."""
+    m = np.ascontiguousarray(mask, np.uint8)
+    lib().fdo_jetaa_merge(*[_p(to[k]) for k in ("class", "nu", "de", "normal")],
+                          *[_p(frm[k]) for k in ("class", "nu", "de", "normal")], _p(m), m.size)

@@ -1869,6 +1869,174 @@ int fdo_render_series(const char *re, const char *im, const char *width, double 
                        oc, cols, stats, kernel_id, kcap, err, errcap);
 }
 
+/* ---------------------------------------------------------------- jetaa.rs */
+
+/* crates/fd-cli/src/jetaa.rs:38-44 first */
+void fdo_jetaa_first(uint64_t w, uint64_t h, uint8_t *mask) {
+    for (uint64_t i = 0; i < 4 * w * h; i++) mask[i] = (i % (2 * w)) % 2 == 0 && (i / (2 * w)) % 2 == 0;
+}
+
+typedef struct { double r, i; } jc;
+static jc jc_mul(jc a, jc b) { jc o = {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; return o; }
+static jc jc_add(jc a, jc b) { jc o = {a.r + b.r, a.i + b.i}; return o; }
+static jc jc_sub(jc a, jc b) { jc o = {a.r - b.r, a.i - b.i}; return o; }
+static jc jc_scale(jc a, double k) { jc o = {a.r * k, a.i * k}; return o; }
+
+/* jetaa.rs:63-76 Off */
+typedef struct { jc d, d2h, d3s; } joff;
+static joff joff_new(double dx, double dy) {
+    joff o;
+    o.d.r = dx;
+    o.d.i = -dy;
+    jc d2 = jc_mul(o.d, o.d);
+    o.d2h = jc_scale(d2, 0.5);
+    o.d3s = jc_scale(jc_mul(d2, o.d), 1.0 / 6.0);
+    return o;
+}
+
+/* jetaa.rs:78-103 Jet / Jet::at */
+typedef struct { int ok; double nu; jc f1, f2, f3; } jjet;
+static int jjet_at(const jjet *j, const joff *o, double *nu, float *de, double *x, double *y) {
+    double re = 1.0 + jc_mul(j->f1, o->d).r + jc_mul(j->f2, o->d2h).r + jc_mul(j->f3, o->d3s).r;
+    jc fp = jc_add(jc_add(j->f1, jc_mul(j->f2, o->d)), jc_mul(j->f3, o->d2h));
+    double dd = 2.0 * re / sqrt(fp.r * fp.r + fp.i * fp.i);
+    double n2 = j->nu - log2(re);
+    if (!(re > 0.0 && isfinite(n2) && isfinite(dd) && dd > 0.0)) return 0;
+    *nu = n2;
+    *de = (float)dd;
+    *x = fp.r;
+    *y = fp.i;
+    return 1;
+}
+
+/* samples.rs:53-58 Samples::unit */
+static void samples_unit(uint16_t a, float *c, float *s) {
+    float t = (float)a * (6.2831855f / 65536.0f);
+    *s = sinf(t);
+    *c = cosf(t);
+}
+
+/* jetaa.rs:123-245 rebuild */
+void fdo_jetaa_rebuild(uint8_t *cls, double *nu, float *de, uint16_t *normal, uint64_t w,
+                       uint64_t h, uint8_t *todo) {
+    const uint64_t nx = 2 * w, n = w * h;
+#define JIDX(px, py, k) ((2 * (py) + (k) / 2) * nx + 2 * (px) + (k) % 2)
+#define JREAL(q) JIDX((q) % w, (q) / w, 0)
+    int *gok = (int *)calloc(n ? n : 1, sizeof(int));
+    double *gnu = (double *)calloc(n ? n : 1, sizeof(double));
+    jc *gc = (jc *)calloc(n ? n : 1, sizeof(jc));
+    jjet *jets = (jjet *)calloc(n ? n : 1, sizeof(jjet));
+    for (uint64_t q = 0; q < n; q++) {
+        uint64_t i = JREAL(q);
+        if ((cls[i] & 3) != FDO_ESCAPED) continue;
+        float x, y;
+        samples_unit(normal[i], &x, &y);
+        gok[q] = 1;
+        gnu[q] = nu[i];
+        jc v = {(double)x, (double)y};
+        gc[q] = jc_scale(v, 2.0 / (double)de[i]);
+    }
+    for (uint64_t q = 0; q < n; q++) {
+        uint64_t px = q % w, py = q / w;
+        if (px == 0 || py == 0 || px + 1 >= w || py + 1 >= h || !gok[q]) continue;
+        uint64_t nbr[4] = {q + 1, q - 1, q - w, q + w};
+        jc f[4];
+        int all = 1;
+        for (int k = 0; k < 4; k++) {
+            if (!gok[nbr[k]]) { all = 0; break; }
+            f[k] = jc_scale(gc[nbr[k]], exp2(gnu[q] - gnu[nbr[k]]));
+        }
+        if (!all) continue;
+        jc a = jc_sub(f[0], f[1]), b = jc_sub(f[2], f[3]);
+        jjet *j = &jets[q];
+        j->nu = gnu[q];
+        j->f1 = gc[q];
+        jc t = {a.r + b.i, a.i - b.r};
+        j->f2 = jc_scale(t, 0.25);
+        j->f3 = jc_scale(jc_sub(jc_sub(jc_add(f[0], f[1]), f[2]), f[3]), 0.5);
+        j->ok = isfinite(j->f2.r) && isfinite(j->f2.i) && isfinite(j->f3.r) && isfinite(j->f3.i);
+    }
+    joff own[3] = {joff_new(0.5, 0.0), joff_new(0.0, 0.5), joff_new(0.5, 0.5)};
+    joff oth[3] = {joff_new(-0.5, 0.0), joff_new(0.0, -0.5), joff_new(-0.5, -0.5)};
+    const double M = 3.0;
+    /* pred: 0 compute, 1 own kind, 2 escaped (jetaa.rs:119-121) */
+    uint8_t *what = (uint8_t *)calloc(3 * (n ? n : 1), 1);
+    double *pnu = (double *)calloc(3 * (n ? n : 1), sizeof(double));
+    float *pde = (float *)calloc(3 * (n ? n : 1), sizeof(float));
+    uint16_t *pnm = (uint16_t *)calloc(3 * (n ? n : 1), sizeof(uint16_t));
+    for (uint64_t q = 0; q < n; q++) {
+        uint64_t px = q % w, py = q / w;
+        if (px == 0 || py == 0 || px + 2 >= w || py + 2 >= h) continue;
+        uint64_t r0 = JREAL(q);
+        if (!gok[q]) {
+            int k0 = cls[r0] & 3, same = 1;
+            uint64_t rs[5] = {q + 1, q + w, q + w + 1, q - 1, q - w};
+            for (int k = 0; k < 5; k++) same &= (cls[JREAL(rs[k])] & 3) == k0;
+            if (same) what[3 * q] = what[3 * q + 1] = what[3 * q + 2] = 1;
+            continue;
+        }
+        if (de[r0] < 0.3f || !jets[q].ok) continue;
+        uint64_t nb[3] = {q + 1, q + w, q + w + 1};
+        for (int k = 0; k < 3; k++) {
+            double an, ax, ay, bn, bx, by;
+            float ad, bd;
+            if (!jjet_at(&jets[q], &own[k], &an, &ad, &ax, &ay)) continue;
+            if (!jets[nb[k]].ok || !jjet_at(&jets[nb[k]], &oth[k], &bn, &bd, &bx, &by)) continue;
+            if (fabs(an - bn) >= M * 1e-3 || fabs(log((double)(ad / bd))) >= M * 1e-2) continue;
+            double na = sqrt(ax * ax + ay * ay), nb2 = sqrt(bx * bx + by * by);
+            jc ua = {ax / na, ay / na}, ub = {bx / nb2, by / nb2};
+            double cross = ua.r * ub.i - ua.i * ub.r, dot = ua.r * ub.r + ua.i * ub.i;
+            if (dot > 0.0 && fabs(cross) < sin(M * 3e-3)) {
+                what[3 * q + k] = 2;
+                pnu[3 * q + k] = 0.5 * (an + bn);
+                pde[3 * q + k] = 0.5f * (ad + bd);
+                pnm[3 * q + k] = sample_angle(ua.r + ub.r, ua.i + ub.i);
+            }
+        }
+    }
+    memset(todo, 0, 4 * n);
+    for (uint64_t q = 0; q < n; q++) {
+        uint64_t px = q % w, py = q / w;
+        uint8_t inside = cls[JREAL(q)];
+        for (int k = 0; k < 3; k++) {
+            uint64_t i = JIDX(px, py, k + 1);
+            if (what[3 * q + k] == 0) {
+                todo[i] = 1;
+            } else if (what[3 * q + k] == 1) {
+                cls[i] = (inside & 3) == 3 ? inside : (uint8_t)(inside & 3);
+            } else {
+                cls[i] = FDO_ESCAPED;
+                nu[i] = pnu[3 * q + k];
+                de[i] = pde[3 * q + k];
+                normal[i] = pnm[3 * q + k];
+            }
+        }
+    }
+#undef JIDX
+#undef JREAL
+    free(gok);
+    free(gnu);
+    free(gc);
+    free(jets);
+    free(what);
+    free(pnu);
+    free(pde);
+    free(pnm);
+}
+
+/* jetaa.rs:248-261 merge */
+void fdo_jetaa_merge(uint8_t *cls, double *nu, float *de, uint16_t *normal, const uint8_t *fcls,
+                     const double *fnu, const float *fde, const uint16_t *fnormal,
+                     const uint8_t *mask, uint64_t n) {
+    for (uint64_t i = 0; i < n; i++) {
+        if (!mask[i]) continue;
+        cls[i] = fcls[i];
+        nu[i] = fnu[i];
+        de[i] = fde[i];
+        normal[i] = fnormal[i];
+    }
+}
+
 /* ---------------------------------------------------------------- fd-samples */
 
 typedef struct {

@@ -156,6 +156,14 @@ int fdo_render_series(const char *re, const char *im, const char *width, double 
                       fdo_outcomes *oc, fdo_columns *cols, fdo_stats *stats, char *kernel_id,
                       size_t kcap, char *err, size_t errcap);
 
+/* jetaa.rs first / rebuild / merge on an ss 2 grid of w x h pixels. */
+void fdo_jetaa_first(uint64_t w, uint64_t h, uint8_t *mask);
+void fdo_jetaa_rebuild(uint8_t *cls, double *nu, float *de, uint16_t *normal, uint64_t w,
+                       uint64_t h, uint8_t *todo);
+void fdo_jetaa_merge(uint8_t *cls, double *nu, float *de, uint16_t *normal, const uint8_t *fcls,
+                     const double *fnu, const float *fde, const uint16_t *fnormal,
+                     const uint8_t *mask, uint64_t n);
+
 int fdo_bla_find_many(const double *re, const double *im, uint64_t len, double eps, double dc_max,
                       int64_t dc_exp, const uint64_t *ms, uint64_t nm, double d2, uint64_t budget,
                       int32_t *out_level);
