@@ -353,6 +353,14 @@ int fdg_create(fdg_ctx** out, int device) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
+    if (hipHostMalloc((void**)&c->small, fdg_ctx::kSmallBytes, hipHostMallocDefault) !=
+        hipSuccess) {
+        g_create_error = "cannot allocate page-locked host memory";
+        for (auto& x : c->ev) (void)hipEventDestroy(x);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return FDG_ENOMEM;
+    }
     *out = c;
     return FDG_OK;
 }
@@ -365,6 +373,7 @@ void fdg_destroy(fdg_ctx* c) {
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->small) (void)hipHostFree(c->small);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -618,8 +627,13 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
     bool deriv = (p->columns & (FDG_COL_DE | FDG_COL_NORMAL)) != 0;
     int grid = (int)std::min<uint64_t>((f->count + 255) / 256, (uint64_t)ctx->cus * 8);
     if (grid < 1) grid = 1;
-    FDG_TRY(ctx, hipMemcpyAsync(b.fdev.p, &d, sizeof(d), hipMemcpyHostToDevice, ctx->stream),
-            "frame upload");
+    static_assert(sizeof(FrameDev) <= 4096, "kernel descriptor slot");
+    auto upload = [&](DevBuf& dst, const FrameDev& src, size_t slot) {
+        std::memcpy(ctx->small + slot, &src, sizeof(src));
+        return hipMemcpyAsync(dst.p, ctx->small + slot, sizeof(src), hipMemcpyHostToDevice,
+                              ctx->stream);
+    };
+    FDG_TRY(ctx, upload(b.fdev, d, fdg_ctx::kSlotDesc0), "frame upload");
     auto count_levels = [&](DevBuf& fdev) {
         return launch_count_levels(b.bla_valid.as<uint32_t>(), b.bla_steps, kMaxLevels,
                                    fdev.as<FrameDev>(), ctx->stream);
@@ -641,9 +655,7 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
         d.ab32 = b.ab32.as<float>();
         d.abexp32 = b.abexp32.as<int32_t>();
         d.defer_escaped = mixed ? 1u : 0u;
-        FDG_TRY(ctx,
-                hipMemcpyAsync(b.fdev.p, &d, sizeof(d), hipMemcpyHostToDevice, ctx->stream),
-                "frame upload");
+        FDG_TRY(ctx, upload(b.fdev, d, fdg_ctx::kSlotDesc1), "frame upload");
         if (built) FDG_TRY(ctx, count_levels(b.fdev), "level count");
     }
     FDG_TRY(ctx, hipEventRecord(ctx->ev[2], ctx->stream), "event");
@@ -684,30 +696,35 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
         d2.defer_escaped = 0;
         d2.next = b.counters2.as<uint32_t>();
         d2.stats = reinterpret_cast<unsigned long long*>(b.counters2.as<uint8_t>() + 64);
-        FDG_TRY(ctx,
-                hipMemcpyAsync(b.fdev2.p, &d2, sizeof(d2), hipMemcpyHostToDevice, ctx->stream),
-                "frame upload");
+        FDG_TRY(ctx, upload(b.fdev2, d2, fdg_ctx::kSlotDesc2), "frame upload");
         if (built) FDG_TRY(ctx, count_levels(b.fdev2), "level count");
         FDG_TRY(ctx,
                 launch_perturb_f64(b.fdev2.as<FrameDev>(), f->fr.tier == TIER_SCALED, deriv,
                                    built, false, grid, ctx->stream, true),
                 "refinement launch");
         FDG_TRY(ctx,
-                hipMemcpyAsync(cnt2, d2.stats, sizeof(cnt2), hipMemcpyDeviceToHost, ctx->stream),
+                hipMemcpyAsync(ctx->small + fdg_ctx::kSlotCounts2, d2.stats, sizeof(cnt2),
+                               hipMemcpyDeviceToHost, ctx->stream),
                 "readback");
     }
     FDG_TRY(ctx, hipEventRecord(ctx->ev[3], ctx->stream), "event");
     unsigned long long cnt[ST_COUNT];
+    static_assert(sizeof(cnt) <= 1024, "counter slot");
     FDG_TRY(ctx,
-            hipMemcpyAsync(cnt, d.stats, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream),
+            hipMemcpyAsync(ctx->small + fdg_ctx::kSlotCounts, d.stats, sizeof(cnt),
+                           hipMemcpyDeviceToHost, ctx->stream),
             "readback");
     uint32_t valid[kMaxLevels];
+    static_assert(sizeof(valid) <= 1024, "level flag slot");
     if (built)
         FDG_TRY(ctx,
-                hipMemcpyAsync(valid, b.bla_valid.p, sizeof(valid), hipMemcpyDeviceToHost,
-                               ctx->stream),
+                hipMemcpyAsync(ctx->small + fdg_ctx::kSlotValid, b.bla_valid.p, sizeof(valid),
+                               hipMemcpyDeviceToHost, ctx->stream),
                 "readback");
     FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "perturbation kernel");
+    std::memcpy(cnt, ctx->small + fdg_ctx::kSlotCounts, sizeof(cnt));
+    if (mixed) std::memcpy(cnt2, ctx->small + fdg_ctx::kSlotCounts2, sizeof(cnt2));
+    if (built) std::memcpy(valid, ctx->small + fdg_ctx::kSlotValid, sizeof(valid));
     uint32_t levels = 0;
     if (built) {
         int rc = finish_table(ctx, b, valid, &levels, &s.bla_seconds);

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <thread>
+#include <utility>
 
 namespace fdg {
 
@@ -630,6 +631,107 @@ void orbit_worker(OrbitShared* sh, int first_role, int last_role, bool emit) {
     }
 }
 
+// The same loop with the limb count a compile-time constant, for the shallow tiers (up to
+// 16 limbs, ~960 bits: the f64-to-1e-280 frames), where the per-step overhead of the
+// general loop (calls, length tests, the high-half check) outweighs the squares
+// themselves; the fixed loops unroll.  Same operations, same bits.
+template <size_t L>
+struct Small {
+    static void sqr(const uint64_t* a, uint64_t* o) {
+        uint64_t prod[2 * L] = {};
+        for (size_t i = 0; i + 1 < L; i++) {
+            uint64_t carry = 0;
+            for (size_t j = i + 1; j < L; j++) {
+                u128 t = (u128)a[i] * a[j] + prod[i + j] + carry;
+                prod[i + j] = (uint64_t)t;
+                carry = (uint64_t)(t >> 64);
+            }
+            prod[i + L] = carry;
+        }
+        uint64_t top = 0;
+        for (size_t i = 0; i < 2 * L; i++) {
+            uint64_t nt = prod[i] >> 63;
+            prod[i] = (prod[i] << 1) | top;
+            top = nt;
+        }
+        uint64_t carry = 0;
+        for (size_t i = 0; i < L; i++) {
+            u128 sq = (u128)a[i] * a[i];
+            u128 t = (u128)prod[2 * i] + (uint64_t)sq + carry;
+            prod[2 * i] = (uint64_t)t;
+            u128 t2 = (u128)prod[2 * i + 1] + (uint64_t)(sq >> 64) + (uint64_t)(t >> 64);
+            prod[2 * i + 1] = (uint64_t)t2;
+            carry = (uint64_t)(t2 >> 64);
+        }
+        for (size_t i = 0; i < L; i++) o[i] = prod[L - 1 + i];
+    }
+    static void add(const uint64_t* a, const uint64_t* b, uint64_t* o) {
+        uint64_t carry = 0;
+        for (size_t i = 0; i < L; i++) {
+            u128 t = (u128)a[i] + b[i] + carry;
+            o[i] = (uint64_t)t;
+            carry = (uint64_t)(t >> 64);
+        }
+    }
+    static void sub(const uint64_t* a, const uint64_t* b, uint64_t* o) {
+        uint64_t borrow = 0;
+        for (size_t i = 0; i < L; i++) {
+            u128 d = (u128)a[i] - b[i] - borrow;
+            o[i] = (uint64_t)d;
+            borrow = (uint64_t)(d >> 64) & 1;
+        }
+    }
+    static bool nonzero(const uint64_t* a) {
+        uint64_t any = 0;
+        for (size_t i = 0; i < L; i++) any |= a[i];
+        return any != 0;
+    }
+    static bool less(const uint64_t* a, const uint64_t* b) {
+        for (size_t i = L; i-- > 0;)
+            if (a[i] != b[i]) return a[i] < b[i];
+        return false;
+    }
+    static bool add_signed(bool an, const uint64_t* a, bool bn, const uint64_t* b, uint64_t* o) {
+        if (an == bn) {
+            add(a, b, o);
+            return an && nonzero(o);
+        }
+        if (less(a, b)) {
+            sub(b, a, o);
+            return bn;
+        }
+        sub(a, b, o);
+        return an && nonzero(o);
+    }
+    static void orbit(const Fixed& cr, const Fixed& ci, uint64_t steps, Orbit* out) {
+        uint64_t x[L] = {}, y[L] = {}, x2[L], y2[L], s[L], s2[L], t[L];
+        bool xn = false, yn = false;
+        for (uint64_t k = 0; k < steps; k++) {
+            sqr(x, x2);
+            sqr(y, y2);
+            add_signed(xn, x, yn, y, s);
+            sqr(s, s2);
+            bool tn = add_signed(false, x2, true, y2, t);
+            xn = add_signed(tn, t, cr.neg, cr.mag.data(), x);
+            add(x2, y2, s);
+            tn = add_signed(false, s2, true, s, t);
+            yn = add_signed(tn, t, ci.neg, ci.mag.data(), y);
+            double fx = to_f64_limbs(xn, x, L), fy = to_f64_limbs(yn, y, L);
+            out->re.push_back(fx);
+            out->im.push_back(fy);
+            if (fx * fx + fy * fy > 4294967296.0) break;
+        }
+    }
+};
+
+template <size_t... Ls>
+bool orbit_small(size_t l, const Fixed& cr, const Fixed& ci, uint64_t steps, Orbit* out,
+                 std::index_sequence<Ls...>) {
+    bool done = false;
+    ((l == Ls + 3 ? (Small<Ls + 3>::orbit(cr, ci, steps, out), done = true) : false), ...);
+    return done;
+}
+
 }  // namespace
 
 bool Orbit::fixed(const std::string& re, const std::string& im, uint64_t bits, uint64_t max_iter,
@@ -643,6 +745,11 @@ bool Orbit::fixed(const std::string& re, const std::string& im, uint64_t bits, u
     o.im.reserve(steps + 1);
     o.re.push_back(0.0);
     o.im.push_back(0.0);
+    if (orbit_small(l, cr, ci, steps, &o, std::make_index_sequence<14>())) {  // 3..16 limbs
+        o.bits = (uint32_t)bits;
+        *out = std::move(o);
+        return true;
+    }
     // Three threads from kOrbitThreadLimbs limbs up, where a square outweighs a barrier.
     const bool threaded = l >= limbs::kOrbitThreadLimbs && std::thread::hardware_concurrency() >= 3;
     OrbitShared sh(l, threaded ? 3 : 1);

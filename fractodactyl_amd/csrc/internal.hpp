@@ -83,6 +83,14 @@ struct fdg_ctx {
     std::string err;
     void* pinned = nullptr;  // grow-only page-locked staging for outcome readback
     size_t pinned_bytes = 0;
+    // Page-locked slots for a frame pass's small transfers (kernel descriptors up, counters
+    // and level flags down): copies from pageable memory are staged by the runtime and
+    // cost more than these few hundred bytes.
+    uint8_t* small = nullptr;
+    static constexpr size_t kSmallBytes = 16384;
+    static constexpr size_t kSlotDesc0 = 0, kSlotDesc1 = 4096, kSlotDesc2 = 8192,
+                            kSlotCounts = 12288, kSlotCounts2 = 12288 + 1024,
+                            kSlotValid = 12288 + 2048;
 
     int fail(int code, const std::string& m) {
         err = m;
