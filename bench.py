@@ -142,6 +142,27 @@ def end_to_end(fa, view, params, route, reps=2):
                 parts_ms={k.replace("_seconds", ""): v for k, v in parts.items()})
 
 
+def end_to_end_device_encode(fa, view, params, route, reps=2):
+    """end_to_end with Store::put on the GPU (fdg_frame_read_columns): the frame opened,
+    run and read as 15-byte columns."""
+    def once():
+        t = time.perf_counter()
+        with fa.ResidentFrame(view, params, route) as f:
+            f.run()
+            _, _, st = f.read_columns()
+        return time.perf_counter() - t, st
+    once()
+    walls, st = [], None
+    for _ in range(reps):
+        w, st = once()
+        walls.append(w)
+    w = sum(walls) / len(walls)
+    return dict(ms_per_frame=w * 1e3, value=st["equivalent"] / w,
+                parts_ms={k.replace("_seconds", ""): st[k] * 1e3
+                          for k in ("reference_seconds", "bla_seconds", "kernel_seconds",
+                                    "transfer_seconds", "encode_seconds")})
+
+
 def agreement(ref, other):
     """Per-sample agreement of another delta form with the FP64 form's outcomes."""
     import numpy as np
@@ -451,6 +472,7 @@ def main():
     }
     # PCIe- and host-inclusive: the whole fdg_render call on the headline frame
     out["config"]["end_to_end"] = end_to_end(fa, view, params, route)
+    out["config"]["end_to_end_device_encode"] = end_to_end_device_encode(fa, view, params, route)
     if not a.no_cpu_baseline:
         t = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline(view, params, route == fa.ROUTE_BLA, a.cpu_seconds,

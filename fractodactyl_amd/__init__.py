@@ -135,6 +135,8 @@ SIGNATURES = {
     "fdg_frame_read": (C.c_int, [C.c_void_p, C.POINTER(_Columns), C.POINTER(_Outcomes),
                                  C.POINTER(_Stats), C.c_char_p, C.c_size_t]),
     "fdg_frame_close": (None, [C.c_void_p]),
+    "fdg_frame_read_columns": (C.c_int, [C.c_void_p, C.POINTER(_Columns), C.POINTER(_Stats),
+                                         C.c_char_p, C.c_size_t]),
     "fdg_frame_mask": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fdg_render_masked": (C.c_int, [C.c_void_p, C.POINTER(_View), C.POINTER(_Params), C.c_int,
                                     C.c_int, C.c_double, C.c_void_p, C.POINTER(_Columns),
@@ -361,6 +363,18 @@ class ResidentFrame:
         sr = _series_from(out, buf)
         self._series = sr if ok.value else None
         return sr, bool(ok.value)
+
+    def read_columns(self):
+        """The columns encoded on the GPU (fdg_frame_read_columns): 15 bytes per sample
+        read back; nu/de/normal within an ulp or a quantum of read()'s host encoding."""
+        cols, cs, _ = _out_buffers(self.params, False)
+        st = _Stats()
+        kid = C.create_string_buffer(256)
+        self.ctx._check(lib().fdg_frame_read_columns(self._h, C.byref(cs), C.byref(st), kid,
+                                                     256))
+        header = dict(view=self.view, params=self.params, kernel=kid.value.decode(),
+                      columns=self.params.columns | COL_CLASS)
+        return header, cols, {k: getattr(st, k) for k, _ in _Stats._fields_}
 
     def jet(self):
         """A jet-AA frame (fdg_frame_jet; crates/fd-cli/src/film.rs:1359-1457): pass 1 on

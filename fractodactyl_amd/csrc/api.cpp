@@ -787,6 +787,96 @@ int fdg_frame_read(fdg_frame* f, fdg_columns* out, fdg_outcomes* raw, fdg_stats*
     return FDG_OK;
 }
 
+int fdg_frame_read_columns(fdg_frame* f, fdg_columns* out, fdg_stats* st, char* kernel_id,
+                           size_t kernel_cap) {
+    if (!f) return FDG_EINVAL;
+    fdg_ctx* ctx = f->ctx;
+    if (!f->ran) return ctx->fail(FDG_EINVAL, "frame has not been run");
+    const fdg_params* p = &f->params;
+    if (!out || !out->class_) return ctx->fail(FDG_EINVAL, "the class column is required");
+    if ((p->columns & FDG_COL_NU && !out->nu) || (p->columns & FDG_COL_DE && !out->de) ||
+        (p->columns & FDG_COL_NORMAL && !out->normal))
+        return ctx->fail(FDG_EINVAL, "a requested column has no buffer");
+    FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    fdg_stats s = f->last;
+    const size_t n = f->count;
+    const bool want_nu = p->columns & FDG_COL_NU, want_de = p->columns & FDG_COL_DE,
+               want_nm = p->columns & FDG_COL_NORMAL;
+    Buffers& b = f->buf;
+    FDG_TRY(ctx, b.c_cls.ensure(n), "hipMalloc");
+    if (want_nu) FDG_TRY(ctx, b.c_nu.ensure(8 * n), "hipMalloc");
+    if (want_de) FDG_TRY(ctx, b.c_de.ensure(4 * n), "hipMalloc");
+    if (want_nm) FDG_TRY(ctx, b.c_nm.ensure(2 * n), "hipMalloc");
+    const Plane& pl = f->fr.pl;
+    EncodeDev e{};
+    e.kind = b.o_kind.as<uint8_t>();
+    e.n = b.o_n.as<uint64_t>();
+    e.zr = b.o_zr.as<double>();
+    e.zi = b.o_zi.as<double>();
+    e.dr = b.o_dr.as<double>();
+    e.di = b.o_di.as<double>();
+    e.dexp = b.o_dexp.as<int64_t>();
+    e.cls = b.c_cls.as<uint8_t>();
+    e.nu = want_nu ? b.c_nu.as<double>() : nullptr;
+    e.de = want_de ? b.c_de.as<float>() : nullptr;
+    e.normal = want_nm ? b.c_nm.as<uint16_t>() : nullptr;
+    e.count = (uint32_t)n;
+    e.px_m = pl.h_m * (double)p->ss;
+    e.px_e = pl.h_e;
+    e.cos_r = pl.cos_r;
+    e.sin_r = pl.sin_r;
+    double t0 = now_s();
+    FDG_TRY(ctx, launch_encode_columns(e, ctx->cus, ctx->stream), "column encoding");
+    // columns into the pinned staging area, 8-byte aligned each, then into the caller's
+    const size_t a1 = (n + 7) & ~(size_t)7, a8 = 8 * n, a4 = (4 * n + 7) & ~(size_t)7;
+    const size_t need = a1 + (want_nu ? a8 : 0) + (want_de ? a4 : 0) + (want_nm ? 2 * n : 0);
+    int rc = ctx->staging(need);
+    if (rc) return rc;
+    uint8_t* base = static_cast<uint8_t*>(ctx->pinned);
+    uint8_t* h_cls = base;
+    double* h_nu = reinterpret_cast<double*>(base + a1);
+    float* h_de = reinterpret_cast<float*>(base + a1 + (want_nu ? a8 : 0));
+    uint16_t* h_nm = reinterpret_cast<uint16_t*>(base + a1 + (want_nu ? a8 : 0) +
+                                                 (want_de ? a4 : 0));
+    FDG_TRY(ctx, hipMemcpyAsync(h_cls, e.cls, n, hipMemcpyDeviceToHost, ctx->stream), "readback");
+    if (want_nu)
+        FDG_TRY(ctx, hipMemcpyAsync(h_nu, e.nu, 8 * n, hipMemcpyDeviceToHost, ctx->stream),
+                "readback");
+    if (want_de)
+        FDG_TRY(ctx, hipMemcpyAsync(h_de, e.de, 4 * n, hipMemcpyDeviceToHost, ctx->stream),
+                "readback");
+    if (want_nm)
+        FDG_TRY(ctx, hipMemcpyAsync(h_nm, e.normal, 2 * n, hipMemcpyDeviceToHost, ctx->stream),
+                "readback");
+    FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "column encoding");
+    double t1 = now_s();
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < (1u << 16)) nt = 1;
+    std::vector<std::thread> th;
+    auto part = [&](unsigned w) {
+        size_t lo = n * w / nt, hi = n * (w + 1) / nt;
+        std::memcpy(out->class_ + lo, h_cls + lo, hi - lo);
+        if (want_nu) std::memcpy(out->nu + lo, h_nu + lo, 8 * (hi - lo));
+        if (want_de) std::memcpy(out->de + lo, h_de + lo, 4 * (hi - lo));
+        if (want_nm) std::memcpy(out->normal + lo, h_nm + lo, 2 * (hi - lo));
+    };
+    for (unsigned w = 1; w < nt; w++) th.emplace_back(part, w);
+    part(0);
+    for (auto& t : th) t.join();
+    s.transfer_seconds += t1 - t0;  // device encoding + the 15-byte columns' readback
+    s.encode_seconds = now_s() - t1;
+    if (st) *st = s;
+    if (kernel_id && kernel_cap) {
+        std::string k = f->fr.pl.kernel(f->fr.tier);
+        if (f->use_bla) k += " bla/1";
+        if (f->has_series) k += " series/0";
+        k += f->form == FDG_FORM_FE32 ? " gpu-hip-fe32/1"
+             : f->form == FDG_FORM_MIXED ? " gpu-hip-mixed/1" : " gpu-hip-f64/1";
+        std::snprintf(kernel_id, kernel_cap, "%s", k.c_str());
+    }
+    return FDG_OK;
+}
+
 int fdg_frame_mask(fdg_frame* f, const uint8_t* mask) {
     if (!f) return FDG_EINVAL;
     fdg_ctx* ctx = f->ctx;

@@ -34,6 +34,8 @@ hipError_t launch_escaped_list(const uint8_t* kind, uint32_t* list, uint32_t* co
 // (FrameDev::series_*); unresolved series starts retry from zero in the same launch.
 hipError_t launch_perturb_f64_series(const FrameDev* f, bool deriv, bool bla, int grid,
                                      hipStream_t stream);
+// Store::put on the device for e.count samples (encode.hip).
+hipError_t launch_encode_columns(const EncodeDev& e, int cus, hipStream_t stream);
 hipError_t launch_perturb_fe32(const FrameDev* f, bool deriv, bool bla, int grid,
                                hipStream_t stream);
 
@@ -128,11 +130,13 @@ struct Buffers {
     DevBuf o_kind, o_n, o_zr, o_zi, o_dr, o_di, o_dexp, o_ops, counters, fdev;
     DevBuf orbit32, ab32, abexp32;  // FP32 copies for the FE32 and mixed forms
     DevBuf list, counters2, fdev2;  // the mixed form's FP64 refinement pass
+    DevBuf c_cls, c_nu, c_de, c_nm;  // columns encoded on the device
     uint64_t bla_stride = 0, bla_steps = 0;
     void release() {
         for (DevBuf* b : {&orbit, &bla_r, &bla_r2, &bla_ab, &bla_alpha, &bla_beta, &bla_valid,
                           &o_kind, &o_n, &o_zr, &o_zi, &o_dr, &o_di, &o_dexp, &o_ops,
-                          &counters, &fdev, &orbit32, &ab32, &abexp32, &list, &counters2, &fdev2})
+                          &counters, &fdev, &orbit32, &ab32, &abexp32, &list, &counters2, &fdev2,
+                          &c_cls, &c_nu, &c_de, &c_nm})
             b->release();
     }
     // Per-sample outcome arrays for n samples, the counters and one FrameDev.

@@ -94,4 +94,21 @@ struct FrameDev {
     uint32_t nframes, per_frame;    // count = nframes * per_frame
 };
 
+// One frame pass's outcomes and the columns the device encoder writes from them
+// (Store::put, store.rs:66-153); nu / de / normal are null when not requested.
+struct EncodeDev {
+    const uint8_t* kind;
+    const uint64_t* n;
+    const double *zr, *zi, *dr, *di;
+    const int64_t* dexp;
+    uint8_t* cls;
+    double* nu;
+    float* de;
+    uint16_t* normal;
+    uint32_t count;
+    double px_m;            // output pixel size h_m ss, times 2^px_e
+    int64_t px_e;
+    double cos_r, sin_r;
+};
+
 }  // namespace fdg

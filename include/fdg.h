@@ -152,6 +152,16 @@ int fdg_frame_read(fdg_frame *frame, fdg_columns *out, fdg_outcomes *raw, fdg_st
                    char *kernel_id, size_t kernel_cap);
 void fdg_frame_close(fdg_frame *frame);
 
+/* fdg_frame_read with Store::put (crates/fd-kernel/src/store.rs:66-153) on the GPU: the
+ * outcomes are encoded into the columns in HBM and only the 15 bytes per sample of the
+ * columns are read back (instead of 53 of raw outcomes).  The class column equals
+ * fdg_frame_read's; nu, de and normal are computed with the device's log2 / atan2, so they
+ * may differ from the host encoding in the last bit (nu, de) or by one quantum (normal) —
+ * as the reference's own columns differ between platforms' libms.  Non-escaped samples
+ * read nu 0, de 0, normal 0. */
+int fdg_frame_read_columns(fdg_frame *frame, fdg_columns *out, fdg_stats *stats,
+                           char *kernel_id, size_t kernel_cap);
+
 /* fd_kernel::render_with_masked / render_bla_masked (crates/fd-kernel/src/grid.rs:173-209):
  * compute only the samples whose mask entry (one byte per sample, row-major over the
  * sample grid, non-zero = compute) is set; the others are left unset — class 0, nu 0, de 0,

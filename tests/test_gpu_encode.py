@@ -1,0 +1,65 @@
+"""Store::put on the GPU (fdg_frame_read_columns) against the host encoder (fdg_frame_read,
+which equals the reference's bytes): the class column exactly, nu and de to a few ulps,
+the normal to one quantum.  The device's log2 / atan2 are not glibc's, so the last bits
+may differ (the reference's own KERN-05 bytes differ from glibc's in 8 nu values);
+the north star's bar for smooth iteration is 1e-3.
+"""
+import numpy as np
+import pytest
+
+import fractodactyl_amd as fa
+from fdsfile import kern05, parse_fds
+
+pytestmark = pytest.mark.gpu
+
+COLS = fa.COL_NU | fa.COL_DE | fa.COL_NORMAL
+FRAMES = ["seahorse", "seahorse-fx", "i-1e-50-rot", "tip-1e-200", "i-1e-1000", "rung-7676"]
+
+
+def compare(host, dev):
+    assert np.array_equal(host["class"], dev["class"])
+    esc = (host["class"] & 3) == 0
+    nh, nd = host["nu"][esc], dev["nu"][esc]
+    ulps = np.abs(nh - nd) / np.spacing(np.abs(nh))
+    assert ulps.max() <= 8, ulps.max()
+    dh, dd = host["de"][esc], dev["de"][esc]
+    fulps = np.abs(dh.astype(np.float64) - dd) / np.spacing(np.abs(dh)).astype(np.float64)
+    assert fulps.max() <= 2, fulps.max()
+    d = (host["normal"][esc].astype(np.int64) - dev["normal"][esc].astype(np.int64)) % 65536
+    assert np.all((d == 0) | (d == 1) | (d == 65535))
+    assert np.all(dev["nu"][~esc] == 0) and np.all(dev["de"][~esc] == 0)
+    return dict(escaped=int(esc.sum()), nu_equal=float((nh == nd).mean()),
+                de_equal=float((dh == dd).mean()),
+                normal_equal=float((host["normal"][esc] == dev["normal"][esc]).mean()))
+
+
+@pytest.mark.parametrize("name", FRAMES)
+@pytest.mark.parametrize("route", [fa.ROUTE_PLAIN, fa.ROUTE_BLA])
+def test_device_columns_match_the_host_encoder(name, route):
+    h, _ = parse_fds(dict(kern05())[name])
+    v = fa.View(h["re"], h["im"], h["width"], h["rotation"])
+    p = fa.Params(h["nx"], h["ny"], 1, h["max_iter"], h["escape_radius"], COLS)
+    with fa.ResidentFrame(v, p, route) as f:
+        f.run()
+        hh, host, _ = f.read()
+        hd, dev, st = f.read_columns()
+    assert hh["kernel"] == hd["kernel"]
+    r = compare(host, dev)
+    print(name, route, r, flush=True)
+    # on these frames the device libm agrees almost everywhere
+    assert r["nu_equal"] > 0.9 and r["normal_equal"] > 0.99
+
+
+def test_device_columns_under_a_mask_and_ss2():
+    h, _ = parse_fds(dict(kern05())["seahorse"])
+    v = fa.View(h["re"], h["im"], h["width"], h["rotation"])
+    p = fa.Params(h["nx"], h["ny"], 2, h["max_iter"], h["escape_radius"], COLS)
+    mask = fa.jetaa_first(h["nx"] // 2, h["ny"] // 2)
+    with fa.ResidentFrame(v, p, fa.ROUTE_BLA) as f:
+        f.mask(mask)
+        f.run()
+        _, host, _ = f.read()
+        _, dev, _ = f.read_columns()
+    compare(host, dev)
+    off = mask == 0
+    assert np.all(dev["class"][off] == 0) and np.all(dev["nu"][off] == 0)
