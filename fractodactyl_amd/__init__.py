@@ -160,6 +160,28 @@ SIGNATURES = {
                                   C.c_uint64]),
     "fdg_frame_jet": (C.c_int, [C.c_void_p, C.POINTER(_Columns), C.POINTER(_Stats), C.c_char_p,
                                 C.c_size_t, C.POINTER(C.c_uint64)]),
+    "fdg_frame_open_orbit": (C.c_int, [C.c_void_p, C.POINTER(_View), C.POINTER(_Params),
+                                       C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_void_p,
+                                       C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "fdg_chunk_id": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
+    "fdg_chunk_orbit_slab": (C.c_int64, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32,
+                                         C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint64]),
+    "fdg_chunk_orbit_slab_read": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64),
+                                            C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                            C.POINTER(C.c_uint32), C.c_void_p, C.c_void_p,
+                                            C.c_uint64]),
+    "fdg_chunk_orbit_manifest": (C.c_int64, [C.c_char_p, C.c_char_p, C.c_uint32, C.c_void_p,
+                                             C.c_uint64, C.c_void_p, C.c_uint64]),
+    "fdg_chunk_orbit_manifest_read": (C.c_int, [C.c_void_p, C.c_uint64, C.c_char_p,
+                                                C.c_char_p, C.c_uint64,
+                                                C.POINTER(C.c_uint32), C.c_void_p, C.c_uint64,
+                                                C.POINTER(C.c_uint64)]),
+    "fdg_chunk_bla": (C.c_int64, [C.c_void_p, C.c_double, C.c_double, C.c_uint64, C.c_uint32,
+                                  C.c_void_p, C.c_void_p, C.c_uint64]),
+    "fdg_chunk_bla_read": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), C.c_void_p,
+                                     C.c_uint64]),
     "fdg_film_open": (C.c_int, [C.c_void_p, C.POINTER(_View), C.c_uint32, C.POINTER(_Params),
                                 C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_void_p)]),
     "fdg_film_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(_Stats)]),
@@ -314,15 +336,29 @@ def _out_buffers(p, raw):
 
 class ResidentFrame:
     """A frame whose orbit, table and outcomes stay in HBM between passes
-    (fdg_frame_open / run / read / close)."""
+    (fdg_frame_open / run / read / close).  orbit = (re, im, bits) supplies the reference
+    orbit instead of computing it (fdg_frame_open_orbit)."""
 
-    def __init__(self, view, params, route=ROUTE_BLA, form=FORM_F64, bla_eps=EPS, ctx=None):
+    def __init__(self, view, params, route=ROUTE_BLA, form=FORM_F64, bla_eps=EPS, ctx=None,
+                 orbit=None):
         self.ctx = ctx or default_context()
         self.view, self.params = view, params
         self._v, self._p = _view(view), _params(params)
         h = C.c_void_p()
-        self.ctx._check(lib().fdg_frame_open(self.ctx._h, C.byref(self._v), C.byref(self._p),
-                                             route, form, bla_eps, C.byref(h)))
+        if orbit is None:
+            self.ctx._check(lib().fdg_frame_open(self.ctx._h, C.byref(self._v),
+                                                 C.byref(self._p), route, form, bla_eps,
+                                                 C.byref(h)))
+        else:
+            re = np.ascontiguousarray(orbit[0], np.float64)
+            im = np.ascontiguousarray(orbit[1], np.float64)
+            if re.size != im.size:
+                raise FdgError(-1, "supplied reference orbit must start at Z_0 = 0 with equal "
+                                   "re/im lengths")
+            self.ctx._check(lib().fdg_frame_open_orbit(self.ctx._h, C.byref(self._v),
+                                                       C.byref(self._p), route, form, bla_eps,
+                                                       _p(re), _p(im), re.size, int(orbit[2]),
+                                                       C.byref(h)))
         self._h = h
 
     def run(self):
@@ -681,3 +717,105 @@ def write(header, samples, minor=1):
     b = bytearray(buf.tobytes())
     b[10:12] = int(minor).to_bytes(2, "little")
     return bytes(b)
+
+
+# ---------------------------------------------------------------- atlas chunks
+
+
+def chunk_id(data):
+    """SHA-256 of a chunk's bytes (fd_atlas::ChunkId::of)."""
+    b = np.frombuffer(bytes(data), np.uint8)
+    out = np.zeros(32, np.uint8)
+    rc = lib().fdg_chunk_id(_p(b) if b.size else None, b.size, _p(out))
+    if rc != 0:
+        raise FdgError(rc, "chunk_id")
+    return out.tobytes()
+
+
+def _sized(call):
+    n = call(None, 0)
+    if n < 0:
+        raise FdgError(int(n), "refused by the chunk format")
+    buf = np.zeros(n, np.uint8)
+    call(_p(buf), n)
+    return buf.tobytes()
+
+
+def orbit_slabs(re, im, bits, size=4096):
+    """OrbitSlab::split + to_chunk: the orbit's slab chunks (bytes), in order."""
+    re = np.ascontiguousarray(re, np.float64)
+    im = np.ascontiguousarray(im, np.float64)
+    out = []
+    for start in range(0, re.size, size):
+        ln = min(size, re.size - start)
+        out.append(_sized(lambda b, c: lib().fdg_chunk_orbit_slab(
+            _p(re[start:]), _p(im[start:]), start, ln, size, bits, b, c)))
+    return out
+
+
+def orbit_slab_read(data):
+    """OrbitSlab::from_chunk: (start, size, bits, re, im)."""
+    b = np.frombuffer(bytes(data), np.uint8)
+    st, ln, sz, bits = C.c_uint64(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+    rc = lib().fdg_chunk_orbit_slab_read(_p(b), b.size, C.byref(st), C.byref(ln), C.byref(sz),
+                                         C.byref(bits), None, None, 0)
+    if rc != 0:
+        raise FdgError(rc, "not a canonical orbit slab chunk")
+    re, im = np.zeros(ln.value), np.zeros(ln.value)
+    lib().fdg_chunk_orbit_slab_read(_p(b), b.size, None, None, None, None, _p(re), _p(im),
+                                    ln.value)
+    return st.value, sz.value, bits.value, re, im
+
+
+def orbit_manifest(center_re, center_im, bits, slab_ids):
+    """OrbitManifest::to_chunk."""
+    ids = np.frombuffer(b"".join(slab_ids), np.uint8)
+    return _sized(lambda b, c: lib().fdg_chunk_orbit_manifest(
+        center_re.encode(), center_im.encode(), bits, _p(ids), len(slab_ids), b, c))
+
+
+def orbit_manifest_read(data):
+    """OrbitManifest::from_chunk: (center_re, center_im, bits, [slab ids])."""
+    b = np.frombuffer(bytes(data), np.uint8)
+    bits, n = C.c_uint32(), C.c_uint64()
+    cap = b.size + 1
+    sre, sim = C.create_string_buffer(cap), C.create_string_buffer(cap)
+    rc = lib().fdg_chunk_orbit_manifest_read(_p(b), b.size, sre, sim, cap, C.byref(bits),
+                                             None, 0, C.byref(n))
+    if rc != 0:
+        raise FdgError(rc, "not a canonical orbit manifest chunk")
+    ids = np.zeros(32 * n.value, np.uint8)
+    lib().fdg_chunk_orbit_manifest_read(_p(b), b.size, None, None, 0, None, _p(ids), n.value,
+                                        None)
+    raw = ids.tobytes()
+    return (sre.value.decode(), sim.value.decode(), bits.value,
+            [raw[32 * k:32 * k + 32] for k in range(n.value)])
+
+
+def bla_chunk(orbit_id, eps, dc_max, points, levels):
+    """BlaTable::to_chunk from build_bla()'s levels (list of (count, 7) arrays)."""
+    blocks = np.ascontiguousarray(np.concatenate(levels, axis=0) if levels else np.zeros((0, 7)))
+    oid = np.frombuffer(bytes(orbit_id), np.uint8)
+    return _sized(lambda b, c: lib().fdg_chunk_bla(_p(oid), eps, dc_max, points, len(levels),
+                                                   _p(blocks), b, c))
+
+
+def bla_chunk_read(data):
+    """BlaTable::from_chunk: (orbit id, eps, dc_max, points, levels)."""
+    b = np.frombuffer(bytes(data), np.uint8)
+    oid = np.zeros(32, np.uint8)
+    eps, dc, pts, nl = C.c_double(), C.c_double(), C.c_uint64(), C.c_uint32()
+    rc = lib().fdg_chunk_bla_read(_p(b), b.size, _p(oid), C.byref(eps), C.byref(dc),
+                                  C.byref(pts), C.byref(nl), None, 0)
+    if rc != 0:
+        raise FdgError(rc, "not a canonical BLA table chunk")
+    steps = max(pts.value - 2, 0)
+    total = sum(steps >> j for j in range(1, nl.value + 1))
+    blocks = np.zeros((max(total, 1), 7))
+    lib().fdg_chunk_bla_read(_p(b), b.size, None, None, None, None, None, _p(blocks), total)
+    levels, off = [], 0
+    for j in range(1, nl.value + 1):
+        c = steps >> j
+        levels.append(blocks[off:off + c].copy())
+        off += c
+    return oid.tobytes(), eps.value, dc.value, pts.value, levels

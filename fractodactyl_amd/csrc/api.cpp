@@ -472,8 +472,25 @@ int fdg_build_bla(fdg_ctx* ctx, const double* re, const double* im, uint64_t len
     return FDG_OK;
 }
 
+static int frame_open(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int route, int form,
+                      double bla_eps, const double* sre, const double* sim, uint64_t slen,
+                      uint32_t sbits, fdg_frame** out);
+
 int fdg_frame_open(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int route, int form,
                    double bla_eps, fdg_frame** out) {
+    return frame_open(ctx, v, p, route, form, bla_eps, nullptr, nullptr, 0, 0, out);
+}
+
+int fdg_frame_open_orbit(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int route,
+                         int form, double bla_eps, const double* re, const double* im,
+                         uint64_t len, uint32_t bits, fdg_frame** out) {
+    if (!re || !im) return ctx ? ctx->fail(FDG_EINVAL, "no orbit") : FDG_EINVAL;
+    return frame_open(ctx, v, p, route, form, bla_eps, re, im, len, bits, out);
+}
+
+static int frame_open(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int route, int form,
+                      double bla_eps, const double* sre, const double* sim, uint64_t slen,
+                      uint32_t sbits, fdg_frame** out) {
     if (!ctx || !out) return FDG_EINVAL;
     *out = nullptr;
     FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
@@ -517,10 +534,34 @@ int fdg_frame_open(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int rou
     f->count = (uint32_t)count64;
     double t0 = now_s();
     Orbit o;
-    rc = compute_orbit(ctx, v, p, fr, &o);
-    if (rc) {
-        delete f;
-        return rc;
+    if (sre) {
+        // grid.rs:458-476: a supplied orbit of the view's precision (exactly 53 bits on the
+        // f64 tier, at least the view's on the deep tiers) starting at Z_0 = 0; the header
+        // then records its precision
+        uint32_t need = fr.tier == TIER_F64 ? 53u : (uint32_t)fr.pl.bits;
+        if (sbits != need && (fr.tier == TIER_F64 || sbits < need)) {
+            delete f;
+            char m[160];
+            std::snprintf(m, sizeof(m), "supplied reference has %u bits; this view needs %s%u",
+                          sbits, fr.tier == TIER_F64 ? "" : "at least ", need);
+            return ctx->fail(FDG_EINVAL, m);
+        }
+        if (slen == 0 || sre[0] != 0.0 || sim[0] != 0.0 || slen > Orbit::kMaxLen) {
+            delete f;
+            return ctx->fail(FDG_EINVAL,
+                             "supplied reference orbit must start at Z_0 = 0 with equal re/im "
+                             "lengths");
+        }
+        o.re.assign(sre, sre + slen);
+        o.im.assign(sim, sim + slen);
+        o.bits = sbits;
+        f->fr.pl.bits = sbits;
+    } else {
+        rc = compute_orbit(ctx, v, p, fr, &o);
+        if (rc) {
+            delete f;
+            return rc;
+        }
     }
     f->reference_seconds = now_s() - t0;
     f->len = o.len();

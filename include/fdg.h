@@ -251,6 +251,46 @@ int fdg_jetaa_merge(fdg_columns *to, const fdg_columns *from, const uint8_t *mas
 int fdg_frame_jet(fdg_frame *frame, fdg_columns *out, fdg_stats *stats, char *kernel_id,
                   size_t kernel_cap, uint64_t *computed);
 
+/* A resident frame over a supplied reference orbit (fd_kernel::render_with / render_bla with
+ * Some((orbit, bits)), crates/fd-kernel/src/grid.rs:95-109,446-476): re/im of len points
+ * computed at `bits` (exactly 53 on the f64 tier, at least the view's precision on the
+ * deep tiers, refused otherwise with the reference's message), starting at Z_0 = 0.  With
+ * the view's own precision the frame equals one whose orbit was computed; the header
+ * records `bits`.  This is how a stored orbit (below) is reused instead of recomputed. */
+int fdg_frame_open_orbit(fdg_ctx *ctx, const fdg_view *view, const fdg_params *params,
+                         int route, int form, double bla_eps, const double *re,
+                         const double *im, uint64_t len, uint32_t bits, fdg_frame **out);
+
+/* Canonical atlas chunks (docs/spec/ATLAS.md:127-333; crates/fd-atlas/src/{chunk,slab,
+ * manifest,bla,sha256}.rs): a 32-byte header stating the numeric contract, the payload
+ * little-endian with canonical floats, zero-padded to 8 bytes; a chunk's id is the SHA-256
+ * of its bytes.  Host-only.  Writers return the byte count (the size needed when buf is
+ * NULL or cap too small) or FDG_EINVAL for content the reference refuses; readers parse,
+ * re-encode and refuse non-canonical bytes (FDG_EINVAL).
+ *   orbit slab (kind 1): points start .. start+len-1 of an orbit cut into slabs of `size`
+ *     (start a multiple of size, 1 <= len <= size, finite), at `bits` precision;
+ *   orbit manifest (kind 9): an orbit bound to its exact decimal centre, its slabs' ids in
+ *     order (32 bytes each);
+ *   BLA table (kind 2): the id of the orbit manifest, eps, dc_max, orbit points and the
+ *     levels 1.. of Bla::build, blocks in fdg_build_bla's layout (7 doubles each). */
+int fdg_chunk_id(const uint8_t *bytes, uint64_t n, uint8_t id[32]);
+int64_t fdg_chunk_orbit_slab(const double *re, const double *im, uint64_t start, uint32_t len,
+                             uint32_t size, uint32_t bits, uint8_t *buf, uint64_t cap);
+int fdg_chunk_orbit_slab_read(const uint8_t *bytes, uint64_t n, uint64_t *start, uint32_t *len,
+                              uint32_t *size, uint32_t *bits, double *re, double *im,
+                              uint64_t cap);
+int64_t fdg_chunk_orbit_manifest(const char *center_re, const char *center_im, uint32_t bits,
+                                 const uint8_t *slab_ids, uint64_t n, uint8_t *buf,
+                                 uint64_t cap);
+int fdg_chunk_orbit_manifest_read(const uint8_t *bytes, uint64_t n, char *center_re,
+                                  char *center_im, uint64_t str_cap, uint32_t *bits,
+                                  uint8_t *slab_ids, uint64_t ids_cap, uint64_t *count);
+int64_t fdg_chunk_bla(const uint8_t orbit_id[32], double eps, double dc_max, uint64_t points,
+                      uint32_t levels, const double *blocks, uint8_t *buf, uint64_t cap);
+int fdg_chunk_bla_read(const uint8_t *bytes, uint64_t n, uint8_t orbit_id[32], double *eps,
+                       double *dc_max, uint64_t *points, uint32_t *levels, double *blocks,
+                       uint64_t cap_blocks);
+
 /* Films: a batch of frames at one centre (the keyframes of a zoom) rendered against ONE
  * reference orbit, as the reference renders a frame with a supplied deeper orbit:
  * fd_kernel::render_with / render_bla with `Some((orbit, bits))` (crates/fd-kernel/src/
