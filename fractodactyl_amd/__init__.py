@@ -189,6 +189,8 @@ SIGNATURES = {
                                 C.POINTER(_Outcomes), C.POINTER(_Stats), C.c_char_p,
                                 C.c_size_t]),
     "fdg_film_close": (None, [C.c_void_p]),
+    "fdg_film_mask": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fdg_film_jet": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
     "fdg_jacobian": (C.c_int, [C.POINTER(_View), C.POINTER(_Params), C.POINTER(_Outcomes),
                                C.c_void_p, C.c_void_p]),
     "fdg_write_fds": (C.c_int64, [C.POINTER(_View), C.POINTER(_Params), C.c_char_p,
@@ -478,6 +480,23 @@ class Film:
         st = _Stats()
         self.ctx._check(lib().fdg_film_run(self._h, launch, grid, C.byref(st)))
         return {k: getattr(st, k) for k, _ in _Stats._fields_}
+
+    def mask(self, masks):
+        """Compute only the set samples in the following runs (fdg_film_mask): masks is
+        nframes x (nx ny) bytes, frame-major; None clears."""
+        self._mask = None if masks is None else np.ascontiguousarray(masks, np.uint8).ravel()
+        n = len(self.views) * self.params.nx * self.params.ny
+        if self._mask is not None and self._mask.size != n:
+            raise FdgError(-1, "film mask must cover every frame's samples")
+        self.ctx._check(lib().fdg_film_mask(self._h, _p(self._mask)))
+
+    def jet(self):
+        """Jet-AA for every frame (fdg_film_jet): returns ([columns per frame], computed)."""
+        outs = [_out_buffers(self.params, False) for _ in self.views]
+        arr = (_Columns * len(outs))(*[cs for _, cs, _ in outs])
+        done = C.c_uint64()
+        self.ctx._check(lib().fdg_film_jet(self._h, arr, C.byref(done)))
+        return [cols for cols, _, _ in outs], done.value
 
     def read(self, k, raw=False):
         cols, cs, oc = _out_buffers(self.params, raw)

@@ -65,6 +65,11 @@ struct fdg_film {
     int graph_grid = 0;
     bool ran = false;
     fdg_stats last;
+    // the kernel descriptors as uploaded (a mask changes them), and the mask itself
+    FrameDev group_desc[NGROUPS];
+    std::vector<FrameDev> frame_desc;
+    DevBuf mask;
+    bool masked = false;
 
     uint64_t count() const { return (uint64_t)nframes * per; }
     int orbit_of(uint32_t k) const { return orbit_of_group(group[k]); }
@@ -87,7 +92,7 @@ struct fdg_film {
         fork = join = nullptr;
         buf.release();
         for (FilmOrbit& o : orb) o.release();
-        for (DevBuf* b : {&batch_frames, &fdev_batch, &fdev_each, &counters, &valid})
+        for (DevBuf* b : {&batch_frames, &fdev_batch, &fdev_each, &counters, &valid, &mask})
             b->release();
     }
 };
@@ -459,6 +464,8 @@ int fdg_film_open(fdg_ctx* ctx, const fdg_view* views, uint32_t nframes, const f
                       hipMemcpyHostToDevice),
             "upload");
     FDG_TRY(ctx, hipMemcpy(f->fdev_batch.p, gd, sizeof(gd), hipMemcpyHostToDevice), "upload");
+    std::memcpy(f->group_desc, gd, sizeof(gd));
+    f->frame_desc = each;
     FDG_TRY(ctx,
             hipMemcpy(f->fdev_each.p, each.data(), nframes * sizeof(FrameDev),
                       hipMemcpyHostToDevice),
@@ -490,6 +497,11 @@ int fdg_film_run(fdg_film* f, int launch, int grid, fdg_stats* st) {
     s.bla_seconds = f->bla_seconds;
     s.bla_levels = *std::max_element(f->levels.begin(), f->levels.end());
     std::vector<unsigned long long> cnt;
+    if (f->masked)
+        // samples outside the mask keep this "not computed" kind through the pass (a fill
+        // on the stream, outside the graph)
+        FDG_TRY(ctx, hipMemsetAsync(f->buf.o_kind.p, kKindUnset, f->count(), ctx->stream),
+                "fill");
     if (launch == FDG_LAUNCH_BATCH) {
         if (!f->exec || f->graph_grid != grid) {
             int rc = film_capture(f, grid);
@@ -585,6 +597,88 @@ int fdg_film_read(fdg_film* f, uint32_t k, fdg_columns* out, fdg_outcomes* raw, 
         std::snprintf(kernel_id, kernel_cap, "%s", kid.c_str());
     }
     return FDG_OK;
+}
+
+int fdg_film_mask(fdg_film* f, const uint8_t* masks) {
+    if (!f) return FDG_EINVAL;
+    fdg_ctx* ctx = f->ctx;
+    FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    const size_t n = f->count();
+    if (masks) {
+        FDG_TRY(ctx, f->mask.ensure(n), "hipMalloc");
+        FDG_TRY(ctx, hipMemcpy(f->mask.p, masks, n, hipMemcpyHostToDevice), "mask upload");
+    }
+    f->masked = masks != nullptr;
+    // the batch kernels index the mask by output sample, the per-frame ones by the
+    // frame's own sample
+    FrameDev gd[NGROUPS];
+    std::memcpy(gd, f->group_desc, sizeof(gd));
+    std::vector<FrameDev> each = f->frame_desc;
+    const uint8_t* m = f->masked ? f->mask.as<uint8_t>() : nullptr;
+    for (auto& d : gd) d.mask = m;
+    for (uint32_t k = 0; k < f->nframes; k++) each[k].mask = m ? m + (size_t)k * f->per : nullptr;
+    FDG_TRY(ctx, hipMemcpy(f->fdev_batch.p, gd, sizeof(gd), hipMemcpyHostToDevice), "upload");
+    FDG_TRY(ctx,
+            hipMemcpy(f->fdev_each.p, each.data(), each.size() * sizeof(FrameDev),
+                      hipMemcpyHostToDevice),
+            "upload");
+    return FDG_OK;
+}
+
+int fdg_film_jet(fdg_film* f, fdg_columns* outs, uint64_t* computed) {
+    if (!f) return FDG_EINVAL;
+    fdg_ctx* ctx = f->ctx;
+    const fdg_params* p = &f->params;
+    if (p->ss != 2) return ctx->fail(FDG_EINVAL, "jet-AA rebuilds ss 2 frames");
+    const uint32_t need = FDG_COL_NU | FDG_COL_DE | FDG_COL_NORMAL;
+    if ((p->columns & need) != need || !outs)
+        return ctx->fail(FDG_EINVAL, "jet-AA needs the nu, de and normal columns");
+    for (uint32_t k = 0; k < f->nframes; k++)
+        if (!outs[k].class_ || !outs[k].nu || !outs[k].de || !outs[k].normal)
+            return ctx->fail(FDG_EINVAL, "jet-AA needs the nu, de and normal columns");
+    const size_t w = p->nx / 2, h = p->ny / 2, per = f->per, n = f->count();
+    // pass 1: every frame's top-left samples, in one batched launch
+    std::vector<uint8_t> masks(n);
+    jetaa_first(w, h, masks.data());
+    for (uint32_t k = 1; k < f->nframes; k++)
+        std::memcpy(masks.data() + (size_t)k * per, masks.data(), per);
+    int rc = fdg_film_mask(f, masks.data());
+    if (!rc) rc = fdg_film_run(f, FDG_LAUNCH_BATCH, 0, nullptr);
+    for (uint32_t k = 0; k < f->nframes && !rc; k++)
+        rc = fdg_film_read(f, k, &outs[k], nullptr, nullptr, nullptr, 0);
+    if (rc) {
+        (void)fdg_film_mask(f, nullptr);
+        return rc;
+    }
+    // the rebuilds, then pass 2: every frame's rejected samples in one launch
+    uint64_t rest = 0;
+    for (uint32_t k = 0; k < f->nframes; k++) {
+        uint8_t* todo = masks.data() + (size_t)k * per;
+        jetaa_rebuild(outs[k].class_, outs[k].nu, outs[k].de, outs[k].normal, w, h, todo);
+        for (size_t i = 0; i < per; i++) rest += todo[i];
+    }
+    if (rest > 0) {
+        rc = fdg_film_mask(f, masks.data());
+        if (!rc) rc = fdg_film_run(f, FDG_LAUNCH_BATCH, 0, nullptr);
+        std::vector<uint8_t> c2(per);
+        std::vector<double> nu2(per);
+        std::vector<float> de2(per);
+        std::vector<uint16_t> nm2(per);
+        fdg_columns second{c2.data(), nu2.data(), de2.data(), nm2.data()};
+        for (uint32_t k = 0; k < f->nframes && !rc; k++) {
+            rc = fdg_film_read(f, k, &second, nullptr, nullptr, nullptr, 0);
+            if (!rc)
+                jetaa_merge(outs[k].class_, outs[k].nu, outs[k].de, outs[k].normal, c2.data(),
+                            nu2.data(), de2.data(), nm2.data(), masks.data() + (size_t)k * per,
+                            per);
+        }
+        if (rc) {
+            (void)fdg_film_mask(f, nullptr);
+            return rc;
+        }
+    }
+    if (computed) *computed = (uint64_t)f->nframes * w * h + rest;
+    return fdg_film_mask(f, nullptr);
 }
 
 void fdg_film_close(fdg_film* f) {

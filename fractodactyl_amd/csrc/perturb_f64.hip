@@ -960,8 +960,11 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
                 if (nb >= count) drained = true;  // the global range is exhausted
             }
             if (!active && have) {
-                if (B) {
-                    uint32_t f = my / fp->per_frame, loc = my - f * fp->per_frame;
+                // batches: the mask (when set) covers every frame's samples, frame-major
+                uint32_t bf = B ? my / fp->per_frame : 0u;
+                bool bmask = B && fp->mask && !fp->mask[fp->frames[bf].out0 + (my - bf * fp->per_frame)];
+                if (B && !bmask) {
+                    uint32_t f = bf, loc = my - f * fp->per_frame;
                     const BatchFrameDev& g = fp->frames[f];
                     idx = g.out0 + loc;
                     uint32_t i = loc % nx, j = loc / nx;

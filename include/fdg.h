@@ -321,6 +321,15 @@ int fdg_film_run(fdg_film *film, int launch, int grid, fdg_stats *stats);
 int fdg_film_read(fdg_film *film, uint32_t frame, fdg_columns *out, fdg_outcomes *raw,
                   fdg_stats *stats, char *kernel_id, size_t kernel_cap);
 void fdg_film_close(fdg_film *film);
+/* fdg_film_mask: compute only the samples whose byte is set in the following runs — masks
+ * holds nframes x nx ny bytes, frame-major (NULL clears); others are left unset as in
+ * fdg_frame_mask.  fdg_film_jet: jet-AA for every frame of an ss 2 film (columns nu, de,
+ * normal; outs holds nframes column sets): each pass is one batched persistent launch over
+ * all frames (the graph of fdg_film_run) — pass 1 every frame's top-left samples, the host
+ * rebuilds, pass 2 every frame's rejected samples — merged as fdg_frame_jet merges;
+ * *computed receives the samples computed over the film.  The mask is cleared after. */
+int fdg_film_mask(fdg_film *film, const uint8_t *masks);
+int fdg_film_jet(fdg_film *film, fdg_columns *outs, uint64_t *computed);
 
 /* fd_kernel::reference (grid.rs:84-87): the orbit the renderer uses and its precision in
  * bits.  With cap too small, only *len and *bits are written.  Host-only: ctx may be NULL. */

@@ -148,3 +148,44 @@ def test_film_refusals_and_centres(centre):
     with fa.Film([fa.View("-0.75000", "0.1", "1e-20"), fa.View("-75e-2", "0.10", "1e-30")],
                  p) as f:
         f.run()
+
+
+def oracle_jet_frames(c, ws, orbit):
+    """Each frame's jet-AA pipeline (film.rs:1359-1457) on the oracle, from the orbit its
+    group shares, per-frame tables."""
+    out, total = [], 0
+    w, h = NX // 2, NY // 2
+    for wd in ws:
+        f64 = oracle.plane(c[0], c[1], wd, 0.0, NX, NY)["tier"] == oracle.TIER_F64
+        kw = dict(columns=COLS, ss=2, bla=True, outcomes=False, orbit=None if f64 else orbit)
+        first = oracle.jetaa_first(w, h)
+        cols = oracle.render(c[0], c[1], wd, 0.0, NX, NY, ITERS, mask=first, **kw)["columns"]
+        todo = oracle.jetaa_rebuild(cols, w, h)
+        if todo.any():
+            s2 = oracle.render(c[0], c[1], wd, 0.0, NX, NY, ITERS, mask=todo, **kw)["columns"]
+            oracle.jetaa_merge(cols, s2, todo)
+        out.append(cols)
+        total += w * h + int(todo.sum())
+    return out, total
+
+
+@pytest.mark.gpu
+def test_film_jet_matches_oracle(centre):
+    # both jet-AA passes of every keyframe in one batched launch each, against each
+    # frame's jet pipeline on the oracle from the same shared orbit
+    ws = widths()
+    orbit = deep_orbit(centre, ws)
+    want, total = oracle_jet_frames(centre, ws, orbit)
+    views = [fa.View(centre[0], centre[1], w) for w in ws]
+    with fa.Film(views, fa.Params(NX, NY, 2, ITERS, 1e10, COLS)) as film:
+        cols, done = film.jet()
+        assert done == total < len(ws) * NX * NY
+        for k in range(len(ws)):
+            for key in ("class", "nu", "de", "normal"):
+                assert_same(cols[k][key], want[k][key], f"frame {k} {key}")
+        # the mask is cleared: a plain run is each frame's exact ss 2 render again
+        film.run(fa.LAUNCH_BATCH)
+        g = film.read(0)
+        exact = oracle.render(centre[0], centre[1], ws[0], 0.0, NX, NY, ITERS, columns=COLS,
+                              ss=2, bla=True, outcomes=False)["columns"]
+        assert_same(g[1]["nu"], exact["nu"], "nu (mask cleared)")
