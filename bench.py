@@ -381,7 +381,7 @@ def main():
         return
     if a.gpus != 1:
         print(f"bench.py: this path runs on one GPU; --gpus {a.gpus} is not supported "
-              "(the frame shards by tiles, see DESIGN.md)", file=sys.stderr)
+              "(a frame shards by tiles of samples; run one process per GPU on a share of the tiles)", file=sys.stderr)
         sys.exit(2)
 
     import fractodactyl_amd as fa

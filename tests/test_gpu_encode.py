@@ -1,8 +1,9 @@
 """Store::put on the GPU (fdg_frame_read_columns) against the host encoder (fdg_frame_read,
 which equals the reference's bytes): the class column exactly, nu and de to a few ulps,
 the normal to one quantum.  The device's log2 / atan2 are not glibc's, so the last bits
-may differ (the reference's own KERN-05 bytes differ from glibc's in 8 nu values);
-the north star's bar for smooth iteration is 1e-3.
+may differ (the reference's own KERN-05 bytes differ from glibc's in 8 nu values), far
+inside the 1e-3 to which smooth iteration is compared against the arbitrary-precision
+truth.
 """
 import numpy as np
 import pytest
