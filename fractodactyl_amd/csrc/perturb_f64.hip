@@ -635,18 +635,20 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         if ((k & 1u) == 0 && (s.nucleus == 0 || s.m != 0)) {
             double d2 = EXT ? (s.xr * s.xr + s.xi * s.xi) * scale_of<EXT>(s) * scale_of<EXT>(s)
                             : s.xr * s.xr + s.xi * s.xi;
-            // holds and deferred saves exist on the ordinary tiers only
-            uint32_t b = EXT ? s.chk
-                             : (s.hold > 0 ? s.sn + s.hold
-                                           : ((s.flags & kPending) ? 0xffffffffu : s.chk));
-            b = b < h.max_iter ? b : h.max_iter;
-            uint32_t nb = s.nucleus ? (s.n == 0 ? 1u : s.nucleus + 1) : 0xffffffffu;
-            b = b < nb ? b : nb;
-            b = b > s.n ? b - s.n : 0;
-            int cap = find_cap(k, h.nlevels, b);
-            // level 0 from the carried radius (0 when no such block exists), then the rest
-            if (cap >= 0 && d2 < s.R0)
-                lvl = find_level_down(h.r2, tb, h.stride32, k, cap, d2);
+            // level 0 from the carried radius (0 when no such block exists) first: a wave
+            // whose lanes all fail it skips the budget arithmetic
+            if (d2 < s.R0) {
+                // holds and deferred saves exist on the ordinary tiers only
+                uint32_t b = EXT ? s.chk
+                                 : (s.hold > 0 ? s.sn + s.hold
+                                               : ((s.flags & kPending) ? 0xffffffffu : s.chk));
+                b = b < h.max_iter ? b : h.max_iter;
+                uint32_t nb = s.nucleus ? (s.n == 0 ? 1u : s.nucleus + 1) : 0xffffffffu;
+                b = b < nb ? b : nb;
+                b = b > s.n ? b - s.n : 0;
+                int cap = find_cap(k, h.nlevels, b);
+                if (cap >= 0) lvl = find_level_down(h.r2, tb, h.stride32, k, cap, d2);
+            }
         }
     }
     double Zmr = s.Zr, Zmi = s.Zi;
