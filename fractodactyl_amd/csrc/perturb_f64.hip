@@ -13,6 +13,7 @@
 #include "bla_find.hpp"
 #include "device_math.hpp"
 #include "kernels.hpp"
+#include "lane_stats.hpp"
 
 namespace fdg {
 namespace {
@@ -882,9 +883,13 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
     // 623-649: that attempt's work counts, minus the skipped prefix, and it retries from 0)
     __shared__ uint32_t series_carry[SER ? kBlock : 1];
     // Statistics accumulate in LDS at each sample's end (rare next to the steps), not in
-    // per-lane registers held across the loop.
+    // per-lane registers held across the loop: per-sample counts in the lane's own slots,
+    // folded into the block's counters `stat` at the end.
     __shared__ unsigned long long stat[ST_COUNT];
     if (threadIdx.x < ST_COUNT) stat[threadIdx.x] = 0;
+    __shared__ unsigned long long lane_slots[kLaneStats * kBlock];
+    LaneStats<kBlock> ls{lane_slots};
+    ls.clear();
     __syncthreads();
     const uint32_t lane = __lane_id();
     Rare r(rare, threadIdx.x);
@@ -985,7 +990,7 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
                         o_dr[idx] = 0.0;
                         o_di[idx] = 0.0;
                         o_dexp[idx] = 0;
-                        if (L) atomicAdd(&stat[ST_CLOSED], 1ull);
+                        if (L) ls.add(ST_CLOSED, 1ull);
                     } else {
                         lane_start<EXT>(s, r, h, nucleus, ar, ai, EXT ? (int32_t)g.h_e : 0);
                         s.tb = g.tb;
@@ -1010,7 +1015,7 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
                             o_dr[idx] = 0.0;
                             o_di[idx] = 0.0;
                             o_dexp[idx] = 0;
-                            if (L) atomicAdd(&stat[ST_CLOSED], 1ull);
+                            if (L) ls.add(ST_CLOSED, 1ull);
                         } else {
                             lane_start<EXT>(s, r, h, nucleus, ar, ai, 0);
                             active = true;
@@ -1074,20 +1079,21 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
                     o_di[idx] = esc ? s.di : 0.0;
                     o_dexp[idx] = esc ? s.derivative_exp : 0;
                     unsigned long long spent = (k == 2 ? h.max_iter : s.n) - prefix + carried;
-                    atomicAdd(&stat[ST_EQUIVALENT], spent);
-                    atomicAdd(&stat[ST_ITERATIONS], L ? spent - s.skipped + s.blocks : spent);
+                    ls.add(ST_EQUIVALENT, spent);
+                    ls.add(ST_ITERATIONS, L ? spent - s.skipped + s.blocks : spent);
                     if (L) {
-                        atomicAdd(&stat[ST_BLOCKS], (unsigned long long)s.blocks);
-                        atomicAdd(&stat[ST_SKIPPED], (unsigned long long)s.skipped);
-                        if (s.blocks == 0) atomicAdd(&stat[ST_FALLBACK], 1ull);
+                        ls.add(ST_BLOCKS, (unsigned long long)s.blocks);
+                        ls.add(ST_SKIPPED, (unsigned long long)s.skipped);
+                        if (s.blocks == 0) ls.add(ST_FALLBACK, 1ull);
                     }
-                    atomicAdd(&stat[ST_PLAIN], (unsigned long long)s.plain);
-                    atomicMax(&stat[ST_MAXOPS], (unsigned long long)(s.plain + s.blocks));
+                    ls.add(ST_PLAIN, (unsigned long long)s.plain);
+                    ls.max(ST_MAXOPS, (unsigned long long)(s.plain + s.blocks));
                     active = false;
                 }
             }
         } while (!__ballot(!active) || (drained && __ballot(active)));
     }
+    ls.fold(stat);
     if (lane == 0) {
         atomicAdd(&stat[ST_WAVE_ITERS], witers);
         atomicAdd(&stat[ST_LANE_ITERS], liters);

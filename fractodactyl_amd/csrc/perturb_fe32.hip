@@ -17,6 +17,7 @@
 #include "bla_find.hpp"
 #include "device_math.hpp"
 #include "kernels.hpp"
+#include "lane_stats.hpp"
 
 namespace fdg {
 namespace {
@@ -298,6 +299,9 @@ template <bool D, bool L>
 __global__ void __launch_bounds__(kBlock32, 4) perturb_fe32(const FrameDev* __restrict__ fp) {
     __shared__ unsigned long long stat[ST_COUNT];
     if (threadIdx.x < ST_COUNT) stat[threadIdx.x] = 0;
+    __shared__ unsigned long long lane_slots[kLaneStats * kBlock32];
+    LaneStats<kBlock32> ls{lane_slots};
+    ls.clear();
     __syncthreads();
     const uint32_t lane = __lane_id();
     Hot32 h;
@@ -375,7 +379,7 @@ __global__ void __launch_bounds__(kBlock32, 4) perturb_fe32(const FrameDev* __re
                     o_n[idx] = 0;
                     o_zr[idx] = o_zi[idx] = o_dr[idx] = o_di[idx] = 0.0;
                     o_dexp[idx] = 0;
-                    if (L) atomicAdd(&stat[ST_CLOSED], 1ull);
+                    if (L) ls.add(ST_CLOSED, 1ull);
                 } else {
                     start32(s, h, nucleus, ux * fp->h_m, uy * fp->h_m, aexp);
                     active = true;
@@ -405,21 +409,22 @@ __global__ void __launch_bounds__(kBlock32, 4) perturb_fe32(const FrameDev* __re
                     o_dexp[idx] = esc ? (int64_t)s.de : 0;
                     unsigned long long spent = k == 2 ? h.max_iter : s.n;
                     if (!(esc && fp->defer_escaped)) {
-                        atomicAdd(&stat[ST_EQUIVALENT], spent);
-                        atomicAdd(&stat[ST_ITERATIONS], L ? spent - s.skipped + s.blocks : spent);
+                        ls.add(ST_EQUIVALENT, spent);
+                        ls.add(ST_ITERATIONS, L ? spent - s.skipped + s.blocks : spent);
                         if (L) {
-                            atomicAdd(&stat[ST_BLOCKS], (unsigned long long)s.blocks);
-                            atomicAdd(&stat[ST_SKIPPED], (unsigned long long)s.skipped);
-                            if (s.blocks == 0) atomicAdd(&stat[ST_FALLBACK], 1ull);
+                            ls.add(ST_BLOCKS, (unsigned long long)s.blocks);
+                            ls.add(ST_SKIPPED, (unsigned long long)s.skipped);
+                            if (s.blocks == 0) ls.add(ST_FALLBACK, 1ull);
                         }
-                        atomicAdd(&stat[ST_PLAIN], (unsigned long long)s.plain);
-                        atomicMax(&stat[ST_MAXOPS], (unsigned long long)(s.plain + s.blocks));
+                        ls.add(ST_PLAIN, (unsigned long long)s.plain);
+                        ls.max(ST_MAXOPS, (unsigned long long)(s.plain + s.blocks));
                     }
                     active = false;
                 }
             }
         } while (!__ballot(!active) || (drained && __ballot(active)));
     }
+    ls.fold(stat);
     if (lane == 0) {
         atomicAdd(&stat[ST_WAVE_ITERS], witers);
         atomicAdd(&stat[ST_LANE_ITERS], liters);
