@@ -193,11 +193,11 @@ using namespace fdg;
 
 namespace {
 
-// Build the BLA table on the GPU over the uploaded orbit.  Returns the level count (0 when
-// no level has a valid block) in *levels.
-// Launch the table build on the context's stream without waiting for it (*built: a
-// table exists, i.e. the orbit has at least 4 points); the level count is read once the
-// stream has been synchronised, by finish_table.
+// Launch the BLA table build over the uploaded orbit on the context's stream without
+// waiting for it (*built: a table exists, i.e. the orbit has at least 4 points); the level
+// count comes from the per-level flags once the build has run (finish_table on the host
+// after a synchronisation, or pass_prologue on the device).  The flags are cleared first
+// unless the last pass set-up left them clear.
 int launch_table(fdg_ctx* ctx, Buffers* bf, uint64_t len, double eps, double dc_max,
                  int64_t dc_exp, bool* built) {
     Buffers& B = *bf;
@@ -215,8 +215,12 @@ int launch_table(fdg_ctx* ctx, Buffers* bf, uint64_t len, double eps, double dc_
     FDG_TRY(ctx, B.bla_ab.ensure(stride * 32), "hipMalloc");
     FDG_TRY(ctx, B.bla_alpha.ensure(stride * 8), "hipMalloc");
     FDG_TRY(ctx, B.bla_beta.ensure(stride * 8), "hipMalloc");
+    void* flags_before = B.bla_valid.p;
     FDG_TRY(ctx, B.bla_valid.ensure(kMaxLevels * 4), "hipMalloc");
-    FDG_TRY(ctx, hipMemsetAsync(B.bla_valid.p, 0, kMaxLevels * 4, ctx->stream), "memset");
+    if (B.bla_valid.p != flags_before) B.valid_clean = false;
+    if (!B.valid_clean)
+        FDG_TRY(ctx, hipMemsetAsync(B.bla_valid.p, 0, kMaxLevels * 4, ctx->stream), "memset");
+    B.valid_clean = false;  // the build below sets flags
     FDG_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream), "event");
     FDG_TRY(ctx,
             launch_bla_build(B.orbit.as<double>(), B.bla_r.as<double>(),
@@ -604,18 +608,16 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
     s.nucleus_period = f->nucleus;
     s.transfer_seconds = f->transfer_seconds;
     // The table is built on the stream and the kernel reads its level count from the
-    // frame descriptor, where a one-thread kernel writes it from the build's flags: no host
-    // round trip between the build and the perturbation kernel.  The host learns the count
-    // with the statistics at the end.
+    // frame descriptor, which one small launch (pass_prologue) stores with the count taken
+    // from the build's flags, zeroing the counters in the same launch: no host round trip
+    // between the build and the perturbation kernel.  The host learns the count with the
+    // statistics at the end.
     bool built = false;
     if (f->route == FDG_ROUTE_BLA) {
         int rc = launch_table(ctx, &f->buf, f->len, f->eps, f->dc, f->dc_exp, &built);
         if (rc) return rc;
     }
     Buffers& b = f->buf;
-    // The kernel writes every output field of every sample; only the counters start at 0.
-    FDG_TRY(ctx, launch_zero_words(b.counters.as<unsigned long long>(), 32, ctx->stream),
-            "counter clear");
     FrameDev d;
     std::memset(&d, 0, sizeof(d));
     const Plane& pl = f->fr.pl;
@@ -669,17 +671,9 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
     int grid = (int)std::min<uint64_t>((f->count + 255) / 256, (uint64_t)ctx->cus * 8);
     if (grid < 1) grid = 1;
     static_assert(sizeof(FrameDev) <= 4096, "kernel descriptor slot");
-    auto upload = [&](DevBuf& dst, const FrameDev& src, size_t slot) {
-        std::memcpy(ctx->small + slot, &src, sizeof(src));
-        return hipMemcpyAsync(dst.p, ctx->small + slot, sizeof(src), hipMemcpyHostToDevice,
-                              ctx->stream);
-    };
-    FDG_TRY(ctx, upload(b.fdev, d, fdg_ctx::kSlotDesc0), "frame upload");
-    auto count_levels = [&](DevBuf& fdev) {
-        return launch_count_levels(b.bla_valid.as<uint32_t>(), b.bla_steps, kMaxLevels,
-                                   fdev.as<FrameDev>(), ctx->stream);
-    };
-    if (built) FDG_TRY(ctx, count_levels(b.fdev), "level count");
+    // counters: words 0-7 the work index, 8-23 the statistics (d.stats), word 24 (bytes
+    // 192-195) the table's level count for the host
+    uint32_t* levels_dev = reinterpret_cast<uint32_t*>(b.counters.as<uint8_t>() + 192);
     bool mixed = f->form == FDG_FORM_MIXED;
     bool fe32 = f->form == FDG_FORM_FE32 || mixed;
     if (fe32) {
@@ -696,9 +690,15 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
         d.ab32 = b.ab32.as<float>();
         d.abexp32 = b.abexp32.as<int32_t>();
         d.defer_escaped = mixed ? 1u : 0u;
-        FDG_TRY(ctx, upload(b.fdev, d, fdg_ctx::kSlotDesc1), "frame upload");
-        if (built) FDG_TRY(ctx, count_levels(b.fdev), "level count");
     }
+    // The kernel writes every output field of every sample; only the counters start at 0.
+    FDG_TRY(ctx,
+            launch_pass_prologue(d, b.fdev.as<FrameDev>(),
+                                 built ? b.bla_valid.as<uint32_t>() : nullptr, nullptr,
+                                 b.bla_steps, kMaxLevels, b.counters.as<unsigned long long>(),
+                                 24, levels_dev, true, ctx->stream),
+            "pass set-up");
+    if (built) b.valid_clean = true;
     FDG_TRY(ctx, hipEventRecord(ctx->ev[2], ctx->stream), "event");
     if (fe32)
         FDG_TRY(ctx,
@@ -722,14 +722,7 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
         FDG_TRY(ctx, b.list.ensure((size_t)f->count * 4), "hipMalloc");
         FDG_TRY(ctx, b.counters2.ensure(256), "hipMalloc");
         FDG_TRY(ctx, b.fdev2.ensure(sizeof(FrameDev)), "hipMalloc");
-        // words 0 (work index) and 1 (list length) and the statistics start at 0
-        FDG_TRY(ctx, launch_zero_words(b.counters2.as<unsigned long long>(), 32, ctx->stream),
-                "counter clear");
         uint32_t* listed = b.counters2.as<uint32_t>() + 1;
-        FDG_TRY(ctx,
-                launch_escaped_list(b.o_kind.as<uint8_t>(), b.list.as<uint32_t>(), listed,
-                                    f->count, ctx->stream),
-                "escaped list");
         FrameDev d2 = d;
         d2.mask = nullptr;  // the list holds only computed samples; it is indexed by position
         d2.list = b.list.as<uint32_t>();
@@ -737,8 +730,18 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
         d2.defer_escaped = 0;
         d2.next = b.counters2.as<uint32_t>();
         d2.stats = reinterpret_cast<unsigned long long*>(b.counters2.as<uint8_t>() + 64);
-        FDG_TRY(ctx, upload(b.fdev2, d2, fdg_ctx::kSlotDesc2), "frame upload");
-        if (built) FDG_TRY(ctx, count_levels(b.fdev2), "level count");
+        // words 0 (work index) and 1 (list length) and the statistics start at 0; the
+        // level count is the first pass's
+        FDG_TRY(ctx,
+                launch_pass_prologue(d2, b.fdev2.as<FrameDev>(), nullptr, levels_dev,
+                                     b.bla_steps, kMaxLevels,
+                                     b.counters2.as<unsigned long long>(), 32, nullptr, false,
+                                     ctx->stream),
+                "pass set-up");
+        FDG_TRY(ctx,
+                launch_escaped_list(b.o_kind.as<uint8_t>(), b.list.as<uint32_t>(), listed,
+                                    f->count, ctx->stream),
+                "escaped list");
         FDG_TRY(ctx,
                 launch_perturb_f64(b.fdev2.as<FrameDev>(), f->fr.tier == TIER_SCALED, deriv,
                                    built, false, grid, ctx->stream, true),
@@ -750,26 +753,22 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
     }
     FDG_TRY(ctx, hipEventRecord(ctx->ev[3], ctx->stream), "event");
     unsigned long long cnt[ST_COUNT];
-    static_assert(sizeof(cnt) <= 1024, "counter slot");
+    // the statistics (counter bytes 64-191) and the level count (bytes 192-195) in one copy
+    constexpr size_t kReadback = sizeof(cnt) + 8;
+    static_assert(64 + kReadback <= 256 && kReadback <= 1024, "counter slot");
     FDG_TRY(ctx,
-            hipMemcpyAsync(ctx->small + fdg_ctx::kSlotCounts, d.stats, sizeof(cnt),
+            hipMemcpyAsync(ctx->small + fdg_ctx::kSlotCounts, d.stats, kReadback,
                            hipMemcpyDeviceToHost, ctx->stream),
             "readback");
-    uint32_t valid[kMaxLevels];
-    static_assert(sizeof(valid) <= 1024, "level flag slot");
-    if (built)
-        FDG_TRY(ctx,
-                hipMemcpyAsync(ctx->small + fdg_ctx::kSlotValid, b.bla_valid.p, sizeof(valid),
-                               hipMemcpyDeviceToHost, ctx->stream),
-                "readback");
     FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "perturbation kernel");
     std::memcpy(cnt, ctx->small + fdg_ctx::kSlotCounts, sizeof(cnt));
     if (mixed) std::memcpy(cnt2, ctx->small + fdg_ctx::kSlotCounts2, sizeof(cnt2));
-    if (built) std::memcpy(valid, ctx->small + fdg_ctx::kSlotValid, sizeof(valid));
     uint32_t levels = 0;
     if (built) {
-        int rc = finish_table(ctx, b, valid, &levels, &s.bla_seconds);
-        if (rc) return rc;
+        std::memcpy(&levels, ctx->small + fdg_ctx::kSlotCounts + sizeof(cnt), sizeof(levels));
+        float ms = 0;
+        FDG_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]), "event timing");
+        s.bla_seconds = ms * 1e-3;
     }
     s.bla_levels = levels;
     f->use_bla = levels > 0;

@@ -21,9 +21,13 @@ hipError_t launch_bla_build(const double* orbit, double* r, double* r2, double* 
 hipError_t launch_perturb_f64(const FrameDev* f, bool ext, bool deriv, bool bla, bool batch,
                               int grid, hipStream_t stream, bool list = false);
 hipError_t launch_zero_words(unsigned long long* p, uint32_t n, hipStream_t stream);
-// f->bla.nlevels from a table build's per-level "any valid block" flags (bla.rs:442-445).
-hipError_t launch_count_levels(const uint32_t* valid, uint64_t steps, uint32_t max_levels,
-                               FrameDev* f, hipStream_t stream);
+// A single frame's pass set-up in one launch: the descriptor d stored at dst with the
+// level count (from the build's flags `valid`, else *levels_in), nzero (<= 64) counter
+// words zeroed, the count copied to *levels_out, and with clear the flags reset.
+hipError_t launch_pass_prologue(const FrameDev& d, FrameDev* dst, uint32_t* valid,
+                                const uint32_t* levels_in, uint64_t steps, uint32_t max_levels,
+                                unsigned long long* counters, uint32_t nzero,
+                                uint32_t* levels_out, bool clear, hipStream_t stream);
 hipError_t launch_orbit_to_f32(const double* orbit, float* o32, uint64_t points,
                                hipStream_t stream);
 hipError_t launch_table_to_f32(const double* ab, float* ab32, int32_t* abexp, uint64_t blocks,
@@ -85,14 +89,12 @@ struct fdg_ctx {
     std::string err;
     void* pinned = nullptr;  // grow-only page-locked staging for outcome readback
     size_t pinned_bytes = 0;
-    // Page-locked slots for a frame pass's small transfers (kernel descriptors up, counters
-    // and level flags down): copies from pageable memory are staged by the runtime and
-    // cost more than these few hundred bytes.
+    // Page-locked slots for a frame pass's small readbacks (statistics and the table's level
+    // count): copies into pageable memory are staged by the runtime and cost more than
+    // these few hundred bytes.
     uint8_t* small = nullptr;
-    static constexpr size_t kSmallBytes = 16384;
-    static constexpr size_t kSlotDesc0 = 0, kSlotDesc1 = 4096, kSlotDesc2 = 8192,
-                            kSlotCounts = 12288, kSlotCounts2 = 12288 + 1024,
-                            kSlotValid = 12288 + 2048;
+    static constexpr size_t kSmallBytes = 4096;
+    static constexpr size_t kSlotCounts = 0, kSlotCounts2 = 1024;
 
     int fail(int code, const std::string& m) {
         err = m;
@@ -132,7 +134,11 @@ struct Buffers {
     DevBuf list, counters2, fdev2;  // the mixed form's FP64 refinement pass
     DevBuf c_cls, c_nu, c_de, c_nm;  // columns encoded on the device
     uint64_t bla_stride = 0, bla_steps = 0;
+    // bla_valid holds zeros (a pass set-up cleared the flags after reading them), so the
+    // next build needs no clearing of its own
+    bool valid_clean = false;
     void release() {
+        valid_clean = false;
         for (DevBuf* b : {&orbit, &bla_r, &bla_r2, &bla_ab, &bla_alpha, &bla_beta, &bla_valid,
                           &o_kind, &o_n, &o_zr, &o_zi, &o_dr, &o_di, &o_dexp, &o_ops,
                           &counters, &fdev, &orbit32, &ab32, &abexp32, &list, &counters2, &fdev2,

@@ -1137,18 +1137,50 @@ __global__ void zero_words(unsigned long long* p, uint32_t n) {
     if (i < n) p[i] = 0ull;
 }
 
-// Levels end before the first level without a valid block (bla.rs:442-445).
-__global__ void count_levels(const uint32_t* valid, uint64_t steps, uint32_t max_levels,
-                             FrameDev* f) {
-    if (threadIdx.x != 0) return;
+// Everything a single frame's pass needs between its table build and its perturbation
+// kernel, in one launch of one wave: the descriptor `d` (a kernel argument) stored at
+// `dst` with the table's level count, which comes from the build's per-level flags
+// `valid` (levels end before the first level without a valid block, bla.rs:442-445) or, with valid null, from *levels_in (0 when
+// both are null); the first `nzero` (<= 64) counter words zeroed; the count also written
+// to *levels_out (nullable) for the host's readback; and, with clear, the flags reset to 0
+// for the next build.
+__global__ void __launch_bounds__(64) pass_prologue(FrameDev d, FrameDev* dst, uint32_t* valid,
+                                                    const uint32_t* levels_in, uint64_t steps,
+                                                    uint32_t max_levels,
+                                                    unsigned long long* counters,
+                                                    uint32_t nzero, uint32_t* levels_out,
+                                                    bool clear) {
+    const uint32_t t = threadIdx.x;
+    if (t < nzero) counters[t] = 0ull;
     uint32_t n = 0;
-    while (n < max_levels && (steps >> (n + 1)) > 0 && valid[n]) n++;
-    f->bla.nlevels = n;
+    if (valid) {
+        while (n < max_levels && (steps >> (n + 1)) > 0 && valid[n]) n++;
+    } else if (levels_in) {
+        n = *levels_in;
+    }
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(&d);
+    uint64_t* out = reinterpret_cast<uint64_t*>(dst);
+    for (uint32_t i = t; i < sizeof(FrameDev) / 8; i += 64) out[i] = src[i];
+    // every lane has read the flags and stored its words before lane 0 patches the count
+    // and the flags are cleared
+    __syncthreads();
+    if (t == 0) {
+        dst->bla.nlevels = n;
+        if (levels_out) *levels_out = n;
+    }
+    if (valid && clear)
+        for (uint32_t i = t; i < max_levels; i += 64) valid[i] = 0u;
 }
 
-hipError_t launch_count_levels(const uint32_t* valid, uint64_t steps, uint32_t max_levels,
-                               FrameDev* f, hipStream_t stream) {
-    hipLaunchKernelGGL(count_levels, dim3(1), dim3(64), 0, stream, valid, steps, max_levels, f);
+static_assert(sizeof(FrameDev) % 8 == 0, "descriptor copied in 8-byte words");
+
+hipError_t launch_pass_prologue(const FrameDev& d, FrameDev* dst, uint32_t* valid,
+                                const uint32_t* levels_in, uint64_t steps, uint32_t max_levels,
+                                unsigned long long* counters, uint32_t nzero,
+                                uint32_t* levels_out, bool clear, hipStream_t stream) {
+    if (nzero > 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(pass_prologue, dim3(1), dim3(64), 0, stream, d, dst, valid, levels_in,
+                       steps, max_levels, counters, nzero, levels_out, clear);
     return hipGetLastError();
 }
 
