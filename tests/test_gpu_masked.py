@@ -86,6 +86,8 @@ def test_two_passes_on_a_resident_frame_make_the_full_render():
         assert_same(merged, full[1][k], k)
         assert_same(c3[k], full[1][k], k + " (mask cleared)")
     assert s1["equivalent"] + s2["equivalent"] == full[2]["equivalent"]
+    # later passes reuse the build flags the first pass's set-up cleared
+    assert s1["bla_levels"] == s2["bla_levels"] == full[2]["bla_levels"]
 
 
 def test_mixed_form_with_a_mask():

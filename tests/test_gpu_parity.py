@@ -61,6 +61,30 @@ def test_kern05_bla_route_is_bit_exact(ctx, name):
                        h["max_iter"], TIER_OF.get(name, -1), bla=True)
     compare_frames(g, c)
     assert g[0]["kernel"] == c["kernel"] + " gpu-hip-f64/1"
+    # the level count the device derives from the build's flags (read back beside the
+    # statistics) is the reference's Bla::build level count
+    assert g[2]["bla_levels"] == c["stats"]["bla_levels"]
+
+
+@pytest.mark.parametrize("name", ["tip-1e-200", "i-1e-1000"])
+def test_repeated_passes_on_a_resident_frame(ctx, name):
+    """Passes after the first reuse the table's level flags that the previous pass's set-up
+    cleared: every pass reports the oracle's level count and counters and the same
+    outcomes."""
+    h, _ = parse_fds(dict(kern05())[name])
+    print(f"case {name} repeated", flush=True)
+    v = fa.View(h["re"], h["im"], h["width"], h["rotation"])
+    cols = fa.COL_NU | fa.COL_DE | fa.COL_NORMAL
+    p = fa.Params(h["nx"], h["ny"], 1, h["max_iter"], h["escape_radius"], cols)
+    c = oracle.render(h["re"], h["im"], h["width"], h["rotation"], h["nx"], h["ny"],
+                      h["max_iter"], h["escape_radius"], columns=cols, bla=True)
+    assert c["stats"]["bla_levels"] > 0
+    with fa.ResidentFrame(v, p, fa.ROUTE_BLA, ctx=ctx) as f:
+        for _ in range(3):
+            st = f.run()
+            gh, gs, _ = f.read(raw=True)
+            compare_frames((gh, gs, st), c)
+            assert st["bla_levels"] == c["stats"]["bla_levels"]
 
 
 @pytest.mark.parametrize("cr,ci,iters,eps,dc,dc_exp", [
