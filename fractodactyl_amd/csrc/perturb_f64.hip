@@ -938,17 +938,24 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
     // Each wave claims samples in chunks, one global atomic per chunk rather than per
     // refill; the chunk shrinks with the work left (guided self-scheduling) so the last
     // chunks are small and the tail short.  [cb, ce) is the wave's unclaimed range.
-    uint32_t cb = 0, ce = 0, seen = 0;
+    // finished: lanes refilled so far (the first 64 included), i.e. samples done + 64
+    uint32_t cb = 0, ce = 0, seen = 0, finished = 0;
     const uint32_t waves = gridDim.x * (kBlock / 64);
     for (;;) {
         uint64_t idle = __ballot(!active);
         if (idle && !drained) {
             uint32_t want = __popcll(idle);
+            if (!LIST) finished += want;
             uint32_t avail = ce - cb;
             uint32_t nb = 0, ne = 0;
             if (avail < want) {
                 uint32_t left = count > seen ? count - seen : 0;
-                uint32_t size = max(min(left / (4 * waves), 1024u), want - avail);
+                // Samples shorter than 64 loop iterations (fewer wave iterations so far
+                // than samples this wave finished) make the claims themselves the
+                // bottleneck: then the chunk is the work left per wave rather than a
+                // quarter of it.  (The mixed form's list holds escaped samples, long ones.)
+                uint32_t div = !LIST && witers + 64 < finished ? 1u : 4u;
+                uint32_t size = max(min(left / (div * waves), 1024u), want - avail);
                 uint32_t leader = __ffsll((unsigned long long)idle) - 1;
                 if (lane == leader) nb = atomicAdd(next, size);
                 // readlane keeps the range wave-uniform (scalar registers)

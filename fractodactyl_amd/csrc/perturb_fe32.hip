@@ -338,17 +338,24 @@ __global__ void __launch_bounds__(kBlock32, 4) perturb_fe32(const FrameDev* __re
     bool active = false, drained = false;
     uint32_t idx = 0;
     unsigned long long witers = 0, liters = 0;
-    uint32_t cb = 0, ce = 0, seen = 0;
+    // finished: lanes refilled so far (the first 64 included), i.e. samples done + 64
+    uint32_t cb = 0, ce = 0, seen = 0, finished = 0;
     const uint32_t waves = gridDim.x * (kBlock32 / 64);
     for (;;) {
         uint64_t idle = __ballot(!active);
         if (idle && !drained) {
             uint32_t want = __popcll(idle);
+            finished += want;
             uint32_t avail = ce - cb;
             uint32_t nb = 0, ne = 0;
             if (avail < want) {
                 uint32_t left = count > seen ? count - seen : 0;
-                uint32_t size = max(min(left / (4 * waves), 1024u), want - avail);
+                // Samples shorter than 64 loop iterations (fewer wave iterations so far
+                // than samples this wave finished) make the claims themselves the
+                // bottleneck: then the chunk is the work left per wave rather than a
+                // quarter of it.
+                uint32_t div = witers + 64 < finished ? 1u : 4u;
+                uint32_t size = max(min(left / (div * waves), 1024u), want - avail);
                 uint32_t leader = __ffsll((unsigned long long)idle) - 1;
                 if (lane == leader) nb = atomicAdd(next, size);
                 nb = __builtin_amdgcn_readlane(nb, leader);
