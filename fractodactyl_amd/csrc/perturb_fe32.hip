@@ -356,6 +356,9 @@ __global__ void __launch_bounds__(kBlock32, 4) perturb_fe32(const FrameDev* __re
                 // quarter of it.
                 uint32_t div = witers + 64 < finished ? 1u : 4u;
                 uint32_t size = max(min(left / (div * waves), 1024u), want - avail);
+                // (whole 64-sample groups keep the claimed ranges aligned, so no two
+                // waves write parts of one cache line of the 1-byte outcome kinds)
+                if (div == 1u) size = (size + 63u) & ~63u;
                 uint32_t leader = __ffsll((unsigned long long)idle) - 1;
                 if (lane == leader) nb = atomicAdd(next, size);
                 nb = __builtin_amdgcn_readlane(nb, leader);
