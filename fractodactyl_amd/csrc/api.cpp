@@ -11,7 +11,7 @@
 
 namespace fdg {
 
-std::string g_create_error;
+thread_local std::string g_create_error;
 
 int Buffers::outputs(fdg_ctx* ctx, size_t n) {
     FDG_TRY(ctx, o_kind.ensure(n), "hipMalloc");
@@ -65,12 +65,14 @@ uint64_t next_pow2_host(uint64_t x) {
 
 int upload_orbit(fdg_ctx* ctx, DevBuf* dst, const Orbit& o, double* seconds) {
     double t0 = now_s();
-    std::vector<double> inter(2 * o.len());
+    // The kernels read up to two points past the orbit's end (their next-step prefetch);
+    // the 64-byte pad after the last point is uploaded as zeros with it.
+    std::vector<double> inter(2 * o.len() + 8, 0.0);
     for (uint64_t k = 0; k < o.len(); k++) {
         inter[2 * k] = o.re[k];
         inter[2 * k + 1] = o.im[k];
     }
-    FDG_TRY(ctx, dst->ensure(inter.size() * 8 + 64), "hipMalloc orbit");
+    FDG_TRY(ctx, dst->ensure(inter.size() * 8), "hipMalloc orbit");
     FDG_TRY(ctx,
             hipMemcpyAsync(dst->p, inter.data(), inter.size() * 8, hipMemcpyHostToDevice,
                            ctx->stream),
@@ -156,23 +158,20 @@ int read_encode(fdg_ctx* ctx, const Buffers& b, size_t off, size_t n, const Plan
     float* de = (p->columns & FDG_COL_DE) ? out->de : nullptr;
     uint16_t* nm = (p->columns & FDG_COL_NORMAL) ? out->normal : nullptr;
     unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
-    std::vector<std::thread> th;
-    for (unsigned w = 0; w < nt; w++)
-        th.emplace_back([&, w] {
-            size_t lo = n * w / nt, hi = n * (w + 1) / nt;
-            for (size_t i = lo; i < hi; i++) {
-                if (nu) nu[i] = 0.0;
-                if (de) de[i] = 0.0f;
-                if (nm) nm[i] = 0;
-                if (kind[i] == kKindUnset) {
-                    out->class_[i] = 0;
-                    continue;
-                }
-                enc.put(i, kind[i], nn[i], zr[i], zi[i], dr[i], di[i], dexp[i], out->class_, nu,
-                        de, nm);
+    run_parts(nt, [&](unsigned w) {
+        size_t lo = n * w / nt, hi = n * (w + 1) / nt;
+        for (size_t i = lo; i < hi; i++) {
+            if (nu) nu[i] = 0.0;
+            if (de) de[i] = 0.0f;
+            if (nm) nm[i] = 0;
+            if (kind[i] == kKindUnset) {
+                out->class_[i] = 0;
+                continue;
             }
-        });
-    for (auto& t : th) t.join();
+            enc.put(i, kind[i], nn[i], zr[i], zi[i], dr[i], di[i], dexp[i], out->class_, nu,
+                    de, nm);
+        }
+    });
     *encode_s = now_s() - t2;
     if (raw) {
         if (raw->kind) std::memcpy(raw->kind, kind, n);
@@ -370,16 +369,18 @@ int fdg_create(fdg_ctx** out, int device) {
 }
 
 void fdg_destroy(fdg_ctx* c) {
-    if (!c) return;
-    // Teardown has no caller to report to: each step runs whatever the previous returned.
-    (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
-    for (auto& e : c->ev)
-        if (e) (void)hipEventDestroy(e);
-    if (c->pinned) (void)hipHostFree(c->pinned);
-    if (c->small) (void)hipHostFree(c->small);
-    (void)hipStreamDestroy(c->stream);
-    delete c;
+    guarded_void([&] {
+        if (!c) return;
+        // Teardown has no caller to report to: each step runs whatever the previous returned.
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        for (auto& e : c->ev)
+            if (e) (void)hipEventDestroy(e);
+        if (c->pinned) (void)hipHostFree(c->pinned);
+        if (c->small) (void)hipHostFree(c->small);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+    });
 }
 
 const char* fdg_last_error(const fdg_ctx* c) {
@@ -387,93 +388,99 @@ const char* fdg_last_error(const fdg_ctx* c) {
 }
 
 int fdg_enabled(const char* value, int* enabled, int* route) {
-    std::string v = value ? value : "off";
-    if (v == "off") {
-        if (enabled) *enabled = 0;
-        return FDG_OK;
-    }
-    if (v == "on" || v == "bla") {
-        if (enabled) *enabled = 1;
-        if (route) *route = v == "bla" ? FDG_ROUTE_BLA : FDG_ROUTE_PLAIN;
-        return FDG_OK;
-    }
-    return FDG_EINVAL;
+    return guarded<int>(nullptr, [&]() -> int {
+        std::string v = value ? value : "off";
+        if (v == "off") {
+            if (enabled) *enabled = 0;
+            return FDG_OK;
+        }
+        if (v == "on" || v == "bla") {
+            if (enabled) *enabled = 1;
+            if (route) *route = v == "bla" ? FDG_ROUTE_BLA : FDG_ROUTE_PLAIN;
+            return FDG_OK;
+        }
+        return FDG_EINVAL;
+    });
 }
 
 int fdg_reference(fdg_ctx* ctx_in, const fdg_view* v, const fdg_params* p, double* re,
                   double* im, uint64_t cap, uint64_t* len, uint32_t* bits) {
-    // Host-only: a NULL context is allowed (errors then go to fdg_last_error(NULL)).
-    fdg_ctx local;
-    fdg_ctx* ctx = ctx_in ? ctx_in : &local;
-    struct Report {
-        fdg_ctx* c;
-        bool own;
-        ~Report() {
-            if (own && !c->err.empty()) g_create_error = c->err;
+    return guarded<int>(ctx_in ? &ctx_in->err : nullptr, [&]() -> int {
+        // Host-only: a NULL context is allowed (errors then go to fdg_last_error(NULL)).
+        fdg_ctx local;
+        fdg_ctx* ctx = ctx_in ? ctx_in : &local;
+        struct Report {
+            fdg_ctx* c;
+            bool own;
+            ~Report() {
+                if (own && !c->err.empty()) g_create_error = c->err;
+            }
+        } report{ctx, ctx_in == nullptr};
+        Frame fr;
+        int rc = setup(ctx, v, p, &fr);
+        if (rc) return rc;
+        Orbit o;
+        rc = compute_orbit(ctx, v, p, fr, &o);
+        if (rc) return rc;
+        if (len) *len = o.len();
+        if (bits) *bits = o.bits;
+        if (re && im && cap >= o.len()) {
+            std::memcpy(re, o.re.data(), o.len() * 8);
+            std::memcpy(im, o.im.data(), o.len() * 8);
         }
-    } report{ctx, ctx_in == nullptr};
-    Frame fr;
-    int rc = setup(ctx, v, p, &fr);
-    if (rc) return rc;
-    Orbit o;
-    rc = compute_orbit(ctx, v, p, fr, &o);
-    if (rc) return rc;
-    if (len) *len = o.len();
-    if (bits) *bits = o.bits;
-    if (re && im && cap >= o.len()) {
-        std::memcpy(re, o.re.data(), o.len() * 8);
-        std::memcpy(im, o.im.data(), o.len() * 8);
-    }
-    return FDG_OK;
+        return FDG_OK;
+    });
 }
 
 int fdg_build_bla(fdg_ctx* ctx, const double* re, const double* im, uint64_t len, double eps,
                   double dc_max, int64_t dc_exp, double* blocks, uint64_t cap_blocks,
                   uint32_t* levels) {
-    if (!ctx || !re || !im) return FDG_EINVAL;
-    FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
-    Orbit o;
-    o.re.assign(re, re + len);
-    o.im.assign(im, im + len);
-    double t = 0;
-    Buffers B;
-    struct Free {
-        Buffers* b;
-        ~Free() { b->release(); }
-    } guard{&B};
-    int rc = upload_orbit(ctx, &B.orbit, o, &t);
-    if (rc) return rc;
-    uint32_t nl = 0;
-    rc = build_table(ctx, &B, len, eps, dc_max, dc_exp, &nl, &t);
-    if (rc) return rc;
-    if (levels) *levels = nl;
-    if (!blocks) return FDG_OK;
-    uint64_t steps = B.bla_steps, total = 0;
-    for (uint32_t i = 0; i < nl; i++) total += steps >> (i + 1);
-    if (total > cap_blocks) return ctx->fail(FDG_EINVAL, "block buffer too small");
-    uint64_t stride = B.bla_stride;
-    std::vector<double> r(stride), ab(4 * stride), al(stride), be(stride);
-    if (nl) {
-        FDG_TRY(ctx, hipMemcpy(r.data(), B.bla_r.p, stride * 8, hipMemcpyDeviceToHost), "copy");
-        FDG_TRY(ctx, hipMemcpy(ab.data(), B.bla_ab.p, stride * 32, hipMemcpyDeviceToHost), "copy");
-        FDG_TRY(ctx, hipMemcpy(al.data(), B.bla_alpha.p, stride * 8, hipMemcpyDeviceToHost), "copy");
-        FDG_TRY(ctx, hipMemcpy(be.data(), B.bla_beta.p, stride * 8, hipMemcpyDeviceToHost), "copy");
-    }
-    uint64_t w = 0;
-    for (uint32_t i = 0; i < nl; i++) {
-        uint64_t off = bla_level_offset(stride, i), c = steps >> (i + 1);
-        for (uint64_t k = 0; k < c; k++, w++) {
-            double* b = blocks + 7 * w;
-            b[0] = ab[4 * (off + k)];
-            b[1] = ab[4 * (off + k) + 1];
-            b[2] = ab[4 * (off + k) + 2];
-            b[3] = ab[4 * (off + k) + 3];
-            b[4] = r[off + k];
-            b[5] = al[off + k];
-            b[6] = be[off + k];
+    return guarded<int>(ctx ? &ctx->err : nullptr, [&]() -> int {
+        if (!ctx || !re || !im) return FDG_EINVAL;
+        FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+        Orbit o;
+        o.re.assign(re, re + len);
+        o.im.assign(im, im + len);
+        double t = 0;
+        Buffers B;
+        struct Free {
+            Buffers* b;
+            ~Free() { b->release(); }
+        } guard{&B};
+        int rc = upload_orbit(ctx, &B.orbit, o, &t);
+        if (rc) return rc;
+        uint32_t nl = 0;
+        rc = build_table(ctx, &B, len, eps, dc_max, dc_exp, &nl, &t);
+        if (rc) return rc;
+        if (levels) *levels = nl;
+        if (!blocks) return FDG_OK;
+        uint64_t steps = B.bla_steps, total = 0;
+        for (uint32_t i = 0; i < nl; i++) total += steps >> (i + 1);
+        if (total > cap_blocks) return ctx->fail(FDG_EINVAL, "block buffer too small");
+        uint64_t stride = B.bla_stride;
+        std::vector<double> r(stride), ab(4 * stride), al(stride), be(stride);
+        if (nl) {
+            FDG_TRY(ctx, hipMemcpy(r.data(), B.bla_r.p, stride * 8, hipMemcpyDeviceToHost), "copy");
+            FDG_TRY(ctx, hipMemcpy(ab.data(), B.bla_ab.p, stride * 32, hipMemcpyDeviceToHost), "copy");
+            FDG_TRY(ctx, hipMemcpy(al.data(), B.bla_alpha.p, stride * 8, hipMemcpyDeviceToHost), "copy");
+            FDG_TRY(ctx, hipMemcpy(be.data(), B.bla_beta.p, stride * 8, hipMemcpyDeviceToHost), "copy");
         }
-    }
-    return FDG_OK;
+        uint64_t w = 0;
+        for (uint32_t i = 0; i < nl; i++) {
+            uint64_t off = bla_level_offset(stride, i), c = steps >> (i + 1);
+            for (uint64_t k = 0; k < c; k++, w++) {
+                double* b = blocks + 7 * w;
+                b[0] = ab[4 * (off + k)];
+                b[1] = ab[4 * (off + k) + 1];
+                b[2] = ab[4 * (off + k) + 2];
+                b[3] = ab[4 * (off + k) + 3];
+                b[4] = r[off + k];
+                b[5] = al[off + k];
+                b[6] = be[off + k];
+            }
+        }
+        return FDG_OK;
+    });
 }
 
 static int frame_open(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int route, int form,
@@ -482,14 +489,18 @@ static int frame_open(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int 
 
 int fdg_frame_open(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int route, int form,
                    double bla_eps, fdg_frame** out) {
-    return frame_open(ctx, v, p, route, form, bla_eps, nullptr, nullptr, 0, 0, out);
+    return guarded<int>(ctx ? &ctx->err : nullptr, [&]() -> int {
+        return frame_open(ctx, v, p, route, form, bla_eps, nullptr, nullptr, 0, 0, out);
+    });
 }
 
 int fdg_frame_open_orbit(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int route,
                          int form, double bla_eps, const double* re, const double* im,
                          uint64_t len, uint32_t bits, fdg_frame** out) {
-    if (!re || !im) return ctx ? ctx->fail(FDG_EINVAL, "no orbit") : FDG_EINVAL;
-    return frame_open(ctx, v, p, route, form, bla_eps, re, im, len, bits, out);
+    return guarded<int>(ctx ? &ctx->err : nullptr, [&]() -> int {
+        if (!re || !im) return ctx ? ctx->fail(FDG_EINVAL, "no orbit") : FDG_EINVAL;
+        return frame_open(ctx, v, p, route, form, bla_eps, re, im, len, bits, out);
+    });
 }
 
 static int frame_open(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int route, int form,
@@ -522,6 +533,17 @@ static int frame_open(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int 
             return ctx->fail(FDG_EINVAL, "BLA needs 0 < eps <= 2^-24");
     }
     fdg_frame* f = new fdg_frame();
+    // Owned by `guard` until it is handed out: every early return (and an exception
+    // unwinding to the ABI barrier) frees its device buffers and the frame.
+    struct Guard {
+        fdg_frame* f;
+        ~Guard() {
+            if (f) {
+                f->buf.release();
+                delete f;
+            }
+        }
+    } guard{f};
     f->ctx = ctx;
     f->re = v->center_re;
     f->im = v->center_im;
@@ -544,30 +566,34 @@ static int frame_open(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int 
         // then records its precision
         uint32_t need = fr.tier == TIER_F64 ? 53u : (uint32_t)fr.pl.bits;
         if (sbits != need && (fr.tier == TIER_F64 || sbits < need)) {
-            delete f;
             char m[160];
             std::snprintf(m, sizeof(m), "supplied reference has %u bits; this view needs %s%u",
                           sbits, fr.tier == TIER_F64 ? "" : "at least ", need);
             return ctx->fail(FDG_EINVAL, m);
         }
-        if (slen == 0 || sre[0] != 0.0 || sim[0] != 0.0 || slen > Orbit::kMaxLen) {
-            delete f;
+        if (slen < 2 || sre[0] != 0.0 || sim[0] != 0.0 || slen > Orbit::kMaxLen)
             return ctx->fail(FDG_EINVAL,
-                             "supplied reference orbit must start at Z_0 = 0 with equal re/im "
-                             "lengths");
-        }
+                             "supplied reference orbit must start at Z_0 = 0 and hold at least 2 "
+                             "points");
         o.re.assign(sre, sre + slen);
         o.im.assign(sim, sim + slen);
         o.bits = sbits;
         f->fr.pl.bits = sbits;
     } else {
         rc = compute_orbit(ctx, v, p, fr, &o);
-        if (rc) {
-            delete f;
-            return rc;
-        }
+        if (rc) return rc;
     }
     f->reference_seconds = now_s() - t0;
+    // The loop reads Z_1 from its first step, and the scaled tier's rebase of a delta
+    // below 1e-270 continues from Z_1 to Z_2 (sample.rs:422-449): shorter orbits are
+    // refused here, where the reference would index past its orbit and panic
+    // (sample.rs:367).
+    if (o.len() < (fr.tier == TIER_SCALED ? 3u : 2u)) {
+        return ctx->fail(FDG_EINVAL,
+                         fr.tier == TIER_SCALED
+                             ? "reference orbit too short: the scaled tier needs 3 points"
+                             : "reference orbit too short: needs 2 points");
+    }
     f->len = o.len();
     f->bits = o.bits;
     f->nucleus = o.nucleus_period();
@@ -586,699 +612,736 @@ static int frame_open(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int 
                           "orbit conversion");
         if (!rc) rc = ctx->hip(hipStreamSynchronize(ctx->stream), "orbit conversion");
     }
-    if (rc) {
-        f->buf.release();
-        delete f;
-        return rc;
-    }
+    if (rc) return rc;
     dc_bound(fr.pl, fr.tier, p->nx, p->ny, &f->dc, &f->dc_exp);
+    guard.f = nullptr;
     *out = f;
     return FDG_OK;
 }
 
 int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
-    if (!f) return FDG_EINVAL;
-    fdg_ctx* ctx = f->ctx;
-    FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
-    fdg_stats s;
-    std::memset(&s, 0, sizeof(s));
-    s.reference_seconds = f->reference_seconds;
-    s.reference_len = f->len;
-    s.reference_bits = f->bits;
-    s.nucleus_period = f->nucleus;
-    s.transfer_seconds = f->transfer_seconds;
-    // The table is built on the stream and the kernel reads its level count from the
-    // frame descriptor, which one small launch (pass_prologue) stores with the count taken
-    // from the build's flags, zeroing the counters in the same launch: no host round trip
-    // between the build and the perturbation kernel.  The host learns the count with the
-    // statistics at the end.
-    bool built = false;
-    if (f->route == FDG_ROUTE_BLA) {
-        int rc = launch_table(ctx, &f->buf, f->len, f->eps, f->dc, f->dc_exp, &built);
-        if (rc) return rc;
-    }
-    Buffers& b = f->buf;
-    FrameDev d;
-    std::memset(&d, 0, sizeof(d));
-    const Plane& pl = f->fr.pl;
-    const fdg_params* p = &f->params;
-    d.orbit = b.orbit.as<double>();
-    d.len = f->len;
-    d.nucleus = f->nucleus;
-    d.bla.r2 = b.bla_r2.as<double>();
-    d.bla.ab = b.bla_ab.as<double>();
-    d.bla.steps = b.bla_steps;
-    d.bla.stride = b.bla_stride;
-    d.bla.nlevels = 0;  // written on the device from the build's flags (built tables)
-    d.c_re = pl.c_re;
-    d.c_im = pl.c_im;
-    d.h = pl.h();
-    d.h_m = pl.h_m;
-    d.h_e = pl.h_e;
-    d.cos_r = pl.cos_r;
-    d.sin_r = pl.sin_r;
-    d.half_x = pl.half_x;
-    d.half_y = pl.half_y;
-    d.nx = p->nx;
-    d.ny = p->ny;
-    d.count = f->count;
-    d.tier = f->fr.tier;
-    d.max_iter = f->run_max_iter ? f->run_max_iter : p->max_iter;
-    d.r2 = p->escape_radius * p->escape_radius;
-    d.mask = f->masked ? f->mask.as<uint8_t>() : nullptr;
-    if (f->masked)
-        // samples outside the mask keep this "not computed" kind through the pass, which
-        // the mixed form's escaped list and the encoder both skip
-        FDG_TRY(ctx, hipMemsetAsync(b.o_kind.p, kKindUnset, f->count, ctx->stream), "fill");
-    d.kind = b.o_kind.as<uint8_t>();
-    d.n = b.o_n.as<uint64_t>();
-    d.zr = b.o_zr.as<double>();
-    d.zi = b.o_zi.as<double>();
-    d.dr = b.o_dr.as<double>();
-    d.di = b.o_di.as<double>();
-    d.dexp = b.o_dexp.as<int64_t>();
-    d.ops = b.o_ops.as<uint32_t>();
-    d.next = b.counters.as<uint32_t>();
-    d.stats = reinterpret_cast<unsigned long long*>(b.counters.as<uint8_t>() + 64);
-    bool series = f->has_series && f->series_n > 0;
-    if (series) {
-        d.series_b = f->series_b.as<double>();
-        d.series_n = f->series_n;
-        d.series_len = f->series_len;
-        d.series_inv_rho = f->series_inv_rho;
-    }
-    bool deriv = (p->columns & (FDG_COL_DE | FDG_COL_NORMAL)) != 0;
-    int grid = (int)std::min<uint64_t>((f->count + 255) / 256, (uint64_t)ctx->cus * 8);
-    if (grid < 1) grid = 1;
-    static_assert(sizeof(FrameDev) <= 4096, "kernel descriptor slot");
-    // counters: words 0-7 the work index, 8-23 the statistics (d.stats), word 24 (bytes
-    // 192-195) the table's level count for the host
-    uint32_t* levels_dev = reinterpret_cast<uint32_t*>(b.counters.as<uint8_t>() + 192);
-    bool mixed = f->form == FDG_FORM_MIXED;
-    bool fe32 = f->form == FDG_FORM_FE32 || mixed;
-    if (fe32) {
-        // FP32 table coefficients (A, B), same padded layout
-        if (built) {
-            FDG_TRY(ctx, b.ab32.ensure(b.bla_stride * 16), "hipMalloc");
-            FDG_TRY(ctx, b.abexp32.ensure(b.bla_stride * 8), "hipMalloc");
-            FDG_TRY(ctx,
-                    launch_table_to_f32(b.bla_ab.as<double>(), b.ab32.as<float>(),
-                                        b.abexp32.as<int32_t>(), b.bla_stride, ctx->stream),
-                    "table conversion");
+    return guarded<int>(f && f->ctx ? &f->ctx->err : nullptr, [&]() -> int {
+        if (!f) return FDG_EINVAL;
+        fdg_ctx* ctx = f->ctx;
+        FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+        fdg_stats s;
+        std::memset(&s, 0, sizeof(s));
+        s.reference_seconds = f->reference_seconds;
+        s.reference_len = f->len;
+        s.reference_bits = f->bits;
+        s.nucleus_period = f->nucleus;
+        s.transfer_seconds = f->transfer_seconds;
+        // The table is built on the stream and the kernel reads its level count from the
+        // frame descriptor, which one small launch (pass_prologue) stores with the count taken
+        // from the build's flags, zeroing the counters in the same launch: no host round trip
+        // between the build and the perturbation kernel.  The host learns the count with the
+        // statistics at the end.
+        bool built = false;
+        if (f->route == FDG_ROUTE_BLA) {
+            int rc = launch_table(ctx, &f->buf, f->len, f->eps, f->dc, f->dc_exp, &built);
+            if (rc) return rc;
         }
-        d.orbit32 = b.orbit32.as<float>();
-        d.ab32 = b.ab32.as<float>();
-        d.abexp32 = b.abexp32.as<int32_t>();
-        d.defer_escaped = mixed ? 1u : 0u;
-    }
-    // The kernel writes every output field of every sample; only the counters start at 0.
-    FDG_TRY(ctx,
-            launch_pass_prologue(d, b.fdev.as<FrameDev>(),
-                                 built ? b.bla_valid.as<uint32_t>() : nullptr, nullptr,
-                                 b.bla_steps, kMaxLevels, b.counters.as<unsigned long long>(),
-                                 24, levels_dev, true, ctx->stream),
-            "pass set-up");
-    if (built) b.valid_clean = true;
-    FDG_TRY(ctx, hipEventRecord(ctx->ev[2], ctx->stream), "event");
-    if (fe32)
+        Buffers& b = f->buf;
+        FrameDev d;
+        std::memset(&d, 0, sizeof(d));
+        const Plane& pl = f->fr.pl;
+        const fdg_params* p = &f->params;
+        d.orbit = b.orbit.as<double>();
+        d.len = f->len;
+        d.nucleus = f->nucleus;
+        d.bla.r2 = b.bla_r2.as<double>();
+        d.bla.ab = b.bla_ab.as<double>();
+        d.bla.steps = b.bla_steps;
+        d.bla.stride = b.bla_stride;
+        d.bla.nlevels = 0;  // written on the device from the build's flags (built tables)
+        d.c_re = pl.c_re;
+        d.c_im = pl.c_im;
+        d.h = pl.h();
+        d.h_m = pl.h_m;
+        d.h_e = pl.h_e;
+        d.cos_r = pl.cos_r;
+        d.sin_r = pl.sin_r;
+        d.half_x = pl.half_x;
+        d.half_y = pl.half_y;
+        d.nx = p->nx;
+        d.ny = p->ny;
+        d.count = f->count;
+        d.tier = f->fr.tier;
+        d.max_iter = f->run_max_iter ? f->run_max_iter : p->max_iter;
+        d.r2 = p->escape_radius * p->escape_radius;
+        d.mask = f->masked ? f->mask.as<uint8_t>() : nullptr;
+        if (f->masked)
+            // samples outside the mask keep this "not computed" kind through the pass, which
+            // the mixed form's escaped list and the encoder both skip
+            FDG_TRY(ctx, hipMemsetAsync(b.o_kind.p, kKindUnset, f->count, ctx->stream), "fill");
+        d.kind = b.o_kind.as<uint8_t>();
+        d.n = b.o_n.as<uint64_t>();
+        d.zr = b.o_zr.as<double>();
+        d.zi = b.o_zi.as<double>();
+        d.dr = b.o_dr.as<double>();
+        d.di = b.o_di.as<double>();
+        d.dexp = b.o_dexp.as<int64_t>();
+        d.ops = b.o_ops.as<uint32_t>();
+        d.next = b.counters.as<uint32_t>();
+        d.stats = reinterpret_cast<unsigned long long*>(b.counters.as<uint8_t>() + 64);
+        bool series = f->has_series && f->series_n > 0;
+        if (series) {
+            d.series_b = f->series_b.as<double>();
+            d.series_n = f->series_n;
+            d.series_len = f->series_len;
+            d.series_inv_rho = f->series_inv_rho;
+        }
+        bool deriv = (p->columns & (FDG_COL_DE | FDG_COL_NORMAL)) != 0;
+        int grid = (int)std::min<uint64_t>((f->count + 255) / 256, (uint64_t)ctx->cus * 8);
+        if (grid < 1) grid = 1;
+        static_assert(sizeof(FrameDev) <= 4096, "kernel descriptor slot");
+        // counters: words 0-7 the work index, 8-23 the statistics (d.stats), word 24 (bytes
+        // 192-195) the table's level count for the host
+        uint32_t* levels_dev = reinterpret_cast<uint32_t*>(b.counters.as<uint8_t>() + 192);
+        bool mixed = f->form == FDG_FORM_MIXED;
+        bool fe32 = f->form == FDG_FORM_FE32 || mixed;
+        if (fe32) {
+            // FP32 table coefficients (A, B), same padded layout
+            if (built) {
+                FDG_TRY(ctx, b.ab32.ensure(b.bla_stride * 16), "hipMalloc");
+                FDG_TRY(ctx, b.abexp32.ensure(b.bla_stride * 8), "hipMalloc");
+                FDG_TRY(ctx,
+                        launch_table_to_f32(b.bla_ab.as<double>(), b.ab32.as<float>(),
+                                            b.abexp32.as<int32_t>(), b.bla_stride, ctx->stream),
+                        "table conversion");
+            }
+            d.orbit32 = b.orbit32.as<float>();
+            d.ab32 = b.ab32.as<float>();
+            d.abexp32 = b.abexp32.as<int32_t>();
+            d.defer_escaped = mixed ? 1u : 0u;
+        }
+        // The kernel writes every output field of every sample; only the counters start at 0.
         FDG_TRY(ctx,
-                launch_perturb_fe32(b.fdev.as<FrameDev>(), deriv, built, grid,
-                                    ctx->stream),
-                "perturbation launch");
-    else if (series)
-        FDG_TRY(ctx, launch_perturb_f64_series(b.fdev.as<FrameDev>(), deriv, built, grid,
-                                               ctx->stream),
-                "perturbation launch");
-    else
-        FDG_TRY(ctx,
-                launch_perturb_f64(b.fdev.as<FrameDev>(), f->fr.tier == TIER_SCALED, deriv,
-                                   built, false, grid, ctx->stream),
-                "perturbation launch");
-    unsigned long long cnt2[ST_COUNT];
-    std::memset(cnt2, 0, sizeof(cnt2));
-    if (mixed) {
-        // Second pass: the samples the FP32 pass found escaped, recomputed in FP64 (the
-        // outcomes of every other sample are the FP32 pass's).
-        FDG_TRY(ctx, b.list.ensure((size_t)f->count * 4), "hipMalloc");
-        FDG_TRY(ctx, b.counters2.ensure(256), "hipMalloc");
-        FDG_TRY(ctx, b.fdev2.ensure(sizeof(FrameDev)), "hipMalloc");
-        uint32_t* listed = b.counters2.as<uint32_t>() + 1;
-        FrameDev d2 = d;
-        d2.mask = nullptr;  // the list holds only computed samples; it is indexed by position
-        d2.list = b.list.as<uint32_t>();
-        d2.list_count = listed;
-        d2.defer_escaped = 0;
-        d2.next = b.counters2.as<uint32_t>();
-        d2.stats = reinterpret_cast<unsigned long long*>(b.counters2.as<uint8_t>() + 64);
-        // words 0 (work index) and 1 (list length) and the statistics start at 0; the
-        // level count is the first pass's
-        FDG_TRY(ctx,
-                launch_pass_prologue(d2, b.fdev2.as<FrameDev>(), nullptr, levels_dev,
-                                     b.bla_steps, kMaxLevels,
-                                     b.counters2.as<unsigned long long>(), 32, nullptr, false,
-                                     ctx->stream),
+                launch_pass_prologue(d, b.fdev.as<FrameDev>(),
+                                     built ? b.bla_valid.as<uint32_t>() : nullptr, nullptr,
+                                     b.bla_steps, kMaxLevels, b.counters.as<unsigned long long>(),
+                                     24, levels_dev, true, ctx->stream),
                 "pass set-up");
+        if (built) b.valid_clean = true;
+        FDG_TRY(ctx, hipEventRecord(ctx->ev[2], ctx->stream), "event");
+        if (fe32)
+            FDG_TRY(ctx,
+                    launch_perturb_fe32(b.fdev.as<FrameDev>(), deriv, built, grid,
+                                        ctx->stream),
+                    "perturbation launch");
+        else if (series)
+            FDG_TRY(ctx, launch_perturb_f64_series(b.fdev.as<FrameDev>(), deriv, built, grid,
+                                                   ctx->stream),
+                    "perturbation launch");
+        else
+            FDG_TRY(ctx,
+                    launch_perturb_f64(b.fdev.as<FrameDev>(), f->fr.tier == TIER_SCALED, deriv,
+                                       built, false, grid, ctx->stream),
+                    "perturbation launch");
+        unsigned long long cnt2[ST_COUNT];
+        std::memset(cnt2, 0, sizeof(cnt2));
+        if (mixed) {
+            // Second pass: the samples the FP32 pass found escaped, recomputed in FP64 (the
+            // outcomes of every other sample are the FP32 pass's).
+            FDG_TRY(ctx, b.list.ensure((size_t)f->count * 4), "hipMalloc");
+            FDG_TRY(ctx, b.counters2.ensure(256), "hipMalloc");
+            FDG_TRY(ctx, b.fdev2.ensure(sizeof(FrameDev)), "hipMalloc");
+            uint32_t* listed = b.counters2.as<uint32_t>() + 1;
+            FrameDev d2 = d;
+            d2.mask = nullptr;  // the list holds only computed samples; it is indexed by position
+            d2.list = b.list.as<uint32_t>();
+            d2.list_count = listed;
+            d2.defer_escaped = 0;
+            d2.next = b.counters2.as<uint32_t>();
+            d2.stats = reinterpret_cast<unsigned long long*>(b.counters2.as<uint8_t>() + 64);
+            // words 0 (work index) and 1 (list length) and the statistics start at 0; the
+            // level count is the first pass's
+            FDG_TRY(ctx,
+                    launch_pass_prologue(d2, b.fdev2.as<FrameDev>(), nullptr, levels_dev,
+                                         b.bla_steps, kMaxLevels,
+                                         b.counters2.as<unsigned long long>(), 32, nullptr, false,
+                                         ctx->stream),
+                    "pass set-up");
+            FDG_TRY(ctx,
+                    launch_escaped_list(b.o_kind.as<uint8_t>(), b.list.as<uint32_t>(), listed,
+                                        f->count, ctx->stream),
+                    "escaped list");
+            FDG_TRY(ctx,
+                    launch_perturb_f64(b.fdev2.as<FrameDev>(), f->fr.tier == TIER_SCALED, deriv,
+                                       built, false, grid, ctx->stream, true),
+                    "refinement launch");
+            FDG_TRY(ctx,
+                    hipMemcpyAsync(ctx->small + fdg_ctx::kSlotCounts2, d2.stats, sizeof(cnt2),
+                                   hipMemcpyDeviceToHost, ctx->stream),
+                    "readback");
+        }
+        FDG_TRY(ctx, hipEventRecord(ctx->ev[3], ctx->stream), "event");
+        unsigned long long cnt[ST_COUNT];
+        // the statistics (counter bytes 64-191) and the level count (bytes 192-195) in one copy
+        constexpr size_t kReadback = sizeof(cnt) + 8;
+        static_assert(64 + kReadback <= 256 && kReadback <= 1024, "counter slot");
         FDG_TRY(ctx,
-                launch_escaped_list(b.o_kind.as<uint8_t>(), b.list.as<uint32_t>(), listed,
-                                    f->count, ctx->stream),
-                "escaped list");
-        FDG_TRY(ctx,
-                launch_perturb_f64(b.fdev2.as<FrameDev>(), f->fr.tier == TIER_SCALED, deriv,
-                                   built, false, grid, ctx->stream, true),
-                "refinement launch");
-        FDG_TRY(ctx,
-                hipMemcpyAsync(ctx->small + fdg_ctx::kSlotCounts2, d2.stats, sizeof(cnt2),
+                hipMemcpyAsync(ctx->small + fdg_ctx::kSlotCounts, d.stats, kReadback,
                                hipMemcpyDeviceToHost, ctx->stream),
                 "readback");
-    }
-    FDG_TRY(ctx, hipEventRecord(ctx->ev[3], ctx->stream), "event");
-    unsigned long long cnt[ST_COUNT];
-    // the statistics (counter bytes 64-191) and the level count (bytes 192-195) in one copy
-    constexpr size_t kReadback = sizeof(cnt) + 8;
-    static_assert(64 + kReadback <= 256 && kReadback <= 1024, "counter slot");
-    FDG_TRY(ctx,
-            hipMemcpyAsync(ctx->small + fdg_ctx::kSlotCounts, d.stats, kReadback,
-                           hipMemcpyDeviceToHost, ctx->stream),
-            "readback");
-    FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "perturbation kernel");
-    std::memcpy(cnt, ctx->small + fdg_ctx::kSlotCounts, sizeof(cnt));
-    if (mixed) std::memcpy(cnt2, ctx->small + fdg_ctx::kSlotCounts2, sizeof(cnt2));
-    uint32_t levels = 0;
-    if (built) {
-        std::memcpy(&levels, ctx->small + fdg_ctx::kSlotCounts + sizeof(cnt), sizeof(levels));
+        FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "perturbation kernel");
+        std::memcpy(cnt, ctx->small + fdg_ctx::kSlotCounts, sizeof(cnt));
+        if (mixed) std::memcpy(cnt2, ctx->small + fdg_ctx::kSlotCounts2, sizeof(cnt2));
+        uint32_t levels = 0;
+        if (built) {
+            std::memcpy(&levels, ctx->small + fdg_ctx::kSlotCounts + sizeof(cnt), sizeof(levels));
+            float ms = 0;
+            FDG_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]), "event timing");
+            s.bla_seconds = ms * 1e-3;
+        }
+        s.bla_levels = levels;
+        f->use_bla = levels > 0;
+        if (built && levels == 0) {
+            // A table without a valid block: the kernel ran its BLA form with no level to
+            // search, which counts every sample as a fallback and the closed-form ones apart;
+            // report what the plain kernel (fd_kernel::render_with) reports.
+            cnt[ST_FALLBACK] = cnt2[ST_FALLBACK] = 0;
+            cnt[ST_CLOSED] = cnt2[ST_CLOSED] = 0;
+        }
+        for (int q = 0; q < ST_COUNT; q++)
+            cnt[q] = q == ST_MAXOPS ? std::max(cnt[q], cnt2[q]) : cnt[q] + cnt2[q];
         float ms = 0;
-        FDG_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]), "event timing");
-        s.bla_seconds = ms * 1e-3;
-    }
-    s.bla_levels = levels;
-    f->use_bla = levels > 0;
-    if (built && levels == 0) {
-        // A table without a valid block: the kernel ran its BLA form with no level to
-        // search, which counts every sample as a fallback and the closed-form ones apart;
-        // report what the plain kernel (fd_kernel::render_with) reports.
-        cnt[ST_FALLBACK] = cnt2[ST_FALLBACK] = 0;
-        cnt[ST_CLOSED] = cnt2[ST_CLOSED] = 0;
-    }
-    for (int q = 0; q < ST_COUNT; q++)
-        cnt[q] = q == ST_MAXOPS ? std::max(cnt[q], cnt2[q]) : cnt[q] + cnt2[q];
-    float ms = 0;
-    FDG_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]), "event timing");
-    s.kernel_seconds = ms * 1e-3;
-    s.iterations = cnt[ST_ITERATIONS];
-    s.equivalent = cnt[ST_EQUIVALENT];
-    s.blocks = cnt[ST_BLOCKS];
-    s.skipped = cnt[ST_SKIPPED];
-    s.fallback_samples = cnt[ST_FALLBACK];
-    s.closed_form_samples = cnt[ST_CLOSED];
-    s.plain_steps = cnt[ST_PLAIN];
-    s.max_sample_ops = cnt[ST_MAXOPS];
-    s.wave_iterations = cnt[ST_WAVE_ITERS];
-    s.lane_iterations = cnt[ST_LANE_ITERS];
-    f->last = s;
-    f->ran = true;
-    if (st) *st = s;
-    return FDG_OK;
+        FDG_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]), "event timing");
+        s.kernel_seconds = ms * 1e-3;
+        s.iterations = cnt[ST_ITERATIONS];
+        s.equivalent = cnt[ST_EQUIVALENT];
+        s.blocks = cnt[ST_BLOCKS];
+        s.skipped = cnt[ST_SKIPPED];
+        s.fallback_samples = cnt[ST_FALLBACK];
+        s.closed_form_samples = cnt[ST_CLOSED];
+        s.plain_steps = cnt[ST_PLAIN];
+        s.max_sample_ops = cnt[ST_MAXOPS];
+        s.wave_iterations = cnt[ST_WAVE_ITERS];
+        s.lane_iterations = cnt[ST_LANE_ITERS];
+        f->last = s;
+        f->ran = true;
+        if (st) *st = s;
+        return FDG_OK;
+    });
 }
 
 int fdg_frame_read(fdg_frame* f, fdg_columns* out, fdg_outcomes* raw, fdg_stats* st,
                    char* kernel_id, size_t kernel_cap) {
-    if (!f) return FDG_EINVAL;
-    fdg_ctx* ctx = f->ctx;
-    if (!f->ran) return ctx->fail(FDG_EINVAL, "frame has not been run");
-    const fdg_params* p = &f->params;
-    if (!out || !out->class_) return ctx->fail(FDG_EINVAL, "the class column is required");
-    if ((p->columns & FDG_COL_NU && !out->nu) || (p->columns & FDG_COL_DE && !out->de) ||
-        (p->columns & FDG_COL_NORMAL && !out->normal))
-        return ctx->fail(FDG_EINVAL, "a requested column has no buffer");
-    FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
-    fdg_stats s = f->last;
-    int rc = read_encode(ctx, f->buf, 0, f->count, f->fr.pl, p, out, raw, &s.transfer_seconds,
-                         &s.encode_seconds, nullptr, nullptr, nullptr);
-    if (rc) return rc;
-    if (st) *st = s;
-    if (kernel_id && kernel_cap) {
-        std::string k = f->fr.pl.kernel(f->fr.tier);
-        if (f->use_bla) k += " bla/1";
-        if (f->has_series) k += " series/0";  // series.rs:371-372
-        k += f->form == FDG_FORM_FE32 ? " gpu-hip-fe32/1"
-             : f->form == FDG_FORM_MIXED ? " gpu-hip-mixed/1" : " gpu-hip-f64/1";
-        std::snprintf(kernel_id, kernel_cap, "%s", k.c_str());
-    }
-    return FDG_OK;
+    return guarded<int>(f && f->ctx ? &f->ctx->err : nullptr, [&]() -> int {
+        if (!f) return FDG_EINVAL;
+        fdg_ctx* ctx = f->ctx;
+        if (!f->ran) return ctx->fail(FDG_EINVAL, "frame has not been run");
+        const fdg_params* p = &f->params;
+        if (!out || !out->class_) return ctx->fail(FDG_EINVAL, "the class column is required");
+        if ((p->columns & FDG_COL_NU && !out->nu) || (p->columns & FDG_COL_DE && !out->de) ||
+            (p->columns & FDG_COL_NORMAL && !out->normal))
+            return ctx->fail(FDG_EINVAL, "a requested column has no buffer");
+        FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+        fdg_stats s = f->last;
+        int rc = read_encode(ctx, f->buf, 0, f->count, f->fr.pl, p, out, raw, &s.transfer_seconds,
+                             &s.encode_seconds, nullptr, nullptr, nullptr);
+        if (rc) return rc;
+        if (st) *st = s;
+        if (kernel_id && kernel_cap) {
+            std::string k = f->fr.pl.kernel(f->fr.tier);
+            if (f->use_bla) k += " bla/1";
+            if (f->has_series) k += " series/0";  // series.rs:371-372
+            k += f->form == FDG_FORM_FE32 ? " gpu-hip-fe32/1"
+                 : f->form == FDG_FORM_MIXED ? " gpu-hip-mixed/1" : " gpu-hip-f64/1";
+            std::snprintf(kernel_id, kernel_cap, "%s", k.c_str());
+        }
+        return FDG_OK;
+    });
 }
 
 int fdg_frame_read_columns(fdg_frame* f, fdg_columns* out, fdg_stats* st, char* kernel_id,
                            size_t kernel_cap) {
-    if (!f) return FDG_EINVAL;
-    fdg_ctx* ctx = f->ctx;
-    if (!f->ran) return ctx->fail(FDG_EINVAL, "frame has not been run");
-    const fdg_params* p = &f->params;
-    if (!out || !out->class_) return ctx->fail(FDG_EINVAL, "the class column is required");
-    if ((p->columns & FDG_COL_NU && !out->nu) || (p->columns & FDG_COL_DE && !out->de) ||
-        (p->columns & FDG_COL_NORMAL && !out->normal))
-        return ctx->fail(FDG_EINVAL, "a requested column has no buffer");
-    FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
-    fdg_stats s = f->last;
-    const size_t n = f->count;
-    const bool want_nu = p->columns & FDG_COL_NU, want_de = p->columns & FDG_COL_DE,
-               want_nm = p->columns & FDG_COL_NORMAL;
-    Buffers& b = f->buf;
-    FDG_TRY(ctx, b.c_cls.ensure(n), "hipMalloc");
-    if (want_nu) FDG_TRY(ctx, b.c_nu.ensure(8 * n), "hipMalloc");
-    if (want_de) FDG_TRY(ctx, b.c_de.ensure(4 * n), "hipMalloc");
-    if (want_nm) FDG_TRY(ctx, b.c_nm.ensure(2 * n), "hipMalloc");
-    const Plane& pl = f->fr.pl;
-    EncodeDev e{};
-    e.kind = b.o_kind.as<uint8_t>();
-    e.n = b.o_n.as<uint64_t>();
-    e.zr = b.o_zr.as<double>();
-    e.zi = b.o_zi.as<double>();
-    e.dr = b.o_dr.as<double>();
-    e.di = b.o_di.as<double>();
-    e.dexp = b.o_dexp.as<int64_t>();
-    e.cls = b.c_cls.as<uint8_t>();
-    e.nu = want_nu ? b.c_nu.as<double>() : nullptr;
-    e.de = want_de ? b.c_de.as<float>() : nullptr;
-    e.normal = want_nm ? b.c_nm.as<uint16_t>() : nullptr;
-    e.count = (uint32_t)n;
-    e.px_m = pl.h_m * (double)p->ss;
-    e.px_e = pl.h_e;
-    e.cos_r = pl.cos_r;
-    e.sin_r = pl.sin_r;
-    double t0 = now_s();
-    FDG_TRY(ctx, launch_encode_columns(e, ctx->cus, ctx->stream), "column encoding");
-    // columns into the pinned staging area, 8-byte aligned each, then into the caller's
-    const size_t a1 = (n + 7) & ~(size_t)7, a8 = 8 * n, a4 = (4 * n + 7) & ~(size_t)7;
-    const size_t need = a1 + (want_nu ? a8 : 0) + (want_de ? a4 : 0) + (want_nm ? 2 * n : 0);
-    int rc = ctx->staging(need);
-    if (rc) return rc;
-    uint8_t* base = static_cast<uint8_t*>(ctx->pinned);
-    uint8_t* h_cls = base;
-    double* h_nu = reinterpret_cast<double*>(base + a1);
-    float* h_de = reinterpret_cast<float*>(base + a1 + (want_nu ? a8 : 0));
-    uint16_t* h_nm = reinterpret_cast<uint16_t*>(base + a1 + (want_nu ? a8 : 0) +
-                                                 (want_de ? a4 : 0));
-    FDG_TRY(ctx, hipMemcpyAsync(h_cls, e.cls, n, hipMemcpyDeviceToHost, ctx->stream), "readback");
-    if (want_nu)
-        FDG_TRY(ctx, hipMemcpyAsync(h_nu, e.nu, 8 * n, hipMemcpyDeviceToHost, ctx->stream),
-                "readback");
-    if (want_de)
-        FDG_TRY(ctx, hipMemcpyAsync(h_de, e.de, 4 * n, hipMemcpyDeviceToHost, ctx->stream),
-                "readback");
-    if (want_nm)
-        FDG_TRY(ctx, hipMemcpyAsync(h_nm, e.normal, 2 * n, hipMemcpyDeviceToHost, ctx->stream),
-                "readback");
-    FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "column encoding");
-    double t1 = now_s();
-    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    if (n < (1u << 16)) nt = 1;
-    std::vector<std::thread> th;
-    auto part = [&](unsigned w) {
-        size_t lo = n * w / nt, hi = n * (w + 1) / nt;
-        std::memcpy(out->class_ + lo, h_cls + lo, hi - lo);
-        if (want_nu) std::memcpy(out->nu + lo, h_nu + lo, 8 * (hi - lo));
-        if (want_de) std::memcpy(out->de + lo, h_de + lo, 4 * (hi - lo));
-        if (want_nm) std::memcpy(out->normal + lo, h_nm + lo, 2 * (hi - lo));
-    };
-    for (unsigned w = 1; w < nt; w++) th.emplace_back(part, w);
-    part(0);
-    for (auto& t : th) t.join();
-    s.transfer_seconds += t1 - t0;  // device encoding + the 15-byte columns' readback
-    s.encode_seconds = now_s() - t1;
-    if (st) *st = s;
-    if (kernel_id && kernel_cap) {
-        std::string k = f->fr.pl.kernel(f->fr.tier);
-        if (f->use_bla) k += " bla/1";
-        if (f->has_series) k += " series/0";
-        k += f->form == FDG_FORM_FE32 ? " gpu-hip-fe32/1"
-             : f->form == FDG_FORM_MIXED ? " gpu-hip-mixed/1" : " gpu-hip-f64/1";
-        std::snprintf(kernel_id, kernel_cap, "%s", k.c_str());
-    }
-    return FDG_OK;
+    return guarded<int>(f && f->ctx ? &f->ctx->err : nullptr, [&]() -> int {
+        if (!f) return FDG_EINVAL;
+        fdg_ctx* ctx = f->ctx;
+        if (!f->ran) return ctx->fail(FDG_EINVAL, "frame has not been run");
+        const fdg_params* p = &f->params;
+        if (!out || !out->class_) return ctx->fail(FDG_EINVAL, "the class column is required");
+        if ((p->columns & FDG_COL_NU && !out->nu) || (p->columns & FDG_COL_DE && !out->de) ||
+            (p->columns & FDG_COL_NORMAL && !out->normal))
+            return ctx->fail(FDG_EINVAL, "a requested column has no buffer");
+        FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+        fdg_stats s = f->last;
+        const size_t n = f->count;
+        const bool want_nu = p->columns & FDG_COL_NU, want_de = p->columns & FDG_COL_DE,
+                   want_nm = p->columns & FDG_COL_NORMAL;
+        Buffers& b = f->buf;
+        FDG_TRY(ctx, b.c_cls.ensure(n), "hipMalloc");
+        if (want_nu) FDG_TRY(ctx, b.c_nu.ensure(8 * n), "hipMalloc");
+        if (want_de) FDG_TRY(ctx, b.c_de.ensure(4 * n), "hipMalloc");
+        if (want_nm) FDG_TRY(ctx, b.c_nm.ensure(2 * n), "hipMalloc");
+        const Plane& pl = f->fr.pl;
+        EncodeDev e{};
+        e.kind = b.o_kind.as<uint8_t>();
+        e.n = b.o_n.as<uint64_t>();
+        e.zr = b.o_zr.as<double>();
+        e.zi = b.o_zi.as<double>();
+        e.dr = b.o_dr.as<double>();
+        e.di = b.o_di.as<double>();
+        e.dexp = b.o_dexp.as<int64_t>();
+        e.cls = b.c_cls.as<uint8_t>();
+        e.nu = want_nu ? b.c_nu.as<double>() : nullptr;
+        e.de = want_de ? b.c_de.as<float>() : nullptr;
+        e.normal = want_nm ? b.c_nm.as<uint16_t>() : nullptr;
+        e.count = (uint32_t)n;
+        e.px_m = pl.h_m * (double)p->ss;
+        e.px_e = pl.h_e;
+        e.cos_r = pl.cos_r;
+        e.sin_r = pl.sin_r;
+        double t0 = now_s();
+        FDG_TRY(ctx, launch_encode_columns(e, ctx->cus, ctx->stream), "column encoding");
+        // columns into the pinned staging area, 8-byte aligned each, then into the caller's
+        const size_t a1 = (n + 7) & ~(size_t)7, a8 = 8 * n, a4 = (4 * n + 7) & ~(size_t)7;
+        const size_t need = a1 + (want_nu ? a8 : 0) + (want_de ? a4 : 0) + (want_nm ? 2 * n : 0);
+        int rc = ctx->staging(need);
+        if (rc) return rc;
+        uint8_t* base = static_cast<uint8_t*>(ctx->pinned);
+        uint8_t* h_cls = base;
+        double* h_nu = reinterpret_cast<double*>(base + a1);
+        float* h_de = reinterpret_cast<float*>(base + a1 + (want_nu ? a8 : 0));
+        uint16_t* h_nm = reinterpret_cast<uint16_t*>(base + a1 + (want_nu ? a8 : 0) +
+                                                     (want_de ? a4 : 0));
+        FDG_TRY(ctx, hipMemcpyAsync(h_cls, e.cls, n, hipMemcpyDeviceToHost, ctx->stream), "readback");
+        if (want_nu)
+            FDG_TRY(ctx, hipMemcpyAsync(h_nu, e.nu, 8 * n, hipMemcpyDeviceToHost, ctx->stream),
+                    "readback");
+        if (want_de)
+            FDG_TRY(ctx, hipMemcpyAsync(h_de, e.de, 4 * n, hipMemcpyDeviceToHost, ctx->stream),
+                    "readback");
+        if (want_nm)
+            FDG_TRY(ctx, hipMemcpyAsync(h_nm, e.normal, 2 * n, hipMemcpyDeviceToHost, ctx->stream),
+                    "readback");
+        FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "column encoding");
+        double t1 = now_s();
+        unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        if (n < (1u << 16)) nt = 1;
+        auto part = [&](unsigned w) {
+            size_t lo = n * w / nt, hi = n * (w + 1) / nt;
+            std::memcpy(out->class_ + lo, h_cls + lo, hi - lo);
+            if (want_nu) std::memcpy(out->nu + lo, h_nu + lo, 8 * (hi - lo));
+            if (want_de) std::memcpy(out->de + lo, h_de + lo, 4 * (hi - lo));
+            if (want_nm) std::memcpy(out->normal + lo, h_nm + lo, 2 * (hi - lo));
+        };
+        run_parts(nt, part);
+        s.transfer_seconds += t1 - t0;  // device encoding + the 15-byte columns' readback
+        s.encode_seconds = now_s() - t1;
+        if (st) *st = s;
+        if (kernel_id && kernel_cap) {
+            std::string k = f->fr.pl.kernel(f->fr.tier);
+            if (f->use_bla) k += " bla/1";
+            if (f->has_series) k += " series/0";
+            k += f->form == FDG_FORM_FE32 ? " gpu-hip-fe32/1"
+                 : f->form == FDG_FORM_MIXED ? " gpu-hip-mixed/1" : " gpu-hip-f64/1";
+            std::snprintf(kernel_id, kernel_cap, "%s", k.c_str());
+        }
+        return FDG_OK;
+    });
 }
 
 int fdg_frame_mask(fdg_frame* f, const uint8_t* mask) {
-    if (!f) return FDG_EINVAL;
-    fdg_ctx* ctx = f->ctx;
-    FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
-    f->masked = mask != nullptr;
-    if (!mask) {
-        f->mask_host.clear();
+    return guarded<int>(f && f->ctx ? &f->ctx->err : nullptr, [&]() -> int {
+        if (!f) return FDG_EINVAL;
+        fdg_ctx* ctx = f->ctx;
+        FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+        f->masked = mask != nullptr;
+        if (!mask) {
+            f->mask_host.clear();
+            return FDG_OK;
+        }
+        f->mask_host.assign(mask, mask + f->count);
+        FDG_TRY(ctx, f->mask.ensure(f->count), "hipMalloc");
+        FDG_TRY(ctx, hipMemcpyAsync(f->mask.p, mask, f->count, hipMemcpyHostToDevice, ctx->stream),
+                "mask upload");
+        FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "mask upload");
         return FDG_OK;
-    }
-    f->mask_host.assign(mask, mask + f->count);
-    FDG_TRY(ctx, f->mask.ensure(f->count), "hipMalloc");
-    FDG_TRY(ctx, hipMemcpyAsync(f->mask.p, mask, f->count, hipMemcpyHostToDevice, ctx->stream),
-            "mask upload");
-    FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "mask upload");
-    return FDG_OK;
+    });
 }
 
 int fdg_render_masked(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int route, int form,
                       double bla_eps, const uint8_t* mask, fdg_columns* out, fdg_outcomes* raw,
                       fdg_stats* st, char* kernel_id, size_t kernel_cap) {
-    if (!ctx) return FDG_EINVAL;
-    if (!out || !out->class_) return ctx->fail(FDG_EINVAL, "the class column is required");
-    if (!mask) return ctx->fail(FDG_EINVAL, "sample mask must cover the sample grid");
-    fdg_frame* f = nullptr;
-    int rc = fdg_frame_open(ctx, v, p, route, form, bla_eps, &f);
-    if (rc) return rc;
-    rc = fdg_frame_mask(f, mask);
-    if (!rc) rc = fdg_frame_run(f, nullptr);
-    if (!rc) rc = fdg_frame_read(f, out, raw, st, kernel_id, kernel_cap);
-    fdg_frame_close(f);
-    return rc;
+    return guarded<int>(ctx ? &ctx->err : nullptr, [&]() -> int {
+        if (!ctx) return FDG_EINVAL;
+        if (!out || !out->class_) return ctx->fail(FDG_EINVAL, "the class column is required");
+        if (!mask) return ctx->fail(FDG_EINVAL, "sample mask must cover the sample grid");
+        fdg_frame* f = nullptr;
+        int rc = fdg_frame_open(ctx, v, p, route, form, bla_eps, &f);
+        if (rc) return rc;
+        rc = fdg_frame_mask(f, mask);
+        if (!rc) rc = fdg_frame_run(f, nullptr);
+        if (!rc) rc = fdg_frame_read(f, out, raw, st, kernel_id, kernel_cap);
+        fdg_frame_close(f);
+        return rc;
+    });
 }
 
 void fdg_frame_close(fdg_frame* f) {
-    if (!f) return;
-    (void)hipSetDevice(f->ctx->device);  // teardown: nothing to report to
-    (void)hipStreamSynchronize(f->ctx->stream);
-    f->buf.release();
-    f->mask.release();
-    f->series_b.release();
-    delete f;
+    guarded_void([&] {
+        if (!f) return;
+        (void)hipSetDevice(f->ctx->device);  // teardown: nothing to report to
+        (void)hipStreamSynchronize(f->ctx->stream);
+        f->buf.release();
+        f->mask.release();
+        f->series_b.release();
+        delete f;
+    });
 }
 
 void fdg_series_config_default(fdg_series_config* cfg) {
-    if (!cfg) return;
-    SeriesConfig c;
-    cfg->start_order = c.start_order;
-    cfg->max_order = c.max_order;
-    cfg->grow_at = c.grow_at;
-    cfg->shift_px = c.shift_px;
-    cfg->derivative_rel = c.derivative_rel;
+    guarded_void([&] {
+        if (!cfg) return;
+        SeriesConfig c;
+        cfg->start_order = c.start_order;
+        cfg->max_order = c.max_order;
+        cfg->grow_at = c.grow_at;
+        cfg->shift_px = c.shift_px;
+        cfg->derivative_rel = c.derivative_rel;
+    });
 }
 
 int fdg_series_build(const double* re, const double* im, uint64_t len, double rho, double pixel,
                      uint64_t max_iter, const fdg_series_config* cfg, fdg_series* out) {
-    if (!re || !im || !out || len < 1) return FDG_EINVAL;
-    SeriesConfig c;
-    if (cfg) {
-        c.start_order = cfg->start_order;
-        c.max_order = cfg->max_order;
-        c.grow_at = cfg->grow_at;
-        c.shift_px = cfg->shift_px;
-        c.derivative_rel = cfg->derivative_rel;
-    }
-    if (!out->b || out->b_cap < std::max<uint32_t>(c.max_order, 1)) return FDG_EINVAL;
-    fill_series(Series::build(re, im, len, rho, pixel, max_iter, c), out);
-    return FDG_OK;
+    return guarded<int>(nullptr, [&]() -> int {
+        if (!re || !im || !out || len < 1) return FDG_EINVAL;
+        SeriesConfig c;
+        if (cfg) {
+            c.start_order = cfg->start_order;
+            c.max_order = cfg->max_order;
+            c.grow_at = cfg->grow_at;
+            c.shift_px = cfg->shift_px;
+            c.derivative_rel = cfg->derivative_rel;
+        }
+        if (!out->b || out->b_cap < std::max<uint32_t>(c.max_order, 1)) return FDG_EINVAL;
+        fill_series(Series::build(re, im, len, rho, pixel, max_iter, c), out);
+        return FDG_OK;
+    });
 }
 
 int fdg_bla_dc_max(const fdg_view* v, const fdg_params* p, double* dc_max) {
-    if (!v || !p || !dc_max) return FDG_EINVAL;
-    if (p->ss == 0 || p->nx % p->ss || p->ny % p->ss) return FDG_EINVAL;
-    Plane pl;
-    std::string e;
-    if (!Plane::make(v->center_re, v->center_im, v->width, v->rotation, p->nx, p->ny, &pl, &e))
-        return FDG_EINVAL;
-    Tier t = p->tier < 0 ? pl.tier : (Tier)p->tier;
-    if (t == TIER_SCALED) return FDG_EINVAL;  // BLA tables need f64 deltas
-    int64_t de;
-    dc_bound(pl, t, p->nx, p->ny, dc_max, &de);
-    return FDG_OK;
+    return guarded<int>(nullptr, [&]() -> int {
+        if (!v || !p || !dc_max) return FDG_EINVAL;
+        if (p->ss == 0 || p->nx % p->ss || p->ny % p->ss) return FDG_EINVAL;
+        Plane pl;
+        std::string e;
+        if (!Plane::make(v->center_re, v->center_im, v->width, v->rotation, p->nx, p->ny, &pl, &e))
+            return FDG_EINVAL;
+        Tier t = p->tier < 0 ? pl.tier : (Tier)p->tier;
+        if (t == TIER_SCALED) return FDG_EINVAL;  // BLA tables need f64 deltas
+        int64_t de;
+        dc_bound(pl, t, p->nx, p->ny, dc_max, &de);
+        return FDG_OK;
+    });
 }
 
 int fdg_frame_series(fdg_frame* f, const fdg_series* sr) {
-    if (!f) return FDG_EINVAL;
-    fdg_ctx* ctx = f->ctx;
-    if (!sr) {
-        f->has_series = false;
+    return guarded<int>(f && f->ctx ? &f->ctx->err : nullptr, [&]() -> int {
+        if (!f) return FDG_EINVAL;
+        fdg_ctx* ctx = f->ctx;
+        if (!sr) {
+            f->has_series = false;
+            return FDG_OK;
+        }
+        // series.rs:342-366 render_masked's refusals
+        if (f->fr.tier == TIER_SCALED)
+            return ctx->fail(FDG_EINVAL, "a series start needs f64 deltas and no Bound column");
+        if (f->form != FDG_FORM_F64)
+            return ctx->fail(FDG_EDECLINED, "a series start is served by the FP64 form");
+        if (sr->points != f->len) return ctx->fail(FDG_EINVAL, "series built over another orbit");
+        if (std::isnan(f->dc) || f->dc > sr->rho) {
+            char m[160];
+            std::snprintf(m, sizeof(m), "view reaches |dc| = %e, beyond the series radius %e",
+                          f->dc, sr->rho);
+            return ctx->fail(FDG_EINVAL, m);
+        }
+        if (sr->n > 0 && (sr->n + 1 >= f->len || sr->b_len == 0 || !sr->b))
+            return ctx->fail(FDG_EINVAL, "series start outside its orbit");
+        // (Series::build stops below max_iter; a later start would leave no iterate to run)
+        if (sr->n >= f->params.max_iter)
+            return ctx->fail(FDG_EINVAL, "series start at or beyond max_iter");
+        FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+        if (sr->n > 0) {
+            size_t bytes = (size_t)sr->b_len * 2 * sizeof(double);
+            FDG_TRY(ctx, f->series_b.ensure(bytes), "hipMalloc");
+            FDG_TRY(ctx,
+                    hipMemcpyAsync(f->series_b.p, sr->b, bytes, hipMemcpyHostToDevice, ctx->stream),
+                    "series upload");
+            FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "series upload");
+        }
+        f->has_series = true;
+        f->series_n = (uint32_t)sr->n;
+        f->series_len = sr->b_len;
+        f->series_inv_rho = 1.0 / sr->rho;
         return FDG_OK;
-    }
-    // series.rs:342-366 render_masked's refusals
-    if (f->fr.tier == TIER_SCALED)
-        return ctx->fail(FDG_EINVAL, "a series start needs f64 deltas and no Bound column");
-    if (f->form != FDG_FORM_F64)
-        return ctx->fail(FDG_EDECLINED, "a series start is served by the FP64 form");
-    if (sr->points != f->len) return ctx->fail(FDG_EINVAL, "series built over another orbit");
-    if (std::isnan(f->dc) || f->dc > sr->rho) {
-        char m[160];
-        std::snprintf(m, sizeof(m), "view reaches |dc| = %e, beyond the series radius %e",
-                      f->dc, sr->rho);
-        return ctx->fail(FDG_EINVAL, m);
-    }
-    if (sr->n > 0 && (sr->n + 1 >= f->len || sr->b_len == 0 || !sr->b))
-        return ctx->fail(FDG_EINVAL, "series start outside its orbit");
-    // (Series::build stops below max_iter; a later start would leave no iterate to run)
-    if (sr->n >= f->params.max_iter)
-        return ctx->fail(FDG_EINVAL, "series start at or beyond max_iter");
-    FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
-    if (sr->n > 0) {
-        size_t bytes = (size_t)sr->b_len * 2 * sizeof(double);
-        FDG_TRY(ctx, f->series_b.ensure(bytes), "hipMalloc");
-        FDG_TRY(ctx,
-                hipMemcpyAsync(f->series_b.p, sr->b, bytes, hipMemcpyHostToDevice, ctx->stream),
-                "series upload");
-        FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "series upload");
-    }
-    f->has_series = true;
-    f->series_n = (uint32_t)sr->n;
-    f->series_len = sr->b_len;
-    f->series_inv_rho = 1.0 / sr->rho;
-    return FDG_OK;
+    });
 }
 
 
 int fdg_frame_series_prepare(fdg_frame* f, const fdg_series_config* cfg, fdg_series* out,
                              int* admitted) {
-    if (!f || !out || !admitted) return FDG_EINVAL;
-    fdg_ctx* ctx = f->ctx;
-    *admitted = 0;
-    SeriesConfig c;
-    if (cfg) {
-        c.start_order = cfg->start_order;
-        c.max_order = cfg->max_order;
-        c.grow_at = cfg->grow_at;
-        c.shift_px = cfg->shift_px;
-        c.derivative_rel = cfg->derivative_rel;
-    }
-    if (!out->b || out->b_cap < std::max<uint32_t>(c.max_order, 1))
-        return ctx->fail(FDG_EINVAL, "series coefficient buffer too small");
-    out->n = 0;
-    out->b_len = 0;
-    const fdg_params* p = &f->params;
-    uint64_t active = f->count;
-    if (f->masked) {
-        active = 0;
-        for (uint8_t m : f->mask_host) active += m != 0;
-    }
-    // series.rs:248-261: small frames, the scaled tier and short budgets decline before
-    // building (the FP32 forms take no series start)
-    int rc = fdg_frame_series(f, nullptr);
-    if (rc) return rc;
-    if (active < 16384 || f->fr.tier == TIER_SCALED || p->max_iter < 256 ||
-        f->form != FDG_FORM_F64)
-        return FDG_OK;
-    // series.rs:275-284: the series of the probe's budget (max_iter capped at 8192)
-    Series sr = Series::build(f->orbit_re.data(), f->orbit_im.data(), f->len, f->dc,
-                              f->fr.pl.h() * (double)p->ss,
-                              std::min<uint64_t>(p->max_iter, 8192), c);
-    fill_series(sr, out);
-    if (sr.n == 0) return FDG_OK;
-    // The probe: every 61st active sample plus the 16 x 16 block at the centre (where a
-    // zoom's minibrot and its slow samples sit; the reference's nine probe points include
-    // the centre), rendered without and with the start on this frame's route, at the
-    // reference probe's budget: max_iter capped at 8192, so a long sample is censored and
-    // its series attempt retried, which makes the admission conservative (series.rs:
-    // 266-274).  Costs are predicted from counted work, so the decision is the same on
-    // every run (the reference times the nine samples on the CPU, series.rs:285-339).
-    std::vector<uint8_t> saved = f->mask_host;
-    bool was_masked = f->masked;
-    std::vector<uint8_t> probe(f->count, 0);
-    uint64_t started = 0, seen = 0;
-    const uint32_t nx = p->nx, ny = p->ny;
-    for (uint32_t i = 0; i < f->count; i++) {
-        if (was_masked && !saved[i]) continue;
-        uint32_t x = i % nx, y = i / nx;
-        bool centre = x + 8 >= nx / 2 && x < nx / 2 + 8 && y + 8 >= ny / 2 && y < ny / 2 + 8;
-        if (seen++ % 61 == 0 || centre) {
-            probe[i] = 1;
-            started++;
+    return guarded<int>(f && f->ctx ? &f->ctx->err : nullptr, [&]() -> int {
+        if (!f || !out || !admitted) return FDG_EINVAL;
+        fdg_ctx* ctx = f->ctx;
+        *admitted = 0;
+        SeriesConfig c;
+        if (cfg) {
+            c.start_order = cfg->start_order;
+            c.max_order = cfg->max_order;
+            c.grow_at = cfg->grow_at;
+            c.shift_px = cfg->shift_px;
+            c.derivative_rel = cfg->derivative_rel;
         }
-    }
-    f->run_max_iter = std::min<uint64_t>(p->max_iter, 8192);
-    rc = fdg_frame_mask(f, probe.data());
-    fdg_stats base, with;
-    if (!rc) rc = fdg_frame_run(f, &base);
-    if (!rc) rc = fdg_frame_series(f, out);
-    if (!rc) rc = fdg_frame_run(f, &with);
-    f->run_max_iter = 0;
-    int rc2 = fdg_frame_mask(f, was_masked ? saved.data() : nullptr);
-    if (rc) return rc;
-    if (rc2) return rc2;
-    f->ran = false;  // the probe's outcomes are not the frame's
-    const double scale = (double)active / (double)std::max<uint64_t>(started, 1);
-    const double parallel = 128.0 * ctx->cus;
-    double cost0 = run_cost(base, 0, 0, scale, parallel);
-    double cost1 = run_cost(with, started, out->b_len, scale, parallel);
-    // series.rs:336-339: a predicted net win with a 20% margin
-    if (cost1 < 0.8 * cost0) {
-        *admitted = 1;
-        return FDG_OK;
-    }
-    return fdg_frame_series(f, nullptr);
+        if (!out->b || out->b_cap < std::max<uint32_t>(c.max_order, 1))
+            return ctx->fail(FDG_EINVAL, "series coefficient buffer too small");
+        out->n = 0;
+        out->b_len = 0;
+        const fdg_params* p = &f->params;
+        uint64_t active = f->count;
+        if (f->masked) {
+            active = 0;
+            for (uint8_t m : f->mask_host) active += m != 0;
+        }
+        // series.rs:248-261: small frames, the scaled tier and short budgets decline before
+        // building (the FP32 forms take no series start)
+        int rc = fdg_frame_series(f, nullptr);
+        if (rc) return rc;
+        if (active < 16384 || f->fr.tier == TIER_SCALED || p->max_iter < 256 ||
+            f->form != FDG_FORM_F64)
+            return FDG_OK;
+        // series.rs:275-284: the series of the probe's budget (max_iter capped at 8192)
+        Series sr = Series::build(f->orbit_re.data(), f->orbit_im.data(), f->len, f->dc,
+                                  f->fr.pl.h() * (double)p->ss,
+                                  std::min<uint64_t>(p->max_iter, 8192), c);
+        fill_series(sr, out);
+        if (sr.n == 0) return FDG_OK;
+        // The probe: every 61st active sample plus the 16 x 16 block at the centre (where a
+        // zoom's minibrot and its slow samples sit; the reference's nine probe points include
+        // the centre), rendered without and with the start on this frame's route, at the
+        // reference probe's budget: max_iter capped at 8192, so a long sample is censored and
+        // its series attempt retried, which makes the admission conservative (series.rs:
+        // 266-274).  Costs are predicted from counted work, so the decision is the same on
+        // every run (the reference times the nine samples on the CPU, series.rs:285-339).
+        std::vector<uint8_t> saved = f->mask_host;
+        bool was_masked = f->masked;
+        std::vector<uint8_t> probe(f->count, 0);
+        uint64_t started = 0, seen = 0;
+        const uint32_t nx = p->nx, ny = p->ny;
+        for (uint32_t i = 0; i < f->count; i++) {
+            if (was_masked && !saved[i]) continue;
+            uint32_t x = i % nx, y = i / nx;
+            bool centre = x + 8 >= nx / 2 && x < nx / 2 + 8 && y + 8 >= ny / 2 && y < ny / 2 + 8;
+            if (seen++ % 61 == 0 || centre) {
+                probe[i] = 1;
+                started++;
+            }
+        }
+        f->run_max_iter = std::min<uint64_t>(p->max_iter, 8192);
+        rc = fdg_frame_mask(f, probe.data());
+        fdg_stats base, with;
+        if (!rc) rc = fdg_frame_run(f, &base);
+        if (!rc) rc = fdg_frame_series(f, out);
+        if (!rc) rc = fdg_frame_run(f, &with);
+        f->run_max_iter = 0;
+        int rc2 = fdg_frame_mask(f, was_masked ? saved.data() : nullptr);
+        if (rc) return rc;
+        if (rc2) return rc2;
+        f->ran = false;  // the probe's outcomes are not the frame's
+        const double scale = (double)active / (double)std::max<uint64_t>(started, 1);
+        const double parallel = 128.0 * ctx->cus;
+        double cost0 = run_cost(base, 0, 0, scale, parallel);
+        double cost1 = run_cost(with, started, out->b_len, scale, parallel);
+        // series.rs:336-339: a predicted net win with a 20% margin
+        if (cost1 < 0.8 * cost0) {
+            *admitted = 1;
+            return FDG_OK;
+        }
+        return fdg_frame_series(f, nullptr);
+    });
 }
 
 int fdg_render_series(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int route,
                       double bla_eps, const fdg_series* sr, const uint8_t* mask,
                       fdg_columns* out, fdg_outcomes* raw, fdg_stats* st, char* kernel_id,
                       size_t kernel_cap) {
-    if (!ctx) return FDG_EINVAL;
-    if (!out || !out->class_) return ctx->fail(FDG_EINVAL, "the class column is required");
-    if (!sr) return ctx->fail(FDG_EINVAL, "no series");
-    fdg_frame* f = nullptr;
-    int rc = fdg_frame_open(ctx, v, p, route, FDG_FORM_F64, bla_eps, &f);
-    if (rc) return rc;
-    rc = fdg_frame_series(f, sr);
-    if (!rc && mask) rc = fdg_frame_mask(f, mask);
-    if (!rc) rc = fdg_frame_run(f, nullptr);
-    if (!rc) rc = fdg_frame_read(f, out, raw, st, kernel_id, kernel_cap);
-    fdg_frame_close(f);
-    return rc;
+    return guarded<int>(ctx ? &ctx->err : nullptr, [&]() -> int {
+        if (!ctx) return FDG_EINVAL;
+        if (!out || !out->class_) return ctx->fail(FDG_EINVAL, "the class column is required");
+        if (!sr) return ctx->fail(FDG_EINVAL, "no series");
+        fdg_frame* f = nullptr;
+        int rc = fdg_frame_open(ctx, v, p, route, FDG_FORM_F64, bla_eps, &f);
+        if (rc) return rc;
+        rc = fdg_frame_series(f, sr);
+        if (!rc && mask) rc = fdg_frame_mask(f, mask);
+        if (!rc) rc = fdg_frame_run(f, nullptr);
+        if (!rc) rc = fdg_frame_read(f, out, raw, st, kernel_id, kernel_cap);
+        fdg_frame_close(f);
+        return rc;
+    });
 }
 
 int fdg_jetaa_first(uint32_t w, uint32_t h, uint8_t* mask) {
-    if (!mask) return FDG_EINVAL;
-    jetaa_first(w, h, mask);
-    return FDG_OK;
+    return guarded<int>(nullptr, [&]() -> int {
+        if (!mask) return FDG_EINVAL;
+        jetaa_first(w, h, mask);
+        return FDG_OK;
+    });
 }
 
 int fdg_jetaa_rebuild(fdg_columns* c, uint32_t w, uint32_t h, uint8_t* todo) {
-    if (!c || !c->class_ || !c->nu || !c->de || !c->normal || !todo) return FDG_EINVAL;
-    jetaa_rebuild(c->class_, c->nu, c->de, c->normal, w, h, todo);
-    return FDG_OK;
+    return guarded<int>(nullptr, [&]() -> int {
+        if (!c || !c->class_ || !c->nu || !c->de || !c->normal || !todo) return FDG_EINVAL;
+        jetaa_rebuild(c->class_, c->nu, c->de, c->normal, w, h, todo);
+        return FDG_OK;
+    });
 }
 
 int fdg_jetaa_merge(fdg_columns* to, const fdg_columns* from, const uint8_t* mask, uint64_t n) {
-    if (!to || !from || !mask || !to->class_ || !from->class_) return FDG_EINVAL;
-    jetaa_merge(to->class_, to->nu, to->de, to->normal, from->class_, from->nu, from->de,
-                from->normal, mask, n);
-    return FDG_OK;
+    return guarded<int>(nullptr, [&]() -> int {
+        if (!to || !from || !mask || !to->class_ || !from->class_) return FDG_EINVAL;
+        jetaa_merge(to->class_, to->nu, to->de, to->normal, from->class_, from->nu, from->de,
+                    from->normal, mask, n);
+        return FDG_OK;
+    });
 }
 
 int fdg_frame_jet(fdg_frame* f, fdg_columns* out, fdg_stats* st, char* kernel_id,
                   size_t kernel_cap, uint64_t* computed) {
-    if (!f) return FDG_EINVAL;
-    fdg_ctx* ctx = f->ctx;
-    const fdg_params* p = &f->params;
-    if (p->ss != 2) return ctx->fail(FDG_EINVAL, "jet-AA rebuilds ss 2 frames");
-    const uint32_t need = FDG_COL_NU | FDG_COL_DE | FDG_COL_NORMAL;
-    if ((p->columns & need) != need || !out || !out->class_ || !out->nu || !out->de ||
-        !out->normal)
-        return ctx->fail(FDG_EINVAL, "jet-AA needs the nu, de and normal columns");
-    const size_t w = p->nx / 2, h = p->ny / 2, n = (size_t)f->count;
-    std::vector<uint8_t> first(n), todo(n);
-    jetaa_first(w, h, first.data());
-    fdg_stats s1, s2;
-    std::memset(&s2, 0, sizeof(s2));
-    // pass 1: each pixel's top-left sample
-    int rc = fdg_frame_mask(f, first.data());
-    if (!rc) rc = fdg_frame_run(f, nullptr);
-    if (!rc) rc = fdg_frame_read(f, out, nullptr, &s1, kernel_id, kernel_cap);
-    if (rc) {
-        (void)fdg_frame_mask(f, nullptr);
-        return rc;
-    }
-    double t = now_s();
-    jetaa_rebuild(out->class_, out->nu, out->de, out->normal, w, h, todo.data());
-    double rebuild_s = now_s() - t;
-    uint64_t rest = 0;
-    for (uint8_t v : todo) rest += v;
-    if (rest > 0) {
-        // pass 2: the rejected predictions, then merged over the rebuilt frame
-        std::vector<uint8_t> c2(n);
-        std::vector<double> nu2(n);
-        std::vector<float> de2(n);
-        std::vector<uint16_t> nm2(n);
-        fdg_columns second{c2.data(), nu2.data(), de2.data(), nm2.data()};
-        rc = fdg_frame_mask(f, todo.data());
+    return guarded<int>(f && f->ctx ? &f->ctx->err : nullptr, [&]() -> int {
+        if (!f) return FDG_EINVAL;
+        fdg_ctx* ctx = f->ctx;
+        const fdg_params* p = &f->params;
+        if (p->ss != 2) return ctx->fail(FDG_EINVAL, "jet-AA rebuilds ss 2 frames");
+        const uint32_t need = FDG_COL_NU | FDG_COL_DE | FDG_COL_NORMAL;
+        if ((p->columns & need) != need || !out || !out->class_ || !out->nu || !out->de ||
+            !out->normal)
+            return ctx->fail(FDG_EINVAL, "jet-AA needs the nu, de and normal columns");
+        const size_t w = p->nx / 2, h = p->ny / 2, n = (size_t)f->count;
+        // The passes set masks of their own; a sample mask the caller had set on the frame
+        // is put back on every exit (it does not restrict the jet frame).
+        const bool had_mask = f->masked;
+        const std::vector<uint8_t> saved_mask = had_mask ? f->mask_host : std::vector<uint8_t>();
+        auto restore = [&]() { return fdg_frame_mask(f, had_mask ? saved_mask.data() : nullptr); };
+        std::vector<uint8_t> first(n), todo(n);
+        jetaa_first(w, h, first.data());
+        fdg_stats s1, s2;
+        std::memset(&s2, 0, sizeof(s2));
+        // pass 1: each pixel's top-left sample
+        int rc = fdg_frame_mask(f, first.data());
         if (!rc) rc = fdg_frame_run(f, nullptr);
-        if (!rc) rc = fdg_frame_read(f, &second, nullptr, &s2, nullptr, 0);
+        if (!rc) rc = fdg_frame_read(f, out, nullptr, &s1, kernel_id, kernel_cap);
         if (rc) {
-            (void)fdg_frame_mask(f, nullptr);
+            (void)restore();
             return rc;
         }
-        jetaa_merge(out->class_, out->nu, out->de, out->normal, second.class_, second.nu,
-                    second.de, second.normal, todo.data(), n);
-    }
-    rc = fdg_frame_mask(f, nullptr);
-    if (rc) return rc;
-    if (computed) *computed = w * h + rest;
-    if (st) {
-        fdg_stats s = s1;
-        s.iterations += s2.iterations;
-        s.equivalent += s2.equivalent;
-        s.blocks += s2.blocks;
-        s.skipped += s2.skipped;
-        s.plain_steps += s2.plain_steps;
-        s.fallback_samples += s2.fallback_samples;
-        s.closed_form_samples += s2.closed_form_samples;
-        s.max_sample_ops = std::max(s.max_sample_ops, s2.max_sample_ops);
-        s.wave_iterations += s2.wave_iterations;
-        s.lane_iterations += s2.lane_iterations;
-        s.bla_seconds += s2.bla_seconds;
-        s.kernel_seconds += s2.kernel_seconds;
-        // (each pass's transfers include the frame's orbit upload once)
-        if (rest > 0) s.transfer_seconds += s2.transfer_seconds - f->transfer_seconds;
-        s.encode_seconds += s2.encode_seconds + rebuild_s;
-        *st = s;
-    }
-    return FDG_OK;
+        double t = now_s();
+        jetaa_rebuild(out->class_, out->nu, out->de, out->normal, w, h, todo.data());
+        double rebuild_s = now_s() - t;
+        uint64_t rest = 0;
+        for (uint8_t v : todo) rest += v;
+        if (rest > 0) {
+            // pass 2: the rejected predictions, then merged over the rebuilt frame
+            std::vector<uint8_t> c2(n);
+            std::vector<double> nu2(n);
+            std::vector<float> de2(n);
+            std::vector<uint16_t> nm2(n);
+            fdg_columns second{c2.data(), nu2.data(), de2.data(), nm2.data()};
+            rc = fdg_frame_mask(f, todo.data());
+            if (!rc) rc = fdg_frame_run(f, nullptr);
+            if (!rc) rc = fdg_frame_read(f, &second, nullptr, &s2, nullptr, 0);
+            if (rc) {
+                (void)restore();
+                return rc;
+            }
+            jetaa_merge(out->class_, out->nu, out->de, out->normal, second.class_, second.nu,
+                        second.de, second.normal, todo.data(), n);
+        }
+        rc = restore();
+        if (rc) return rc;
+        if (computed) *computed = w * h + rest;
+        if (st) {
+            fdg_stats s = s1;
+            s.iterations += s2.iterations;
+            s.equivalent += s2.equivalent;
+            s.blocks += s2.blocks;
+            s.skipped += s2.skipped;
+            s.plain_steps += s2.plain_steps;
+            s.fallback_samples += s2.fallback_samples;
+            s.closed_form_samples += s2.closed_form_samples;
+            s.max_sample_ops = std::max(s.max_sample_ops, s2.max_sample_ops);
+            s.wave_iterations += s2.wave_iterations;
+            s.lane_iterations += s2.lane_iterations;
+            s.bla_seconds += s2.bla_seconds;
+            s.kernel_seconds += s2.kernel_seconds;
+            // (each pass's transfers include the frame's orbit upload once)
+            if (rest > 0) s.transfer_seconds += s2.transfer_seconds - f->transfer_seconds;
+            s.encode_seconds += s2.encode_seconds + rebuild_s;
+            *st = s;
+        }
+        return FDG_OK;
+    });
 }
 
 int fdg_render(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int route, int form,
                double bla_eps, fdg_columns* out, fdg_outcomes* raw, fdg_stats* st,
                char* kernel_id, size_t kernel_cap) {
-    if (!ctx) return FDG_EINVAL;
-    if (!out || !out->class_) return ctx->fail(FDG_EINVAL, "the class column is required");
-    fdg_frame* f = nullptr;
-    int rc = fdg_frame_open(ctx, v, p, route, form, bla_eps, &f);
-    if (rc) return rc;
-    rc = fdg_frame_run(f, nullptr);
-    if (!rc) rc = fdg_frame_read(f, out, raw, st, kernel_id, kernel_cap);
-    fdg_frame_close(f);
-    return rc;
+    return guarded<int>(ctx ? &ctx->err : nullptr, [&]() -> int {
+        if (!ctx) return FDG_EINVAL;
+        if (!out || !out->class_) return ctx->fail(FDG_EINVAL, "the class column is required");
+        fdg_frame* f = nullptr;
+        int rc = fdg_frame_open(ctx, v, p, route, form, bla_eps, &f);
+        if (rc) return rc;
+        rc = fdg_frame_run(f, nullptr);
+        if (!rc) rc = fdg_frame_read(f, out, raw, st, kernel_id, kernel_cap);
+        fdg_frame_close(f);
+        return rc;
+    });
 }
 
 int fdg_jacobian(const fdg_view* v, const fdg_params* p, const fdg_outcomes* raw, double* j,
                  int64_t* jexp) {
-    if (!v || !p || !raw || !raw->kind || !raw->dr || !raw->di || !raw->dexp || !j || !jexp)
-        return FDG_EINVAL;
-    Plane pl;
-    std::string e;
-    if (!Plane::make(v->center_re, v->center_im, v->width, v->rotation, p->nx, p->ny, &pl, &e))
-        return FDG_EINVAL;
-    size_t n = (size_t)p->nx * p->ny;
-    for (size_t i = 0; i < n; i++) {
-        double* o = j + 4 * i;
-        if (raw->kind[i] != FDG_ESCAPED) {
-            o[0] = o[1] = o[2] = o[3] = 0.0;
-            jexp[i] = 0;
-            continue;
+    return guarded<int>(nullptr, [&]() -> int {
+        if (!v || !p || !raw || !raw->kind || !raw->dr || !raw->di || !raw->dexp || !j || !jexp)
+            return FDG_EINVAL;
+        Plane pl;
+        std::string e;
+        if (!Plane::make(v->center_re, v->center_im, v->width, v->rotation, p->nx, p->ny, &pl, &e))
+            return FDG_EINVAL;
+        size_t n = (size_t)p->nx * p->ny;
+        for (size_t i = 0; i < n; i++) {
+            double* o = j + 4 * i;
+            if (raw->kind[i] != FDG_ESCAPED) {
+                o[0] = o[1] = o[2] = o[3] = 0.0;
+                jexp[i] = 0;
+                continue;
+            }
+            // dz/dc h_m, then the grid's basis: d c / d x = h (cos, sin), d c / d y = h (sin, -cos)
+            double ar = raw->dr[i] * pl.h_m, ai = raw->di[i] * pl.h_m;
+            double xr = ar * pl.cos_r - ai * pl.sin_r, xi = ar * pl.sin_r + ai * pl.cos_r;
+            double yr = ar * pl.sin_r + ai * pl.cos_r, yi = ai * pl.sin_r - ar * pl.cos_r;
+            o[0] = xr;  // d Re z / d x
+            o[1] = yr;  // d Re z / d y
+            o[2] = xi;  // d Im z / d x
+            o[3] = yi;  // d Im z / d y
+            jexp[i] = raw->dexp[i] + pl.h_e;
         }
-        // dz/dc h_m, then the grid's basis: d c / d x = h (cos, sin), d c / d y = h (sin, -cos)
-        double ar = raw->dr[i] * pl.h_m, ai = raw->di[i] * pl.h_m;
-        double xr = ar * pl.cos_r - ai * pl.sin_r, xi = ar * pl.sin_r + ai * pl.cos_r;
-        double yr = ar * pl.sin_r + ai * pl.cos_r, yi = ai * pl.sin_r - ar * pl.cos_r;
-        o[0] = xr;  // d Re z / d x
-        o[1] = yr;  // d Re z / d y
-        o[2] = xi;  // d Im z / d x
-        o[3] = yi;  // d Im z / d y
-        jexp[i] = raw->dexp[i] + pl.h_e;
-    }
-    return FDG_OK;
+        return FDG_OK;
+    });
 }
 
 int64_t fdg_write_fds(const fdg_view* v, const fdg_params* p, const char* kernel_id,
                       const fdg_columns* c, uint8_t* buf, uint64_t cap) {
-    if (!v || !p || !c || !c->class_ || !kernel_id) return FDG_EINVAL;
-    std::vector<uint8_t> b = write_fds(v->center_re, v->center_im, v->width, v->rotation, p->nx,
-                                       p->ny, p->ss, p->max_iter, p->escape_radius, p->columns,
-                                       kernel_id, c->class_, c->nu, c->de, c->normal);
-    if (buf && cap >= b.size()) std::memcpy(buf, b.data(), b.size());
-    return (int64_t)b.size();
+    return guarded<int64_t>(nullptr, [&]() -> int64_t {
+        if (!v || !p || !c || !c->class_ || !kernel_id) return FDG_EINVAL;
+        std::vector<uint8_t> b = write_fds(v->center_re, v->center_im, v->width, v->rotation, p->nx,
+                                           p->ny, p->ss, p->max_iter, p->escape_radius, p->columns,
+                                           kernel_id, c->class_, c->nu, c->de, c->normal);
+        if (buf && cap >= b.size()) std::memcpy(buf, b.data(), b.size());
+        return (int64_t)b.size();
+    });
 }
 
 }  // extern "C"

@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "abi_guard.hpp"
 #include "fdg.h"
 
 namespace {
@@ -277,149 +278,172 @@ bool same_bytes(const std::vector<uint8_t>& a, const uint8_t* b, uint64_t n) {
 
 }  // namespace
 
+using fdg::guarded;
+
 extern "C" {
 
 int fdg_chunk_id(const uint8_t* bytes, uint64_t n, uint8_t id[32]) {
-    if ((!bytes && n) || !id) return FDG_EINVAL;
-    Sha256 s;
-    s.update(bytes, n);
-    s.finish(id);
-    return FDG_OK;
+    return guarded<int>(nullptr, [&]() -> int {
+        if ((!bytes && n) || !id) return FDG_EINVAL;
+        Sha256 s;
+        s.update(bytes, n);
+        s.finish(id);
+        return FDG_OK;
+    });
 }
 
 int64_t fdg_chunk_orbit_slab(const double* re, const double* im, uint64_t start, uint32_t len,
                              uint32_t size, uint32_t bits, uint8_t* buf, uint64_t cap) {
-    if (!re || !im) return FDG_EINVAL;
-    std::vector<uint8_t> b;
-    if (!slab_bytes(re, im, start, len, size, bits, &b)) return FDG_EINVAL;
-    return emit(b, buf, cap);
+    return guarded<int64_t>(nullptr, [&]() -> int64_t {
+        if (!re || !im) return FDG_EINVAL;
+        std::vector<uint8_t> b;
+        if (!slab_bytes(re, im, start, len, size, bits, &b)) return FDG_EINVAL;
+        return emit(b, buf, cap);
+    });
 }
 
 int fdg_chunk_orbit_slab_read(const uint8_t* bytes, uint64_t n, uint64_t* start, uint32_t* len,
                               uint32_t* size, uint32_t* bits, double* re, double* im,
                               uint64_t cap) {
-    Contract c;
-    const uint8_t* p;
-    uint64_t plen;
-    if (!bytes || !parse(bytes, n, &c, &p, &plen) || !(c == slab_contract(c.precision_bits)) ||
-        plen < 16 || rd16(bytes + 10) != 0)
-        return FDG_EINVAL;
-    uint32_t sz = rd32(p), ln = rd32(p + 4);
-    uint64_t st = rd64(p + 8);
-    if (plen != 16 + 16 * (uint64_t)ln) return FDG_EINVAL;
-    std::vector<double> r(ln), m(ln);
-    for (uint32_t i = 0; i < ln; i++) {
-        uint64_t a = rd64(p + 16 + 8 * (uint64_t)i), b = rd64(p + 16 + 8 * ((uint64_t)ln + i));
-        std::memcpy(&r[i], &a, 8);
-        std::memcpy(&m[i], &b, 8);
-    }
-    std::vector<uint8_t> again;
-    if (!slab_bytes(r.data(), m.data(), st, ln, sz, c.precision_bits, &again) ||
-        !same_bytes(again, bytes, n))
-        return FDG_EINVAL;  // not canonical
-    if (start) *start = st;
-    if (len) *len = ln;
-    if (size) *size = sz;
-    if (bits) *bits = c.precision_bits;
-    if (re && im && cap >= ln) {
-        std::memcpy(re, r.data(), 8 * (size_t)ln);
-        std::memcpy(im, m.data(), 8 * (size_t)ln);
-    }
-    return FDG_OK;
+    return guarded<int>(nullptr, [&]() -> int {
+        Contract c;
+        const uint8_t* p;
+        uint64_t plen;
+        if (!bytes || !parse(bytes, n, &c, &p, &plen) || !(c == slab_contract(c.precision_bits)) ||
+            plen < 16 || rd16(bytes + 10) != 0)
+            return FDG_EINVAL;
+        uint32_t sz = rd32(p), ln = rd32(p + 4);
+        uint64_t st = rd64(p + 8);
+        if (plen != 16 + 16 * (uint64_t)ln) return FDG_EINVAL;
+        std::vector<double> r(ln), m(ln);
+        for (uint32_t i = 0; i < ln; i++) {
+            uint64_t a = rd64(p + 16 + 8 * (uint64_t)i), b = rd64(p + 16 + 8 * ((uint64_t)ln + i));
+            std::memcpy(&r[i], &a, 8);
+            std::memcpy(&m[i], &b, 8);
+        }
+        std::vector<uint8_t> again;
+        if (!slab_bytes(r.data(), m.data(), st, ln, sz, c.precision_bits, &again) ||
+            !same_bytes(again, bytes, n))
+            return FDG_EINVAL;  // not canonical
+        if (start) *start = st;
+        if (len) *len = ln;
+        if (size) *size = sz;
+        if (bits) *bits = c.precision_bits;
+        if (re && im && cap >= ln) {
+            std::memcpy(re, r.data(), 8 * (size_t)ln);
+            std::memcpy(im, m.data(), 8 * (size_t)ln);
+        }
+        return FDG_OK;
+    });
 }
 
 int64_t fdg_chunk_orbit_manifest(const char* center_re, const char* center_im, uint32_t bits,
                                  const uint8_t* slab_ids, uint64_t n, uint8_t* buf,
                                  uint64_t cap) {
-    if (!center_re || !center_im || !slab_ids || n == 0) return FDG_EINVAL;
-    Builder b{manifest_contract(bits), {}};
-    b.str(center_re);
-    b.str(center_im);
-    b.u64(n);
-    b.raw(slab_ids, 32 * n);
-    return emit(b.finish(), buf, cap);
+    return guarded<int64_t>(nullptr, [&]() -> int64_t {
+        if (!center_re || !center_im || !slab_ids || n == 0) return FDG_EINVAL;
+        Builder b{manifest_contract(bits), {}};
+        b.str(center_re);
+        b.str(center_im);
+        b.u64(n);
+        b.raw(slab_ids, 32 * n);
+        return emit(b.finish(), buf, cap);
+    });
 }
 
 int fdg_chunk_orbit_manifest_read(const uint8_t* bytes, uint64_t n, char* center_re,
                                   char* center_im, uint64_t str_cap, uint32_t* bits,
                                   uint8_t* slab_ids, uint64_t ids_cap, uint64_t* count) {
-    Contract c;
-    const uint8_t* p;
-    uint64_t plen;
-    if (!bytes || !parse(bytes, n, &c, &p, &plen) ||
-        !(c == manifest_contract(c.precision_bits)) || rd16(bytes + 10) != 0)
-        return FDG_EINVAL;
-    uint64_t at = 0;
-    std::string s[2];
-    for (auto& x : s) {
+    return guarded<int>(nullptr, [&]() -> int {
+        Contract c;
+        const uint8_t* p;
+        uint64_t plen;
+        if (!bytes || !parse(bytes, n, &c, &p, &plen) ||
+            !(c == manifest_contract(c.precision_bits)) || rd16(bytes + 10) != 0)
+            return FDG_EINVAL;
+        uint64_t at = 0;
+        std::string s[2];
+        for (auto& x : s) {
+            if (at + 8 > plen) return FDG_EINVAL;
+            uint64_t k = rd64(p + at);
+            at += 8;
+            if (k > plen - at) return FDG_EINVAL;
+            x.assign(reinterpret_cast<const char*>(p + at), k);
+            at += k;
+        }
         if (at + 8 > plen) return FDG_EINVAL;
-        uint64_t k = rd64(p + at);
+        uint64_t ns = rd64(p + at);
         at += 8;
-        if (k > plen - at) return FDG_EINVAL;
-        x.assign(reinterpret_cast<const char*>(p + at), k);
-        at += k;
-    }
-    if (at + 8 > plen) return FDG_EINVAL;
-    uint64_t ns = rd64(p + at);
-    at += 8;
-    if (ns == 0 || ns > (plen - at) / 32 || at + 32 * ns != plen) return FDG_EINVAL;
-    if (center_re && center_im && str_cap > std::max(s[0].size(), s[1].size())) {
-        std::memcpy(center_re, s[0].c_str(), s[0].size() + 1);
-        std::memcpy(center_im, s[1].c_str(), s[1].size() + 1);
-    }
-    if (bits) *bits = c.precision_bits;
-    if (count) *count = ns;
-    if (slab_ids && ids_cap >= ns) std::memcpy(slab_ids, p + at, 32 * ns);
-    return FDG_OK;
+        if (ns == 0 || ns > (plen - at) / 32 || at + 32 * ns != plen) return FDG_EINVAL;
+        if (center_re && center_im && str_cap > std::max(s[0].size(), s[1].size())) {
+            std::memcpy(center_re, s[0].c_str(), s[0].size() + 1);
+            std::memcpy(center_im, s[1].c_str(), s[1].size() + 1);
+        }
+        if (bits) *bits = c.precision_bits;
+        if (count) *count = ns;
+        if (slab_ids && ids_cap >= ns) std::memcpy(slab_ids, p + at, 32 * ns);
+        return FDG_OK;
+    });
 }
 
 int64_t fdg_chunk_bla(const uint8_t orbit_id[32], double eps, double dc_max, uint64_t points,
                       uint32_t levels, const double* blocks, uint8_t* buf, uint64_t cap) {
-    if (!orbit_id || (!blocks && levels)) return FDG_EINVAL;
-    std::vector<uint8_t> b;
-    if (!bla_bytes(orbit_id, eps, dc_max, points, levels, blocks, &b)) return FDG_EINVAL;
-    return emit(b, buf, cap);
+    return guarded<int64_t>(nullptr, [&]() -> int64_t {
+        if (!orbit_id || (!blocks && levels)) return FDG_EINVAL;
+        std::vector<uint8_t> b;
+        if (!bla_bytes(orbit_id, eps, dc_max, points, levels, blocks, &b)) return FDG_EINVAL;
+        return emit(b, buf, cap);
+    });
 }
 
 int fdg_chunk_bla_read(const uint8_t* bytes, uint64_t n, uint8_t orbit_id[32], double* eps,
                        double* dc_max, uint64_t* points, uint32_t* levels, double* blocks,
                        uint64_t cap_blocks) {
-    Contract c;
-    const uint8_t* p;
-    uint64_t plen;
-    if (!bytes || !parse(bytes, n, &c, &p, &plen) || !(c == kBlaContract) ||
-        rd16(bytes + 10) != 0 || plen < 64)
-        return FDG_EINVAL;
-    uint64_t a = rd64(p + 32), d = rd64(p + 40), pts = rd64(p + 48), nl = rd64(p + 56);
-    if (nl >= 63) return FDG_EINVAL;
-    double e, dc;
-    std::memcpy(&e, &a, 8);
-    std::memcpy(&dc, &d, 8);
-    const uint64_t steps = pts >= 2 ? pts - 2 : 0;
-    uint64_t total = 0;
-    for (uint64_t j = 1; j <= nl; j++) total += steps >> j;
-    if (plen != 64 + 56 * total) return FDG_EINVAL;
-    std::vector<double> blk(7 * total);
-    uint64_t at = 64, off = 0;
-    for (uint64_t j = 1; j <= nl; j++) {
-        uint64_t cnt = steps >> j;
-        for (int f = 0; f < 7; f++)
-            for (uint64_t t = 0; t < cnt; t++, at += 8) {
-                uint64_t v = rd64(p + at);
-                std::memcpy(&blk[7 * (off + t) + f], &v, 8);
-            }
-        off += cnt;
-    }
-    std::vector<uint8_t> again;
-    if (!bla_bytes(p, e, dc, pts, (uint32_t)nl, blk.data(), &again) || !same_bytes(again, bytes, n))
-        return FDG_EINVAL;  // not canonical (or not a valid table)
-    if (orbit_id) std::memcpy(orbit_id, p, 32);
-    if (eps) *eps = e;
-    if (dc_max) *dc_max = dc;
-    if (points) *points = pts;
-    if (levels) *levels = (uint32_t)nl;
-    if (blocks && cap_blocks >= total) std::memcpy(blocks, blk.data(), 8 * blk.size());
-    return FDG_OK;
+    return guarded<int>(nullptr, [&]() -> int {
+        Contract c;
+        const uint8_t* p;
+        uint64_t plen;
+        if (!bytes || !parse(bytes, n, &c, &p, &plen) || !(c == kBlaContract) ||
+            rd16(bytes + 10) != 0 || plen < 64)
+            return FDG_EINVAL;
+        uint64_t a = rd64(p + 32), d = rd64(p + 40), pts = rd64(p + 48), nl = rd64(p + 56);
+        if (nl >= 63) return FDG_EINVAL;
+        double e, dc;
+        std::memcpy(&e, &a, 8);
+        std::memcpy(&dc, &d, 8);
+        const uint64_t steps = pts >= 2 ? pts - 2 : 0;
+        uint64_t total = 0;
+        // The block count comes from the chunk's own header: bound it by the payload before
+        // any product with it (56 * total can wrap for a forged points field), as the
+        // reference checks lengths before reading (crates/fd-atlas/src/bla.rs:100).
+        const uint64_t fits = (plen - 64) / 56;
+        for (uint64_t j = 1; j <= nl; j++) {
+            total += steps >> j;
+            if (total > fits) return FDG_EINVAL;
+        }
+        if (plen != 64 + 56 * total) return FDG_EINVAL;
+        std::vector<double> blk(7 * total);
+        uint64_t at = 64, off = 0;
+        for (uint64_t j = 1; j <= nl; j++) {
+            uint64_t cnt = steps >> j;
+            for (int f = 0; f < 7; f++)
+                for (uint64_t t = 0; t < cnt; t++, at += 8) {
+                    uint64_t v = rd64(p + at);
+                    std::memcpy(&blk[7 * (off + t) + f], &v, 8);
+                }
+            off += cnt;
+        }
+        std::vector<uint8_t> again;
+        if (!bla_bytes(p, e, dc, pts, (uint32_t)nl, blk.data(), &again) || !same_bytes(again, bytes, n))
+            return FDG_EINVAL;  // not canonical (or not a valid table)
+        if (orbit_id) std::memcpy(orbit_id, p, 32);
+        if (eps) *eps = e;
+        if (dc_max) *dc_max = dc;
+        if (points) *points = pts;
+        if (levels) *levels = (uint32_t)nl;
+        if (blocks && cap_blocks >= total) std::memcpy(blocks, blk.data(), 8 * blk.size());
+        return FDG_OK;
+    });
 }
 
 }  // extern "C"

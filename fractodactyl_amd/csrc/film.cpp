@@ -303,391 +303,407 @@ extern "C" {
 
 int fdg_film_open(fdg_ctx* ctx, const fdg_view* views, uint32_t nframes, const fdg_params* p,
                   int route, int form, double bla_eps, int tables, fdg_film** out) {
-    if (!ctx || !out) return FDG_EINVAL;
-    *out = nullptr;
-    if (!views || nframes == 0) return ctx->fail(FDG_EINVAL, "a film needs at least one view");
-    if (!p) return ctx->fail(FDG_EINVAL, "null params");
-    FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
-    if (p->columns & (FDG_COL_BOUND | FDG_COL_STRIPE))
-        return ctx->fail(FDG_EDECLINED,
-                         "bound/stripe columns are CPU-kernel columns (frame.rs:133-144)");
-    if (form != FDG_FORM_F64) return ctx->fail(FDG_EINVAL, "unknown delta form");
-    if (route != FDG_ROUTE_PLAIN && route != FDG_ROUTE_BLA)
-        return ctx->fail(FDG_EINVAL, "unknown route");
-    if (tables != FDG_TABLES_PER_FRAME && tables != FDG_TABLES_SHARED)
-        return ctx->fail(FDG_EINVAL, "unknown table mode");
-    if (p->max_iter >= (1ull << 31)) return ctx->fail(FDG_EINVAL, "max_iter must be < 2^31");
-    uint64_t per = (uint64_t)p->nx * p->ny;
-    if (per * nframes >= (1ull << 31)) return ctx->fail(FDG_EINVAL, "film too large");
-    if (route == FDG_ROUTE_BLA) {
-        if (std::isnan(p->escape_radius) || p->escape_radius < 4.0)
-            return ctx->fail(FDG_EINVAL, "escape radius is below 4: BLA needs >= 4");
-        if (!(bla_eps > 0.0 && bla_eps <= kEpsLimit))
-            return ctx->fail(FDG_EINVAL, "BLA needs 0 < eps <= 2^-24");
-    }
-    std::vector<Frame> fr(nframes);
-    int lead[2] = {-1, -1};  // per orbit: the frame it is computed for (deepest, earliest)
-    for (uint32_t k = 0; k < nframes; k++) {
-        int rc = setup(ctx, &views[k], p, &fr[k]);
+    return guarded<int>(ctx ? &ctx->err : nullptr, [&]() -> int {
+        if (!ctx || !out) return FDG_EINVAL;
+        *out = nullptr;
+        if (!views || nframes == 0) return ctx->fail(FDG_EINVAL, "a film needs at least one view");
+        if (!p) return ctx->fail(FDG_EINVAL, "null params");
+        FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+        if (p->columns & (FDG_COL_BOUND | FDG_COL_STRIPE))
+            return ctx->fail(FDG_EDECLINED,
+                             "bound/stripe columns are CPU-kernel columns (frame.rs:133-144)");
+        if (form != FDG_FORM_F64) return ctx->fail(FDG_EINVAL, "unknown delta form");
+        if (route != FDG_ROUTE_PLAIN && route != FDG_ROUTE_BLA)
+            return ctx->fail(FDG_EINVAL, "unknown route");
+        if (tables != FDG_TABLES_PER_FRAME && tables != FDG_TABLES_SHARED)
+            return ctx->fail(FDG_EINVAL, "unknown table mode");
+        if (p->max_iter >= (1ull << 31)) return ctx->fail(FDG_EINVAL, "max_iter must be < 2^31");
+        uint64_t per = (uint64_t)p->nx * p->ny;
+        if (per * nframes >= (1ull << 31)) return ctx->fail(FDG_EINVAL, "film too large");
+        if (route == FDG_ROUTE_BLA) {
+            if (std::isnan(p->escape_radius) || p->escape_radius < 4.0)
+                return ctx->fail(FDG_EINVAL, "escape radius is below 4: BLA needs >= 4");
+            if (!(bla_eps > 0.0 && bla_eps <= kEpsLimit))
+                return ctx->fail(FDG_EINVAL, "BLA needs 0 < eps <= 2^-24");
+        }
+        std::vector<Frame> fr(nframes);
+        int lead[2] = {-1, -1};  // per orbit: the frame it is computed for (deepest, earliest)
+        for (uint32_t k = 0; k < nframes; k++) {
+            int rc = setup(ctx, &views[k], p, &fr[k]);
+            if (rc) return rc;
+            if (!same_decimal(views[k].center_re, views[0].center_re) ||
+                !same_decimal(views[k].center_im, views[0].center_im))
+                return ctx->fail(FDG_EINVAL, "film frames must share one centre");
+            int o = orbit_of_group(group_of(fr[k].tier));
+            if (lead[o] < 0 || fr[k].pl.bits > fr[lead[o]].pl.bits) lead[o] = (int)k;
+        }
+        fdg_film* f = new fdg_film();
+        struct Guard {
+            fdg_film* f;
+            bool keep = false;
+            ~Guard() {
+                if (!keep) {
+                    f->release();
+                    delete f;
+                }
+            }
+        } guard{f};
+        f->ctx = ctx;
+        f->params = *p;
+        f->route = route;
+        f->form = form;
+        f->tables = tables;
+        f->eps = bla_eps;
+        f->nframes = nframes;
+        f->per = (uint32_t)per;
+        f->fr = fr;
+        for (uint32_t k = 0; k < nframes; k++) f->group.push_back(group_of(fr[k].tier));
+        for (int o = 0; o < 2; o++) {
+            if (lead[o] < 0) continue;
+            double t0 = now_s();
+            Orbit orbit;
+            int rc = compute_orbit(ctx, &views[lead[o]], p, fr[lead[o]], &orbit);
+            if (rc) return rc;
+            // as a single frame refuses them: the loop reads Z_1, and the scaled tier's
+            // rebase from a delta below 1e-270 continues to Z_2
+            if (orbit.len() < (o == 1 ? 3u : 2u))
+                return ctx->fail(FDG_EINVAL, "reference orbit too short for the film's frames");
+            f->reference_seconds += now_s() - t0;
+            FilmOrbit& ob = f->orb[o];
+            ob.len = orbit.len();
+            ob.bits = orbit.bits;
+            ob.nucleus = orbit.nucleus_period();
+            ob.steps = ob.len >= 2 ? ob.len - 2 : 0;
+            ob.stride = next_pow2_host(std::max<uint64_t>(ob.steps, 2));
+            // grid.rs:474-476: the plane records the supplied orbit's precision
+            for (uint32_t k = 0; k < nframes; k++)
+                if (f->orbit_of(k) == o && o == 1) f->fr[k].pl.bits = orbit.bits;
+            rc = upload_orbit(ctx, &ob.orbit, orbit, &f->transfer_seconds);
+            if (rc) return rc;
+        }
+        int rc = f->buf.outputs(ctx, f->count());
         if (rc) return rc;
-        if (!same_decimal(views[k].center_re, views[0].center_re) ||
-            !same_decimal(views[k].center_im, views[0].center_im))
-            return ctx->fail(FDG_EINVAL, "film frames must share one centre");
-        int o = orbit_of_group(group_of(fr[k].tier));
-        if (lead[o] < 0 || fr[k].pl.bits > fr[lead[o]].pl.bits) lead[o] = (int)k;
-    }
-    fdg_film* f = new fdg_film();
-    struct Guard {
-        fdg_film* f;
-        bool keep = false;
-        ~Guard() {
-            if (!keep) {
-                f->release();
-                delete f;
+        f->levels.assign(nframes, 0);
+        f->tb.assign(nframes, 0);
+        if (route == FDG_ROUTE_BLA) {
+            rc = film_tables(f);
+            if (rc) return rc;
+        }
+        // batch groups, frames in order within each; one BatchFrameDev array
+        std::vector<BatchFrameDev> bd;
+        uint32_t first[NGROUPS];
+        for (int g = 0; g < NGROUPS; g++) {
+            first[g] = (uint32_t)bd.size();
+            for (uint32_t k = 0; k < nframes; k++) {
+                if (f->group[k] != g) continue;
+                const Plane& pl = f->fr[k].pl;
+                BatchFrameDev b;
+                b.h = pl.h();
+                b.h_m = pl.h_m;
+                b.h_e = pl.h_e;
+                b.cos_r = pl.cos_r;
+                b.sin_r = pl.sin_r;
+                b.half_x = pl.half_x;
+                b.half_y = pl.half_y;
+                b.out0 = k * f->per;
+                b.tb = f->tb[k];
+                bd.push_back(b);
+                f->group_frames[g]++;
+                f->group_levels[g] = std::max(f->group_levels[g], f->levels[k]);
             }
         }
-    } guard{f};
-    f->ctx = ctx;
-    f->params = *p;
-    f->route = route;
-    f->form = form;
-    f->tables = tables;
-    f->eps = bla_eps;
-    f->nframes = nframes;
-    f->per = (uint32_t)per;
-    f->fr = fr;
-    for (uint32_t k = 0; k < nframes; k++) f->group.push_back(group_of(fr[k].tier));
-    for (int o = 0; o < 2; o++) {
-        if (lead[o] < 0) continue;
-        double t0 = now_s();
-        Orbit orbit;
-        int rc = compute_orbit(ctx, &views[lead[o]], p, fr[lead[o]], &orbit);
-        if (rc) return rc;
-        f->reference_seconds += now_s() - t0;
-        FilmOrbit& ob = f->orb[o];
-        ob.len = orbit.len();
-        ob.bits = orbit.bits;
-        ob.nucleus = orbit.nucleus_period();
-        ob.steps = ob.len >= 2 ? ob.len - 2 : 0;
-        ob.stride = next_pow2_host(std::max<uint64_t>(ob.steps, 2));
-        // grid.rs:474-476: the plane records the supplied orbit's precision
-        for (uint32_t k = 0; k < nframes; k++)
-            if (f->orbit_of(k) == o && o == 1) f->fr[k].pl.bits = orbit.bits;
-        rc = upload_orbit(ctx, &ob.orbit, orbit, &f->transfer_seconds);
-        if (rc) return rc;
-    }
-    int rc = f->buf.outputs(ctx, f->count());
-    if (rc) return rc;
-    f->levels.assign(nframes, 0);
-    f->tb.assign(nframes, 0);
-    if (route == FDG_ROUTE_BLA) {
-        rc = film_tables(f);
-        if (rc) return rc;
-    }
-    // batch groups, frames in order within each; one BatchFrameDev array
-    std::vector<BatchFrameDev> bd;
-    uint32_t first[NGROUPS];
-    for (int g = 0; g < NGROUPS; g++) {
-        first[g] = (uint32_t)bd.size();
+        FDG_TRY(ctx, f->batch_frames.ensure(bd.size() * sizeof(BatchFrameDev)), "hipMalloc");
+        FDG_TRY(ctx, f->fdev_batch.ensure(NGROUPS * sizeof(FrameDev)), "hipMalloc");
+        FDG_TRY(ctx, f->fdev_each.ensure(nframes * sizeof(FrameDev)), "hipMalloc");
+        FDG_TRY(ctx, f->counters.ensure((size_t)(NGROUPS + nframes) * 256), "hipMalloc");
+        FrameDev gd[NGROUPS];
+        for (int g = 0; g < NGROUPS; g++) {
+            gd[g] = orbit_frame_dev(f, orbit_of_group(g));
+            gd[g].tier = g;  // G_F64 / G_FIXED / G_SCALED are the tiers' values
+            gd[g].frames = f->batch_frames.as<BatchFrameDev>() + first[g];
+            gd[g].nframes = f->group_frames[g];
+            gd[g].per_frame = f->per;
+            gd[g].count = f->group_frames[g] * f->per;
+            gd[g].bla.nlevels = f->group_levels[g];
+            gd[g].next = reinterpret_cast<uint32_t*>(f->group_counters(g));
+            gd[g].stats = f->group_counters(g) + 8;
+            if (f->group_frames[g]) {
+                const Plane& pl = f->fr[bd[first[g]].out0 / f->per].pl;
+                gd[g].c_re = pl.c_re;  // the rounded centre, the same for every frame
+                gd[g].c_im = pl.c_im;
+            }
+        }
+        std::vector<FrameDev> each(nframes);
         for (uint32_t k = 0; k < nframes; k++) {
-            if (f->group[k] != g) continue;
             const Plane& pl = f->fr[k].pl;
-            BatchFrameDev b;
-            b.h = pl.h();
-            b.h_m = pl.h_m;
-            b.h_e = pl.h_e;
-            b.cos_r = pl.cos_r;
-            b.sin_r = pl.sin_r;
-            b.half_x = pl.half_x;
-            b.half_y = pl.half_y;
-            b.out0 = k * f->per;
-            b.tb = f->tb[k];
-            bd.push_back(b);
-            f->group_frames[g]++;
-            f->group_levels[g] = std::max(f->group_levels[g], f->levels[k]);
+            FrameDev d = orbit_frame_dev(f, f->orbit_of(k));
+            uint64_t off = (uint64_t)k * f->per;
+            d.bla.r2 += f->tb[k];
+            d.bla.ab += 4 * (uint64_t)f->tb[k];
+            d.bla.nlevels = f->levels[k];
+            d.c_re = pl.c_re;
+            d.c_im = pl.c_im;
+            d.h = pl.h();
+            d.h_m = pl.h_m;
+            d.h_e = pl.h_e;
+            d.cos_r = pl.cos_r;
+            d.sin_r = pl.sin_r;
+            d.half_x = pl.half_x;
+            d.half_y = pl.half_y;
+            d.count = f->per;
+            d.tier = f->fr[k].tier;
+            d.kind += off;
+            d.n += off;
+            d.zr += off;
+            d.zi += off;
+            d.dr += off;
+            d.di += off;
+            d.dexp += off;
+            d.ops += off;
+            d.next = reinterpret_cast<uint32_t*>(f->frame_counters(k));
+            d.stats = f->frame_counters(k) + 8;
+            each[k] = d;
         }
-    }
-    FDG_TRY(ctx, f->batch_frames.ensure(bd.size() * sizeof(BatchFrameDev)), "hipMalloc");
-    FDG_TRY(ctx, f->fdev_batch.ensure(NGROUPS * sizeof(FrameDev)), "hipMalloc");
-    FDG_TRY(ctx, f->fdev_each.ensure(nframes * sizeof(FrameDev)), "hipMalloc");
-    FDG_TRY(ctx, f->counters.ensure((size_t)(NGROUPS + nframes) * 256), "hipMalloc");
-    FrameDev gd[NGROUPS];
-    for (int g = 0; g < NGROUPS; g++) {
-        gd[g] = orbit_frame_dev(f, orbit_of_group(g));
-        gd[g].tier = g;  // G_F64 / G_FIXED / G_SCALED are the tiers' values
-        gd[g].frames = f->batch_frames.as<BatchFrameDev>() + first[g];
-        gd[g].nframes = f->group_frames[g];
-        gd[g].per_frame = f->per;
-        gd[g].count = f->group_frames[g] * f->per;
-        gd[g].bla.nlevels = f->group_levels[g];
-        gd[g].next = reinterpret_cast<uint32_t*>(f->group_counters(g));
-        gd[g].stats = f->group_counters(g) + 8;
-        if (f->group_frames[g]) {
-            const Plane& pl = f->fr[bd[first[g]].out0 / f->per].pl;
-            gd[g].c_re = pl.c_re;  // the rounded centre, the same for every frame
-            gd[g].c_im = pl.c_im;
-        }
-    }
-    std::vector<FrameDev> each(nframes);
-    for (uint32_t k = 0; k < nframes; k++) {
-        const Plane& pl = f->fr[k].pl;
-        FrameDev d = orbit_frame_dev(f, f->orbit_of(k));
-        uint64_t off = (uint64_t)k * f->per;
-        d.bla.r2 += f->tb[k];
-        d.bla.ab += 4 * (uint64_t)f->tb[k];
-        d.bla.nlevels = f->levels[k];
-        d.c_re = pl.c_re;
-        d.c_im = pl.c_im;
-        d.h = pl.h();
-        d.h_m = pl.h_m;
-        d.h_e = pl.h_e;
-        d.cos_r = pl.cos_r;
-        d.sin_r = pl.sin_r;
-        d.half_x = pl.half_x;
-        d.half_y = pl.half_y;
-        d.count = f->per;
-        d.tier = f->fr[k].tier;
-        d.kind += off;
-        d.n += off;
-        d.zr += off;
-        d.zi += off;
-        d.dr += off;
-        d.di += off;
-        d.dexp += off;
-        d.ops += off;
-        d.next = reinterpret_cast<uint32_t*>(f->frame_counters(k));
-        d.stats = f->frame_counters(k) + 8;
-        each[k] = d;
-    }
-    FDG_TRY(ctx,
-            hipMemcpy(f->batch_frames.p, bd.data(), bd.size() * sizeof(BatchFrameDev),
-                      hipMemcpyHostToDevice),
-            "upload");
-    FDG_TRY(ctx, hipMemcpy(f->fdev_batch.p, gd, sizeof(gd), hipMemcpyHostToDevice), "upload");
-    std::memcpy(f->group_desc, gd, sizeof(gd));
-    f->frame_desc = each;
-    FDG_TRY(ctx,
-            hipMemcpy(f->fdev_each.p, each.data(), nframes * sizeof(FrameDev),
-                      hipMemcpyHostToDevice),
-            "upload");
-    FDG_TRY(ctx, hipStreamCreate(&f->side), "side stream");
-    FDG_TRY(ctx, hipEventCreateWithFlags(&f->fork, hipEventDisableTiming), "event");
-    FDG_TRY(ctx, hipEventCreateWithFlags(&f->join, hipEventDisableTiming), "event");
-    guard.keep = true;
-    *out = f;
-    return FDG_OK;
+        FDG_TRY(ctx,
+                hipMemcpy(f->batch_frames.p, bd.data(), bd.size() * sizeof(BatchFrameDev),
+                          hipMemcpyHostToDevice),
+                "upload");
+        FDG_TRY(ctx, hipMemcpy(f->fdev_batch.p, gd, sizeof(gd), hipMemcpyHostToDevice), "upload");
+        std::memcpy(f->group_desc, gd, sizeof(gd));
+        f->frame_desc = each;
+        FDG_TRY(ctx,
+                hipMemcpy(f->fdev_each.p, each.data(), nframes * sizeof(FrameDev),
+                          hipMemcpyHostToDevice),
+                "upload");
+        FDG_TRY(ctx, hipStreamCreate(&f->side), "side stream");
+        FDG_TRY(ctx, hipEventCreateWithFlags(&f->fork, hipEventDisableTiming), "event");
+        FDG_TRY(ctx, hipEventCreateWithFlags(&f->join, hipEventDisableTiming), "event");
+        guard.keep = true;
+        *out = f;
+        return FDG_OK;
+    });
 }
 
 int fdg_film_run(fdg_film* f, int launch, int grid, fdg_stats* st) {
-    if (!f) return FDG_EINVAL;
-    fdg_ctx* ctx = f->ctx;
-    FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
-    if (launch != FDG_LAUNCH_BATCH && launch != FDG_LAUNCH_PER_FRAME)
-        return ctx->fail(FDG_EINVAL, "unknown launch mode");
-    if (grid < 0 || grid > 65536) return ctx->fail(FDG_EINVAL, "grid must be in [0, 65536]");
-    if (grid == 0) grid = ctx->cus * 8;
-    fdg_stats s;
-    std::memset(&s, 0, sizeof(s));
-    int o = f->orb[1].len ? 1 : 0;
-    s.reference_seconds = f->reference_seconds;
-    s.reference_len = f->orb[o].len;
-    s.reference_bits = f->orb[o].bits;
-    s.nucleus_period = f->orb[o].nucleus;
-    s.transfer_seconds = f->transfer_seconds;
-    s.bla_seconds = f->bla_seconds;
-    s.bla_levels = *std::max_element(f->levels.begin(), f->levels.end());
-    std::vector<unsigned long long> cnt;
-    if (f->masked)
-        // samples outside the mask keep this "not computed" kind through the pass (a fill
-        // on the stream, outside the graph)
-        FDG_TRY(ctx, hipMemsetAsync(f->buf.o_kind.p, kKindUnset, f->count(), ctx->stream),
-                "fill");
-    if (launch == FDG_LAUNCH_BATCH) {
-        if (!f->exec || f->graph_grid != grid) {
-            int rc = film_capture(f, grid);
-            if (rc) return rc;
-        }
-        FDG_TRY(ctx, hipEventRecord(ctx->ev[2], ctx->stream), "event");
-        FDG_TRY(ctx, hipGraphLaunch(f->exec, ctx->stream), "graph launch");
-        FDG_TRY(ctx, hipEventRecord(ctx->ev[3], ctx->stream), "event");
-        cnt.resize(32 * NGROUPS);
-        FDG_TRY(ctx,
-                hipMemcpyAsync(cnt.data(), f->group_counters(0), cnt.size() * 8,
-                               hipMemcpyDeviceToHost, ctx->stream),
-                "readback");
-        FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "film kernels");
-        for (int g = 0; g < NGROUPS; g++) add_counters(cnt.data() + 32 * g + 8, &s);
-    } else {
-        bool deriv = deriv_of(f);
-        FDG_TRY(ctx, launch_zero_words(f->frame_counters(0), 32 * f->nframes, ctx->stream),
-                "counter clear");
-        FDG_TRY(ctx, hipEventRecord(ctx->ev[2], ctx->stream), "event");
-        int g = (int)std::min<uint64_t>((uint64_t)grid, (f->per + 255) / 256);
-        for (uint32_t k = 0; k < f->nframes; k++)
+    return guarded<int>(f && f->ctx ? &f->ctx->err : nullptr, [&]() -> int {
+        if (!f) return FDG_EINVAL;
+        fdg_ctx* ctx = f->ctx;
+        FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+        if (launch != FDG_LAUNCH_BATCH && launch != FDG_LAUNCH_PER_FRAME)
+            return ctx->fail(FDG_EINVAL, "unknown launch mode");
+        if (grid < 0 || grid > 65536) return ctx->fail(FDG_EINVAL, "grid must be in [0, 65536]");
+        if (grid == 0) grid = ctx->cus * 8;
+        fdg_stats s;
+        std::memset(&s, 0, sizeof(s));
+        int o = f->orb[1].len ? 1 : 0;
+        s.reference_seconds = f->reference_seconds;
+        s.reference_len = f->orb[o].len;
+        s.reference_bits = f->orb[o].bits;
+        s.nucleus_period = f->orb[o].nucleus;
+        s.transfer_seconds = f->transfer_seconds;
+        s.bla_seconds = f->bla_seconds;
+        s.bla_levels = *std::max_element(f->levels.begin(), f->levels.end());
+        std::vector<unsigned long long> cnt;
+        if (f->masked)
+            // samples outside the mask keep this "not computed" kind through the pass (a fill
+            // on the stream, outside the graph)
+            FDG_TRY(ctx, hipMemsetAsync(f->buf.o_kind.p, kKindUnset, f->count(), ctx->stream),
+                    "fill");
+        if (launch == FDG_LAUNCH_BATCH) {
+            if (!f->exec || f->graph_grid != grid) {
+                int rc = film_capture(f, grid);
+                if (rc) return rc;
+            }
+            FDG_TRY(ctx, hipEventRecord(ctx->ev[2], ctx->stream), "event");
+            FDG_TRY(ctx, hipGraphLaunch(f->exec, ctx->stream), "graph launch");
+            FDG_TRY(ctx, hipEventRecord(ctx->ev[3], ctx->stream), "event");
+            cnt.resize(32 * NGROUPS);
             FDG_TRY(ctx,
-                    launch_perturb_f64(f->fdev_each.as<FrameDev>() + k,
-                                       f->fr[k].tier == TIER_SCALED, deriv, f->levels[k] > 0,
-                                       false, std::max(g, 1), ctx->stream),
-                    "perturbation launch");
-        FDG_TRY(ctx, hipEventRecord(ctx->ev[3], ctx->stream), "event");
-        cnt.resize(32 * (size_t)f->nframes);
-        FDG_TRY(ctx,
-                hipMemcpyAsync(cnt.data(), f->frame_counters(0), cnt.size() * 8,
-                               hipMemcpyDeviceToHost, ctx->stream),
-                "readback");
-        FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "film kernels");
-        for (uint32_t k = 0; k < f->nframes; k++) add_counters(cnt.data() + 32 * k + 8, &s);
-    }
-    float ms = 0;
-    FDG_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]), "event timing");
-    s.kernel_seconds = ms * 1e-3;
-    f->last = s;
-    f->ran = true;
-    if (st) *st = s;
-    return FDG_OK;
+                    hipMemcpyAsync(cnt.data(), f->group_counters(0), cnt.size() * 8,
+                                   hipMemcpyDeviceToHost, ctx->stream),
+                    "readback");
+            FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "film kernels");
+            for (int g = 0; g < NGROUPS; g++) add_counters(cnt.data() + 32 * g + 8, &s);
+        } else {
+            bool deriv = deriv_of(f);
+            FDG_TRY(ctx, launch_zero_words(f->frame_counters(0), 32 * f->nframes, ctx->stream),
+                    "counter clear");
+            FDG_TRY(ctx, hipEventRecord(ctx->ev[2], ctx->stream), "event");
+            int g = (int)std::min<uint64_t>((uint64_t)grid, (f->per + 255) / 256);
+            for (uint32_t k = 0; k < f->nframes; k++)
+                FDG_TRY(ctx,
+                        launch_perturb_f64(f->fdev_each.as<FrameDev>() + k,
+                                           f->fr[k].tier == TIER_SCALED, deriv, f->levels[k] > 0,
+                                           false, std::max(g, 1), ctx->stream),
+                        "perturbation launch");
+            FDG_TRY(ctx, hipEventRecord(ctx->ev[3], ctx->stream), "event");
+            cnt.resize(32 * (size_t)f->nframes);
+            FDG_TRY(ctx,
+                    hipMemcpyAsync(cnt.data(), f->frame_counters(0), cnt.size() * 8,
+                                   hipMemcpyDeviceToHost, ctx->stream),
+                    "readback");
+            FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "film kernels");
+            for (uint32_t k = 0; k < f->nframes; k++) add_counters(cnt.data() + 32 * k + 8, &s);
+        }
+        float ms = 0;
+        FDG_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]), "event timing");
+        s.kernel_seconds = ms * 1e-3;
+        f->last = s;
+        f->ran = true;
+        if (st) *st = s;
+        return FDG_OK;
+    });
 }
 
 int fdg_film_read(fdg_film* f, uint32_t k, fdg_columns* out, fdg_outcomes* raw, fdg_stats* st,
                   char* kernel_id, size_t kernel_cap) {
-    if (!f) return FDG_EINVAL;
-    fdg_ctx* ctx = f->ctx;
-    if (!f->ran) return ctx->fail(FDG_EINVAL, "film has not been run");
-    if (k >= f->nframes) return ctx->fail(FDG_EINVAL, "frame index out of range");
-    const fdg_params* p = &f->params;
-    if (!out || !out->class_) return ctx->fail(FDG_EINVAL, "the class column is required");
-    if ((p->columns & FDG_COL_NU && !out->nu) || (p->columns & FDG_COL_DE && !out->de) ||
-        (p->columns & FDG_COL_NORMAL && !out->normal))
-        return ctx->fail(FDG_EINVAL, "a requested column has no buffer");
-    FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
-    fdg_stats s = f->last;
-    s.transfer_seconds = f->transfer_seconds;
-    std::vector<uint64_t> nn;
-    std::vector<uint8_t> kind;
-    std::vector<uint32_t> ops;
-    int rc = read_encode(ctx, f->buf, (size_t)k * f->per, f->per, f->fr[k].pl, p, out, raw,
-                         &s.transfer_seconds, &s.encode_seconds, &nn, &kind, &ops);
-    if (rc) return rc;
-    const FilmOrbit& ob = f->orb[f->orbit_of(k)];
-    s.reference_len = ob.len;
-    s.reference_bits = ob.bits;
-    s.nucleus_period = ob.nucleus;
-    s.bla_levels = f->levels[k];
-    // This frame's Stats (grid.rs:640-650): iterations counts a block as one iterate and
-    // stops at max_iter; equivalent counts every step.  n holds every sample's stop
-    // iterate, so spent - skipped + blocks = spent - n + (plain steps + blocks).
-    bool bla = f->levels[k] > 0;
-    uint64_t its = 0, eq = 0, mx = 0;
-    for (size_t i = 0; i < f->per; i++) {
-        uint64_t spent = kind[i] == KIND_UNRESOLVED ? p->max_iter : nn[i];
-        eq += spent;
-        its += bla ? spent - nn[i] + ops[i] : spent;
-        mx = std::max<uint64_t>(mx, ops[i]);
-    }
-    s.iterations = its;
-    s.equivalent = eq;
-    s.max_sample_ops = mx;
-    // launch-wide counters are not split by frame
-    s.blocks = s.skipped = s.plain_steps = s.fallback_samples = s.closed_form_samples = 0;
-    s.wave_iterations = s.lane_iterations = 0;
-    if (st) *st = s;
-    if (kernel_id && kernel_cap) {
-        std::string kid = f->fr[k].pl.kernel(f->fr[k].tier);
-        if (f->levels[k] > 0) kid += " bla/1";
-        kid += " gpu-hip-f64/1";
-        std::snprintf(kernel_id, kernel_cap, "%s", kid.c_str());
-    }
-    return FDG_OK;
+    return guarded<int>(f && f->ctx ? &f->ctx->err : nullptr, [&]() -> int {
+        if (!f) return FDG_EINVAL;
+        fdg_ctx* ctx = f->ctx;
+        if (!f->ran) return ctx->fail(FDG_EINVAL, "film has not been run");
+        if (k >= f->nframes) return ctx->fail(FDG_EINVAL, "frame index out of range");
+        const fdg_params* p = &f->params;
+        if (!out || !out->class_) return ctx->fail(FDG_EINVAL, "the class column is required");
+        if ((p->columns & FDG_COL_NU && !out->nu) || (p->columns & FDG_COL_DE && !out->de) ||
+            (p->columns & FDG_COL_NORMAL && !out->normal))
+            return ctx->fail(FDG_EINVAL, "a requested column has no buffer");
+        FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+        fdg_stats s = f->last;
+        s.transfer_seconds = f->transfer_seconds;
+        std::vector<uint64_t> nn;
+        std::vector<uint8_t> kind;
+        std::vector<uint32_t> ops;
+        int rc = read_encode(ctx, f->buf, (size_t)k * f->per, f->per, f->fr[k].pl, p, out, raw,
+                             &s.transfer_seconds, &s.encode_seconds, &nn, &kind, &ops);
+        if (rc) return rc;
+        const FilmOrbit& ob = f->orb[f->orbit_of(k)];
+        s.reference_len = ob.len;
+        s.reference_bits = ob.bits;
+        s.nucleus_period = ob.nucleus;
+        s.bla_levels = f->levels[k];
+        // This frame's Stats (grid.rs:640-650): iterations counts a block as one iterate and
+        // stops at max_iter; equivalent counts every step.  n holds every sample's stop
+        // iterate, so spent - skipped + blocks = spent - n + (plain steps + blocks).
+        bool bla = f->levels[k] > 0;
+        uint64_t its = 0, eq = 0, mx = 0;
+        for (size_t i = 0; i < f->per; i++) {
+            uint64_t spent = kind[i] == KIND_UNRESOLVED ? p->max_iter : nn[i];
+            eq += spent;
+            its += bla ? spent - nn[i] + ops[i] : spent;
+            mx = std::max<uint64_t>(mx, ops[i]);
+        }
+        s.iterations = its;
+        s.equivalent = eq;
+        s.max_sample_ops = mx;
+        // launch-wide counters are not split by frame
+        s.blocks = s.skipped = s.plain_steps = s.fallback_samples = s.closed_form_samples = 0;
+        s.wave_iterations = s.lane_iterations = 0;
+        if (st) *st = s;
+        if (kernel_id && kernel_cap) {
+            std::string kid = f->fr[k].pl.kernel(f->fr[k].tier);
+            if (f->levels[k] > 0) kid += " bla/1";
+            kid += " gpu-hip-f64/1";
+            std::snprintf(kernel_id, kernel_cap, "%s", kid.c_str());
+        }
+        return FDG_OK;
+    });
 }
 
 int fdg_film_mask(fdg_film* f, const uint8_t* masks) {
-    if (!f) return FDG_EINVAL;
-    fdg_ctx* ctx = f->ctx;
-    FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
-    const size_t n = f->count();
-    if (masks) {
-        FDG_TRY(ctx, f->mask.ensure(n), "hipMalloc");
-        FDG_TRY(ctx, hipMemcpy(f->mask.p, masks, n, hipMemcpyHostToDevice), "mask upload");
-    }
-    f->masked = masks != nullptr;
-    // the batch kernels index the mask by output sample, the per-frame ones by the
-    // frame's own sample
-    FrameDev gd[NGROUPS];
-    std::memcpy(gd, f->group_desc, sizeof(gd));
-    std::vector<FrameDev> each = f->frame_desc;
-    const uint8_t* m = f->masked ? f->mask.as<uint8_t>() : nullptr;
-    for (auto& d : gd) d.mask = m;
-    for (uint32_t k = 0; k < f->nframes; k++) each[k].mask = m ? m + (size_t)k * f->per : nullptr;
-    FDG_TRY(ctx, hipMemcpy(f->fdev_batch.p, gd, sizeof(gd), hipMemcpyHostToDevice), "upload");
-    FDG_TRY(ctx,
-            hipMemcpy(f->fdev_each.p, each.data(), each.size() * sizeof(FrameDev),
-                      hipMemcpyHostToDevice),
-            "upload");
-    return FDG_OK;
+    return guarded<int>(f && f->ctx ? &f->ctx->err : nullptr, [&]() -> int {
+        if (!f) return FDG_EINVAL;
+        fdg_ctx* ctx = f->ctx;
+        FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+        const size_t n = f->count();
+        if (masks) {
+            FDG_TRY(ctx, f->mask.ensure(n), "hipMalloc");
+            FDG_TRY(ctx, hipMemcpy(f->mask.p, masks, n, hipMemcpyHostToDevice), "mask upload");
+        }
+        f->masked = masks != nullptr;
+        // the batch kernels index the mask by output sample, the per-frame ones by the
+        // frame's own sample
+        FrameDev gd[NGROUPS];
+        std::memcpy(gd, f->group_desc, sizeof(gd));
+        std::vector<FrameDev> each = f->frame_desc;
+        const uint8_t* m = f->masked ? f->mask.as<uint8_t>() : nullptr;
+        for (auto& d : gd) d.mask = m;
+        for (uint32_t k = 0; k < f->nframes; k++) each[k].mask = m ? m + (size_t)k * f->per : nullptr;
+        FDG_TRY(ctx, hipMemcpy(f->fdev_batch.p, gd, sizeof(gd), hipMemcpyHostToDevice), "upload");
+        FDG_TRY(ctx,
+                hipMemcpy(f->fdev_each.p, each.data(), each.size() * sizeof(FrameDev),
+                          hipMemcpyHostToDevice),
+                "upload");
+        return FDG_OK;
+    });
 }
 
 int fdg_film_jet(fdg_film* f, fdg_columns* outs, uint64_t* computed) {
-    if (!f) return FDG_EINVAL;
-    fdg_ctx* ctx = f->ctx;
-    const fdg_params* p = &f->params;
-    if (p->ss != 2) return ctx->fail(FDG_EINVAL, "jet-AA rebuilds ss 2 frames");
-    const uint32_t need = FDG_COL_NU | FDG_COL_DE | FDG_COL_NORMAL;
-    if ((p->columns & need) != need || !outs)
-        return ctx->fail(FDG_EINVAL, "jet-AA needs the nu, de and normal columns");
-    for (uint32_t k = 0; k < f->nframes; k++)
-        if (!outs[k].class_ || !outs[k].nu || !outs[k].de || !outs[k].normal)
+    return guarded<int>(f && f->ctx ? &f->ctx->err : nullptr, [&]() -> int {
+        if (!f) return FDG_EINVAL;
+        fdg_ctx* ctx = f->ctx;
+        const fdg_params* p = &f->params;
+        if (p->ss != 2) return ctx->fail(FDG_EINVAL, "jet-AA rebuilds ss 2 frames");
+        const uint32_t need = FDG_COL_NU | FDG_COL_DE | FDG_COL_NORMAL;
+        if ((p->columns & need) != need || !outs)
             return ctx->fail(FDG_EINVAL, "jet-AA needs the nu, de and normal columns");
-    const size_t w = p->nx / 2, h = p->ny / 2, per = f->per, n = f->count();
-    // pass 1: every frame's top-left samples, in one batched launch
-    std::vector<uint8_t> masks(n);
-    jetaa_first(w, h, masks.data());
-    for (uint32_t k = 1; k < f->nframes; k++)
-        std::memcpy(masks.data() + (size_t)k * per, masks.data(), per);
-    int rc = fdg_film_mask(f, masks.data());
-    if (!rc) rc = fdg_film_run(f, FDG_LAUNCH_BATCH, 0, nullptr);
-    for (uint32_t k = 0; k < f->nframes && !rc; k++)
-        rc = fdg_film_read(f, k, &outs[k], nullptr, nullptr, nullptr, 0);
-    if (rc) {
-        (void)fdg_film_mask(f, nullptr);
-        return rc;
-    }
-    // the rebuilds, then pass 2: every frame's rejected samples in one launch
-    uint64_t rest = 0;
-    for (uint32_t k = 0; k < f->nframes; k++) {
-        uint8_t* todo = masks.data() + (size_t)k * per;
-        jetaa_rebuild(outs[k].class_, outs[k].nu, outs[k].de, outs[k].normal, w, h, todo);
-        for (size_t i = 0; i < per; i++) rest += todo[i];
-    }
-    if (rest > 0) {
-        rc = fdg_film_mask(f, masks.data());
+        for (uint32_t k = 0; k < f->nframes; k++)
+            if (!outs[k].class_ || !outs[k].nu || !outs[k].de || !outs[k].normal)
+                return ctx->fail(FDG_EINVAL, "jet-AA needs the nu, de and normal columns");
+        const size_t w = p->nx / 2, h = p->ny / 2, per = f->per, n = f->count();
+        // pass 1: every frame's top-left samples, in one batched launch
+        std::vector<uint8_t> masks(n);
+        jetaa_first(w, h, masks.data());
+        for (uint32_t k = 1; k < f->nframes; k++)
+            std::memcpy(masks.data() + (size_t)k * per, masks.data(), per);
+        int rc = fdg_film_mask(f, masks.data());
         if (!rc) rc = fdg_film_run(f, FDG_LAUNCH_BATCH, 0, nullptr);
-        std::vector<uint8_t> c2(per);
-        std::vector<double> nu2(per);
-        std::vector<float> de2(per);
-        std::vector<uint16_t> nm2(per);
-        fdg_columns second{c2.data(), nu2.data(), de2.data(), nm2.data()};
-        for (uint32_t k = 0; k < f->nframes && !rc; k++) {
-            rc = fdg_film_read(f, k, &second, nullptr, nullptr, nullptr, 0);
-            if (!rc)
-                jetaa_merge(outs[k].class_, outs[k].nu, outs[k].de, outs[k].normal, c2.data(),
-                            nu2.data(), de2.data(), nm2.data(), masks.data() + (size_t)k * per,
-                            per);
-        }
+        for (uint32_t k = 0; k < f->nframes && !rc; k++)
+            rc = fdg_film_read(f, k, &outs[k], nullptr, nullptr, nullptr, 0);
         if (rc) {
             (void)fdg_film_mask(f, nullptr);
             return rc;
         }
-    }
-    if (computed) *computed = (uint64_t)f->nframes * w * h + rest;
-    return fdg_film_mask(f, nullptr);
+        // the rebuilds, then pass 2: every frame's rejected samples in one launch
+        uint64_t rest = 0;
+        for (uint32_t k = 0; k < f->nframes; k++) {
+            uint8_t* todo = masks.data() + (size_t)k * per;
+            jetaa_rebuild(outs[k].class_, outs[k].nu, outs[k].de, outs[k].normal, w, h, todo);
+            for (size_t i = 0; i < per; i++) rest += todo[i];
+        }
+        if (rest > 0) {
+            rc = fdg_film_mask(f, masks.data());
+            if (!rc) rc = fdg_film_run(f, FDG_LAUNCH_BATCH, 0, nullptr);
+            std::vector<uint8_t> c2(per);
+            std::vector<double> nu2(per);
+            std::vector<float> de2(per);
+            std::vector<uint16_t> nm2(per);
+            fdg_columns second{c2.data(), nu2.data(), de2.data(), nm2.data()};
+            for (uint32_t k = 0; k < f->nframes && !rc; k++) {
+                rc = fdg_film_read(f, k, &second, nullptr, nullptr, nullptr, 0);
+                if (!rc)
+                    jetaa_merge(outs[k].class_, outs[k].nu, outs[k].de, outs[k].normal, c2.data(),
+                                nu2.data(), de2.data(), nm2.data(), masks.data() + (size_t)k * per,
+                                per);
+            }
+            if (rc) {
+                (void)fdg_film_mask(f, nullptr);
+                return rc;
+            }
+        }
+        if (computed) *computed = (uint64_t)f->nframes * w * h + rest;
+        return fdg_film_mask(f, nullptr);
+    });
 }
 
 void fdg_film_close(fdg_film* f) {
-    if (!f) return;
-    (void)hipSetDevice(f->ctx->device);  // teardown: nothing to report to
-    (void)hipStreamSynchronize(f->ctx->stream);
-    if (f->side) (void)hipStreamSynchronize(f->side);
-    f->release();
-    delete f;
+    guarded_void([&] {
+        if (!f) return;
+        (void)hipSetDevice(f->ctx->device);  // teardown: nothing to report to
+        (void)hipStreamSynchronize(f->ctx->stream);
+        if (f->side) (void)hipStreamSynchronize(f->side);
+        f->release();
+        delete f;
+    });
 }
 
 }  // extern "C"

@@ -585,23 +585,37 @@ struct SpinBarrier {
 // double-buffered by step parity, and a thread can only write a buffer again after all
 // have passed the next barrier, by which time they have read it).  With roles {0, 1, 2}
 // on one thread this is the serial loop.
+// Every buffer is allocated by the calling thread before a worker starts, so no worker
+// allocates (an exception in a worker thread would end the process), and the workers
+// wait at a start gate: when a second thread cannot be started the first is told to
+// leave before it reaches the step barrier, and the caller runs the serial loop.
 struct OrbitShared {
     size_t l;
     uint64_t steps;
     const Fixed *cr, *ci;
-    std::vector<uint64_t> sq;  // [parity][role] squares, l limbs each
+    std::vector<uint64_t> sq;       // [parity][role] squares, l limbs each
+    std::vector<uint64_t> buf[3];   // each worker's registers and scratch, by first role
+    std::atomic<int> gate{0};       // 0 wait, 1 run, 2 leave
     SpinBarrier bar;
     Orbit* out;
-    OrbitShared(size_t limbs, uint32_t threads) : l(limbs), sq(6 * limbs), bar(threads) {}
+    OrbitShared(size_t limbs, uint32_t threads) : l(limbs), sq(6 * limbs), bar(threads) {
+        for (uint32_t w = 0; w < threads; w++)
+            buf[w].assign(6 * limbs + limbs::sqr_scratch(limbs), 0);
+    }
 };
 
 void orbit_worker(OrbitShared* sh, int first_role, int last_role, bool emit) {
     const size_t l = sh->l;
-    std::vector<uint64_t> buf(6 * l + limbs::sqr_scratch(l), 0);
+    const bool threaded = !(first_role == 0 && last_role == 2);
+    if (threaded) {
+        int g;
+        while ((g = sh->gate.load(std::memory_order_acquire)) == 0) __builtin_ia32_pause();
+        if (g == 2) return;
+    }
+    std::vector<uint64_t>& buf = sh->buf[first_role];
     uint64_t *x = &buf[0], *y = &buf[l], *s = &buf[2 * l], *t = &buf[3 * l],
              *prod = &buf[4 * l], *scratch = &buf[6 * l];
     bool xn = false, yn = false;
-    const bool threaded = !(first_role == 0 && last_role == 2);
     for (uint64_t k = 0; k < sh->steps; k++) {
         uint64_t* sq = &sh->sq[(k & 1) * 3 * l];
         uint64_t *x2 = sq, *y2 = sq + l, *s2 = sq + 2 * l;
@@ -751,18 +765,36 @@ bool Orbit::fixed(const std::string& re, const std::string& im, uint64_t bits, u
         return true;
     }
     // Three threads from kOrbitThreadLimbs limbs up, where a square outweighs a barrier.
-    const bool threaded = l >= limbs::kOrbitThreadLimbs && std::thread::hardware_concurrency() >= 3;
-    OrbitShared sh(l, threaded ? 3 : 1);
-    sh.steps = steps;
-    sh.cr = &cr;
-    sh.ci = &ci;
-    sh.out = &o;
+    bool threaded = l >= limbs::kOrbitThreadLimbs && std::thread::hardware_concurrency() >= 3;
     if (threaded) {
-        std::thread t1(orbit_worker, &sh, 1, 1, false), t2(orbit_worker, &sh, 2, 2, false);
-        orbit_worker(&sh, 0, 0, true);
-        t1.join();
-        t2.join();
-    } else {
+        OrbitShared sh(l, 3);
+        sh.steps = steps;
+        sh.cr = &cr;
+        sh.ci = &ci;
+        sh.out = &o;
+        std::thread t1, t2;
+        try {
+            t1 = std::thread(orbit_worker, &sh, 1, 1, false);
+            t2 = std::thread(orbit_worker, &sh, 2, 2, false);
+        } catch (...) {
+            // (std::system_error: no thread for it) the started one leaves at the gate
+            sh.gate.store(2, std::memory_order_release);
+            if (t1.joinable()) t1.join();
+            threaded = false;
+        }
+        if (threaded) {
+            sh.gate.store(1, std::memory_order_release);
+            orbit_worker(&sh, 0, 0, true);
+            t1.join();
+            t2.join();
+        }
+    }
+    if (!threaded) {
+        OrbitShared sh(l, 1);
+        sh.steps = steps;
+        sh.cr = &cr;
+        sh.ci = &ci;
+        sh.out = &o;
         orbit_worker(&sh, 0, 2, true);
     }
     o.bits = (uint32_t)bits;

@@ -7,9 +7,28 @@
 #pragma once
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace fdg {
+
+// part(w) for every w in [0, n): parts 1.. on threads of their own, part 0 on the caller.
+// A thread that cannot be started (std::system_error, or no memory for its handle) runs
+// its part on the caller instead, so every part runs once and every started thread is
+// joined — a joinable std::thread is never destroyed.  Parts must not throw.
+template <class F>
+void run_parts(unsigned n, F part) {
+    std::vector<std::thread> th;
+    unsigned w = 1;
+    try {
+        th.reserve(n);
+        for (; w < n; w++) th.emplace_back(part, w);
+    } catch (...) {
+    }
+    for (unsigned v = w; v < n; v++) part(v);
+    part(0);
+    for (auto& t : th) t.join();
+}
 
 // 2^k as a double: exact where representable, 0 below the subnormals, inf above
 // (crates/fd-fixed/src/lib.rs:30-42).

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "abi_guard.hpp"
 #include "fdg.h"
 #include "host.hpp"
 #include "kernels.hpp"
@@ -48,7 +49,6 @@ constexpr uint32_t kMaxLevels = 48;
 constexpr uint8_t kKindUnset = 0xff;
 constexpr double kEpsLimit = 1.0 / 16777216.0;  // bla.rs:288-292 EPS_LIMIT = 2^-24
 
-extern std::string g_create_error;
 
 // A grow-only device allocation.
 struct DevBuf {

@@ -89,15 +89,10 @@ void par_for(size_t n, F f) {
         for (size_t i = 0; i < n; i++) f(i);
         return;
     }
-    std::vector<std::thread> pool;
-    for (unsigned t = 0; t < threads; t++) {
+    run_parts(threads, [&f, chunk, n](unsigned t) {
         size_t b = t * chunk, e = std::min(n, b + chunk);
-        if (b >= e) break;
-        pool.emplace_back([&f, b, e] {
-            for (size_t i = b; i < e; i++) f(i);
-        });
-    }
-    for (auto& th : pool) th.join();
+        for (size_t i = b; i < e; i++) f(i);
+    });
 }
 
 // A predicted sample (jetaa.rs:119-121): none (compute it), the pixel's own non-escaped

@@ -9,6 +9,11 @@
  *
  * Threading: a context owns one HIP device, one stream and its resident buffers; use a
  * context from one host thread at a time.  Several contexts may coexist.
+ *
+ * Exceptions: every entry point catches what the C++ runtime may throw inside it — a
+ * host allocation that fails returns FDG_ENOMEM, any other exception FDG_EDEVICE — and
+ * keeps the message for fdg_last_error (of the context, or of the calling thread when the
+ * call has none).
  */
 #ifndef FDG_H
 #define FDG_H
@@ -116,8 +121,8 @@ int fdg_abi_version(void);
  * wgpu device open in crates/fd-shade/src/gpu/testkit.rs:12-47. */
 int fdg_create(fdg_ctx **out, int device);
 void fdg_destroy(fdg_ctx *ctx);
-/* Message for the last non-zero status on this context (or of fdg_create when ctx is
- * NULL). */
+/* Message for the last non-zero status on this context, or, with ctx NULL, of the calling
+ * thread's last failed call without a context (fdg_create, host-only helpers). */
 const char *fdg_last_error(const fdg_ctx *ctx);
 
 /* perturb_gpu::enabled (frame.rs:62-68): "on" -> 1 route plain, "bla" -> 1 route BLA,
@@ -243,7 +248,8 @@ int fdg_render_series(fdg_ctx *ctx, const fdg_view *view, const fdg_params *para
  * fdg_frame_jet runs the whole jet-AA frame on a resident ss 2 frame: pass 1 on the GPU,
  * the rebuild on the host, pass 2 on the GPU for the rejected samples only, merged into
  * out; *computed receives the samples the GPU computed (w h + rejected), stats the sum of
- * both passes (encode_seconds includes the rebuild).  The frame's mask is cleared after;
+ * both passes (encode_seconds includes the rebuild).  A sample mask set on the frame does
+ * not restrict the jet frame and is set again afterwards;
  * a series start set on the frame applies to both passes. */
 int fdg_jetaa_first(uint32_t w, uint32_t h, uint8_t *mask);
 int fdg_jetaa_rebuild(fdg_columns *columns, uint32_t w, uint32_t h, uint8_t *todo);

@@ -122,3 +122,35 @@ def test_writer_matches_golden_bytes():
                                        h["escape_radius"], h["columns"] & ~1),
                       kernel=h["kernel"])
         assert fa.write(header, cols, minor=0) == data, name
+
+
+def test_a_failed_host_allocation_returns_enomem():
+    # In a child process whose address space is capped a little above what it uses, a
+    # host-only call that needs far more memory (the jet-AA rebuild of a 2^20 x 2^20 pixel
+    # grid: its first per-pixel array is 1 TiB) must return FDG_ENOMEM with a message, not
+    # throw into the caller or abort.  No column memory is touched before that array.
+    import subprocess
+    import sys
+    code = f"""
+import ctypes as C, resource, sys
+import numpy as np
+sys.path.insert(0, {ROOT!r})
+import fractodactyl_amd as fa
+L = fa.lib()
+def vm():
+    for l in open("/proc/self/status"):
+        if l.startswith("VmSize:"):
+            return int(l.split()[1]) * 1024
+lim = vm() + (512 << 20)
+resource.setrlimit(resource.RLIMIT_AS, (lim, lim))
+cols = dict(**{{"class": np.zeros(4, np.uint8), "nu": np.zeros(4), "de": np.zeros(4, np.float32),
+            "normal": np.zeros(4, np.uint16)}})
+todo = np.zeros(4, np.uint8)
+rc = L.fdg_jetaa_rebuild(C.byref(fa._cols_struct(cols)), 1 << 20, 1 << 20, fa._p(todo))
+print(rc, L.fdg_last_error(None).decode())
+"""
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    rc, msg = r.stdout.strip().split(" ", 1)
+    assert int(rc) == -4, r.stdout  # FDG_ENOMEM
+    assert "memory" in msg

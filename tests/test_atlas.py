@@ -124,3 +124,23 @@ def test_bla_chunk_equals_the_restatement_and_round_trips():
     bad[10] = 1
     with pytest.raises(fa.FdgError):
         fa.bla_chunk_read(bytes(bad))
+
+
+def test_bla_chunk_with_a_forged_block_count_is_refused():
+    # A points field chosen so that 56 x (blocks it implies) wraps to the payload's real
+    # length: the reader must bound the count by the payload before any product with it
+    # (crates/fd-atlas/src/bla.rs:100 checks lengths before reading).
+    fa = _product()
+    zr, zi, bits = orbit()
+    dc = 1e-28 * np.hypot(32, 18) / 64
+    levels = oracle.bla_build(zr, zi, 2.0 ** -50, dc)
+    oid = atlas.chunk_id(b"orbit")
+    c = bytearray(fa.bla_chunk(oid, 2.0 ** -50, dc, len(zr), levels))
+    t0 = sum(len(lv) for lv in levels)
+    forged_points = 2 * (t0 + (1 << 61)) + 2      # one level of t0 + 2^61 blocks
+    assert (56 * ((forged_points - 2) >> 1)) % (1 << 64) == 56 * t0
+    c[32 + 48:32 + 56] = forged_points.to_bytes(8, "little")
+    c[32 + 56:32 + 64] = (1).to_bytes(8, "little")
+    with pytest.raises(fa.FdgError) as e:
+        fa.bla_chunk_read(bytes(c))
+    assert e.value.code == -1  # FDG_EINVAL
