@@ -326,46 +326,48 @@ extern "C" {
 int fdg_abi_version(void) { return FDG_ABI_VERSION; }
 
 int fdg_create(fdg_ctx** out, int device) {
-    if (!out) return FDG_EINVAL;
-    *out = nullptr;
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
-        g_create_error = "no HIP device";
-        return FDG_EDEVICE;
-    }
-    if (device < 0 || device >= n) {
-        g_create_error = "device index out of range";
-        return FDG_EINVAL;
-    }
-    fdg_ctx* c = new fdg_ctx();
-    c->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
-        g_create_error = "cannot open the HIP device";
-        delete c;
-        return FDG_EDEVICE;
-    }
-    for (auto& e : c->ev) {
-        if (hipEventCreate(&e) != hipSuccess) {
-            g_create_error = "cannot create HIP events";
-            for (auto& x : c->ev)
-                if (x) (void)hipEventDestroy(x);
-            (void)hipStreamDestroy(c->stream);
+    return guarded<int>(nullptr, [&]() -> int {
+        if (!out) return FDG_EINVAL;
+        *out = nullptr;
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+            g_create_error = "no HIP device";
+            return FDG_EDEVICE;
+        }
+        if (device < 0 || device >= n) {
+            g_create_error = "device index out of range";
+            return FDG_EINVAL;
+        }
+        fdg_ctx* c = new fdg_ctx();
+        c->device = device;
+        if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+            g_create_error = "cannot open the HIP device";
             delete c;
             return FDG_EDEVICE;
         }
-    }
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
-    if (hipHostMalloc((void**)&c->small, fdg_ctx::kSmallBytes, hipHostMallocDefault) !=
-        hipSuccess) {
-        g_create_error = "cannot allocate page-locked host memory";
-        for (auto& x : c->ev) (void)hipEventDestroy(x);
-        (void)hipStreamDestroy(c->stream);
-        delete c;
-        return FDG_ENOMEM;
-    }
-    *out = c;
-    return FDG_OK;
+        for (auto& e : c->ev) {
+            if (hipEventCreate(&e) != hipSuccess) {
+                g_create_error = "cannot create HIP events";
+                for (auto& x : c->ev)
+                    if (x) (void)hipEventDestroy(x);
+                (void)hipStreamDestroy(c->stream);
+                delete c;
+                return FDG_EDEVICE;
+            }
+        }
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
+        if (hipHostMalloc((void**)&c->small, fdg_ctx::kSmallBytes, hipHostMallocDefault) !=
+            hipSuccess) {
+            g_create_error = "cannot allocate page-locked host memory";
+            for (auto& x : c->ev) (void)hipEventDestroy(x);
+            (void)hipStreamDestroy(c->stream);
+            delete c;
+            return FDG_ENOMEM;
+        }
+        *out = c;
+        return FDG_OK;
+    });
 }
 
 void fdg_destroy(fdg_ctx* c) {

@@ -80,3 +80,17 @@ def test_supplied_orbit_refusals():
         fa.ResidentFrame(deeper, p, fa.ROUTE_PLAIN, orbit=(re, im, bits))
     with pytest.raises(fa.FdgError, match="Z_0 = 0"):
         fa.ResidentFrame(deep, p, fa.ROUTE_PLAIN, orbit=(re[1:], im[1:], bits))
+
+
+def test_orbits_shorter_than_the_loop_reads_are_refused():
+    # Z_1 is read from the first step, and the scaled tier's rebase from a delta below
+    # 1e-270 continues to Z_2: 1 point (2 on the scaled tier) is refused before any launch,
+    # where the reference would index past its orbit (sample.rs:367)
+    for name, short in (("valley-1e-28", 1), ("i-1e-300", 2)):
+        h, _ = parse_fds(dict(kern05())[name])
+        v = fa.View(h["re"], h["im"], h["width"], h["rotation"])
+        p = fa.Params(h["nx"], h["ny"], 1, h["max_iter"], h["escape_radius"], COLS)
+        re, im, bits = fa.reference(v, p)
+        with pytest.raises(fa.FdgError, match="at least 2|too short") as e:
+            fa.ResidentFrame(v, p, fa.ROUTE_PLAIN, orbit=(re[:short], im[:short], bits))
+        assert e.value.code == -1

@@ -67,3 +67,21 @@ def test_jet_frame_refusals():
     with fa.ResidentFrame(v, fa.Params(96, 54, 2, 2000, 1e10, fa.COL_NU), fa.ROUTE_PLAIN) as f:
         with pytest.raises(fa.FdgError, match="nu, de and normal"):
             f.jet()
+
+
+def test_a_sample_mask_set_before_a_jet_frame_is_kept():
+    # fdg_frame_jet sets its own pass masks; a mask the caller had set is set again after,
+    # so the next plain run computes exactly the caller's samples
+    v = fa.View("-0.7453", "0.1127", "0.0065", 0.0)
+    p = fa.Params(96, 54, 2, 2000, 1e10, COLS)
+    rng = np.random.default_rng(5)
+    mask = (rng.random(96 * 54) < 0.3).astype(np.uint8)
+    with fa.ResidentFrame(v, p, fa.ROUTE_PLAIN) as f:
+        f.mask(mask)
+        f.jet()
+        f.run()
+        _, after, _ = f.read()
+    want = oracle.render("-0.7453", "0.1127", "0.0065", 0.0, 96, 54, 2000, 1e10, columns=COLS,
+                         ss=2, bla=False, outcomes=False, mask=mask)["columns"]
+    for k in ("class", "nu", "de", "normal"):
+        assert_same(after[k], want[k], k)
