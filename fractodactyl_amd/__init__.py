@@ -163,6 +163,10 @@ SIGNATURES = {
     "fdg_frame_open_orbit": (C.c_int, [C.c_void_p, C.POINTER(_View), C.POINTER(_Params),
                                        C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_void_p,
                                        C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "fdg_render_orbit_noalloc": (C.c_int, [C.c_void_p, C.POINTER(_View), C.POINTER(_Params),
+                                           C.c_int, C.c_double, C.c_void_p, C.c_void_p,
+                                           C.c_uint64, C.c_uint32, C.POINTER(_Outcomes),
+                                           C.POINTER(_Stats)]),
     "fdg_chunk_id": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
     "fdg_chunk_orbit_slab": (C.c_int64, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32,
                                          C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint64]),

@@ -357,6 +357,7 @@ int fdg_create(fdg_ctx** out, int device) {
         }
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
+        noalloc_attach(device);  // (the device is current)
         if (hipHostMalloc((void**)&c->small, fdg_ctx::kSmallBytes, hipHostMallocDefault) !=
             hipSuccess) {
             g_create_error = "cannot allocate page-locked host memory";

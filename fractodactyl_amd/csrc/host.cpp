@@ -802,11 +802,11 @@ bool Orbit::fixed(const std::string& re, const std::string& im, uint64_t bits, u
     return true;
 }
 
-uint64_t Orbit::nucleus_period() const {
-    if (len() < 2) return 0;
+uint64_t Orbit::nucleus_period_of(const double* re, const double* im, uint64_t len) {
+    if (len < 2) return 0;
     double record = hypot_ref(re[1], im[1]);
     double log_beta = 0.0;
-    for (uint64_t p = 2; p < len() / 2; p++) {
+    for (uint64_t p = 2; p < len / 2; p++) {
         double previous = hypot_ref(re[p - 1], im[p - 1]);
         log_beta += 1.0 + std::log2(previous);
         double radius = hypot_ref(re[p], im[p]);

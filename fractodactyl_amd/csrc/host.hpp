@@ -91,7 +91,8 @@ struct Orbit {
                       uint64_t max_iter, Orbit* out, uint64_t max_len = kMaxLen);
     uint64_t len() const { return re.size(); }
     // Reference::nucleus_period (reference.rs:102-120); 0 for None.
-    uint64_t nucleus_period() const;
+    uint64_t nucleus_period() const { return nucleus_period_of(re.data(), im.data(), len()); }
+    static uint64_t nucleus_period_of(const double* re, const double* im, uint64_t len);
 };
 
 // Upper bound m 2^e on |dc| over a view's samples (grid.rs:423-444).

@@ -44,6 +44,10 @@ hipError_t launch_encode_columns(const EncodeDev& e, int cus, hipStream_t stream
 hipError_t launch_perturb_fe32(const FrameDev* f, bool deriv, bool bla, int grid,
                                hipStream_t stream);
 
+// The no-allocation entry point's set-up (noalloc.hip): looks up the scratch array's address
+// on `device` once, with that device current; fdg_create calls it.
+void noalloc_attach(int device);
+
 constexpr uint32_t kMaxLevels = 48;
 // Outcome kind of a sample outside a sample mask (not computed; its columns stay unset).
 constexpr uint8_t kKindUnset = 0xff;

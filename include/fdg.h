@@ -365,6 +365,26 @@ int fdg_jacobian(const fdg_view *view, const fdg_params *params, const fdg_outco
 int64_t fdg_write_fds(const fdg_view *view, const fdg_params *params, const char *kernel_id,
                       const fdg_columns *columns, uint8_t *buf, uint64_t cap);
 
+/* No counterpart in the reference: an entry point for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The FP64 form of fdg_frame_open_orbit + fdg_frame_run in one synchronous
+ * call: the same view, params, route, eps and supplied orbit (re, im: len host points at
+ * `bits`, the same refusals), with the raw outcomes written by the kernel straight into
+ * the caller's DEVICE arrays (every member of out non-NULL, nx*ny elements each).  Every
+ * other buffer — the orbit's device copy, the BLA table, the counters and the kernel
+ * descriptor — sits in one static device array of FDG_NOALLOC_SCRATCH_BYTES per GPU, so
+ * the call allocates no device or page-locked memory; orbits longer than
+ * FDG_NOALLOC_MAX_POINTS (2^17 + 2: a table of 2^17 steps) are FDG_EINVAL.  Outcomes are
+ * the kernel's: an Unresolved sample's n is the iterate it stopped at (fdg_frame_read
+ * reports 0, as Outcome::Unresolved carries none).  Every call on a GPU shares that GPU's
+ * array and leaves its contents there: calls that overlap on one GPU (several contexts or
+ * threads) race, so the caller serialises them. */
+#define FDG_NOALLOC_MAX_POINTS ((1u << 17) + 2u)
+#define FDG_NOALLOC_SCRATCH_BYTES (13u << 20)
+int fdg_render_orbit_noalloc(fdg_ctx *ctx, const fdg_view *view, const fdg_params *params,
+                             int route, double bla_eps, const double *re, const double *im,
+                             uint64_t len, uint32_t bits, fdg_outcomes *out, fdg_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif
