@@ -1,12 +1,15 @@
 """Benchmark: effective pixel-iterations/s of the deep-zoom perturbation/BLA path on one GPU.
 
-Workload (default): the 1920x1080 frame of width 1e-100 centred 0.3 widths from a seeded
-Newton nucleus, max_iter 100,000, escape radius 1e10, BLA on (eps 2^-50), columns
-nu/de/normal (so dz/dc is carried), FP64 deltas.  A step is one pass of the hot path over
-the frame with its reference orbit resident in HBM: the GPU BLA table build plus the
-persistent perturbation kernel over all 2,073,600 samples.  value = plain-equivalent
-iterations (BLA-skipped steps included, the reference's `iterations.equivalent`) per
-second of wall time over the timed steps.
+Workload (default, config 3): the 1920x1080 frame of width 1e-1000 at the period-16116
+ladder nucleus of the reference's rung-7676 frame (its minibrot is ~1e-1000 wide, so the
+frame holds its interior and its exterior), max_iter 1,000,000, escape radius 1e10, BLA on
+(eps 2^-50), columns nu/de/normal (so dz/dc is carried), FP64 mantissa x 2^e deltas (the
+scaled tier).  A step is one pass of the hot path over the frame with its reference orbit
+resident in HBM: the GPU BLA table build plus the persistent perturbation kernel over all
+2,073,600 samples.  value = plain-equivalent iterations (BLA-skipped steps included, the
+reference's `iterations.equivalent`) per second of wall time over the timed steps.  The
+FE32 and mixed forms run the same frame (with their agreement), and the 1e-100 frame, the
+escape-dominated frame and the 64-keyframe film run as companions.
 
 Prints ONE JSON line.  --gpus above 1 exits: this path runs on one GPU here.
 """
@@ -20,7 +23,8 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_FP64_TFLOPS = 78.6      # MI355X vector FP64
+PEAK_FP64_TFLOPS = 78.6      # MI355X vector FP64 (and FP32 without packed instructions)
+PEAK_FP32_TFLOPS = 157.3     # MI355X vector FP32 with packed (v_pk_*) instructions
 F_PLAIN, F_BLOCK = 18, 22    # algorithmic flops per plain step / applied block
 F_PLAIN_D, F_BLOCK_D = 27, 30  # with dz/dc carried
 
@@ -29,9 +33,33 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(view, params, route_bla, target_seconds, threads, orbit=None):
-    """The oracle (C port of the reference CPU BLA renderer) on a row sample of the frame
-    (orbit: a supplied (re, im, bits) reference, as the film's frames use)."""
+def host_cpu():
+    """The host's CPU: model name, online cores and the cores this process may run on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    online = os.cpu_count() or 1
+    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else online
+    return model, online, allowed
+
+
+def cpu_threads():
+    """Worker threads for the CPU baseline: every core this process may use, capped at 16,
+    the host CPU share that comes with one GPU on the box."""
+    return max(1, min(16, host_cpu()[2]))
+
+
+def cpu_baseline(view, params, route_bla, target_seconds, threads, orbit=None, runs=3):
+    """The oracle (C port of the reference CPU BLA renderer, its dynamic row queue on
+    `threads` workers) on a row sample of the frame sized for ~target_seconds of render
+    per run: `runs` runs of the same sample, the first labelled cold, the median reported
+    (render-only; end to end adds the host reference orbit and the table build, each
+    timed apart).  orbit: a supplied (re, im, bits) reference, as the film's frames use."""
     import numpy as np
     import oracle
     nx, ny = params.nx, params.ny
@@ -47,18 +75,30 @@ def cpu_baseline(view, params, route_bla, target_seconds, threads, orbit=None):
                           orbit=orbit)
         return f["stats"], len(rows)
 
-    st, rows = run(ny // 4, ny // 8)   # probe: 4 rows
+    st, rows = run(ny // 4, ny // 8)   # sizing probe: 4 rows
     per_row = st["render_seconds"] / rows
     want_rows = max(4, min(ny, int(target_seconds / max(per_row, 1e-6))))
     stride = max(1, ny // want_rows)
-    st, rows = run(stride, stride // 2)
-    return dict(value=st["equivalent"] / st["render_seconds"], unit="pixel-iter/s",
-                cores=threads, kind="port",
+    each = []
+    for k in range(runs):
+        st, rows = run(stride, stride // 2)
+        e2e = st["render_seconds"] + st["reference_seconds"] + st["bla_seconds"]
+        each.append(dict(run=k + 1, cold=k == 0, render_seconds=st["render_seconds"],
+                         reference_seconds=st["reference_seconds"],
+                         bla_seconds=st["bla_seconds"],
+                         value=st["equivalent"] / st["render_seconds"],
+                         end_to_end_value=st["equivalent"] / e2e))
+    med = float(np.median([r["value"] for r in each]))
+    model, online, allowed = host_cpu()
+    return dict(value=med, unit="pixel-iter/s", cores=threads, kind="port",
+                end_to_end_value=float(np.median([r["end_to_end_value"] for r in each])),
+                runs=each, statistic=f"median of {runs} runs (run 1 cold)",
+                cpu_model=model, nproc=online, affinity_cores=allowed,
                 sample=f"{rows} of {ny} rows (every {stride}th) x {nx} samples of the same "
-                       f"frame, {'BLA eps 2^-50' if route_bla else 'plain'}, oracle render "
-                       f"{st['render_seconds']:.2f}s (reference orbit "
-                       f"{st['reference_seconds']:.3f}s, table {st['bla_seconds']:.3f}s "
-                       f"reported apart)")
+                       f"frame, {'BLA eps 2^-50' if route_bla else 'plain'}, per run: oracle "
+                       f"render {np.median([r['render_seconds'] for r in each]):.2f}s "
+                       f"(median; reference orbit and table build timed apart), {threads} "
+                       f"threads of {allowed} allowed / {online} online cores")
 
 
 def measure(fa, view, params, route, steps, warmup, label, form=None, keep=None):
@@ -260,7 +300,7 @@ def measure_film(fa, sc, cols, steps, warmup, threads, cpu_seconds):
                                         0.0, params.nx, params.ny, params.max_iter,
                                         params.escape_radius)
         vals = [cpu_baseline(views[k], params, True, cpu_seconds / 8, threads,
-                             orbit=(zr, zi, bits)) for k in range(0, len(views), 8)]
+                             orbit=(zr, zi, bits), runs=1) for k in range(0, len(views), 8)]
         out["cpu_baseline"] = dict(value=float(np.mean([v["value"] for v in vals])),
                                    unit="pixel-iter/s", cores=threads, kind="port",
                                    sample="mean over keyframes 0, 8, .., 56 of: " +
@@ -354,25 +394,53 @@ def copy_probe(nbytes=1 << 28, reps=3):
     print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
 
 
-def deep_view(fa, s3, cols):
-    c3 = s3["config"]
-    return (fa.View(s3["center_re"], s3["center_im"], c3["width"], 0.0),
-            fa.Params(c3["nx"], c3["ny"], 1, c3["max_iter"], c3["escape_radius"], cols))
+def scene_view(fa, sc, cols):
+    c = sc["config"]
+    return (fa.View(sc["center_re"], sc["center_im"], c["width"], 0.0),
+            fa.Params(c["nx"], c["ny"], 1, c["max_iter"], c["escape_radius"], cols))
+
+
+def workload_name(k, sc, params):
+    c = sc["config"]
+    if k == 3:
+        return (f"config3: {params.nx}x{params.ny} width {c['width']} at the period-"
+                f"{sc['nucleus']['period']} ladder nucleus (its minibrot ~1e-1000), max_iter "
+                f"{params.max_iter}, BLA eps 2^-50, scaled tier (f64 mantissa x 2^e deltas), "
+                f"dz/dc carried, nu/de/normal")
+    return (f"config{k}: {params.nx}x{params.ny} width {c['width']} at nucleus+{c['shift']}w "
+            f"(period {sc['nucleus']['period']}, minibrot ~1e{sc['nucleus']['log10_size']:.0f}), "
+            f"max_iter {params.max_iter}, BLA eps 2^-50, FP64 deltas, nu/de/normal")
+
+
+def fe32_roofline(m, forms):
+    """The FE32 form's kernel against the FP32 vector peak (same algorithmic flops)."""
+    r = m["roofline"]
+    out = {}
+    for name in ("fe32", "mixed"):
+        if name in forms:
+            ach = r["flops_per_launch"] / (forms[name]["kernel_ms"] * 1e-3) / 1e12
+            out[name] = {"bound": "valu-fp32", "achieved": ach, "peak": PEAK_FP32_TFLOPS,
+                         "unit": "TFLOP/s", "frac": ach / PEAK_FP32_TFLOPS,
+                         "frac_of_unpacked_fp32": ach / PEAK_FP64_TFLOPS,
+                         "flops_model": r["flops_model"]}
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", type=int, default=2)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=3, choices=(2, 3),
+                    help="the headline frame: 3 (1e-1000, 1M iterations, the north star's "
+                         "target) or 2 (1e-100)")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--frames", default="config,escape,deep,film",
-                    help="which frames to run: config (the JSON line); companions escape "
-                         "(reference zoom path at 2e-49), deep (1e-1000, 1M iterations), "
-                         "film (64 keyframes sharing one orbit); series (the reference's "
-                         "24 series-study frames in four arms, companion mode only)")
+    ap.add_argument("--frames", default="config,other,escape,film",
+                    help="which frames to run: config (the JSON line); companions other "
+                         "(the other of configs 2 and 3), escape (reference zoom path at "
+                         "2e-49), film (64 keyframes sharing one orbit); series (the "
+                         "reference's 24 series-study frames, companion mode only)")
     ap.add_argument("--forms", action="store_true",
                     help="with companions only: also run the FE32 and mixed forms")
     ap.add_argument("--copy-probe", action="store_true",
@@ -383,19 +451,21 @@ def main():
         return
     if a.gpus != 1:
         print(f"bench.py: this path runs on one GPU; --gpus {a.gpus} is not supported "
-              "(a frame shards by tiles of samples; run one process per GPU on a share of the tiles)", file=sys.stderr)
+              "(a frame shards by tiles of samples; run one process per GPU on a share of the "
+              "tiles)", file=sys.stderr)
         sys.exit(2)
 
     import fractodactyl_amd as fa
     from fractodactyl_amd.scenes import scene as get_scene
-    sc = get_scene(a.config)
-    cfg = sc["config"]
     cols = fa.COL_NU | fa.COL_DE | fa.COL_NORMAL
-    view = fa.View(sc["center_re"], sc["center_im"], cfg["width"], 0.0)
-    params = fa.Params(cfg["nx"], cfg["ny"], 1, cfg["max_iter"], cfg["escape_radius"], cols)
-    route = fa.ROUTE_BLA if cfg["bla"] else fa.ROUTE_PLAIN
+    sc = get_scene(a.config)
+    view, params = scene_view(fa, sc, cols)
+    route = fa.ROUTE_BLA if sc["config"]["bla"] else fa.ROUTE_PLAIN
+    other = 2 if a.config == 3 else 3
     fa.default_context()
     frames = a.frames.split(",")
+    threads = cpu_threads()
+    cpu_s = 0.0 if a.no_cpu_baseline else a.cpu_seconds
     ka = json.load(open(os.path.join(ROOT, "tests", "golden", "known_answers.json")))
     # Escape-dominated companion: the reference's own deep-zoom path (bench/path-atlas-v0.txt,
     # the period-764 minibrot's neighbourhood) at its frame-749 width, at 1080p.
@@ -403,9 +473,14 @@ def main():
     p0 = fa.Params(1920, 1080, 1, 100_000, 1e10, cols)
     if "config" not in frames:
         # profiling aid: companions alone, each printed as JSON on stderr, no bench line
-        for name, (v, p) in (("escape", (v0, p0)), ("deep", deep_view(fa, get_scene(3), cols))):
-            if name not in frames:
-                continue
+        todo = []
+        if "escape" in frames:
+            todo.append(("escape", v0, p0))
+        if "deep" in frames or "config3" in frames:
+            todo.append(("deep",) + scene_view(fa, get_scene(3), cols))
+        if "config2" in frames:
+            todo.append(("config2",) + scene_view(fa, get_scene(2), cols))
+        for name, v, p in todo:
             keep = []
             m = measure(fa, v, p, fa.ROUTE_BLA, a.steps, a.warmup, name, keep=keep)
             if a.forms:
@@ -421,29 +496,32 @@ def main():
     m = measure(fa, view, params, route, a.steps, a.warmup, f"config{a.config}", keep=keep)
     forms = measure_forms(fa, view, params, route, a.steps, a.warmup, f"config{a.config}", m,
                           keep[0])
-    threads = min(16, os.cpu_count() or 1)
-    cpu_s = 0.0 if a.no_cpu_baseline else a.cpu_seconds
-    deep = {}
-    if "deep" in frames:
-        v3, p3 = deep_view(fa, get_scene(3), cols)
-        keep3 = []
-        deep = measure(fa, v3, p3, fa.ROUTE_BLA, a.steps, a.warmup, "deep-1e-1000", keep=keep3)
-        deep["precision_forms"] = measure_forms(fa, v3, p3, fa.ROUTE_BLA, a.steps, a.warmup,
-                                                "deep-1e-1000", deep, keep3[0])
-        deep["workload"] = (f"1920x1080 width 1e-1000 at the period-"
-                            f"{get_scene(3)['nucleus']['period']} ladder nucleus, max_iter 1e6, "
-                            f"BLA eps 2^-50, scaled tier (f64 mantissa x 2^e deltas), dz/dc "
-                            f"carried")
+    comp = {}
+    if "other" in frames:
+        so = get_scene(other)
+        vo, po = scene_view(fa, so, cols)
+        keepo = []
+        mo = measure(fa, vo, po, fa.ROUTE_BLA, a.steps, a.warmup, f"config{other}", keep=keepo)
+        mo["precision_forms"] = measure_forms(fa, vo, po, fa.ROUTE_BLA, a.steps, a.warmup,
+                                              f"config{other}", mo, keepo[0])
+        mo["roofline"]["traffic"] = measured_traffic(other)
+        mo["workload"] = workload_name(other, so, po)
         if cpu_s > 0:
-            deep["cpu_baseline"] = cpu_baseline(v3, p3, True, cpu_s, threads)
-    film = measure_film(fa, get_scene(4), cols, a.steps, a.warmup, threads, cpu_s) \
-        if "film" in frames else {}
-    m0 = {}
+            mo["cpu_baseline"] = cpu_baseline(vo, po, True, cpu_s / 2, threads, runs=1)
+        comp[f"config{other}_frame"] = mo
     if "escape" in frames:
         keep0 = []
         m0 = measure(fa, v0, p0, fa.ROUTE_BLA, a.steps, a.warmup, "v0-2e-49", keep=keep0)
         m0["precision_forms"] = measure_forms(fa, v0, p0, fa.ROUTE_BLA, a.steps, a.warmup,
                                               "v0-2e-49", m0, keep0[0])
+        m0["workload"] = ("reference deep-zoom path (v0 centre, period-764 minibrot) at width "
+                          "2e-49, 1920x1080, max_iter 1e5, BLA")
+        if cpu_s > 0:
+            m0["cpu_baseline"] = cpu_baseline(v0, p0, True, cpu_s / 2, threads, runs=1)
+        comp["escape_frame"] = m0
+    if "film" in frames:
+        comp["film"] = measure_film(fa, get_scene(4), cols, a.steps, a.warmup, threads,
+                                    cpu_s / 2)
     out = {
         "metric": "effective pixel-iter/s at 1080p deep zoom (BLA-skipped steps included)",
         "value": m["value"],
@@ -456,33 +534,24 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded Newton nucleus scene, fractodactyl_amd/scenes.json)",
+        "data": "synthetic (scene fractodactyl_amd/scenes.json: the reference's ladder nucleus "
+                "for config 3, seeded Newton nuclei for configs 2 and 4)",
         "config": {
-            "workload": f"config{a.config}: {params.nx}x{params.ny} width {cfg['width']} at "
-                        f"nucleus+{cfg['shift']}w (period {sc['nucleus']['period']}, minibrot "
-                        f"~1e{sc['nucleus']['log10_size']:.0f}), max_iter {params.max_iter}, BLA "
-                        f"eps 2^-50, FP64 deltas, nu/de/normal",
+            "workload": workload_name(a.config, sc, params),
             **{k: v for k, v in m.items() if k not in ("value", "ms_per_step", "roofline")},
             "precision_forms": forms,
-            "escape_frame": {"workload": "reference deep-zoom path (v0 centre, period-764 "
-                                         "minibrot) at width 2e-49, 1920x1080, max_iter 1e5, BLA",
-                             **{k: v for k, v in m0.items()}},
-            "deep_frame": deep,
-            "film": film,
+            "precision_forms_roofline": fe32_roofline(m, forms),
+            **comp,
         },
         "roofline": dict(m["roofline"], traffic=measured_traffic(a.config)),
     }
     # PCIe- and host-inclusive: the whole fdg_render call on the headline frame
     out["config"]["end_to_end"] = end_to_end(fa, view, params, route)
     out["config"]["end_to_end_device_encode"] = end_to_end_device_encode(fa, view, params, route)
-    if not a.no_cpu_baseline:
+    if cpu_s > 0:
         t = time.perf_counter()
-        out["cpu_baseline"] = cpu_baseline(view, params, route == fa.ROUTE_BLA, a.cpu_seconds,
-                                           threads)
-        if "escape" in frames:
-            out["config"]["escape_frame"]["cpu_baseline"] = cpu_baseline(
-                v0, p0, True, a.cpu_seconds, threads)
-        log(f"cpu baselines: {time.perf_counter() - t:.1f}s")
+        out["cpu_baseline"] = cpu_baseline(view, params, route == fa.ROUTE_BLA, cpu_s, threads)
+        log(f"cpu baseline: {time.perf_counter() - t:.1f}s")
     print(json.dumps(out), flush=True)
 
 
