@@ -301,16 +301,23 @@ __device__ inline void con_multiply(double& re_, double& im_, int32_t& ex, bool&
 // CAP: the delta's form, max_e = 0, so k <= -e; the derivative's max_e (i64::MAX / 4)
 // never binds.  The two half-power factors are the reference's (a single 2^-k can
 // overflow for subnormal mantissas).
+//
+// The two scalings are ldexps: each rounds as the product with that (exact) power of two,
+// so the result equals re * 2^-(k/2) * 2^-(k - k/2) bit for bit.  Returns true when the
+// result is canonical: normalising it again would find k = 0 and change nothing (the
+// exponent came from frexp, so the larger magnitude now lies in [1, 2) away from the log2
+// tie region, or the cap left the exponent at 0, or there was nothing to scale).
 template <bool CAP>
-__device__ inline void normalise(double& re, double& im, int32_t& e) {
+__device__ inline bool normalise(double& re, double& im, int32_t& e) {
     double a = fmax(fabs(re), fabs(im));
-    if (a == 0.0 || !isfinite(a)) return;
-    int32_t k = floor_log2_fast(a);
+    if (a == 0.0 || !isfinite(a)) return true;
+    bool tie = __builtin_amdgcn_frexp_mant(a) > 1.0 - 0x1p-40;
+    int32_t k = tie ? (int32_t)floor_log2(a) : __builtin_amdgcn_frexp_exp(a) - 1;
     if (CAP && -e < k) k = -e;
-    double h = exp2i32(-(k / 2)), l = exp2i32(-(k - k / 2));
-    re = re * h * l;
-    im = im * h * l;
+    re = ldexp(ldexp(re, -(k / 2)), -(k - k / 2));
+    im = ldexp(ldexp(im, -(k / 2)), -(k - k / 2));
     e = e + k;
+    return !tie;
 }
 
 // sample.rs:1094-1114 sum2 over two terms
@@ -380,6 +387,10 @@ struct Lane {
 };
 
 constexpr uint32_t kPending = 1, kPredicted = 2, kConStarted = 4;
+// Scaled tier: the delta (kCanonX) or dz/dc (kCanonD) is as the last block's normalisation
+// left it and canonical, so the next block's first normalisation of it is the identity
+// (bits 8 and 9, above the close-return count in bits 4-7).
+constexpr uint32_t kCanonX = 256, kCanonD = 512;
 // series_carry's mark of a sample running from the series start
 constexpr uint32_t kSeriesTry = 0xffffffffu;
 
@@ -717,11 +728,13 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
             con_multiply(r.conr, r.coni, r.cone, st, a_r, a_i);
             s.flags |= kConStarted;
         }
-        normalise<true>(s.xr, s.xi, s.delta_exp);
-        if (D) {
-            normalise<false>(s.dr, s.di, s.derivative_exp);
+        // (skipped where it is the identity: the state is still the last block's
+        // canonical result)
+        if (!(s.flags & kCanonX)) {
+            normalise<true>(s.xr, s.xi, s.delta_exp);
+            refresh_dc<EXT>(s, oe);
         }
-        refresh_dc<EXT>(s, oe);
+        if (D && !(s.flags & kCanonD)) normalise<false>(s.dr, s.di, s.derivative_exp);
         if (D) {
             double inv = inverse_of<EXT>(s);
             double t = a_r * s.dr - a_i * s.di + b_r * inv;
@@ -731,11 +744,10 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         double t = a_r * s.xr - a_i * s.xi + b_r * s.dc_r - b_i * s.dc_i;
         s.xi = a_r * s.xi + a_i * s.xr + b_r * s.dc_i + b_i * s.dc_r;
         s.xr = t;
-        normalise<true>(s.xr, s.xi, s.delta_exp);
-        if (D) {
-            normalise<false>(s.dr, s.di, s.derivative_exp);
-        }
+        bool cx = normalise<true>(s.xr, s.xi, s.delta_exp);
+        bool cd = D ? normalise<false>(s.dr, s.di, s.derivative_exp) : false;
         refresh_dc<EXT>(s, oe);
+        s.flags = (s.flags & ~(kCanonX | kCanonD)) | (cx ? kCanonX : 0u) | (cd ? kCanonD : 0u);
         s.m += blen;
         s.n += blen;
         s.blocks += 1;
@@ -761,6 +773,7 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         double xr = s.xr, xi = s.xi;
         s.xr = 2.0 * (Zmr * xr - Zmi * xi) + scale * (xr * xr - xi * xi) + s.dc_r;
         s.xi = 2.0 * (Zmr * xi + Zmi * xr) + scale * (2.0 * xr * xi) + s.dc_i;
+        s.flags &= ~(kCanonX | kCanonD);
         s.m += 1;
         s.n += 1;
         s.plain += 1;
@@ -788,6 +801,7 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         if (rebase) {
             double R00 = B ? (L && h.count0 > 0 ? h.r2[tb] : 0.0) : h.R00;
             ext_rebase<D>(s, h, oe, R00, fr, fi, f2, Zn.x, Zn.y);
+            s.flags &= ~(kCanonX | kCanonD);
         } else {
             double magnitude = fmax(fabs(s.xr), fabs(s.xi));
             int32_t k = 0;
@@ -801,10 +815,11 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
                 s.xi = s.xi * factor;
                 s.delta_exp = s.delta_exp + k;
                 refresh_dc<EXT>(s, oe);
+                s.flags &= ~kCanonX;
             }
         }
         // extended tier: judged on the physical delta before rebase/rescale
-        uint32_t returns = s.flags >> 4;
+        uint32_t returns = (s.flags >> 4) & 15u;
         bool res = returns < 4 && pr * pr + pi * pi > kResolvable * kResolvable * f2;
         if (res) {
             double dist = fabs(fr - s.sr) + fabs(fi - s.si);
