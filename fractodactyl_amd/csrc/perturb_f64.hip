@@ -304,13 +304,14 @@ __device__ inline void con_multiply(double& re_, double& im_, int32_t& ex, bool&
 //
 // The two scalings are ldexps: each rounds as the product with that (exact) power of two,
 // so the result equals re * 2^-(k/2) * 2^-(k - k/2) bit for bit.  Returns true when the
-// result is canonical: normalising it again would find k = 0 and change nothing (the
-// exponent came from frexp, so the larger magnitude now lies in [1, 2) away from the log2
-// tie region, or the cap left the exponent at 0, or there was nothing to scale).
+// result is canonical: normalising it again would find k = 0 and change nothing, and the
+// lazy rescale after a step would not fire (the exponent came from frexp, so the larger
+// magnitude now lies in [1, 2) away from the log2 tie region; or the cap left the exponent
+// at 0 with the magnitude above 2; or both parts are zero).
 template <bool CAP>
 __device__ inline bool normalise(double& re, double& im, int32_t& e) {
     double a = fmax(fabs(re), fabs(im));
-    if (a == 0.0 || !isfinite(a)) return true;
+    if (a == 0.0 || !isfinite(a)) return a == 0.0;  // (a non-finite delta stays unmarked)
     bool tie = __builtin_amdgcn_frexp_mant(a) > 1.0 - 0x1p-40;
     int32_t k = tie ? (int32_t)floor_log2(a) : __builtin_amdgcn_frexp_exp(a) - 1;
     if (CAP && -e < k) k = -e;
@@ -802,7 +803,8 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
             double R00 = B ? (L && h.count0 > 0 ? h.r2[tb] : 0.0) : h.R00;
             ext_rebase<D>(s, h, oe, R00, fr, fi, f2, Zn.x, Zn.y);
             s.flags &= ~(kCanonX | kCanonD);
-        } else {
+        } else if (!(s.flags & kCanonX)) {
+            // (a block's canonical delta has its larger part in [1, 2): nothing to rescale)
             double magnitude = fmax(fabs(s.xr), fabs(s.xi));
             int32_t k = 0;
             if (magnitude > kHi && s.delta_exp < 0)
