@@ -144,3 +144,68 @@ def test_bla_chunk_with_a_forged_block_count_is_refused():
     with pytest.raises(fa.FdgError) as e:
         fa.bla_chunk_read(bytes(c))
     assert e.value.code == -1  # FDG_EINVAL
+
+
+# Chunks assembled by hand, field by field, from the layouts the reference documents:
+# the 32-byte header of crates/fd-atlas/src/chunk.rs (magic "FDCHUNK\0", major u16 = 1,
+# minor u16 = 0, kind u16, encoding u16, formula u16, rounding u8, a zero byte, precision
+# bits u32, payload length u64; payload zero-padded to 8 bytes), the orbit slab table of
+# docs/spec/ATLAS.md:127-140 (size u32, len u32, start u64, re[len], im[len]), the orbit
+# manifest of ATLAS.md:145-152 / manifest.rs:270-278 (centre strings as u64 length + bytes,
+# slab count u64, 32-byte ids) and the BLA table of fd-atlas bla.rs:59-70 (orbit id,
+# eps f64, dc_max f64, points u64, levels u64, then per level the 7 fields a.re a.im b.re
+# b.im r alpha beta, each over the level's blocks).  Every number is little-endian; the
+# f64 patterns are written out: 0.5 = 3fe0.., -2 = c000.., 0.25 = 3fd0.., 1 = 3ff0..,
+# 2^-50 = 3cd0.., 0.125 = 3fc0...  No encoder of this repository builds these bytes.
+def _hex(s):
+    return bytes.fromhex("".join(s.split()))
+
+
+HAND_SLAB = _hex("""
+    46444348554e4b00 0100 0000 0100 0100 0100 01 00 35000000 3000000000000000
+    04000000 02000000 0400000000000000
+    000000000000e03f 00000000000000c0
+    000000000000d03f 000000000000f03f
+""")  # slab of size 4, points Z_4 = 0.5 + 0.25i, Z_5 = -2 + i, 53 bits
+
+HAND_MANIFEST = _hex("""
+    46444348554e4b00 0100 0000 0900 0100 0100 01 00 e9000000 4000000000000000
+    0500000000000000 2d302e3735
+    0300000000000000 302e31
+    0100000000000000
+""") + bytes(range(32))  # centre -0.75 + 0.1i, 233 bits, one slab id 00..1f (64 bytes)
+
+HAND_BLA = _hex("""
+    46444348554e4b00 0100 0000 0200 0100 0100 01 00 35000000 b000000000000000
+""") + bytes([0x11] * 32) + _hex("""
+    000000000000d03c 000000000000e03f 0600000000000000 0100000000000000
+    0000000000000040 0000000000000000
+    0000000000000000 0000000000000000
+    000000000000f03f 0000000000000000
+    0000000000000000 0000000000000000
+    000000000000e03f 0000000000000000
+    000000000000d03f 0000000000000000
+    000000000000c03f 0000000000000000
+""")  # points 6: level 1 holds blocks (A 2, B 1, r 0.5, alpha 0.25, beta 0.125) and zero
+
+
+def test_hand_assembled_chunks_equal_the_encoders_bytes():
+    import hashlib
+    fa = _product()
+    assert len(HAND_SLAB) == 32 + 48 and len(HAND_MANIFEST) == 32 + 64
+    assert len(HAND_BLA) == 32 + 176
+    # orbit Z_0 .. Z_5 cut into slabs of 4: the second slab holds Z_4, Z_5
+    re = np.array([0.0, 0.1, 0.2, 0.3, 0.5, -2.0])
+    im = np.array([0.0, 0.1, 0.2, 0.3, 0.25, 1.0])
+    slabs = fa.orbit_slabs(re, im, 53, size=4)
+    assert len(slabs) == 2 and slabs[1] == HAND_SLAB
+    assert fa.chunk_id(HAND_SLAB) == hashlib.sha256(HAND_SLAB).digest()
+    start, size, bits, r, i = fa.orbit_slab_read(HAND_SLAB)
+    assert (start, size, bits) == (4, 4, 53)
+    assert list(r) == [0.5, -2.0] and list(i) == [0.25, 1.0]
+    assert fa.orbit_manifest("-0.75", "0.1", 233, [bytes(range(32))]) == HAND_MANIFEST
+    levels = [np.array([[2.0, 0.0, 1.0, 0.0, 0.5, 0.25, 0.125], [0.0] * 7])]
+    assert fa.bla_chunk(bytes([0x11] * 32), 2.0 ** -50, 0.5, 6, levels) == HAND_BLA
+    oid, eps, dc, pts, lv = fa.bla_chunk_read(HAND_BLA)
+    assert (oid, eps, dc, pts) == (bytes([0x11] * 32), 2.0 ** -50, 0.5, 6)
+    assert np.array_equal(lv[0], levels[0])
