@@ -132,6 +132,12 @@ def lib():
         L.fdo_jetaa_rebuild.argtypes = [C.c_void_p] * 4 + [C.c_uint64, C.c_uint64, C.c_void_p]
         L.fdo_jetaa_merge.restype = None
         L.fdo_jetaa_merge.argtypes = [C.c_void_p] * 9 + [C.c_uint64]
+        L.fdo_sample_many.restype = None
+        L.fdo_sample_many.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int,
+                                      C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_void_p,
+                                      C.c_uint64, C.c_int, C.c_int64, C.c_int, C.c_uint64,
+                                      C.c_double, C.c_uint64, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p]
         L.fdo_sample.restype = C.c_int
         L.fdo_sample.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int, C.c_int,
                                  C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
@@ -219,6 +225,14 @@ def bla_build(zr, zi, eps, dc_max, dc_exp=0):
     return levels
 
 
+class PackedLevels:
+    """bla_build()'s levels concatenated once, for many sample() calls over one table."""
+
+    def __init__(self, levels):
+        self.blocks = np.ascontiguousarray(np.concatenate(levels, axis=0)) if levels else None
+        self.count = len(levels)
+
+
 def sample(zr, zi, ar, ai, max_iter, r2, c=None, extended=False, offset_exp=0,
            derivative=True, bla_levels=None, nucleus=0):
     """sample.rs iterate for one sample; returns a dict with the outcome."""
@@ -226,7 +240,9 @@ def sample(zr, zi, ar, ai, max_iter, r2, c=None, extended=False, offset_exp=0,
     zi = np.ascontiguousarray(zi, np.float64)
     blocks = None
     nl = 0
-    if bla_levels:
+    if isinstance(bla_levels, PackedLevels):
+        blocks, nl = bla_levels.blocks, bla_levels.count
+    elif bla_levels:
         blocks = np.ascontiguousarray(np.concatenate(bla_levels, axis=0))
         nl = len(bla_levels)
     n = C.c_uint64()
@@ -241,6 +257,28 @@ def sample(zr, zi, ar, ai, max_iter, r2, c=None, extended=False, offset_exp=0,
                             _p(z), _p(d), C.byref(dexp), C.byref(nb), C.byref(ns))
     return dict(kind=kind, n=n.value, zr=z[0], zi=z[1], dr=d[0], di=d[1], dexp=dexp.value,
                 blocks=nb.value, skipped=ns.value)
+
+
+def sample_many(zr, zi, ar, ai, max_iter, r2, c=None, extended=False, offset_exp=0,
+                derivative=True, bla_levels=None, nucleus=0):
+    """sample() at many offsets (arrays ar, ai) over one table built once; returns a dict
+    of arrays kind, n, zr, zi, dr, di, dexp."""
+    zr = np.ascontiguousarray(zr, np.float64)
+    zi = np.ascontiguousarray(zi, np.float64)
+    ar = np.ascontiguousarray(ar, np.float64)
+    ai = np.ascontiguousarray(ai, np.float64)
+    pk = bla_levels if isinstance(bla_levels, PackedLevels) else PackedLevels(bla_levels or [])
+    m = ar.size
+    kind = np.zeros(m, np.uint8)
+    n = np.zeros(m, np.uint64)
+    z = np.zeros((m, 2))
+    d = np.zeros((m, 2))
+    dexp = np.zeros(m, np.int64)
+    lib().fdo_sample_many(_p(zr), _p(zi), len(zr), _p(pk.blocks), pk.count,
+                          c[0] if c else 0.0, c[1] if c else 0.0, c is not None, _p(ar), _p(ai),
+                          m, int(extended), offset_exp, int(derivative), max_iter, r2, nucleus,
+                          _p(kind), _p(n), _p(z), _p(d), _p(dexp))
+    return dict(kind=kind, n=n, zr=z[:, 0], zi=z[:, 1], dr=d[:, 0], di=d[:, 1], dexp=dexp)
 
 
 def series_build(zr, zi, rho, pixel, max_iter, config=None, stalled_steps=32):

@@ -1261,6 +1261,47 @@ int fdo_sample(const double *re, const double *im, uint64_t len, const double *b
     return o.kind;
 }
 
+/* fdo_sample over many offsets (ar[i], ai[i]) against one table, built once: outputs are
+ * arrays of count entries (kind as the return value of fdo_sample). */
+void fdo_sample_many(const double *re, const double *im, uint64_t len, const double *bla_blocks,
+                     int bla_levels, double c_re, double c_im, int have_c, const double *ar,
+                     const double *ai, uint64_t count, int extended, int64_t offset_exp,
+                     int derivative, uint64_t max_iter, double r2, uint64_t nucleus,
+                     uint8_t *kind_out, uint64_t *n_out, double *z_out, double *d_out,
+                     int64_t *dexp_out) {
+    bla_t t;
+    memset(&t, 0, sizeof(t));
+    int L = bla_blocks != NULL && bla_levels > 0;
+    if (L) {
+        uint64_t steps = len - 2, off = 0;
+        t.nlevels = bla_levels;
+        t.levels = (block_t **)malloc(bla_levels * sizeof(block_t *));
+        t.r2 = (double **)malloc(bla_levels * sizeof(double *));
+        t.count = (uint64_t *)malloc(bla_levels * sizeof(uint64_t));
+        for (int i = 0; i < bla_levels; i++) {
+            uint64_t c = steps >> (i + 1);
+            t.count[i] = c;
+            t.levels[i] = (block_t *)malloc((c ? c : 1) * sizeof(block_t));
+            t.r2[i] = (double *)malloc((c ? c : 1) * sizeof(double));
+            memcpy(t.levels[i], bla_blocks + 7 * off, c * sizeof(block_t));
+            for (uint64_t k = 0; k < c; k++) t.r2[i][k] = t.levels[i][k].r * t.levels[i][k].r;
+            off += c;
+        }
+    }
+    for (uint64_t i = 0; i < count; i++) {
+        skip_t sk = {0, 0};
+        outcome_t o = pick_iter(extended, derivative, L)(re, im, len, L ? &t : NULL, &sk, have_c,
+                                                        c_re, c_im, ar[i], ai[i], offset_exp,
+                                                        max_iter, r2, nucleus, NULL);
+        kind_out[i] = (uint8_t)o.kind;
+        n_out[i] = o.n;
+        z_out[2 * i] = o.zr, z_out[2 * i + 1] = o.zi;
+        d_out[2 * i] = o.dr, d_out[2 * i + 1] = o.di;
+        dexp_out[i] = o.dexp;
+    }
+    if (L) bla_free(&t);
+}
+
 /* ---------------------------------------------------------------- series.rs */
 
 /* series.rs:99-234 Series::build_stalled: the growing-order parameter series

@@ -113,6 +113,12 @@ int fdo_sample(const double *re, const double *im, uint64_t len, const double *b
                int extended, int64_t offset_exp, int derivative, uint64_t max_iter, double r2,
                uint64_t nucleus, uint64_t *n_out, double *z_out, double *d_out,
                int64_t *dexp_out, uint64_t *blocks_out, uint64_t *skipped_out);
+void fdo_sample_many(const double *re, const double *im, uint64_t len, const double *bla_blocks,
+                     int bla_levels, double c_re, double c_im, int have_c, const double *ar,
+                     const double *ai, uint64_t count, int extended, int64_t offset_exp,
+                     int derivative, uint64_t max_iter, double r2, uint64_t nucleus,
+                     uint8_t *kind_out, uint64_t *n_out, double *z_out, double *d_out,
+                     int64_t *dexp_out);
 
 /* fd_samples::write of the given columns.  Returns bytes written, or needed size when
  * buf is too small (call with cap 0 to size). */

@@ -157,3 +157,118 @@ def test_host_jacobian_formula():
         want = jac_from_d(pl, raw["dr"][i], raw["di"][i], 3330)
         got = j[i] * 2.0 ** int(e[i])
         assert np.allclose(got, want, rtol=1e-15, atol=0)
+
+
+# ---------------------------------------------------------------- the 1e-1000 frame at 1080p
+#
+# 256 seeded escaped samples of the 1920x1080 frame of width 1e-1000 (max_iter 1e6, BLA,
+# scaled tier, the bench's headline frame).  The north star asks for central differences at
+# a 1e-3-pixel step; on this frame z_n(c) (n ~ 1e5 .. 1e6 near a 1e-1000 minibrot) bends on
+# far smaller scales: at a 1e-3-pixel step only ~8% of these samples keep their escape count
+# at +-h and the difference quotients are O(1) off.  Each sample therefore takes the largest
+# step of 1e-3, 1e-4, .., 1e-8 pixel whose Richardson estimate agrees with the next smaller
+# step's to FD_TOL (a converged difference, found without looking at the analytic value);
+# below ~1e-6 pixel the differences of z (|z| ~ 1e5 at escape) lose digits to rounding, so
+# about two thirds of the samples converge to 1e-4.
+
+C3_STEPS = (1e-3, 1e-4, 1e-5, 1e-6, 1e-7, 1e-8)
+FD_TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def c3():
+    from fractodactyl_amd.scenes import scene
+    sc = scene(3)
+    c = sc["config"]
+    nx, ny, it = c["nx"], c["ny"], c["max_iter"]
+    pl = oracle.plane(sc["center_re"], sc["center_im"], c["width"], 0.0, nx, ny)
+    zr, zi, _ = oracle.reference(sc["center_re"], sc["center_im"], c["width"], 0.0, nx, ny, it,
+                                 1e10)
+    levels = oracle.bla_build(zr, zi, oracle.EPS_EXACT, pl["h_m"] * math.hypot(nx / 2, ny / 2),
+                              pl["h_e"])
+    pk = oracle.PackedLevels(levels)
+    nuc = oracle.nucleus_period(zr, zi)
+
+    def run(ux, uy):
+        return oracle.sample_many(zr, zi, np.asarray(ux) * pl["h_m"], np.asarray(uy) * pl["h_m"],
+                                  it, 1e20, extended=True, offset_exp=pl["h_e"], bla_levels=pk,
+                                  nucleus=nuc)
+
+    rng = np.random.default_rng(20261204)
+    ii, jj = rng.integers(0, nx, 20000), rng.integers(0, ny, 20000)
+    u = np.array([unit(pl, int(a), int(b)) for a, b in zip(ii, jj)])
+    r = run(u[:, 0], u[:, 1])
+    pick = np.nonzero(r["kind"] == oracle.ESCAPED)[0][:256]
+    assert pick.size == 256
+    ux, uy, n0 = u[pick, 0], u[pick, 1], r["n"][pick]
+    c_, s_ = pl["cos_r"], pl["sin_r"]
+    # d(ux, uy)/dx = (c, s) and /dy = (s, -c): the Jacobian's columns x, y; rows Re z, Im z
+    est = {}
+    for h in C3_STEPS:
+        cols = []
+        ok = np.ones(pick.size, bool)
+        for dx, dy in ((c_, s_), (s_, -c_)):
+            q = []
+            for hh in (h, h / 2):
+                p, m = run(ux + hh * dx, uy + hh * dy), run(ux - hh * dx, uy - hh * dy)
+                ok &= (p["kind"] == 0) & (m["kind"] == 0) & (p["n"] == n0) & (m["n"] == n0)
+                q.append(((p["zr"] - m["zr"]) + 1j * (p["zi"] - m["zi"])) / (2 * hh))
+            cols.append((4 * q[1] - q[0]) / 3)
+        jac = np.stack([np.stack([cols[0].real, cols[1].real], -1),
+                        np.stack([cols[0].imag, cols[1].imag], -1)], -2)
+        est[h] = (ok, jac)
+    fd = np.full((pick.size, 2, 2), np.nan)
+    step = np.full(pick.size, np.nan)
+    for a, b in zip(C3_STEPS, C3_STEPS[1:]):
+        oka, ja = est[a]
+        okb, jb = est[b]
+        scale = np.abs(ja).reshape(-1, 4).max(1)
+        conv = oka & okb & (np.abs(ja - jb).reshape(-1, 4).max(1) <= FD_TOL * scale)
+        new = conv & np.isnan(step)
+        fd[new], step[new] = ja[new], a
+    ii, jj = ii[pick], jj[pick]
+    return dict(pl=pl, nx=nx, ny=ny, view=(sc["center_re"], sc["center_im"], c["width"]),
+                it=it, idx=jj * nx + ii, n=n0, fd=fd, step=step, oracle=r, pick=pick)
+
+
+def _jac_err(jac, fd):
+    return np.abs(jac - fd).reshape(-1, 4).max(1) / np.abs(fd).reshape(-1, 4).max(1)
+
+
+def test_config3_oracle_derivative_matches_finite_differences(c3):
+    conv = ~np.isnan(c3["step"])
+    print(f"config 3: {conv.sum()} of 256 samples converge; steps "
+          f"{dict(zip(*np.unique(c3['step'][conv], return_counts=True)))}", flush=True)
+    assert conv.sum() >= 150
+    r = c3["oracle"]
+    p = c3["pick"]
+    jd = np.array([jac_from_d(c3["pl"], r["dr"][k], r["di"][k], int(r["dexp"][k])) for k in p])
+    err = _jac_err(jd[conv], c3["fd"][conv])
+    print(f"relative Jacobian error median {np.median(err):.2e}, max {err.max():.2e}")
+    assert err.max() < 1e-3, np.sort(err)[-5:]
+
+
+@pytest.mark.gpu
+def test_config3_gpu_jacobian_fp64_and_fe32(c3):
+    re_, im_, w = c3["view"]
+    view = fa.View(re_, im_, w, 0.0)
+    params = fa.Params(c3["nx"], c3["ny"], 1, c3["it"], 1e10, fa.COL_NU | fa.COL_DE | fa.COL_NORMAL)
+    conv = ~np.isnan(c3["step"])
+    out = {}
+    for form, name in ((fa.FORM_F64, "fp64"), (fa.FORM_FE32, "fe32")):
+        with fa.ResidentFrame(view, params, fa.ROUTE_BLA, form) as f:
+            f.run()
+            h, s, _ = f.read(raw=True)
+        j, e = fa.jacobian(h, s)
+        idx = c3["idx"]
+        same = (s["raw"]["kind"][idx] == fa.ESCAPED) & (s["raw"]["n"][idx] == c3["n"]) & conv
+        jg = j[idx] * (2.0 ** e[idx].astype(np.float64))[:, None]
+        err = _jac_err(jg.reshape(-1, 2, 2)[same], c3["fd"][same])
+        out[name] = (int(same.sum()), err)
+        print(f"{name}: {same.sum()} samples with the oracle's escape count and a converged "
+              f"difference; relative Jacobian error median {np.median(err):.2e}, max "
+              f"{err.max():.2e}", flush=True)
+    # FP64: the oracle's own dz/dc, bit for bit, so the difference quotient bound holds
+    assert out["fp64"][0] == conv.sum() and out["fp64"][1].max() < 1e-3
+    # FE32: FP32 mantissas; where it keeps the escape count its derivative is within 1e-3
+    assert out["fe32"][0] >= 0.3 * conv.sum() and np.median(out["fe32"][1]) < 1e-3
