@@ -138,6 +138,16 @@ def lib():
                                       C.c_uint64, C.c_int, C.c_int64, C.c_int, C.c_uint64,
                                       C.c_double, C.c_uint64, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_void_p]
+        L.fdo_error_radius.restype = None
+        L.fdo_error_radius.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int64, C.c_double,
+                                       C.c_double, C.c_void_p]
+        L.fdo_guarded_build.restype = C.c_int
+        L.fdo_guarded_build.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+                                        C.c_double, C.c_double, C.c_void_p, C.c_uint64]
+        L.fdo_external_escape_ok.restype = C.c_int
+        L.fdo_external_escape_ok.argtypes = [C.c_uint64, C.c_double, C.c_double, C.c_double,
+                                             C.c_double, C.c_int64, C.c_double, C.c_double,
+                                             C.c_double, C.c_int64]
         L.fdo_sample.restype = C.c_int
         L.fdo_sample.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int, C.c_int,
                                  C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
@@ -257,6 +267,42 @@ def sample(zr, zi, ar, ai, max_iter, r2, c=None, extended=False, offset_exp=0,
                             _p(z), _p(d), C.byref(dexp), C.byref(nb), C.byref(ns))
     return dict(kind=kind, n=n.value, zr=z[0], zi=z[1], dr=d[0], di=d[1], dexp=dexp.value,
                 blocks=nb.value, skipped=ns.value)
+
+
+def error_radius(zr, zi, bits, c_re, c_im):
+    """reference.rs Reference::error_radius; bits None for an f64 orbit."""
+    zr = np.ascontiguousarray(zr, np.float64)
+    zi = np.ascontiguousarray(zi, np.float64)
+    q = np.zeros(zr.size)
+    lib().fdo_error_radius(_p(zr), _p(zi), zr.size, -1 if bits is None else int(bits), c_re,
+                           c_im, _p(q))
+    return q
+
+
+def guarded_build(zr, zi, q, eps, dc_max):
+    """bla/guarded.rs GuardedBla::build: a list of (count, 9) arrays per level (a.re a.im
+    b.re b.im r alpha beta dalpha dbeta); raises ValueError where the reference refuses."""
+    zr = np.ascontiguousarray(zr, np.float64)
+    zi = np.ascontiguousarray(zi, np.float64)
+    q = np.ascontiguousarray(q, np.float64)
+    steps = max(zr.size - 2, 0)
+    cap = max(steps, 1)
+    out = np.zeros((cap, 9))
+    nl = lib().fdo_guarded_build(_p(zr), _p(zi), _p(q), zr.size, eps, dc_max, _p(out), cap)
+    if nl < 0:
+        raise ValueError("guarded BLA refused")
+    levels, off = [], 0
+    for j in range(1, nl + 1):
+        c = steps >> j
+        levels.append(out[off:off + c].copy())
+        off += c
+    return levels
+
+
+def external_escape_ok(n, z, d, dexp, ez, ed, px_m, px_e):
+    """store.rs external_escape_ok for an escaped state (n, z, dz/dc = d 2^dexp)."""
+    return bool(lib().fdo_external_escape_ok(n, z[0], z[1], d[0], d[1], dexp, ez, ed, px_m,
+                                             px_e))
 
 
 def sample_many(zr, zi, ar, ai, max_iter, r2, c=None, extended=False, offset_exp=0,

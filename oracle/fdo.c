@@ -17,6 +17,7 @@
  */
 #include "fdo.h"
 
+#include <float.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdarg.h>
@@ -1300,6 +1301,174 @@ void fdo_sample_many(const double *re, const double *im, uint64_t len, const dou
         dexp_out[i] = o.dexp;
     }
     if (L) bla_free(&t);
+}
+
+/* ------------------------------------------------- certified error control (GPU route) */
+
+/* reference.rs:122-148 Reference::error_radius: q[m] >= |A_m - Z_m| for the stored orbit Z
+ * and the exact orbit A of the exact view centre; bits < 0: an f64 orbit from (c_re, c_im),
+ * the correctly rounded centre, else the fixed-point fraction bits it was computed with. */
+void fdo_error_radius(const double *re, const double *im, uint64_t len, int64_t bits,
+                      double c_re, double c_im, double *q) {
+    const double U = DBL_EPSILON / 2.0;
+    double c1 = fabs(c_re) + fabs(c_im);
+    double fixed = 0.0;
+    if (bits >= 0) {
+        fixed = exp2i(4 - bits);
+        if (fixed < DBL_MIN) fixed = DBL_MIN; /* f64::max with MIN_POSITIVE */
+    }
+    double a = 0.0;
+    for (uint64_t m = 0; m < len; m++) {
+        double s = sqrt(re[m] * re[m] + im[m] * im[m]) * (1.0 + 4.0 * U);
+        double round, defect;
+        if (bits >= 0) {
+            round = 2.0 * U * s + DBL_MIN;
+            defect = fixed;
+        } else {
+            round = 0.0;
+            defect = 8.0 * U * (s * s + c1) + 2.0 * U * c1 + DBL_MIN;
+        }
+        q[m] = a + round;
+        a = (a * (2.0 * (s + round) + a) + defect) * (1.0 + 8.0 * U);
+    }
+}
+
+/* bla/guarded.rs:9-18 norm, up, remainder */
+static double g_norm(double re, double im) { return fabs(re) + fabs(im); }
+static double g_up(double x) { return x * (1.0 + 64.0 * DBL_EPSILON) + DBL_MIN; }
+
+typedef struct {
+    block_t b;
+    double dalpha, dbeta;
+} gblock_t;
+
+/* guarded.rs:34-50 GuardedBlock::single */
+static gblock_t gblock_single(double zr, double zi, double q, double eps) {
+    gblock_t g;
+    memset(&g, 0, sizeof(g));
+    block_t b = block_single(zr, zi, eps);
+    if (b.r == 0.0 || !isfinite(q) || q < 0.0 || g_norm(zr, zi) + q > 2.0) return g;
+    double da = g_up(2.0 * b.r + 2.0 * q);
+    b.alpha = g_up(b.r + 2.0 * q);
+    g.b = b;
+    g.dalpha = da;
+    g.dbeta = 0.0;
+    return g;
+}
+
+/* guarded.rs:51-97 GuardedBlock::merge (x then y, |dc| <= c) */
+static gblock_t gblock_merge(const gblock_t *dx, const gblock_t *dy, double c) {
+    gblock_t zero;
+    memset(&zero, 0, sizeof(zero));
+    const block_t *x = &dx->b, *y = &dy->b;
+    if (x->r == 0.0 || y->r == 0.0) return zero;
+    double ax = g_norm(x->ar, x->ai), ay = g_norm(y->ar, y->ai), bx = g_norm(x->br, x->bi);
+    double r = fmin(x->r, (y->r - g_up((bx + x->beta) * c)) / g_up(ax + x->alpha)) *
+               (1.0 - 64.0 * DBL_EPSILON);
+    gblock_t z;
+    /* affine(x.a, y.a, 0) and affine(x.b, y.a, y.b), bla.rs:124-129 */
+    z.b.ar = y->ar * x->ar - y->ai * x->ai + 0.0 + 0.0;
+    z.b.ai = y->ar * x->ai + y->ai * x->ar + 0.0 + 0.0;
+    z.b.br = y->ar * x->br - y->ai * x->bi + y->br + 0.0;
+    z.b.bi = y->ar * x->bi + y->ai * x->br + y->bi + 0.0;
+    double ra = g_up(32.0 * DBL_EPSILON * ay * ax);
+    double rb = g_up(32.0 * DBL_EPSILON * (ay * bx + g_norm(y->br, y->bi)));
+    /* remainder(ay, ax, bx, xa, xb, ya, yb) = (up(ay xa + ya (ax + xa)),
+     *                                          up(ay xb + ya (bx + xb) + yb)) */
+    double alpha = g_up(ay * x->alpha + y->alpha * (ax + x->alpha));
+    double beta = g_up(ay * x->beta + y->alpha * (bx + x->beta) + y->beta);
+    double da = g_up(ay * dx->dalpha + dy->dalpha * (ax + dx->dalpha));
+    double db = g_up(ay * dx->dbeta + dy->dalpha * (bx + dx->dbeta) + dy->dbeta);
+    z.b.r = r;
+    z.b.alpha = g_up(alpha + ra);
+    z.b.beta = g_up(beta + rb);
+    z.dalpha = g_up(da + ra);
+    z.dbeta = g_up(db + rb);
+    if (r > 0.0 && isfinite(z.b.ar) && isfinite(z.b.ai) && isfinite(z.b.br) && isfinite(z.b.bi) &&
+        isfinite(z.b.r) && isfinite(z.b.alpha) && isfinite(z.b.beta) && isfinite(z.dalpha) &&
+        isfinite(z.dbeta))
+        return z;
+    return zero;
+}
+
+/* guarded.rs:103-134 GuardedBla::build: levels 1.. of guarded blocks, 9 doubles each
+ * (a.re a.im b.re b.im r alpha beta dalpha dbeta), level-major into out (cap blocks);
+ * returns the level count, or -1 for a refused request (bla.rs check, non-finite or
+ * negative radii). */
+int fdo_guarded_build(const double *re, const double *im, const double *q, uint64_t len,
+                      double eps, double dc_max, double *out, uint64_t cap) {
+    if (!(eps > 0.0 && eps <= 1.0 / 16777216.0 && dc_max > 0.0 && isfinite(dc_max))) return -1;
+    for (uint64_t k = 0; k < len; k++)
+        if (!isfinite(q[k]) || q[k] < 0.0) return -1;
+    uint64_t n = len >= 2 ? len - 2 : 0;
+    gblock_t *next = (gblock_t *)malloc((n ? n : 1) * sizeof(gblock_t));
+    for (uint64_t k = 1; k <= n; k++) next[k - 1] = gblock_single(re[k], im[k], q[k], eps);
+    int levels = 0;
+    uint64_t w = 0;
+    for (;;) {
+        uint64_t m = n / 2;
+        int any = 0;
+        for (uint64_t i = 0; i < m; i++) {
+            next[i] = gblock_merge(&next[2 * i], &next[2 * i + 1], dc_max);
+            any |= next[i].b.r > 0.0;
+        }
+        n = m;
+        if (!any) break;
+        for (uint64_t i = 0; i < n; i++, w++) {
+            if (w >= cap) continue;
+            double *o = out + 9 * w;
+            o[0] = next[i].b.ar, o[1] = next[i].b.ai, o[2] = next[i].b.br, o[3] = next[i].b.bi;
+            o[4] = next[i].b.r, o[5] = next[i].b.alpha, o[6] = next[i].b.beta;
+            o[7] = next[i].dalpha, o[8] = next[i].dbeta;
+        }
+        levels++;
+    }
+    free(next);
+    return levels;
+}
+
+static float de_px(double az, double log2z, double dr, double di, int64_t dexp, double px_m,
+                   int64_t px_e);
+
+/* store.rs:119-139 bounds: (lo, bound) or 0 when the radii do not prove escape. */
+static int escape_bounds(uint64_t n, double az, double ez, double ad, double ed, double *lo_out,
+                         double *bound_out) {
+    const double U = DBL_EPSILON / 2.0;
+    double lo = az - ez;
+    if (!(lo > 2.0 && ed < 0.5 * ad)) return 0;
+    double lnz = log(az), llo = log(lo), lhi = log(az + ez);
+    double nu = log1p(-log1p(-ez / az) / llo) / 0.693147180559945309417232121458176568 +
+                8.0 * U * ((double)n + 8.0);
+    double g = lhi * exp2(-(double)n);
+    double de_lo = (lo * llo) / (az * lnz) * ad / (ad + ed) * exp(-g) * (1.0 - 4.0 / (lo * lo)) *
+                   (1.0 - 1e-6);
+    double de_hi = ((az + ez) * lhi) / (az * lnz) * ad / (ad - ed) * (1.0 + 1e-6);
+    *lo_out = de_lo;
+    *bound_out = nu * de_hi / de_lo * (1.0 + 1e-6);
+    return 1;
+}
+
+/* store.rs:155-198 external_escape_ok: admit an externally computed escaped state with
+ * radii ez (z units) and ed (derivative mantissa units) against output pixel px_m 2^px_e. */
+int fdo_external_escape_ok(uint64_t n, double zr, double zi, double dr, double di, int64_t dexp,
+                           double ez, double ed, double px_m, int64_t px_e) {
+    const double LN2 = 0.693147180559945309417232121458176568;
+    const double TAU = 6.283185307179586476925286766559005768;
+    double az = hypot(zr, zi), ad = hypot(dr, di);
+    if (!(ez >= 0.0 && ed >= 0.0 && isfinite(ez) && isfinite(ed) && isfinite(az) &&
+          isfinite(ad)))
+        return 0;
+    double lo, bound;
+    if (!escape_bounds(n, az, ez, ad, ed, &lo, &bound)) return 0;
+    double de = (double)de_px(az, log2(az), dr, di, dexp, px_m, px_e);
+    double displacement = bound * de * lo * LN2 / 2.0;
+    double base = az * log(az);
+    double de_hi = (az + ez) * log(az + ez) / base * ad / (ad - ed);
+    double de_lo = (az - ez) * log(az - ez) / base * ad / (ad + ed);
+    double angle = asin(ez / az) + asin(ed / ad);
+    return isfinite(displacement) && displacement <= 5e-4 && isfinite(de) &&
+           de_hi - 1.0 + (double)FLT_EPSILON <= 1e-3 && 1.0 - de_lo + (double)FLT_EPSILON <= 1e-3 &&
+           angle + TAU / 65536.0 <= 0.1 * (3.141592653589793238462643383279502884 / 180.0);
 }
 
 /* ---------------------------------------------------------------- series.rs */

@@ -178,4 +178,13 @@ int fdo_fixed_parse(const char *s, uint64_t limbs, uint64_t *out);
 #ifdef __cplusplus
 }
 #endif
+/* certified error control of the GPU route: reference.rs error_radius, bla/guarded.rs
+ * GuardedBla::build, store.rs external_escape_ok */
+void fdo_error_radius(const double *re, const double *im, uint64_t len, int64_t bits,
+                      double c_re, double c_im, double *q);
+int fdo_guarded_build(const double *re, const double *im, const double *q, uint64_t len,
+                      double eps, double dc_max, double *out, uint64_t cap);
+int fdo_external_escape_ok(uint64_t n, double zr, double zi, double dr, double di, int64_t dexp,
+                           double ez, double ed, double px_m, int64_t px_e);
+
 #endif
