@@ -84,6 +84,11 @@ class _Outcomes(C.Structure):
                 ("dexp", C.c_void_p), ("ops", C.c_void_p)]
 
 
+class _EscapedState(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("zr", C.c_double), ("zi", C.c_double), ("dr", C.c_double),
+                ("di", C.c_double), ("dexp", C.c_int64)]
+
+
 class _Stats(C.Structure):
     _fields_ = [("reference_len", C.c_uint64), ("reference_bits", C.c_uint32),
                 ("bla_levels", C.c_uint32), ("nucleus_period", C.c_uint64),
@@ -167,6 +172,14 @@ SIGNATURES = {
                                            C.c_int, C.c_double, C.c_void_p, C.c_void_p,
                                            C.c_uint64, C.c_uint32, C.POINTER(_Outcomes),
                                            C.POINTER(_Stats)]),
+    "fdg_error_radius": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int64, C.c_double,
+                                   C.c_double, C.c_void_p]),
+    "fdg_guarded_bla": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_double,
+                                  C.c_double, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint32)]),
+    "fdg_external_escape_ok": (C.c_int, [C.POINTER(_EscapedState), C.c_double, C.c_double,
+                                         C.c_double, C.c_int64]),
+    "fdg_overlay_escaped": (C.c_int, [C.POINTER(_View), C.POINTER(_Params), C.POINTER(_Columns),
+                                      C.c_void_p, C.c_void_p]),
     "fdg_chunk_id": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
     "fdg_chunk_orbit_slab": (C.c_int64, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32,
                                          C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint64]),
@@ -842,3 +855,61 @@ def bla_chunk_read(data):
         levels.append(blocks[off:off + c].copy())
         off += c
     return oid.tobytes(), eps.value, dc.value, pts.value, levels
+
+
+def error_radius(re, im, bits, c_re, c_im):
+    """Reference::error_radius (fdg_error_radius); bits None for an f64 orbit."""
+    re = np.ascontiguousarray(re, np.float64)
+    im = np.ascontiguousarray(im, np.float64)
+    q = np.zeros(re.size)
+    rc = lib().fdg_error_radius(_p(re), _p(im), re.size, -1 if bits is None else int(bits),
+                                c_re, c_im, _p(q))
+    if rc != 0:
+        raise FdgError(rc, "error_radius")
+    return q
+
+
+def guarded_bla(re, im, q, eps, dc_max):
+    """GuardedBla::build (fdg_guarded_bla): a list of (count, 9) arrays, levels 1.."""
+    re = np.ascontiguousarray(re, np.float64)
+    im = np.ascontiguousarray(im, np.float64)
+    q = np.ascontiguousarray(q, np.float64)
+    steps = max(re.size - 2, 0)
+    cap = max(steps, 1)
+    out = np.zeros((cap, 9))
+    nl = C.c_uint32()
+    rc = lib().fdg_guarded_bla(_p(re), _p(im), _p(q), re.size, eps, dc_max, _p(out), cap,
+                               C.byref(nl))
+    if rc != 0:
+        raise FdgError(rc, lib().fdg_last_error(None).decode())
+    levels, off = [], 0
+    for j in range(1, nl.value + 1):
+        c = steps >> j
+        levels.append(out[off:off + c].copy())
+        off += c
+    return levels
+
+
+def external_escape_ok(n, z, d, dexp, ez, ed, px_m, px_e):
+    """external_escape_ok (fdg_external_escape_ok) for an escaped state."""
+    st = _EscapedState(n, z[0], z[1], d[0], d[1], dexp)
+    rc = lib().fdg_external_escape_ok(C.byref(st), ez, ed, px_m, px_e)
+    if rc < 0:
+        raise FdgError(rc, "external_escape_ok")
+    return rc == 1
+
+
+def overlay_escaped(view, params, cols, states):
+    """overlay_escaped (fdg_overlay_escaped): states is a dict of per-sample arrays n, zr,
+    zi, dr, di, dexp and a boolean `present`; writes the present ones into cols."""
+    n = params.nx * params.ny
+    arr = np.zeros(n, dtype=[("n", "<u8"), ("zr", "<f8"), ("zi", "<f8"), ("dr", "<f8"),
+                             ("di", "<f8"), ("dexp", "<i8")])
+    for k in ("n", "zr", "zi", "dr", "di", "dexp"):
+        arr[k] = states[k]
+    present = np.ascontiguousarray(states["present"], np.uint8)
+    v, p = _view(view), _params(params)
+    rc = lib().fdg_overlay_escaped(C.byref(v), C.byref(p), C.byref(_cols_struct(cols)),
+                                   arr.ctypes.data, _p(present))
+    if rc != 0:
+        raise FdgError(rc, lib().fdg_last_error(None).decode())

@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libfdg.so")
-SOURCES = ["host.cpp", "jetaa.cpp", "atlas.cpp", "api.cpp", "film.cpp", "perturb_f64.hip", "perturb_fe32.hip", "encode.hip", "noalloc.hip"]
+SOURCES = ["host.cpp", "jetaa.cpp", "atlas.cpp", "certify.cpp", "api.cpp", "film.cpp", "perturb_f64.hip", "perturb_fe32.hip", "encode.hip", "noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-I" + os.path.join(os.path.dirname(HERE), "include"), "-O3", "-std=c++17", "-fPIC", "-shared",
          "-ffp-contract=off", "-fno-fast-math",

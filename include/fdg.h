@@ -365,6 +365,42 @@ int fdg_jacobian(const fdg_view *view, const fdg_params *params, const fdg_outco
 int64_t fdg_write_fds(const fdg_view *view, const fdg_params *params, const char *kernel_id,
                       const fdg_columns *columns, uint8_t *buf, uint64_t cap);
 
+/* Certified error control of the reference's GPU route (host only).
+ * fdg_error_radius = Reference::error_radius (crates/fd-kernel/src/reference.rs:122-148):
+ *   q[m] >= |A_m - Z_m| for the stored orbit Z (len points) and the exact orbit A of the
+ *   exact centre; bits < 0 for an f64 orbit from (c_re, c_im), the correctly rounded
+ *   centre, else the fixed-point fraction bits it was computed with.
+ * fdg_guarded_bla = GuardedBla::build (crates/fd-kernel/src/bla/guarded.rs:103-134): BLA
+ *   levels 1.. whose blocks carry outward-inflated state envelopes and derivative
+ *   envelopes over the orbit's radii q; written level-major, 9 doubles per block (a.re
+ *   a.im b.re b.im r alpha beta dalpha dbeta), level j holding (len-2) >> j blocks;
+ *   *levels receives the level count (the table ends before the first level without a
+ *   valid block).  Refusals as the reference's: eps outside (0, 2^-24], dc_max not finite
+ *   and positive, a non-finite or negative radius.
+ * fdg_external_escape_ok = external_escape_ok (crates/fd-kernel/src/store.rs:155-198):
+ *   1 when an externally computed escaped state with radii ez (z units) and ed
+ *   (derivative mantissa units) is admitted for output pixel px_m 2^px_e (displacement
+ *   <= 5e-4 px, DE within 1e-3, angle within 0.1 degree, escape proven), else 0.
+ * fdg_overlay_escaped = overlay_escaped (crates/fd-kernel/src/grid.rs:111-171): writes the
+ *   states whose `present` byte is set through the column encoder (Store::put), leaving
+ *   the other samples' columns as they are; refuses Bound/Stripe columns and invalid
+ *   states (n beyond max_iter, non-finite z or dz/dc, |z| within the escape radius,
+ *   dz/dc = 0). */
+typedef struct {
+    uint64_t n;
+    double zr, zi;      /* z at escape */
+    double dr, di;      /* dz/dc mantissa */
+    int64_t dexp;       /* dz/dc = (dr, di) 2^dexp */
+} fdg_escaped_state;
+int fdg_error_radius(const double *re, const double *im, uint64_t len, int64_t bits, double c_re,
+                     double c_im, double *q);
+int fdg_guarded_bla(const double *re, const double *im, const double *q, uint64_t len, double eps,
+                    double dc_max, double *blocks, uint64_t cap_blocks, uint32_t *levels);
+int fdg_external_escape_ok(const fdg_escaped_state *state, double ez, double ed, double px_m,
+                           int64_t px_e);
+int fdg_overlay_escaped(const fdg_view *view, const fdg_params *params, fdg_columns *out,
+                        const fdg_escaped_state *states, const uint8_t *present);
+
 /* No counterpart in the reference: an entry point for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  The FP64 form of fdg_frame_open_orbit + fdg_frame_run in one synchronous

@@ -163,3 +163,83 @@ def test_external_escape_ok_cases():
     # non-finite or negative radii
     for ez, ed in ((math.nan, 0.0), (0.0, math.inf), (-1.0, 0.0)):
         assert not oracle.external_escape_ok(n, z, d, dexp, ez, ed, *px)
+
+
+# ------------------------------------------------------------ the library against the oracle
+
+def _product():
+    fa = pytest.importorskip("fractodactyl_amd")
+    try:
+        fa.lib()
+    except Exception as e:
+        pytest.skip(str(e))
+    return fa
+
+
+def test_library_error_radius_and_guarded_table_equal_the_oracle():
+    fa = _product()
+    cases = [("-0.743643887037158704752191506114774", "0.131825904205311970493132056385139",
+              "1e-28", 30_000)]
+    for re_, im_, w, it in cases:
+        zr, zi, bits = oracle.reference(re_, im_, w, 0.0, 64, 36, it, 1e10)
+        pl = oracle.plane(re_, im_, w, 0.0, 64, 36)
+        for b in (None, bits):
+            qa = fa.error_radius(zr, zi, b, pl["c_re"], pl["c_im"])
+            qo = oracle.error_radius(zr, zi, b, pl["c_re"], pl["c_im"])
+            assert np.array_equal(qa.view(np.uint64), qo.view(np.uint64))
+        q = oracle.error_radius(zr, zi, bits, pl["c_re"], pl["c_im"])
+        dc = pl["h_m"] * 2.0 ** pl["h_e"] * math.hypot(32, 18)
+        ga = fa.guarded_bla(zr, zi, q, 2.0 ** -50, dc)
+        go = oracle.guarded_build(zr, zi, q, 2.0 ** -50, dc)
+        assert len(ga) == len(go) >= 5
+        for a, b in zip(ga, go):
+            assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    with pytest.raises(fa.FdgError):
+        fa.guarded_bla(zr, zi, np.full(len(zr), -1.0), 2.0 ** -50, 1e-20)
+
+
+def test_library_admission_equals_the_oracle():
+    fa = _product()
+    rng = np.random.default_rng(7)
+    agree = admitted = 0
+    for _ in range(3000):
+        n = int(rng.integers(10, 10**6))
+        az = 10.0 ** rng.uniform(0.5, 6)
+        th = rng.uniform(0, 2 * math.pi)
+        z = (az * math.cos(th), az * math.sin(th))
+        d = tuple(rng.normal(size=2))
+        dexp = int(rng.integers(-50, 3000))
+        ez = az * 10.0 ** rng.uniform(-16, -1)
+        ed = math.hypot(*d) * 10.0 ** rng.uniform(-16, -1)
+        px = (10.0 ** rng.uniform(-3, 3), -dexp + int(rng.integers(-20, 20)))
+        a = fa.external_escape_ok(n, z, d, dexp, ez, ed, *px)
+        b = oracle.external_escape_ok(n, z, d, dexp, ez, ed, *px)
+        agree += a == b
+        admitted += a
+    assert agree == 3000
+    assert 100 < admitted < 2900
+
+
+def test_overlay_writes_admitted_states_through_the_encoder():
+    fa = _product()
+    re_, im_, w = "-0.7453", "0.1127", "0.0065"
+    cols = fa.COL_NU | fa.COL_DE | fa.COL_NORMAL
+    full = oracle.render(re_, im_, w, 0.0, 48, 27, 5000, 1e10, columns=cols, bla=False)
+    o = full["outcomes"]
+    present = o["kind"] == oracle.ESCAPED
+    present[::3] = False  # overlay a subset; the others keep what the columns held
+    out = {"class": np.full(48 * 27, 9, np.uint8), "nu": np.full(48 * 27, -1.0),
+           "de": np.full(48 * 27, -1.0, np.float32), "normal": np.full(48 * 27, 7, np.uint16)}
+    states = {k: o[k] for k in ("n", "zr", "zi", "dr", "di", "dexp")}
+    states["present"] = present
+    fa.overlay_escaped(fa.View(re_, im_, w, 0.0), fa.Params(48, 27, 1, 5000, 1e10, cols), out,
+                       states)
+    for k in ("class", "nu", "de", "normal"):
+        assert np.array_equal(out[k][present].view(np.uint8),
+                              full["columns"][k][present].view(np.uint8)), k
+    assert np.all(out["class"][~present] == 9) and np.all(out["nu"][~present] == -1.0)
+    bad = dict(states)
+    bad["zr"] = np.where(present, 0.0, bad["zr"])  # |z| within the escape radius
+    with pytest.raises(fa.FdgError):
+        fa.overlay_escaped(fa.View(re_, im_, w, 0.0), fa.Params(48, 27, 1, 5000, 1e10, cols),
+                           out, bad)
