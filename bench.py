@@ -85,6 +85,7 @@ def cpu_baseline(view, params, route_bla, target_seconds, threads, orbit=None, r
         e2e = st["render_seconds"] + st["reference_seconds"] + st["bla_seconds"]
         each.append(dict(run=k + 1, cold=k == 0, render_seconds=st["render_seconds"],
                          reference_seconds=st["reference_seconds"],
+                uncertified_samples=st.get("uncertified_samples", 0),
                          bla_seconds=st["bla_seconds"],
                          value=st["equivalent"] / st["render_seconds"],
                          end_to_end_value=st["equivalent"] / e2e))
@@ -149,6 +150,7 @@ def measure(fa, view, params, route, steps, warmup, label, form=None, keep=None)
                 wave_iterations=st["wave_iterations"],
                 lane_utilisation=st["lane_iterations"] / max(1, 64 * st["wave_iterations"]),
                 reference_seconds=st["reference_seconds"],
+                uncertified_samples=st.get("uncertified_samples", 0),
                 roofline={"bound": "valu-fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                           "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
                           "traffic": None, "flops_per_launch": flops,
@@ -211,6 +213,7 @@ def agreement(ref, other):
     ka, kb = ref["raw"]["kind"], other["raw"]["kind"]
     esc = (ka == 0) & (kb == 0)
     out = {"samples": int(ka.size), "class_agreement": float((ka == kb).mean()),
+           "exterior_agreement": float(((ka == 0) == (kb == 0)).mean()),
            "escaped_in_both": int(esc.sum())}
     if esc.any():
         dn = other["raw"]["n"][esc].astype(np.int64) - ref["raw"]["n"][esc].astype(np.int64)
@@ -234,6 +237,7 @@ def measure_forms(fa, view, params, route, steps, warmup, label, ref_metrics, re
         out[name] = {"kernel_ms": m["kernel_ms"], "ms_per_step": m["ms_per_step"],
                      "value": m["value"], "kernel_id": m["kernel_id"],
                      "speedup_over_fp64_kernel": ref_metrics["kernel_ms"] / m["kernel_ms"],
+                     "uncertified_samples": m["uncertified_samples"],
                      "agreement_with_fp64": agreement(ref_cols, keep[0])}
     return out
 

@@ -601,20 +601,33 @@ static int frame_open(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int 
     f->bits = o.bits;
     f->nucleus = o.nucleus_period();
     rc = upload_orbit(ctx, &f->buf.orbit, o, &f->transfer_seconds);
-    f->orbit_re = std::move(o.re);
-    f->orbit_im = std::move(o.im);
     if (!rc) rc = f->buf.outputs(ctx, f->count);
     if (!rc && form != FDG_FORM_F64) {
-        // FP32 floatexp orbit, 16 bytes per point, with two zero points of pad past the end
+        // FP32 floatexp orbit, 16 bytes per point, with two zero points of pad past the end;
+        // each point carries log2 of its error radius (Reference::error_radius over the
+        // orbit as computed: f64 from the rounded centre, or fixed point at its bits), which
+        // the FP32 kernel charges into its error bound
+        std::vector<double> q(o.len());
+        error_radius(o.re.data(), o.im.data(), o.len(),
+                     fr.tier == TIER_F64 ? -1 : (int64_t)o.bits, fr.pl.c_re, fr.pl.c_im,
+                     q.data());
         uint64_t bytes = 16 * (f->len + 2);
         rc = ctx->hip(f->buf.orbit32.ensure(bytes), "hipMalloc");
+        if (!rc) rc = ctx->hip(f->buf.qrad.ensure(8 * o.len()), "hipMalloc");
+        if (!rc) rc = ctx->hip(f->buf.cert.ensure(f->count), "hipMalloc");
+        if (!rc)
+            rc = ctx->hip(hipMemcpyAsync(f->buf.qrad.p, q.data(), 8 * o.len(),
+                                         hipMemcpyHostToDevice, ctx->stream),
+                          "radius upload");
         if (!rc) rc = ctx->hip(hipMemsetAsync(f->buf.orbit32.p, 0, bytes, ctx->stream), "memset");
         if (!rc)
-            rc = ctx->hip(launch_orbit_to_f32(f->buf.orbit.as<double>(),
+            rc = ctx->hip(launch_orbit_to_f32(f->buf.orbit.as<double>(), f->buf.qrad.as<double>(),
                                               f->buf.orbit32.as<float>(), f->len, ctx->stream),
                           "orbit conversion");
         if (!rc) rc = ctx->hip(hipStreamSynchronize(ctx->stream), "orbit conversion");
     }
+    f->orbit_re = std::move(o.re);
+    f->orbit_im = std::move(o.im);
     if (rc) return rc;
     dc_bound(fr.pl, fr.tier, p->nx, p->ny, &f->dc, &f->dc_exp);
     guard.f = nullptr;
@@ -716,6 +729,9 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
             d.orbit32 = b.orbit32.as<float>();
             d.ab32 = b.ab32.as<float>();
             d.abexp32 = b.abexp32.as<int32_t>();
+            d.bla_alpha = b.bla_alpha.as<double>();
+            d.bla_beta = b.bla_beta.as<double>();
+            d.cert = mixed ? b.cert.as<uint8_t>() : nullptr;
             d.defer_escaped = mixed ? 1u : 0u;
         }
         // The kernel writes every output field of every sample; only the counters start at 0.
@@ -730,7 +746,7 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
         if (fe32)
             FDG_TRY(ctx,
                     launch_perturb_fe32(b.fdev.as<FrameDev>(), deriv, built, grid,
-                                        ctx->stream),
+                                        ctx->stream, mixed),
                     "perturbation launch");
         else if (series)
             FDG_TRY(ctx, launch_perturb_f64_series(b.fdev.as<FrameDev>(), deriv, built, grid,
@@ -766,8 +782,8 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
                                          ctx->stream),
                     "pass set-up");
             FDG_TRY(ctx,
-                    launch_escaped_list(b.o_kind.as<uint8_t>(), b.list.as<uint32_t>(), listed,
-                                        f->count, ctx->stream),
+                    launch_escaped_list(b.o_kind.as<uint8_t>(), b.cert.as<uint8_t>(),
+                                        b.list.as<uint32_t>(), listed, f->count, ctx->stream),
                     "escaped list");
             FDG_TRY(ctx,
                     launch_perturb_f64(b.fdev2.as<FrameDev>(), f->fr.tier == TIER_SCALED, deriv,
@@ -821,6 +837,7 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
         s.max_sample_ops = cnt[ST_MAXOPS];
         s.wave_iterations = cnt[ST_WAVE_ITERS];
         s.lane_iterations = cnt[ST_LANE_ITERS];
+        s.uncertified_samples = cnt[ST_UNCERT];
         f->last = s;
         f->ran = true;
         if (st) *st = s;

@@ -95,6 +95,11 @@ struct Orbit {
     static uint64_t nucleus_period_of(const double* re, const double* im, uint64_t len);
 };
 
+// Reference::error_radius (reference.rs:122-148): q[m] >= |A_m - Z_m|; bits < 0 for an f64
+// orbit from (c_re, c_im), else the fixed-point fraction bits.
+void error_radius(const double* re, const double* im, uint64_t len, int64_t bits, double c_re,
+                  double c_im, double* q);
+
 // Upper bound m 2^e on |dc| over a view's samples (grid.rs:423-444).
 void dc_bound(const Plane& pl, Tier tier, uint32_t nx, uint32_t ny, double* dc, int64_t* e);
 

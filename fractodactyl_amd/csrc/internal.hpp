@@ -29,20 +29,25 @@ hipError_t launch_pass_prologue(const FrameDev& d, FrameDev* dst, uint32_t* vali
                                 const uint32_t* levels_in, uint64_t steps, uint32_t max_levels,
                                 unsigned long long* counters, uint32_t nzero,
                                 uint32_t* levels_out, bool clear, hipStream_t stream);
-hipError_t launch_orbit_to_f32(const double* orbit, float* o32, uint64_t points,
+// The orbit as FP32 mantissa pairs with exponents; with q (nullable, one radius per point)
+// each point also carries log2 of its error radius.
+hipError_t launch_orbit_to_f32(const double* orbit, const double* q, float* o32, uint64_t points,
                                hipStream_t stream);
 hipError_t launch_table_to_f32(const double* ab, float* ab32, int32_t* abexp, uint64_t blocks,
                                hipStream_t stream);
-hipError_t launch_escaped_list(const uint8_t* kind, uint32_t* list, uint32_t* count, uint32_t n,
-                               hipStream_t stream);
+// The mixed form's refinement list: the computed samples the FP32 pass did not certify
+// (cert[i] == 0; with cert null, the escaped ones).
+hipError_t launch_escaped_list(const uint8_t* kind, const uint8_t* cert, uint32_t* list,
+                               uint32_t* count, uint32_t n, hipStream_t stream);
 // The FP64 kernel on the ordinary tiers with every sample starting from a series state
 // (FrameDev::series_*); unresolved series starts retry from zero in the same launch.
 hipError_t launch_perturb_f64_series(const FrameDev* f, bool deriv, bool bla, int grid,
                                      hipStream_t stream);
 // Store::put on the device for e.count samples (encode.hip).
 hipError_t launch_encode_columns(const EncodeDev& e, int cus, hipStream_t stream);
+// cert: the mixed form's certifying pass (error bound per sample, FrameDev::cert written).
 hipError_t launch_perturb_fe32(const FrameDev* f, bool deriv, bool bla, int grid,
-                               hipStream_t stream);
+                               hipStream_t stream, bool cert = false);
 
 // The no-allocation entry point's set-up (noalloc.hip): looks up the scratch array's address
 // on `device` once, with that device current; fdg_create calls it.
@@ -135,6 +140,7 @@ struct Buffers {
     DevBuf orbit, bla_r, bla_r2, bla_ab, bla_alpha, bla_beta, bla_valid;
     DevBuf o_kind, o_n, o_zr, o_zi, o_dr, o_di, o_dexp, o_ops, counters, fdev;
     DevBuf orbit32, ab32, abexp32;  // FP32 copies for the FE32 and mixed forms
+    DevBuf qrad, cert;              // FP32 forms: orbit error radii, per-sample certificates
     DevBuf list, counters2, fdev2;  // the mixed form's FP64 refinement pass
     DevBuf c_cls, c_nu, c_de, c_nm;  // columns encoded on the device
     uint64_t bla_stride = 0, bla_steps = 0;
@@ -145,7 +151,8 @@ struct Buffers {
         valid_clean = false;
         for (DevBuf* b : {&orbit, &bla_r, &bla_r2, &bla_ab, &bla_alpha, &bla_beta, &bla_valid,
                           &o_kind, &o_n, &o_zr, &o_zi, &o_dr, &o_di, &o_dexp, &o_ops,
-                          &counters, &fdev, &orbit32, &ab32, &abexp32, &list, &counters2, &fdev2,
+                          &counters, &fdev, &orbit32, &ab32, &abexp32, &qrad, &cert, &list, &counters2,
+                          &fdev2,
                           &c_cls, &c_nu, &c_de, &c_nm})
             b->release();
     }

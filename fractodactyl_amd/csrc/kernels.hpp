@@ -33,6 +33,8 @@ enum StatSlot : int {
     ST_MAXOPS = 7,      // most operations (plain steps + blocks) one sample executed
     ST_WAVE_ITERS = 8,  // loop iterations summed over waves
     ST_LANE_ITERS = 9,  // active lanes summed over those iterations (utilisation)
+    ST_UNCERT = 10,     // FP32 forms: non-escaped samples whose error bound reached the
+                        // escape radius (not certified)
     ST_COUNT = 16
 };
 
@@ -79,8 +81,14 @@ struct FrameDev {
     const float* orbit32;
     const float* ab32;
     const int32_t* abexp32;         // per block: the exponents of A and B
-    uint32_t defer_escaped;         // FP32 pass of the mixed form: escaped samples are
-                                    // recomputed in FP64, so they add no statistics here
+    uint32_t defer_escaped;         // FP32 pass of the mixed form: the samples it does not
+                                    // certify are recomputed in FP64, so they add no
+                                    // statistics here
+    uint8_t* cert;                  // FP32 forms (nullable): per sample 1 = certified (an
+                                    // error bound shows the exact iteration stays below the
+                                    // escape radius), 0 = escaped or not certified
+    const double* bla_alpha;        // FP32 forms: the table's state envelopes (alpha, beta),
+    const double* bla_beta;         // same layout as bla.r2
     const uint32_t* list;           // nullable: compute only samples list[0 .. *list_count)
     const uint32_t* list_count;
     // Series start (the series kernels only, ordinary tiers): every sample starts at

@@ -89,9 +89,25 @@ __device__ inline bool fx_less2(float xr, float xi, int32_t xe, float yr, float 
     return scale2(x2, d) < y2;
 }
 
+// log2 of a bound that is zero (an exact quantity)
+constexpr float kNoError = -1.0e30f;
+// log2 of the relative rounding charged to one complex FP32 operation: 16 units of 2^-24
+constexpr float kRound = -20.0f;
+
 struct Lane32 {
     float xr, xi, dr, di, ar, ai;
     float Zr, Zi, Z1r, Z1i;     // Z_m and Z_{m+1} mantissas
+    float lq, lq1;              // log2 of their error radii (Reference::error_radius)
+    float le;                   // log2 of a bound on |delta - the exact delta|
+    bool safe;                  // |z| + that bound stayed below the escape radius
+    // Brent/Newton periodicity (ordinary tiers, sample.rs:524-558): the saved iterate
+    // (sr, si) at sn, the next save point chk, the awaited return period hold, the
+    // return multiplier mu = (mr, mi) 2^me since the save, the last cycle-point
+    // prediction (qr, qi) and its step; flags kPending | kPredicted
+    double sr, si, qr, qi, qstep;
+    float mur, mui;
+    int32_t mue;
+    uint32_t sn, chk, hold, flags;
     float cr, ci;               // nucleus contraction product (mantissa)
     double R0;                  // level-0 r2 of the block starting at m (when any)
     int32_t e, de, ce, ze, z1e; // exponents: delta, dz/dc, product, Z_m, Z_{m+1}
@@ -104,13 +120,41 @@ struct Hot32 {
     const FDG_GLOBAL v4f* ab;       // A, B mantissas: (A.re, A.im) 2^ea, (B.re, B.im) 2^eb
     const FDG_GLOBAL v2i* abexp;    // (ea, eb): |A|, |B| exceed FP32's range on long blocks
     const FDG_GLOBAL double* r2;
+    const FDG_GLOBAL double* alpha;  // the table's state envelopes (bla.rs:91-111)
+    const FDG_GLOBAL double* beta;
     uint32_t last, max_iter, nlevels, steps, count0;
     uint64_t stride;
-    float esc2;
+    float esc2, escR;
     int32_t oe;
-    float Z1r, Z1i;
+    float Z1r, Z1i, Z0q, Z1q;
     int32_t Z1e;
+    bool periodic;                 // ordinary tiers: Brent/Newton confirmation on
 };
+
+constexpr uint32_t kPending32 = 1, kPredicted32 = 2;
+// sample.rs:524-558's tolerance term 16 u |s| with the FP32 unit, as the predictions are
+// made from FP32 iterates
+constexpr double kU32 = 1.0 / 16777216.0;
+
+// Error bounds are carried as log2 of the bound, in FP32; each sum of bounds uses an upper
+// bound on log2(2^a + 2^b), so the slack does not compound over long samples.  The hardware
+// log2 / exp2 are accurate to about an ulp; every use is inflated by 2^-17 relative plus
+// 2^-20 absolute (in log2 units) to stay an upper bound.
+__device__ inline float lse(float a, float b) {
+    const float m = fmaxf(a, b), d = fabsf(a - b);
+    const float t = __builtin_amdgcn_logf(1.0f + __builtin_amdgcn_exp2f(-d));
+    return m + t * (1.0f + 0x1p-17f) + 0x1p-20f;
+}
+// log2 |(re, im) 2^e| (kNoError for 0), inflated as above
+__device__ inline float lmag(float re, float im, int32_t e) {
+    const float a = re * re + im * im;
+    if (!(a > 0.0f)) return kNoError;
+    return (float)e + 0.5f * __builtin_amdgcn_logf(a) * (1.0f + 0x1p-17f) + 0x1p-18f;
+}
+// an upper bound on log2 v for v >= 0 (kNoError for 0)
+__device__ inline float lg64(double v) {
+    return v > 0.0 ? (float)__builtin_amdgcn_frexp_exp(v) : kNoError;
+}
 
 // A sample's start: delta = 0, D = 0, dc = a 2^aexp with a rounded to FP32 and its
 // exponent kept apart from the frame's offset exponent, so any depth fits.
@@ -131,6 +175,18 @@ __device__ inline void start32(Lane32& s, const Hot32& h, uint32_t nucleus, doub
     s.Z1r = h.Z1r;
     s.Z1i = h.Z1i;
     s.z1e = h.Z1e;
+    s.lq = h.Z0q;
+    s.lq1 = h.Z1q;
+    s.le = kNoError;
+    s.safe = true;
+    s.sr = s.si = s.qr = s.qi = s.qstep = 0.0;
+    s.mur = 1.0f;
+    s.mui = 0.0f;
+    s.mue = 0;
+    s.sn = 8;    // the ordinary tiers' first save point (sample.rs:216-230)
+    s.chk = 16;
+    s.hold = 0;
+    s.flags = 0;
     s.cr = 1.0f;
     s.ci = 0.0f;
     s.ce = 0;
@@ -143,14 +199,19 @@ __device__ inline void start32(Lane32& s, const Hot32& h, uint32_t nucleus, doub
 
 // One block or plain step plus the checks after it.  Returns -1 running, or the outcome
 // kind (0 escaped, z in ofr/ofi; 1 interior; 2 unresolved).
-template <bool D, bool L>
+template <bool D, bool L, bool CERT>
 __device__ inline int advance32(Lane32& s, const Hot32& h, int32_t aexp, float& ofr, float& ofi) {
     if (s.n >= h.max_iter) return 2;
     v4f pz2 = h.orbit[s.m + 2];
     double pr0 = 0.0;
     if (L && (s.m & 1u) == 0 && (s.m >> 1) < h.count0) pr0 = h.r2[s.m >> 1];
     float Znr = s.Z1r, Zni = s.Z1i, Zxr = pz2.x, Zxi = pz2.y;
+    float Znq = s.lq1, Zxq = pz2.w;
     int32_t Zne = s.z1e, Zxe = __float_as_int(pz2.z);
+    // log2 of this step's input magnitudes (certifying pass only): delta, dc, Z_m
+    const float ld = CERT ? lmag(s.xr, s.xi, s.e) : 0.0f;
+    const float ldc = CERT ? lmag(s.ar, s.ai, aexp) : 0.0f;
+    const float lZ = CERT ? lmag(s.Zr, s.Zi, s.ze) : 0.0f;
     double r0next = pr0;
     int lvl = -1;
     if (L) {
@@ -159,6 +220,13 @@ __device__ inline int advance32(Lane32& s, const Hot32& h, int32_t aexp, float& 
             double xr = (double)s.xr, xi = (double)s.xi;
             double d2 = (xr * xr + xi * xi) * exp2i(2 * (int64_t)s.e);
             uint32_t b = h.max_iter;
+            // with periodicity, a block ends at the next save point or awaited return at
+            // the latest (sample.rs:238-262), so saves land on the plain kernel's iterates
+            if (h.periodic) {
+                uint32_t pb = s.hold > 0 ? s.sn + s.hold
+                                         : ((s.flags & kPending32) ? 0xffffffffu : s.chk);
+                b = b < pb ? b : pb;
+            }
             uint32_t nb = s.nucleus ? (s.n == 0 ? 1u : s.nucleus + 1) : 0xffffffffu;
             b = b < nb ? b : nb;
             b = b > s.n ? b - s.n : 0;
@@ -177,11 +245,25 @@ __device__ inline int advance32(Lane32& s, const Hot32& h, int32_t aexp, float& 
         Znr = zn.x;
         Zni = zn.y;
         Zne = __float_as_int(zn.z);
+        Znq = zn.w;
         Zxr = zx.x;
         Zxi = zx.y;
         Zxe = __float_as_int(zx.z);
+        Zxq = zx.w;
         uint32_t i0 = (s.m + blen - 1) >> 1;
         r0next = i0 < h.count0 ? h.r2[i0] : 0.0;
+        // error bound through the block (the reference route's bla_advance,
+        // crates/fd-cli/shaders/perturb_bla.wgsl:30-35): (|A| + alpha) e + alpha |delta|
+        // + beta |dc| + (|B| + beta) rc + rounding(|A| |delta| + |B| |dc|), rc the FP32
+        // rounding of dc; each sum of k terms bounded by k times its largest, in log2
+        if (CERT) {
+            const float lA = lmag(ab.x, ab.y, ex.x), lB = lmag(ab.z, ab.w, ex.y);
+            const float la = lg64(h.alpha[at]), lb = lg64(h.beta[at]);
+            const float lrc = ldc - 23.0f;
+            float t = lse(lse(lA, la) + s.le, la + ld);
+            t = lse(t, lse(lb + ldc, lse(lB, lb) + lrc));
+            s.le = lse(t, kRound + lse(lA + ld, lB + ldc));
+        }
         // A delta + B dc with A = a 2^ea, B = b 2^eb, each term scaled into the units of
         // the larger (one ldexp each, so neither overflows nor needlessly underflows).
         mr = ab.x;
@@ -226,6 +308,17 @@ __device__ inline int advance32(Lane32& s, const Hot32& h, int32_t aexp, float& 
             s.de = E;
             renorm(s.dr, s.di, s.de);
         }
+        // error bound through the step (the reference route's frame loop,
+        // crates/fd-cli/shaders/perturb_frame.wgsl:33-36): e' = e (2w + e) + rho with
+        // w = |z| + q + rounding(|Z| + |delta|), rho = rounding(2|Z||delta| + |delta|^2 +
+        // |dc|) + 2 q |delta| + rc
+        if (CERT) {
+            const float lz = lmag(z.re, z.im, z.e);
+            const float lw = lse(lse(lz, s.lq), kRound + lse(lZ, ld));
+            const float lrho = lse(ld + lse(kRound + lse(1.0f + lZ, ld), 1.0f + s.lq),
+                                   lse(kRound, -23.0f) + ldc);
+            s.le = lse(s.le + lse(1.0f + lw, s.le), lrho);
+        }
         // delta' = 2 Z delta + delta^2 + dc, in the units of the largest term
         bool zz = s.Zr == 0.0f && s.Zi == 0.0f;
         int32_t E = max(max(2 * s.e + 1, aexp), zz ? INT32_MIN / 2 : s.e + s.ze + 1);
@@ -256,9 +349,19 @@ __device__ inline int advance32(Lane32& s, const Hot32& h, int32_t aexp, float& 
         s.ce += mexp;
         renorm(s.cr, s.ci, s.ce);
     }
+    if (h.periodic && s.hold > 0) {
+        // the return multiplier (sample.rs:597-652), mantissa kept near 1
+        float nr = s.mur * mr - s.mui * mi, ni = s.mur * mi + s.mui * mr;
+        s.mur = nr;
+        s.mui = ni;
+        s.mue += mexp;
+        renorm(s.mur, s.mui, s.mue);
+    }
     // the landing point z = Z_{m'} + delta'
     Fx zl = fx_add(Znr, Zni, Zne, s.xr, s.xi, s.e);
     float fr = scale2(zl.re, zl.e), fi = scale2(zl.im, zl.e);
+    // certified below the escape radius: |z| + e < R (|z| bounded by its L1 norm)
+    if (CERT) s.safe = s.safe && fabsf(fr) + fabsf(fi) + exp2f(s.le) < h.escR;
     if (fr * fr + fi * fi > h.esc2) {
         ofr = fr;
         ofi = fi;
@@ -274,6 +377,10 @@ __device__ inline int advance32(Lane32& s, const Hot32& h, int32_t aexp, float& 
     }
     bool rebase = fx_less2(zl.re, zl.im, zl.e, s.xr, s.xi, s.e) || s.m == h.last;
     if (rebase) {
+        // the new delta is z = Z + delta: e + q + rounding(|Z| + |delta|)
+        if (CERT)
+            s.le = lse(lse(s.le, Znq),
+                       kRound + lse(lmag(Znr, Zni, Zne), lmag(s.xr, s.xi, s.e)));
         s.xr = zl.re;
         s.xi = zl.im;
         s.e = zl.e;
@@ -283,6 +390,8 @@ __device__ inline int advance32(Lane32& s, const Hot32& h, int32_t aexp, float& 
         s.Z1r = h.Z1r;
         s.Z1i = h.Z1i;
         s.z1e = h.Z1e;
+        s.lq = h.Z0q;
+        s.lq1 = h.Z1q;
     } else {
         s.Zr = Znr;
         s.Zi = Zni;
@@ -290,12 +399,74 @@ __device__ inline int advance32(Lane32& s, const Hot32& h, int32_t aexp, float& 
         s.Z1r = Zxr;
         s.Z1i = Zxi;
         s.z1e = Zxe;
+        s.lq = Znq;
+        s.lq1 = Zxq;
     }
     s.R0 = r0next;
+    if (h.periodic) {
+        // Brent saves and Newton's cycle-point prediction (sample.rs:451-575), on the
+        // physical z (FP64 here: ordinary-tier iterates are inside its range)
+        const double zr = ldexp((double)zl.re, zl.e), zi = ldexp((double)zl.im, zl.e);
+        const double f2 = zr * zr + zi * zi;
+        const double xr = ldexp((double)s.xr, s.e), xi = ldexp((double)s.xi, s.e);
+        const bool res = xr * xr + xi * xi > 1e-24 * f2;  // (after a rebase delta = z)
+        const double dist = fabs(zr - s.sr) + fabs(zi - s.si);
+        const double size = fabs(s.sr) + fabs(s.si);
+        if (res && dist < 1e-3 * size) {
+            const uint32_t period = s.n - s.sn;
+            // |mu| < 1 (sample.rs scaled_below)
+            const double m2 = (double)s.mur * s.mur + (double)s.mui * s.mui;
+            const bool below = isfinite(m2) &&
+                               (s.mue < -64 || (s.mue <= 64 && m2 * exp2i(2 * s.mue) < 1.0));
+            if (s.hold == period && below) {
+                const double sc = exp2i(s.mue);
+                const double mr2 = (double)s.mur * sc, mi2 = (double)s.mui * sc;
+                const double gr = zr - s.sr, gi = zi - s.si;
+                const double er = mr2 - 1.0, ei = mi2;
+                const double den = er * er + ei * ei;
+                const double pr2 = s.sr - (gr * er + gi * ei) / den;
+                const double pi2 = s.si - (gi * er - gr * ei) / den;
+                const double tol = fmin(0.1 * sqrt(den), 1e-2) * s.qstep + 16.0 * kU32 * size;
+                if ((s.flags & kPredicted32) && fabs(pr2 - s.qr) + fabs(pi2 - s.qi) <= tol)
+                    return 1;  // Interior: two consecutive predictions agree
+                s.qr = pr2;
+                s.qi = pi2;
+                s.qstep = (fabs(gr) + fabs(gi)) / sqrt(den);
+                s.flags |= kPredicted32;
+            } else {
+                s.flags &= ~kPredicted32;
+            }
+            s.hold = period;
+            s.sr = zr;
+            s.si = zi;
+            s.sn = s.n;
+            s.flags &= ~kPending32;
+            s.mur = 1.0f;
+            s.mui = 0.0f;
+            s.mue = 0;
+            return -1;
+        }
+        const uint32_t h0 = s.hold;
+        const bool expired = h0 > 0 && s.n >= s.sn + h0;
+        if (expired) {
+            s.hold = 0;
+            s.flags = (s.flags & ~kPredicted32) | kPending32;
+        }
+        if (!res && h0 == 0 && s.n >= s.chk) s.flags |= kPending32;  // deferred save
+        if (res && (h0 == 0 || expired) && ((s.flags & kPending32) || s.n >= s.chk)) {
+            const uint32_t np = 1u << (32 - __builtin_clz(s.n));  // next power of two > n
+            s.chk = s.chk > np ? s.chk : np;
+            s.sr = zr;
+            s.si = zi;
+            s.sn = s.n;
+            s.flags &= ~kPending32;
+        }
+    }
     return -1;
 }
 
-template <bool D, bool L>
+// CERT: the mixed form's pass, which also carries the error bound and certifies samples.
+template <bool D, bool L, bool CERT>
 __global__ void __launch_bounds__(kBlock32, 4) perturb_fe32(const FrameDev* __restrict__ fp) {
     __shared__ unsigned long long stat[ST_COUNT];
     if (threadIdx.x < ST_COUNT) stat[threadIdx.x] = 0;
@@ -309,6 +480,8 @@ __global__ void __launch_bounds__(kBlock32, 4) perturb_fe32(const FrameDev* __re
     h.ab = (const FDG_GLOBAL v4f*)(fp->ab32);
     h.abexp = (const FDG_GLOBAL v2i*)(fp->abexp32);
     h.r2 = (const FDG_GLOBAL double*)(fp->bla.r2);
+    h.alpha = (const FDG_GLOBAL double*)(fp->bla_alpha);
+    h.beta = (const FDG_GLOBAL double*)(fp->bla_beta);
     h.last = (uint32_t)fp->len - 1;
     h.max_iter = (uint32_t)fp->max_iter;
     h.nlevels = fp->bla.nlevels;
@@ -316,12 +489,16 @@ __global__ void __launch_bounds__(kBlock32, 4) perturb_fe32(const FrameDev* __re
     h.stride = fp->bla.stride;
     h.count0 = (uint32_t)(fp->bla.steps >> 1);
     h.esc2 = (float)fp->r2;
+    h.escR = (float)sqrt(fp->r2);
+    h.periodic = fp->tier != 2;
     h.oe = (int32_t)fp->h_e;
     {
         v4f z1 = h.orbit[1];
         h.Z1r = z1.x;
         h.Z1i = z1.y;
         h.Z1e = __float_as_int(z1.z);
+        h.Z1q = z1.w;
+        h.Z0q = h.orbit[0].w;
     }
     FDG_GLOBAL uint8_t* o_kind = (FDG_GLOBAL uint8_t*)fp->kind;
     FDG_GLOBAL uint64_t* o_n = (FDG_GLOBAL uint64_t*)fp->n;
@@ -405,9 +582,13 @@ __global__ void __launch_bounds__(kBlock32, 4) perturb_fe32(const FrameDev* __re
             liters += __popcll(__ballot(active));
             if (active) {
                 float fr = 0.0f, fi = 0.0f;
-                int k = advance32<D, L>(s, h, aexp, fr, fi);
+                int k = advance32<D, L, CERT>(s, h, aexp, fr, fi);
                 if (k >= 0) {
                     bool esc = k == 0;
+                    // kept: a non-escaped sample whose bound never reached the escape radius
+                    bool keep = !esc && (!CERT || s.safe);
+                    if (CERT) fp->cert[idx] = keep ? 1 : 0;
+                    if (CERT && !esc && !keep) atomicAdd(&stat[ST_UNCERT], 1ull);
                     o_kind[idx] = (uint8_t)k;
                     o_n[idx] = s.n;
                     o_ops[idx] = s.plain + s.blocks;
@@ -418,7 +599,7 @@ __global__ void __launch_bounds__(kBlock32, 4) perturb_fe32(const FrameDev* __re
                     // dz/dc = d 2^de per unit of c; the outcome's D is per unit of c too
                     o_dexp[idx] = esc ? (int64_t)s.de : 0;
                     unsigned long long spent = k == 2 ? h.max_iter : s.n;
-                    if (!(esc && fp->defer_escaped)) {
+                    if (!(!keep && fp->defer_escaped)) {
                         ls.add(ST_EQUIVALENT, spent);
                         ls.add(ST_ITERATIONS, L ? spent - s.skipped + s.blocks : spent);
                         if (L) {
@@ -462,8 +643,8 @@ __device__ inline void split32(double re, double im, float& mr, float& mi, int32
 
 // The orbit as FP32 mantissa pairs with their exponents: (re, im, bits of e, 0) per point
 // (deep returns take |Z| far below FP32's range).
-__global__ void orbit_to_f32(const double* __restrict__ orbit, float* __restrict__ o32,
-                             uint64_t points) {
+__global__ void orbit_to_f32(const double* __restrict__ orbit, const double* __restrict__ q,
+                             float* __restrict__ o32, uint64_t points) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= points) return;
     float r, im;
@@ -472,7 +653,7 @@ __global__ void orbit_to_f32(const double* __restrict__ orbit, float* __restrict
     o32[4 * i] = r;
     o32[4 * i + 1] = im;
     o32[4 * i + 2] = __int_as_float(e);
-    o32[4 * i + 3] = 0.0f;
+    o32[4 * i + 3] = q ? lg64(q[i]) : kNoError;  // log2 of the point's error radius
 }
 
 // Table coefficients (A, B) per block as FP32 mantissas with their exponents.
@@ -497,10 +678,12 @@ __global__ void table_to_f32(const double* __restrict__ ab, float* __restrict__ 
 // The mixed form's second pass covers the samples the FP32 pass found escaped: their
 // indices, compacted with one atomic per wave (the order within the list is immaterial,
 // every sample being independent).
-__global__ void escaped_list(const uint8_t* __restrict__ kind, uint32_t* __restrict__ list,
-                             uint32_t* __restrict__ count, uint32_t n) {
+__global__ void escaped_list(const uint8_t* __restrict__ kind, const uint8_t* __restrict__ cert,
+                             uint32_t* __restrict__ list, uint32_t* __restrict__ count,
+                             uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool esc = i < n && kind[i] == 0;
+    // (a sample outside a mask keeps kind 0xff: never listed)
+    bool esc = i < n && kind[i] != 0xff && (cert ? cert[i] == 0 : kind[i] == 0);
     uint64_t b = __ballot(esc);
     if (!b) return;
     uint32_t leader = __ffsll((unsigned long long)b) - 1, base = 0;
@@ -509,19 +692,19 @@ __global__ void escaped_list(const uint8_t* __restrict__ kind, uint32_t* __restr
     if (esc) list[base + lanes_below32(b)] = i;
 }
 
-hipError_t launch_escaped_list(const uint8_t* kind, uint32_t* list, uint32_t* count, uint32_t n,
-                               hipStream_t stream) {
+hipError_t launch_escaped_list(const uint8_t* kind, const uint8_t* cert, uint32_t* list,
+                               uint32_t* count, uint32_t n, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(escaped_list, dim3((n + 255) / 256), dim3(256), 0, stream, kind, list,
-                       count, n);
+    hipLaunchKernelGGL(escaped_list, dim3((n + 255) / 256), dim3(256), 0, stream, kind, cert,
+                       list, count, n);
     return hipGetLastError();
 }
 
-hipError_t launch_orbit_to_f32(const double* orbit, float* o32, uint64_t points,
+hipError_t launch_orbit_to_f32(const double* orbit, const double* q, float* o32, uint64_t points,
                                hipStream_t stream) {
     if (points == 0) return hipSuccess;
     hipLaunchKernelGGL(orbit_to_f32, dim3((unsigned)((points + 255) / 256)), dim3(256), 0, stream,
-                       orbit, o32, points);
+                       orbit, q, o32, points);
     return hipGetLastError();
 }
 
@@ -534,13 +717,19 @@ hipError_t launch_table_to_f32(const double* ab, float* ab32, int32_t* abexp, ui
 }
 
 hipError_t launch_perturb_fe32(const FrameDev* f, bool deriv, bool bla, int grid,
-                               hipStream_t stream) {
-#define FDG_LAUNCH32(D, L) \
-    hipLaunchKernelGGL((perturb_fe32<D, L>), dim3(grid), dim3(kBlock32), 0, stream, f)
-    if (deriv && bla) FDG_LAUNCH32(true, true);
-    else if (deriv) FDG_LAUNCH32(true, false);
-    else if (bla) FDG_LAUNCH32(false, true);
-    else FDG_LAUNCH32(false, false);
+                               hipStream_t stream, bool cert) {
+#define FDG_LAUNCH32(D, L, C) \
+    hipLaunchKernelGGL((perturb_fe32<D, L, C>), dim3(grid), dim3(kBlock32), 0, stream, f)
+    switch ((cert ? 4 : 0) | (deriv ? 2 : 0) | (bla ? 1 : 0)) {
+        case 0: FDG_LAUNCH32(false, false, false); break;
+        case 1: FDG_LAUNCH32(false, true, false); break;
+        case 2: FDG_LAUNCH32(true, false, false); break;
+        case 3: FDG_LAUNCH32(true, true, false); break;
+        case 4: FDG_LAUNCH32(false, false, true); break;
+        case 5: FDG_LAUNCH32(false, true, true); break;
+        case 6: FDG_LAUNCH32(true, false, true); break;
+        default: FDG_LAUNCH32(true, true, true); break;
+    }
 #undef FDG_LAUNCH32
     return hipGetLastError();
 }

@@ -113,6 +113,9 @@ typedef struct {
     double kernel_seconds;        /* GPU perturbation kernel (device time) */
     double transfer_seconds;      /* uploads + outcome readback */
     double encode_seconds;        /* Store::put into the columns (host) */
+    uint64_t uncertified_samples; /* FP32 forms: non-escaped samples whose error bound
+                                     reached the escape radius (the mixed form recomputes
+                                     them, and the escaped ones, in FP64) */
 } fdg_stats;
 
 int fdg_abi_version(void);
