@@ -163,7 +163,54 @@ CONFIGS = {
             escape_radius=1e3, bla=False, shift=0.0, max_log10_size=-21),
     2: dict(pmin=1000, pmax=10000, bits=512, width="1e-100", nx=1920, ny=1080,
             max_iter=100_000, escape_radius=1e10, bla=True, shift=0.3),
+    5: dict(pmin=20_000, pmax=200_000, bits=10_500, width="1e-3000", nx=7680, ny=4320, ss=2,
+            max_iter=10_000_000, escape_radius=1e10, bla=True, shift=0.0),
 }
+
+
+def satellite_nucleus(seed, pmin, pmax, bits):
+    """A seeded nucleus of period P in [pmin, pmax]: the satellite component of the main
+    cardioid at internal angle p/P (p seeded, coprime with P) has exact period P and its
+    nucleus lies next to the cardioid point c(t) = e^{it}/2 - e^{2it}/4, t = 2 pi p / P.
+    Newton on z_P(c) = 0 from that point at 96 bits, then once more at each doubled
+    precision up to `bits` (+32 guard bits); the result's period is checked to be exactly P
+    (no return to within 2^-64 at a proper divisor).  Python-integer fixed point, as
+    find_nucleus; a 10,500-bit nucleus of period ~1e5 takes about a minute."""
+    rng = np.random.default_rng(seed)
+    P = int(rng.integers(pmin, pmax + 1))
+    while True:
+        q = int(rng.integers(1, P))
+        if math.gcd(q, P) == 1:
+            break
+    t = 2 * math.pi * q / P
+    c0 = (0.5 * math.cos(t) - 0.25 * math.cos(2 * t), 0.5 * math.sin(t) - 0.25 * math.sin(2 * t))
+    b = 96
+    cr, ci = _fx(repr(c0[0]), b), _fx(repr(c0[1]), b)
+    cr, ci, ok = newton(cr, ci, P, b, iters=60)
+    if not ok:
+        raise RuntimeError("Newton did not converge at 96 bits")
+    target = bits + 32
+    while b < target:
+        nb = min(2 * b, target)
+        cr, ci = cr << (nb - b), ci << (nb - b)
+        b = nb
+        cr, ci, ok = newton(cr, ci, P, b, iters=3)
+        if not ok:
+            raise RuntimeError(f"Newton did not converge at {b} bits")
+    # exact period: no proper divisor returns near 0
+    divs = {d for k in range(1, int(math.isqrt(P)) + 1) if P % k == 0 for d in (k, P // k)}
+    divs.discard(P)
+    x = y = 0
+    near = 1 << (b - 64)
+    for k in range(1, P + 1):
+        x, y = ((x * x - y * y) >> b) + cr, ((x * y) >> (b - 1)) + ci
+        if k in divs and abs(x) < near and abs(y) < near:
+            raise RuntimeError(f"period divides {k}")
+    digits = int(bits * math.log10(2)) + 4
+    return dict(seed=seed, period=P, angle=f"{q}/{P}", bits=bits, re=_dec(cr, b, digits),
+                im=_dec(ci, b, digits), log10_size=-2.0 * math.log10(P),
+                source=f"satellite of the main cardioid at internal angle {q}/{P}, Newton "
+                       f"from the cardioid point, refined by precision doubling")
 
 
 def _offset_centre(nuc, shift, width, seed, bits):
@@ -193,8 +240,11 @@ def scene(k, generate=False):
         if not generate:
             raise KeyError(f"scene {k} not in {CACHE}; run python -m fractodactyl_amd.scenes {k}")
         cfg = CONFIGS[k]
-        nuc = find_nucleus(k, cfg["pmin"], cfg["pmax"], cfg["bits"],
-                           max_log10_size=cfg.get("max_log10_size"), tries=4000)
+        if k == 5:
+            nuc = satellite_nucleus(k, cfg["pmin"], cfg["pmax"], cfg["bits"])
+        else:
+            nuc = find_nucleus(k, cfg["pmin"], cfg["pmax"], cfg["bits"],
+                               max_log10_size=cfg.get("max_log10_size"), tries=4000)
         re_, im_, th = _offset_centre(nuc, cfg["shift"], cfg["width"], k, cfg["bits"])
         cache[key] = dict(config=cfg, nucleus=nuc, center_re=re_, center_im=im_, theta=th)
         json.dump(cache, open(CACHE, "w"), indent=1)
