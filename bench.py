@@ -9,7 +9,8 @@ resident in HBM: the GPU BLA table build plus the persistent perturbation kernel
 2,073,600 samples.  value = plain-equivalent iterations (BLA-skipped steps included, the
 reference's `iterations.equivalent`) per second of wall time over the timed steps.  The
 FE32 and mixed forms run the same frame (with their agreement), and the 1e-100 frame, the
-escape-dominated frame and the 64-keyframe film run as companions.
+escape-dominated frame, the 64-keyframe film and the 1e-3000 jet-AA frame run as
+companions.
 
 Prints ONE JSON line.  --gpus above 1 exits: this path runs on one GPU here.
 """
@@ -85,7 +86,6 @@ def cpu_baseline(view, params, route_bla, target_seconds, threads, orbit=None, r
         e2e = st["render_seconds"] + st["reference_seconds"] + st["bla_seconds"]
         each.append(dict(run=k + 1, cold=k == 0, render_seconds=st["render_seconds"],
                          reference_seconds=st["reference_seconds"],
-                uncertified_samples=st.get("uncertified_samples", 0),
                          bla_seconds=st["bla_seconds"],
                          value=st["equivalent"] / st["render_seconds"],
                          end_to_end_value=st["equivalent"] / e2e))
@@ -416,6 +416,56 @@ def workload_name(k, sc, params):
             f"max_iter {params.max_iter}, BLA eps 2^-50, FP64 deltas, nu/de/normal")
 
 
+def measure_config5(fa, sc, cols, steps, warmup):
+    """The 1e-3000 jet-AA frame at full size: 3840x2160 pixels at 2x2 samples (7680x4320
+    samples), max_iter 1e7, BLA, dz/dc, through fdg_frame_jet (pass 1 on each pixel's
+    top-left sample, the host rebuild, pass 2 on the rejected samples, merge) with the
+    frame's 2^20-point orbit (~10,100 bits) and BLA table resident in HBM.  The host
+    reference orbit is timed apart (frame open).  Every sample of this frame stays bounded:
+    it rebases onto the orbit's exact f64 zero at the nucleus period and restarts from dc,
+    so it is Unresolved at max_iter with 1e7 equivalent iterations, nearly all BLA-skipped."""
+    c = sc["config"]
+    v = fa.View(sc["center_re"], sc["center_im"], c["width"], 0.0)
+    p = fa.Params(c["nx"], c["ny"], c["ss"], c["max_iter"], c["escape_radius"], cols)
+    t = time.perf_counter()
+    f = fa.ResidentFrame(v, p, fa.ROUTE_BLA, fa.FORM_F64, fa.EPS)
+    open_s = time.perf_counter() - t
+    log(f"config5: frame open (orbit + upload) {open_s:.2f}s")
+    for i in range(warmup):
+        _, _, st, done = f.jet()
+        log(f"config5 warmup {i}: kernel {st['kernel_seconds'] * 1e3:.1f} ms, {done} samples")
+    walls, stats = [], []
+    for i in range(steps):
+        t = time.perf_counter()
+        h, out, st, done = f.jet()
+        walls.append(time.perf_counter() - t)
+        stats.append(st)
+    for i, (w, st) in enumerate(zip(walls, stats)):
+        log(f"config5 step {i}: jet frame {w * 1e3:.1f} ms, kernel "
+            f"{st['kernel_seconds'] * 1e3:.1f} ms")
+    f.close()
+    st = stats[-1]
+    wall = sum(walls) / len(walls)
+    cls = out["class"] & 3
+    return dict(
+        workload=(f"config5: {c['nx'] // 2}x{c['ny'] // 2} pixels at 2x2 samples, width "
+                  f"{c['width']} at a seeded satellite nucleus of period "
+                  f"{sc['nucleus']['period']} ({sc['nucleus']['bits']} bits), max_iter "
+                  f"{c['max_iter']}, BLA eps 2^-50, scaled tier with dz/dc, jet-AA frame "
+                  f"(fdg_frame_jet)"),
+        ms_per_frame=wall * 1e3, value=st["equivalent"] / wall, unit="pixel-iter/s",
+        kernel_ms=st["kernel_seconds"] * 1e3, table_ms=st["bla_seconds"] * 1e3,
+        host_rebuild_and_encode_ms=st["encode_seconds"] * 1e3,
+        transfer_ms=st["transfer_seconds"] * 1e3,
+        samples_computed=done, samples=c["nx"] * c["ny"],
+        equivalent_iterations=st["equivalent"], plain_steps=st["plain_steps"],
+        blocks=st["blocks"], skipped=st["skipped"], kernel_id=h["kernel"],
+        reference_len=st["reference_len"], reference_bits=st["reference_bits"],
+        reference_seconds=st["reference_seconds"], frame_open_seconds=open_s,
+        classes={k: float((cls == val).mean()) for k, val in (("escaped", 0), ("interior", 1),
+                                                              ("unresolved", 2))})
+
+
 def fe32_roofline(m, forms):
     """The FE32 form's kernel against the FP32 vector peak (same algorithmic flops)."""
     r = m["roofline"]
@@ -440,11 +490,12 @@ def main():
                          "target) or 2 (1e-100)")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--frames", default="config,other,escape,film",
+    ap.add_argument("--frames", default="config,other,escape,film,config5",
                     help="which frames to run: config (the JSON line); companions other "
                          "(the other of configs 2 and 3), escape (reference zoom path at "
-                         "2e-49), film (64 keyframes sharing one orbit); series (the "
-                         "reference's 24 series-study frames, companion mode only)")
+                         "2e-49), film (64 keyframes sharing one orbit), config5 (the "
+                         "1e-3000 jet-AA frame); series (the reference's 24 "
+                         "series-study frames, companion mode only)")
     ap.add_argument("--forms", action="store_true",
                     help="with companions only: also run the FE32 and mixed forms")
     ap.add_argument("--copy-probe", action="store_true",
@@ -495,6 +546,8 @@ def main():
             log(json.dumps(measure_film(fa, get_scene(4), cols, a.steps, a.warmup, 1, 0.0)))
         if "series" in frames:
             log(json.dumps(measure_series(fa, cols)))
+        if "config5" in frames:
+            log(json.dumps(measure_config5(fa, get_scene(5), cols, a.steps, a.warmup)))
         return
     keep = []
     m = measure(fa, view, params, route, a.steps, a.warmup, f"config{a.config}", keep=keep)
@@ -526,6 +579,8 @@ def main():
     if "film" in frames:
         comp["film"] = measure_film(fa, get_scene(4), cols, a.steps, a.warmup, threads,
                                     cpu_s / 2)
+    if "config5" in frames:
+        comp["config5_frame"] = measure_config5(fa, get_scene(5), cols, min(a.steps, 3), 1)
     out = {
         "metric": "effective pixel-iter/s at 1080p deep zoom (BLA-skipped steps included)",
         "value": m["value"],
@@ -539,7 +594,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (scene fractodactyl_amd/scenes.json: the reference's ladder nucleus "
-                "for config 3, seeded Newton nuclei for configs 2 and 4)",
+                "for configs 3 and 4, seeded Newton nuclei for configs 1, 2 and 5)",
         "config": {
             "workload": workload_name(a.config, sc, params),
             **{k: v for k, v in m.items() if k not in ("value", "ms_per_step", "roofline")},

@@ -1291,6 +1291,7 @@ int fdg_frame_jet(fdg_frame* f, fdg_columns* out, fdg_stats* st, char* kernel_id
             s.plain_steps += s2.plain_steps;
             s.fallback_samples += s2.fallback_samples;
             s.closed_form_samples += s2.closed_form_samples;
+            s.uncertified_samples += s2.uncertified_samples;
             s.max_sample_ops = std::max(s.max_sample_ops, s2.max_sample_ops);
             s.wave_iterations += s2.wave_iterations;
             s.lane_iterations += s2.lane_iterations;

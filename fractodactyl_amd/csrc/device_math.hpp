@@ -28,6 +28,24 @@ __device__ inline double exp2i32(int32_t k) { return ldexp(1.0, k); }
 // adjacent exponents is chosen.
 __device__ inline int64_t floor_log2(double a) { return (int64_t)floor(log2(a)); }
 
+// Rust's `x as i64` for a float: saturating, and 0 for NaN, where a C++ cast is undefined.
+// sum2 (sample.rs:1097) takes a NaN term's exponent this way: the term then sits at its
+// own exponent and drops out of the sum when far below the other term.
+__device__ inline int64_t rust_as_i64(double x) {
+    if (__builtin_isnan(x)) return 0;
+    if (x >= 0x1p63) return INT64_MAX;
+    if (x < -0x1p63) return INT64_MIN;
+    return (int64_t)x;
+}
+
+// i64 + and - as the reference's release build computes them (wrapping, no overflow checks)
+__device__ inline int64_t wrap_add(int64_t a, int64_t b) {
+    return (int64_t)((uint64_t)a + (uint64_t)b);
+}
+__device__ inline int64_t wrap_sub(int64_t a, int64_t b) {
+    return (int64_t)((uint64_t)a - (uint64_t)b);
+}
+
 // The same value without the libm call on all but a sliver of inputs.  a = f 2^e with f in
 // [0.5, 1) (subnormals included) has floor(log2(a)) = e - 1 exactly; the rounded log2 can
 // only differ where log2(a) = e + log2(f) lies within an ulp of the integer e, i.e. for f

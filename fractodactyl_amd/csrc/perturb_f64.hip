@@ -326,13 +326,14 @@ __device__ inline void sum2(double t0r, double t0i, int64_t t0e, double t1r, dou
                             int64_t t1e, int64_t zero_e, double& ore, double& oim, int32_t& oe) {
     bool have = false;
     int64_t top = 0;
+    // (a NaN term passes the != 0 test and takes exponent t.e, as in the reference)
     double a0 = fmax(fabs(t0r), fabs(t0i)), a1 = fmax(fabs(t1r), fabs(t1i));
     if (a0 != 0.0) {
-        top = floor_log2(a0) + t0e;
+        top = wrap_add(rust_as_i64(floor(log2(a0))), t0e);
         have = true;
     }
     if (a1 != 0.0) {
-        int64_t g = floor_log2(a1) + t1e;
+        int64_t g = wrap_add(rust_as_i64(floor(log2(a1))), t1e);
         if (!have || g > top) top = g;
         have = true;
     }
@@ -342,7 +343,7 @@ __device__ inline void sum2(double t0r, double t0i, int64_t t0e, double t1r, dou
         oe = (int32_t)zero_e;
         return;
     }
-    int64_t k0 = t0e - top, k1 = t1e - top;
+    int64_t k0 = wrap_sub(t0e, top), k1 = wrap_sub(t1e, top);
     double r = 0.0, i = 0.0;
     r = r + (k0 < -2046 ? 0.0 : t0r * exp2i(k0 / 2) * exp2i(k0 - k0 / 2));
     i = i + (k0 < -2046 ? 0.0 : t0i * exp2i(k0 / 2) * exp2i(k0 - k0 / 2));

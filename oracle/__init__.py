@@ -92,6 +92,9 @@ def lib():
         L.fdo_reference_fixed.restype = C.c_int64
         L.fdo_reference_fixed.argtypes = [C.c_char_p, C.c_char_p, C.c_uint64, C.c_uint64,
                                           C.c_void_p, C.c_void_p, C.c_uint64]
+        L.fdo_zero_step.restype = None
+        L.fdo_zero_step.argtypes = [C.c_double, C.c_double, C.c_int64, C.c_double, C.c_double,
+                                    C.c_int64, C.c_void_p, C.POINTER(C.c_int64)]
         L.fdo_nucleus_period.restype = C.c_uint64
         L.fdo_nucleus_period.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
         L.fdo_bla_build.restype = C.c_int
@@ -297,6 +300,16 @@ def guarded_build(zr, zi, q, eps, dc_max):
         levels.append(out[off:off + c].copy())
         off += c
     return levels
+
+
+def zero_step(w, a):
+    """sample.rs:1119-1122 zero_step: the scaled step from Z_0 = 0, z' = w^2 2^2e + a 2^ae,
+    for w = (re, im, e) and a = (re, im, e); returns (re, im, e)."""
+    out = np.zeros(2)
+    oe = C.c_int64()
+    lib().fdo_zero_step(float(w[0]), float(w[1]), int(w[2]), float(a[0]), float(a[1]),
+                        int(a[2]), _p(out), C.byref(oe))
+    return float(out[0]), float(out[1]), oe.value
 
 
 def external_escape_ok(n, z, d, dexp, ez, ed, px_m, px_e):

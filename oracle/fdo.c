@@ -61,6 +61,21 @@ double fdo_exp2i(int64_t k) {
 }
 #define exp2i fdo_exp2i
 
+/* Rust's `x as i64` for a float: saturating, NaN to 0 (C's cast is undefined there).  It
+ * matters where a term is NaN: sum2 (sample.rs:1097) then takes the term's own exponent,
+ * and a NaN term far below the other one drops out of the sum. */
+static inline int64_t rs_i64(double x) {
+    if (x != x) return 0;
+    if (x >= 9223372036854775808.0) return INT64_MAX;
+    if (x < -9223372036854775808.0) return INT64_MIN;
+    return (int64_t)x;
+}
+
+/* i64 + and - as the reference's release build computes them (wrapping: the workspace's
+ * release profile leaves overflow checks off, Cargo.toml:11-14) */
+static inline int64_t wadd(int64_t a, int64_t b) { return (int64_t)((uint64_t)a + (uint64_t)b); }
+static inline int64_t wsub(int64_t a, int64_t b) { return (int64_t)((uint64_t)a - (uint64_t)b); }
+
 static int64_t f64_exponent(double v) {
     uint64_t b;
     memcpy(&b, &v, 8);
@@ -814,25 +829,26 @@ static inline void con_multiply(contraction *s, double re, double im, int64_t ex
         s->re = 1.0;
         s->started = 1;
     }
+    /* (a non-finite a reaches the casts too, as in the reference: NaN gives e = 0) */
     double a = fmax(fabs(re), fabs(im));
     if (a != 0.0 && !(LO <= a && a <= HI)) {
-        int64_t e = (int64_t)floor(log2(a));
-        double scale = exp2i(-e);
+        int64_t e = rs_i64(floor(log2(a)));
+        double scale = exp2i(wsub(0, e));
         re *= scale;
         im *= scale;
-        exp += e;
+        exp = wadd(exp, e);
     }
     double nr = s->re * re - s->im * im, ni = s->re * im + s->im * re;
     s->re = nr;
     s->im = ni;
-    s->exp += exp;
+    s->exp = wadd(s->exp, exp);
     a = fmax(fabs(s->re), fabs(s->im));
     if (a != 0.0 && !(LO <= a && a <= HI)) {
-        int64_t e = (int64_t)floor(log2(a));
-        double scale = exp2i(-e);
+        int64_t e = rs_i64(floor(log2(a)));
+        double scale = exp2i(wsub(0, e));
         s->re *= scale;
         s->im *= scale;
-        s->exp += e;
+        s->exp = wadd(s->exp, e);
     }
 }
 static inline int con_small(const contraction *s) {
@@ -866,8 +882,8 @@ static void sum2(const double t[2][2], const int64_t te[2], int64_t zero_e, doub
     int64_t top = 0;
     for (int i = 0; i < 2; i++) {
         double a = fmax(fabs(t[i][0]), fabs(t[i][1]));
-        if (a != 0.0) {
-            int64_t g = (int64_t)floor(log2(a)) + te[i];
+        if (a != 0.0) { /* (NaN included: its exponent is then te[i] alone) */
+            int64_t g = wadd(rs_i64(floor(log2(a))), te[i]);
             if (!have || g > top) top = g;
             have = 1;
         }
@@ -880,7 +896,7 @@ static void sum2(const double t[2][2], const int64_t te[2], int64_t zero_e, doub
     }
     double r = 0.0, im = 0.0;
     for (int i = 0; i < 2; i++) {
-        int64_t k = te[i] - top;
+        int64_t k = wsub(te[i], top);
         double a = k < -2046 ? 0.0 : t[i][0] * exp2i(k / 2) * exp2i(k - k / 2);
         double b = k < -2046 ? 0.0 : t[i][1] * exp2i(k / 2) * exp2i(k - k / 2);
         r = r + a;
@@ -895,15 +911,21 @@ static void sum2(const double t[2][2], const int64_t te[2], int64_t zero_e, doub
 static void zero_step(double wr, double wi, int64_t e, double ar, double ai, int64_t ae,
                       double *ore, double *oim, int64_t *oe) {
     double t[2][2] = {{wr * wr - wi * wi, 2.0 * wr * wi}, {ar, ai}};
-    int64_t te[2] = {2 * e, ae};
+    int64_t te[2] = {wadd(e, e), ae};
     sum2(t, te, e, ore, oim, oe);
+}
+
+/* zero_step on explicit operands, for the tests: out = {re, im}, *oe = exponent */
+void fdo_zero_step(double wr, double wi, int64_t e, double ar, double ai, int64_t ae,
+                   double *out, int64_t *oe) {
+    zero_step(wr, wi, e, ar, ai, ae, &out[0], &out[1], oe);
 }
 
 /* sample.rs:1124-1134 zero_step_dz */
 static void zero_step_dz(double wr, double wi, int64_t we, double vr, double vi, int64_t ve,
                          double *ore, double *oim, int64_t *oe) {
     double t[2][2] = {{2.0 * (wr * vr - wi * vi), 2.0 * (wr * vi + wi * vr)}, {1.0, 0.0}};
-    int64_t te[2] = {we + ve, 0};
+    int64_t te[2] = {wadd(we, ve), 0};
     sum2(t, te, ve, ore, oim, oe);
 }
 

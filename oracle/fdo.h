@@ -187,4 +187,8 @@ int fdo_guarded_build(const double *re, const double *im, const double *q, uint6
 int fdo_external_escape_ok(uint64_t n, double zr, double zi, double dr, double di, int64_t dexp,
                            double ez, double ed, double px_m, int64_t px_e);
 
+/* sample.rs:1119-1122 zero_step on explicit operands (test hook) */
+void fdo_zero_step(double wr, double wi, int64_t e, double ar, double ai, int64_t ae,
+                   double *out, int64_t *oe);
+
 #endif

@@ -390,3 +390,42 @@ def test_truth_pack(name):
         expected = sum(1 for e in edges if e[0] == name)
         assert s["class_mismatches"] == s["known_edges"] == expected, (bla, s["mismatches"])
         assert s["scalar_failures"] == 0, (bla, s)
+
+
+# The reference's float-to-integer casts are Rust's `as`: saturating, NaN to 0.  sum2
+# (sample.rs:1094-1114) meets a NaN term when the scaled tier rebases onto an orbit point
+# that is exactly zero in f64 (a nucleus deeper than 2^-1074): the zero term scaled by
+# 2^k (k > 1023) is 0 x inf.  The step from Z_0 that follows then drops the NaN square,
+# which sits 2e binary orders down, and the sample restarts from dc.
+def test_zero_step_drops_a_nan_square_far_below_dc():
+    assert oracle.zero_step((math.nan, math.nan, -9945), (3.4, 2.3, -9971)) == (1.7, 1.15, -9970)
+    # a NaN square within 2046 binary orders of dc stays in the sum
+    re, im, e = oracle.zero_step((math.nan, math.nan, -10), (3.4, 2.3, -12))
+    assert math.isnan(re) and math.isnan(im) and e == -11
+    # and a finite one is the plain two-term sum at dc's exponent
+    assert oracle.zero_step((1.5, 0.5, -10), (3.4, 2.3, -12)) == (1.70390625, 1.1529296875, -11)
+
+
+def test_a_rebase_onto_an_exact_zero_of_the_orbit_restarts_from_dc():
+    # period-39 satellite nucleus at 4300 bits, frame width 1e-1200: Z_39 is 0 in f64, so
+    # every sample rebases there (|Z + delta| <= |delta| with both terms underflowing) and,
+    # its state being (m = 1, delta = dc) again, repeats its first period, BLA blocks
+    # included; a NaN delta would stop the blocks after the first period
+    from fractodactyl_amd.scenes import satellite_nucleus
+    nuc = satellite_nucleus(7, 20, 40, 4300)
+    P = nuc["period"]
+    assert P == 39
+    zr, zi, bits = oracle.reference(nuc["re"], nuc["im"], "1e-1200", 0.0, 16, 9, 40 * P, 1e10)
+    assert zr[P] == 0.0 and zi[P] == 0.0 and oracle.nucleus_period(zr, zi) == 0
+    pl = oracle.plane(nuc["re"], nuc["im"], "1e-1200", 0.0, 16, 9)
+    lv = oracle.bla_build(zr, zi, oracle.EPS_EXACT, pl["h_m"] * math.hypot(8, 4.5), pl["h_e"])
+    pk = oracle.PackedLevels(lv)
+    ks = np.array([1, 2, 3, 5, 9, 17, 33])
+    skipped = []
+    for k in ks:
+        s = oracle.sample(zr, zi, 3.0 * pl["h_m"], 2.0 * pl["h_m"], int(k) * P + 1, 1e20,
+                          extended=True, offset_exp=pl["h_e"], bla_levels=pk, nucleus=0)
+        assert s["kind"] == oracle.UNRESOLVED
+        skipped.append(s["skipped"])
+    per_period = np.diff(skipped) / np.diff(ks)
+    assert skipped[0] > 0 and (per_period >= skipped[0]).all(), skipped
