@@ -102,13 +102,23 @@ def cpu_baseline(view, params, route_bla, target_seconds, threads, orbit=None, r
                        f"threads of {allowed} allowed / {online} online cores")
 
 
-def measure(fa, view, params, route, steps, warmup, label, form=None, keep=None):
-    """Open the frame (orbit resident in HBM), warm up, time `steps` passes.  keep: a
-    list that receives the frame's columns and raw outcomes."""
+def measure(fa, view, params, route, steps, warmup, label, form=None, keep=None, tune=True):
+    """Open the frame (orbit resident in HBM), autotune its launch (untimed), warm up, time
+    `steps` passes.  keep: a list that receives the frame's columns and raw outcomes."""
     form = fa.FORM_F64 if form is None else form
     t = time.perf_counter()
     frame = fa.ResidentFrame(view, params, route, form, fa.EPS)
     log(f"{label}: frame open (host reference orbit + upload) {time.perf_counter() - t:.2f}s")
+    launch = None
+    if tune:
+        t = time.perf_counter()
+        chosen, tuned_ms, default_ms = frame.tune(reps=2)
+        launch = dict(chosen=chosen, tuned_kernel_ms=tuned_ms, default_kernel_ms=default_ms,
+                      tuning_seconds=time.perf_counter() - t,
+                      candidates="groups/CU {4,8,16} x claim divisor {2,4,8} x claim cap "
+                                 "{256,1024,4096}, plus the default (8, 4, 1024); min of 2 "
+                                 "runs each")
+        log(f"{label}: tuned launch {chosen} {tuned_ms:.3f} ms (default {default_ms:.3f} ms)")
     for i in range(warmup):
         st = frame.run()
         log(f"{label} warmup {i}: kernel {st['kernel_seconds'] * 1e3:.2f} ms, table "
@@ -150,7 +160,7 @@ def measure(fa, view, params, route, steps, warmup, label, form=None, keep=None)
                 wave_iterations=st["wave_iterations"],
                 lane_utilisation=st["lane_iterations"] / max(1, 64 * st["wave_iterations"]),
                 reference_seconds=st["reference_seconds"],
-                uncertified_samples=st.get("uncertified_samples", 0),
+                uncertified_samples=st.get("uncertified_samples", 0), launch=launch,
                 roofline={"bound": "valu-fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                           "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
                           "traffic": None, "flops_per_launch": flops,
@@ -226,10 +236,51 @@ def agreement(ref, other):
     return out
 
 
-def measure_forms(fa, view, params, route, steps, warmup, label, ref_metrics, ref_cols):
+_DIRECT = None
+
+
+def vs_direct(k, cols):
+    """A frame's outcomes at the 256 seeded pixels of tests/golden/direct_hp.json (direct
+    high-precision iteration of z^2 + c) against that fixture, each miss as the displacement
+    of c in pixels that explains it (the reference's tools/oracle.py rule): class misses by
+    the escaped side's distance estimate, smooth-count misses by |d nu| de ln2 / 2.  None
+    when the fixture has no entry for frame k."""
+    global _DIRECT
+    import numpy as np
+    if _DIRECT is None:
+        path = os.path.join(ROOT, "tests", "golden", "direct_hp.json")
+        _DIRECT = json.load(open(path)) if os.path.exists(path) else {}
+    d = _DIRECT.get(str(k))
+    if d is None:
+        return None
+    idx = np.array(d["j"]) * d["nx"] + np.array(d["i"])
+    tn = np.array(d["n"], np.int64)
+    ok = np.array(d["converged"])
+    tz2 = np.array(d["zr"]) ** 2 + np.array(d["zi"]) ** 2
+    tde = np.array([2.0 ** x if x is not None else np.nan for x in d["log2_de_px"]])
+    kind, n = cols["raw"]["kind"][idx], cols["raw"]["n"][idx].astype(np.int64)
+    nu, de = cols["nu"][idx], cols["de"][idx].astype(np.float64)
+    esc_t, esc_r = tn >= 0, kind == 0
+    both, miss = ok & esc_t & esc_r, ok & (esc_t != esc_r)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        tnu = tn + 1.0 - np.log2(0.5 * np.log2(tz2))
+    dnu = np.abs(nu - tnu)[both]
+    px = dnu * tde[both] * math.log(2.0) / 2.0
+    return dict(pixels=int(ok.sum()), escaped=int((ok & esc_t).sum()),
+                class_mismatch=int(miss.sum()),
+                class_px_err=float(np.where(esc_t, tde, de)[miss].max()) if miss.any() else 0.0,
+                nu_px_err=float(px.max()) if both.any() else 0.0,
+                escape_count_equal=float((n[both] == tn[both]).mean()) if both.any() else 1.0,
+                smooth_count_within_1e_3=float((dnu <= 1e-3).mean()) if both.any() else 1.0,
+                bar="class and smooth-count misses within 1e-3 pixel of displacement")
+
+
+def measure_forms(fa, view, params, route, steps, warmup, label, ref_metrics, ref_cols,
+                  direct_k=None):
     """The FP32 floatexp form and the mixed form (FP32 pass + FP64 recompute of its escaped
-    samples) on the same frame: speed against the FP64 form and per-sample agreement with
-    it (the FP64 form equals the oracle bit for bit)."""
+    samples) on the same frame: speed against the FP64 form, per-sample agreement with
+    it (the FP64 form equals the oracle bit for bit), and agreement with direct
+    high-precision iteration at the fixture's pixels of frame direct_k."""
     out = {}
     for form, name in ((fa.FORM_FE32, "fe32"), (fa.FORM_MIXED, "mixed")):
         keep = []
@@ -239,6 +290,10 @@ def measure_forms(fa, view, params, route, steps, warmup, label, ref_metrics, re
                      "speedup_over_fp64_kernel": ref_metrics["kernel_ms"] / m["kernel_ms"],
                      "uncertified_samples": m["uncertified_samples"],
                      "agreement_with_fp64": agreement(ref_cols, keep[0])}
+        if direct_k is not None:
+            out[name]["vs_direct_high_precision"] = vs_direct(direct_k, keep[0])
+    if direct_k is not None:
+        out["fp64_vs_direct_high_precision"] = vs_direct(direct_k, ref_cols)
     return out
 
 
@@ -261,14 +316,21 @@ def measure_film(fa, sc, cols, steps, warmup, threads, cpu_seconds):
         film = fa.Film(views, params, tables=tables)
         open_s = time.perf_counter() - t
         res = {"open_seconds": open_s}
+        # the persistent grid of the batch launch, autotuned (untimed)
+        grid, grid_ms, default_ms = film.tune()
+        res["batch_launch_tuning"] = dict(grid=grid or "default (8 per CU)",
+                                          tuned_kernel_ms=grid_ms,
+                                          default_kernel_ms=default_ms,
+                                          candidates="grids 1024, 2048, 4096 workgroups")
         for launch, lname in ((fa.LAUNCH_BATCH, "one_graph_batch_launch"),
                               (fa.LAUNCH_PER_FRAME, "launch_per_frame")):
+            g = grid if launch == fa.LAUNCH_BATCH else 0
             for i in range(warmup):
-                st = film.run(launch)
+                st = film.run(launch, g)
             walls, kerns = [], []
             for i in range(steps):
                 t0 = time.perf_counter()
-                st = film.run(launch)
+                st = film.run(launch, g)
                 walls.append(time.perf_counter() - t0)
                 kerns.append(st["kernel_seconds"])
                 log(f"film {tname} {lname} step {i}: kernel {kerns[-1] * 1e3:.2f} ms")
@@ -552,7 +614,7 @@ def main():
     keep = []
     m = measure(fa, view, params, route, a.steps, a.warmup, f"config{a.config}", keep=keep)
     forms = measure_forms(fa, view, params, route, a.steps, a.warmup, f"config{a.config}", m,
-                          keep[0])
+                          keep[0], direct_k=a.config)
     comp = {}
     if "other" in frames:
         so = get_scene(other)
@@ -560,7 +622,7 @@ def main():
         keepo = []
         mo = measure(fa, vo, po, fa.ROUTE_BLA, a.steps, a.warmup, f"config{other}", keep=keepo)
         mo["precision_forms"] = measure_forms(fa, vo, po, fa.ROUTE_BLA, a.steps, a.warmup,
-                                              f"config{other}", mo, keepo[0])
+                                              f"config{other}", mo, keepo[0], direct_k=other)
         mo["roofline"]["traffic"] = measured_traffic(other)
         mo["workload"] = workload_name(other, so, po)
         if cpu_s > 0:

@@ -89,6 +89,11 @@ class _EscapedState(C.Structure):
                 ("di", C.c_double), ("dexp", C.c_int64)]
 
 
+class _Launch(C.Structure):
+    _fields_ = [("groups_per_cu", C.c_uint32), ("claim_div", C.c_uint32),
+                ("claim_cap", C.c_uint32)]
+
+
 class _Stats(C.Structure):
     _fields_ = [("reference_len", C.c_uint64), ("reference_bits", C.c_uint32),
                 ("bla_levels", C.c_uint32), ("nucleus_period", C.c_uint64),
@@ -143,6 +148,9 @@ SIGNATURES = {
     "fdg_frame_read_columns": (C.c_int, [C.c_void_p, C.POINTER(_Columns), C.POINTER(_Stats),
                                          C.c_char_p, C.c_size_t]),
     "fdg_frame_mask": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fdg_frame_launch": (C.c_int, [C.c_void_p, C.POINTER(_Launch)]),
+    "fdg_frame_tune": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(_Launch),
+                                 C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "fdg_render_masked": (C.c_int, [C.c_void_p, C.POINTER(_View), C.POINTER(_Params), C.c_int,
                                     C.c_int, C.c_double, C.c_void_p, C.POINTER(_Columns),
                                     C.POINTER(_Outcomes), C.POINTER(_Stats), C.c_char_p,
@@ -394,6 +402,20 @@ class ResidentFrame:
             raise FdgError(-1, "sample mask must cover the sample grid")
         self.ctx._check(lib().fdg_frame_mask(self._h, _p(self._mask)))
 
+    def launch(self, groups_per_cu=0, claim_div=0, claim_cap=0):
+        """Launch tuning of the following runs (fdg_frame_launch; 0 = the default)."""
+        self.ctx._check(lib().fdg_frame_launch(self._h, C.byref(_Launch(groups_per_cu,
+                                                                        claim_div, claim_cap))))
+
+    def tune(self, reps=2):
+        """The autotuner (fdg_frame_tune): times each candidate launch over `reps` runs and
+        keeps the fastest for this frame and later frames of its shape.  Returns
+        (dict of the chosen launch, its kernel ms, the default launch's kernel ms)."""
+        best, bms, dms = _Launch(), C.c_double(), C.c_double()
+        self.ctx._check(lib().fdg_frame_tune(self._h, reps, C.byref(best), C.byref(bms),
+                                             C.byref(dms)))
+        return ({k: getattr(best, k) for k, _ in _Launch._fields_}, bms.value, dms.value)
+
     def series(self, series):
         """Start every sample of the following runs from `series` (a Series; None clears
         it): Series::render_masked (crates/fd-kernel/src/series.rs:342-374)."""
@@ -497,6 +519,21 @@ class Film:
         st = _Stats()
         self.ctx._check(lib().fdg_film_run(self._h, launch, grid, C.byref(st)))
         return {k: getattr(st, k) for k, _ in _Stats._fields_}
+
+    def tune(self, grids=(1024, 2048, 4096), reps=2, launch=LAUNCH_BATCH):
+        """The film's autotuner: times `reps` runs under each persistent grid size (in
+        workgroups; 0 is the default, 8 per CU) by the kernel time and returns (the fastest
+        grid, its kernel ms, the default grid's kernel ms).  A candidate replaces the
+        default only when 1% faster.  Outputs do not depend on the grid."""
+        def best_of(g):
+            return min(self.run(launch, g)["kernel_seconds"] for _ in range(reps)) * 1e3
+        default = best_of(0)
+        win, win_ms = 0, default
+        for g in grids:
+            ms = best_of(g)
+            if ms < (0.99 * win_ms if win == 0 else win_ms):
+                win, win_ms = g, ms
+        return win, win_ms, default
 
     def mask(self, masks):
         """Compute only the set samples in the following runs (fdg_film_mask): masks is

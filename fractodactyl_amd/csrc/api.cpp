@@ -317,9 +317,34 @@ struct fdg_frame {
     double series_inv_rho = 0;
     DevBuf series_b;      // its coefficients, (re, im) pairs
     uint64_t run_max_iter = 0;  // the series probe's budget for a run (0: params.max_iter)
+    fdg_launch launch{};        // launch tuning (0 fields: the defaults)
     fdg_stats last;
     Buffers buf;
 };
+
+namespace {
+
+// The shape a tuned launch is kept under: form, tier, derivative, BLA and the
+// power-of-two buckets of the sample count and max_iter.
+std::string tune_key(const fdg_frame* f) {
+    auto lg = [](uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; };
+    bool deriv = (f->params.columns & (FDG_COL_DE | FDG_COL_NORMAL)) != 0;
+    return std::to_string(f->form) + "/" + std::to_string(f->fr.tier) + "/" +
+           std::to_string(deriv) + "/" + std::to_string(f->use_bla) + "/" +
+           std::to_string(lg(f->count)) + "/" + std::to_string(lg(f->params.max_iter));
+}
+
+int check_launch(fdg_ctx* ctx, const fdg_launch& l) {
+    if (l.groups_per_cu > 32) return ctx->fail(FDG_EINVAL, "groups_per_cu must be in [1, 32]");
+    if (l.claim_div > 64) return ctx->fail(FDG_EINVAL, "claim_div must be in [1, 64]");
+    // (the 32-bit work counter overshoots the sample count, below 2^31, by at most one
+    // claim per wave after it is exhausted: 2^15 waves x 2^14 stays far from wrapping)
+    if (l.claim_cap != 0 && (l.claim_cap % 64 != 0 || l.claim_cap > 16384))
+        return ctx->fail(FDG_EINVAL, "claim_cap must be a multiple of 64 in [64, 16384]");
+    return FDG_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -630,6 +655,8 @@ static int frame_open(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int 
     f->orbit_im = std::move(o.im);
     if (rc) return rc;
     dc_bound(fr.pl, fr.tier, p->nx, p->ny, &f->dc, &f->dc_exp);
+    auto tuned = ctx->tuned.find(tune_key(f));
+    if (tuned != ctx->tuned.end()) f->launch = tuned->second;
     guard.f = nullptr;
     *out = f;
     return FDG_OK;
@@ -708,8 +735,11 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
             d.series_inv_rho = f->series_inv_rho;
         }
         bool deriv = (p->columns & (FDG_COL_DE | FDG_COL_NORMAL)) != 0;
-        int grid = (int)std::min<uint64_t>((f->count + 255) / 256, (uint64_t)ctx->cus * 8);
+        const uint32_t per_cu = f->launch.groups_per_cu ? f->launch.groups_per_cu : 8u;
+        int grid = (int)std::min<uint64_t>((f->count + 255) / 256, (uint64_t)ctx->cus * per_cu);
         if (grid < 1) grid = 1;
+        d.claim_div = f->launch.claim_div;
+        d.claim_cap = f->launch.claim_cap;
         static_assert(sizeof(FrameDev) <= 4096, "kernel descriptor slot");
         // counters: words 0-7 the work index, 8-23 the statistics (d.stats), word 24 (bytes
         // 192-195) the table's level count for the host
@@ -978,6 +1008,71 @@ int fdg_frame_mask(fdg_frame* f, const uint8_t* mask) {
         FDG_TRY(ctx, hipMemcpyAsync(f->mask.p, mask, f->count, hipMemcpyHostToDevice, ctx->stream),
                 "mask upload");
         FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "mask upload");
+        return FDG_OK;
+    });
+}
+
+int fdg_frame_launch(fdg_frame* f, const fdg_launch* launch) {
+    return guarded<int>(f && f->ctx ? &f->ctx->err : nullptr, [&]() -> int {
+        if (!f) return FDG_EINVAL;
+        fdg_launch l{};
+        if (launch) l = *launch;
+        int rc = check_launch(f->ctx, l);
+        if (rc) return rc;
+        f->launch = l;
+        return FDG_OK;
+    });
+}
+
+int fdg_frame_tune(fdg_frame* f, uint32_t reps, fdg_launch* best, double* best_ms,
+                   double* default_ms) {
+    return guarded<int>(f && f->ctx ? &f->ctx->err : nullptr, [&]() -> int {
+        if (!f) return FDG_EINVAL;
+        if (reps < 1 || reps > 16) return f->ctx->fail(FDG_EINVAL, "reps must be in [1, 16]");
+        const fdg_launch before = f->launch;
+        // minimum kernel time over the reps under launch l
+        auto time_of = [&](const fdg_launch& l, double* ms) -> int {
+            f->launch = l;
+            double m = 1e300;
+            for (uint32_t r = 0; r < reps; r++) {
+                fdg_stats st;
+                int rc = fdg_frame_run(f, &st);
+                if (rc) return rc;
+                m = std::min(m, st.kernel_seconds * 1e3);
+            }
+            *ms = m;
+            return FDG_OK;
+        };
+        const fdg_launch dflt{};
+        double dms = 0.0;
+        int rc = time_of(dflt, &dms);
+        fdg_launch win = dflt;
+        double wms = dms;
+        static const uint32_t per_cu[] = {4, 8, 16}, divs[] = {2, 4, 8},
+                              caps[] = {256, 1024, 4096};
+        for (uint32_t pc : per_cu)
+            for (uint32_t dv : divs)
+                for (uint32_t cp : caps) {
+                    if (rc) break;
+                    const fdg_launch l{pc, dv, cp};
+                    double ms = 0.0;
+                    rc = time_of(l, &ms);
+                    // a candidate replaces the default only when it is 1% faster, so that
+                    // timing noise alone never moves the choice away from the default
+                    if (!rc && ms < (win.groups_per_cu == 0 ? 0.99 * wms : wms)) {
+                        win = l;
+                        wms = ms;
+                    }
+                }
+        if (rc) {
+            f->launch = before;
+            return rc;
+        }
+        f->launch = win;
+        f->ctx->tuned[tune_key(f)] = win;
+        if (best) *best = win;
+        if (best_ms) *best_ms = wms;
+        if (default_ms) *default_ms = dms;
         return FDG_OK;
     });
 }

@@ -5,6 +5,7 @@
 
 #include <chrono>
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -96,6 +97,9 @@ struct fdg_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::string err;
+    // the autotuner's choices by workload shape (fdg_frame_tune), applied to frames opened
+    // later with the same shape
+    std::map<std::string, fdg_launch> tuned;
     void* pinned = nullptr;  // grow-only page-locked staging for outcome readback
     size_t pinned_bytes = 0;
     // Page-locked slots for a frame pass's small readbacks (statistics and the table's level

@@ -100,6 +100,9 @@ struct FrameDev {
     // batches only (the single-frame kernels read the geometry above)
     const BatchFrameDev* frames;    // nframes entries
     uint32_t nframes, per_frame;    // count = nframes * per_frame
+    // claim policy (0 = default): a wave's claim is the work left / (claim_div x waves),
+    // at most claim_cap samples (a multiple of 64), at least its idle lanes
+    uint32_t claim_div, claim_cap;
 };
 
 // One frame pass's outcomes and the columns the device encoder writes from them

@@ -956,14 +956,17 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
     // Each wave claims samples in chunks, one global atomic per chunk rather than per
     // refill; the chunk shrinks with the work left (guided self-scheduling) so the last
     // chunks are small and the tail short.  [cb, ce) is the wave's unclaimed range.
-    // finished: lanes refilled so far (the first 64 included), i.e. samples done + 64
+    // finished: samples this wave has started (closed-form interiors included; a masked-out
+    // index or an exhausted claim starts none)
     uint32_t cb = 0, ce = 0, seen = 0, finished = 0;
     const uint32_t waves = gridDim.x * (kBlock / 64);
+    // claim policy (the launch's tuning; 0 = the defaults)
+    const uint32_t cdiv = fp->claim_div ? fp->claim_div : 4u;
+    const uint32_t ccap = fp->claim_cap ? fp->claim_cap : 1024u;
     for (;;) {
         uint64_t idle = __ballot(!active);
         if (idle && !drained) {
             uint32_t want = __popcll(idle);
-            if (!LIST) finished += want;
             uint32_t avail = ce - cb;
             uint32_t nb = 0, ne = 0;
             if (avail < want) {
@@ -972,11 +975,12 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
                 // than samples this wave finished) make the claims themselves the
                 // bottleneck: then the chunk is the work left per wave rather than a
                 // quarter of it.  (The mixed form's list holds escaped samples, long ones.)
-                uint32_t div = !LIST && witers + 64 < finished ? 1u : 4u;
-                uint32_t size = max(min(left / (div * waves), 1024u), want - avail);
-                // (whole 64-sample groups keep the claimed ranges aligned, so no two
-                // waves write parts of one cache line of the 1-byte outcome kinds)
-                if (div == 1u) size = (size + 63u) & ~63u;
+                uint32_t div = !LIST && witers + 64 < finished ? 1u : cdiv;
+                uint32_t size = max(min(left / (div * waves), ccap), want - avail);
+                // Every claim is a whole number of 64-sample groups, so every claimed
+                // range starts on a group boundary: no two waves write parts of one
+                // 64-byte line of the 1-byte outcome kinds.
+                size = (size + 63u) & ~63u;
                 uint32_t leader = __ffsll((unsigned long long)idle) - 1;
                 if (lane == leader) nb = atomicAdd(next, size);
                 // readlane keeps the range wave-uniform (scalar registers)
@@ -994,11 +998,13 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
                 ce = ne;
                 if (nb >= count) drained = true;  // the global range is exhausted
             }
+            bool took = false;
             if (!active && have) {
                 // batches: the mask (when set) covers every frame's samples, frame-major
                 uint32_t bf = B ? my / fp->per_frame : 0u;
                 bool bmask = B && fp->mask && !fp->mask[fp->frames[bf].out0 + (my - bf * fp->per_frame)];
                 if (B && !bmask) {
+                    took = true;
                     uint32_t f = bf, loc = my - f * fp->per_frame;
                     const BatchFrameDev& g = fp->frames[f];
                     idx = g.out0 + loc;
@@ -1026,6 +1032,7 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
                         active = true;
                     }
                 } else if (!B && (!fp->mask || fp->mask[my])) {
+                    took = true;
                     idx = LIST ? fp->list[my] : my;
                     uint32_t i = idx % nx, j = idx / nx;
                     double x = (double)i + 0.5 - fp->half_x;
@@ -1062,6 +1069,7 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
                     }
                 }
             }
+            if (!LIST) finished += __popcll(__ballot(took));
         }
         if (!__ballot(active)) {
             if (drained) break;

@@ -515,14 +515,17 @@ __global__ void __launch_bounds__(kBlock32, 4) perturb_fe32(const FrameDev* __re
     bool active = false, drained = false;
     uint32_t idx = 0;
     unsigned long long witers = 0, liters = 0;
-    // finished: lanes refilled so far (the first 64 included), i.e. samples done + 64
+    // finished: samples this wave has started (closed-form interiors included; a masked-out
+    // index or an exhausted claim starts none)
     uint32_t cb = 0, ce = 0, seen = 0, finished = 0;
     const uint32_t waves = gridDim.x * (kBlock32 / 64);
+    // claim policy (the launch's tuning; 0 = the defaults)
+    const uint32_t cdiv = fp->claim_div ? fp->claim_div : 4u;
+    const uint32_t ccap = fp->claim_cap ? fp->claim_cap : 1024u;
     for (;;) {
         uint64_t idle = __ballot(!active);
         if (idle && !drained) {
             uint32_t want = __popcll(idle);
-            finished += want;
             uint32_t avail = ce - cb;
             uint32_t nb = 0, ne = 0;
             if (avail < want) {
@@ -531,11 +534,12 @@ __global__ void __launch_bounds__(kBlock32, 4) perturb_fe32(const FrameDev* __re
                 // than samples this wave finished) make the claims themselves the
                 // bottleneck: then the chunk is the work left per wave rather than a
                 // quarter of it.
-                uint32_t div = witers + 64 < finished ? 1u : 4u;
-                uint32_t size = max(min(left / (div * waves), 1024u), want - avail);
-                // (whole 64-sample groups keep the claimed ranges aligned, so no two
-                // waves write parts of one cache line of the 1-byte outcome kinds)
-                if (div == 1u) size = (size + 63u) & ~63u;
+                uint32_t div = witers + 64 < finished ? 1u : cdiv;
+                uint32_t size = max(min(left / (div * waves), ccap), want - avail);
+                // Every claim is a whole number of 64-sample groups, so every claimed
+                // range starts on a group boundary: no two waves write parts of one
+                // 64-byte line of the 1-byte outcome kinds.
+                size = (size + 63u) & ~63u;
                 uint32_t leader = __ffsll((unsigned long long)idle) - 1;
                 if (lane == leader) nb = atomicAdd(next, size);
                 nb = __builtin_amdgcn_readlane(nb, leader);
@@ -552,7 +556,8 @@ __global__ void __launch_bounds__(kBlock32, 4) perturb_fe32(const FrameDev* __re
                 ce = ne;
                 if (nb >= count) drained = true;
             }
-            if (!active && have && (!fp->mask || fp->mask[my])) {
+            bool took = !active && have && (!fp->mask || fp->mask[my]);
+            if (took) {
                 idx = my;
                 uint32_t i = idx % nx, j = idx / nx;
                 double x = (double)i + 0.5 - fp->half_x;
@@ -572,6 +577,7 @@ __global__ void __launch_bounds__(kBlock32, 4) perturb_fe32(const FrameDev* __re
                     active = true;
                 }
             }
+            finished += __popcll(__ballot(took));
         }
         if (!__ballot(active)) {
             if (drained) break;

@@ -179,6 +179,33 @@ int fdg_frame_read_columns(fdg_frame *frame, fdg_columns *out, fdg_stats *stats,
  * (crates/fd-cli/src/jetaa.rs:1-45) share one orbit and table upload; fdg_render_masked is
  * the one-call form. */
 int fdg_frame_mask(fdg_frame *frame, const uint8_t *mask);
+
+/* Launch tuning of a frame's persistent perturbation kernel.  No counterpart in the
+ * reference: its CPU renderer has no launch, and its wgpu route a fixed workgroup shape.
+ * Outputs and sample statistics do not depend on it (each sample is computed alone, in
+ * any order); only the wave statistics (wave_iterations, lane_iterations) do.  Every
+ * field 0 means its default. */
+typedef struct fdg_launch {
+    uint32_t groups_per_cu; /* persistent workgroups of 256 lanes per CU, 1-32; default 8 */
+    uint32_t claim_div;     /* a wave claims the work left / (claim_div x waves), 1-64;
+                               default 4 */
+    uint32_t claim_cap;     /* at most this many samples per claim, a multiple of 64 in
+                               [64, 16384]; default 1024 */
+} fdg_launch;
+
+/* Set the launch tuning of the frame's following runs (NULL: the defaults). */
+int fdg_frame_launch(fdg_frame *frame, const fdg_launch *launch);
+
+/* The autotuner: times `reps` (1-16) runs of the frame under each candidate launch
+ * (groups per CU 4, 8, 16 x claim divisor 2, 4, 8 x claim cap 256, 1024, 4096) by the
+ * perturbation kernel's event time, keeps the fastest (its minimum over the reps) for the
+ * frame's following runs and for frames the context opens later with the same shape
+ * (form, tier, derivative, BLA, and the power-of-two buckets of the sample count and
+ * max_iter), and reports it in `best` (nullable) with its time and the default launch's
+ * (best_ms, default_ms: minimum kernel milliseconds; nullable).  The frame's outputs after
+ * it are those of its last run. */
+int fdg_frame_tune(fdg_frame *frame, uint32_t reps, fdg_launch *best, double *best_ms,
+                   double *default_ms);
 int fdg_render_masked(fdg_ctx *ctx, const fdg_view *view, const fdg_params *params, int route,
                       int form, double bla_eps, const uint8_t *mask, fdg_columns *out,
                       fdg_outcomes *raw, fdg_stats *stats, char *kernel_id, size_t kernel_cap);

@@ -92,6 +92,9 @@ def lib():
         L.fdo_reference_fixed.restype = C.c_int64
         L.fdo_reference_fixed.argtypes = [C.c_char_p, C.c_char_p, C.c_uint64, C.c_uint64,
                                           C.c_void_p, C.c_void_p, C.c_uint64]
+        L.fdo_direct.restype = C.c_int64
+        L.fdo_direct.argtypes = [C.c_char_p, C.c_char_p, C.c_uint64, C.c_uint64, C.c_double,
+                                 C.c_void_p, C.POINTER(C.c_int64)]
         L.fdo_zero_step.restype = None
         L.fdo_zero_step.argtypes = [C.c_double, C.c_double, C.c_int64, C.c_double, C.c_double,
                                     C.c_int64, C.c_void_p, C.POINTER(C.c_int64)]
@@ -300,6 +303,18 @@ def guarded_build(zr, zi, q, eps, dc_max):
         levels.append(out[off:off + c].copy())
         off += c
     return levels
+
+
+def direct(re, im, bits, max_iter, r2):
+    """Direct iteration of z^2 + c at `bits` fraction bits (fdo_direct): (n, z_n, dz/dc at n
+    as (re, im, e), |z_{n-1}|^2, |z_n|^2); n = -1 when |z|^2 stays <= r2 through max_iter."""
+    out = np.zeros(6)
+    e = C.c_int64()
+    n = lib().fdo_direct(re.encode(), im.encode(), bits, max_iter, r2, _p(out), C.byref(e))
+    if n == -2:
+        raise ValueError("bad decimal")
+    return int(n), complex(out[0], out[1]), (float(out[2]), float(out[3]), e.value), \
+        float(out[4]), float(out[5])
 
 
 def zero_step(w, a):

@@ -565,6 +565,107 @@ int64_t fdo_reference_fixed(const char *re, const char *im, uint64_t bits, uint6
     return (int64_t)o.len;
 }
 
+/* dz' = 2 z dz + 1 with z = (ar + i ai) 2^ae and dz = (*dr + i *di) 2^*dx, the result kept
+ * as a mantissa in [2^-512, 2^512] (or 0) times 2^*dx */
+static void dz_step(double ar, double ai, int64_t ae, double *dr, double *di, int64_t *dx) {
+    double pr = 2.0 * (ar * *dr - ai * *di), pi = 2.0 * (ar * *di + ai * *dr);
+    int64_t pe = ae + *dx;
+    double a = fmax(fabs(pr), fabs(pi));
+    if (a == 0.0 || (double)pe + log2(a) < -600.0) {
+        /* the product is below 2^-600 of the added 1 */
+        *dr = 1.0 + (a == 0.0 ? 0.0 : pr * exp2i(pe));
+        *di = a == 0.0 ? 0.0 : pi * exp2i(pe);
+        *dx = 0;
+        return;
+    }
+    if (pe < -1000) {
+        /* (the product is at least 2^-600: its mantissa can take the difference) */
+        pr *= exp2i(pe + 1000);
+        pi *= exp2i(pe + 1000);
+        pe = -1000;
+    }
+    pr += exp2i(-pe); /* 0 once pe > 1074: the 1 is far below the product */
+    *dr = pr;
+    *di = pi;
+    *dx = pe;
+    a = fmax(fabs(*dr), fabs(*di));
+    while (a > 0x1p512) {
+        *dr *= 0x1p-512, *di *= 0x1p-512, *dx += 512, a *= 0x1p-512;
+    }
+    while (a != 0.0 && a < 0x1p-512) {
+        *dr *= 0x1p512, *di *= 0x1p512, *dx -= 512, a *= 0x1p512;
+    }
+}
+
+/* Direct iteration of z <- z^2 + c for the decimal c at `bits` fraction bits (the
+ * fd_fixed::orbit arithmetic above) with dz/dc carried alongside in scaled form (each z_k
+ * taken to 53 bits with its own exponent, so passes near 0 far below f64's range keep
+ * their weight), continued in f64 past |z|^2 > 2^32 up to |z|^2 > r2 (|z| >= 2^16 makes
+ * those few steps well conditioned).  Test infrastructure for the direct high-precision
+ * fixture (tools/oracle.py:56-67 iterates the same recurrences in mpmath).  Returns n, the
+ * first iterate with fl(|z_n|^2) > r2 (-1 when none through max_iter, -2 on a bad decimal);
+ * out[0..1] = z_n, out[2..3] = the mantissa of dz/dc at n with *de its exponent, out[4..5] =
+ * fl(|z_{n-1}|^2), fl(|z_n|^2). */
+int64_t fdo_direct(const char *re, const char *im, uint64_t bits, uint64_t max_iter, double r2,
+                   double *out, int64_t *de) {
+    size_t l = limbs_for(bits);
+    /* (prod holds a full 2l-limb product, as in fixed_orbit) */
+    uint64_t *buf = (uint64_t *)calloc(11 * l, sizeof(uint64_t));
+    uint64_t *cr = buf, *ci = buf + l, *x = buf + 2 * l, *y = buf + 3 * l, *x2 = buf + 4 * l,
+             *y2 = buf + 5 * l, *s = buf + 6 * l, *s2 = buf + 7 * l, *t = buf + 8 * l,
+             *prod = buf + 9 * l;
+    int crn, cin;
+    if (fixed_parse(re, l, &crn, cr) != 0 || fixed_parse(im, l, &cin, ci) != 0) {
+        free(buf);
+        return -2;
+    }
+    double c_re = fixed_to_f64(crn, cr, l), c_im = fixed_to_f64(cin, ci, l);
+    int xn = 0, yn = 0;
+    double dr = 0.0, di = 0.0, prev2 = 0.0, fx = 0.0, fy = 0.0;
+    int64_t dx = 0;
+    uint64_t n = 0;
+    int fixed = 1;
+    while (n < max_iter) {
+        if (fixed) {
+            /* dz/dc from z_n, then z_{n+1} */
+            double mx = 0.0, my = 0.0;
+            int64_t ex = INT64_MIN / 4, ey = INT64_MIN / 4;
+            int hx = fixed_frexp(xn, x, l, &mx, &ex), hy = fixed_frexp(yn, y, l, &my, &ey);
+            int64_t ez = hx || hy ? (ex > ey ? ex : ey) : 0;
+            double ar = hx ? mx * exp2i(ex - ez) : 0.0, ai = hy ? my * exp2i(ey - ez) : 0.0;
+            dz_step(ar, ai, ez, &dr, &di, &dx);
+            limbs_sqr(x, prod, x2, l);
+            limbs_sqr(y, prod, y2, l);
+            limbs_add_signed(xn, x, yn, y, s, l);
+            limbs_sqr(s, prod, s2, l);
+            int tn = limbs_add_signed(0, x2, 1, y2, t, l);
+            xn = limbs_add_signed(tn, t, crn, cr, x, l);
+            limbs_add(x2, y2, s, l);
+            tn = limbs_add_signed(0, s2, 1, s, t, l);
+            yn = limbs_add_signed(tn, t, cin, ci, y, l);
+            fx = fixed_to_f64(xn, x, l);
+            fy = fixed_to_f64(yn, y, l);
+        } else {
+            dz_step(fx, fy, 0, &dr, &di, &dx);
+            double t2 = fx * fx - fy * fy + c_re;
+            fy = 2.0 * fx * fy + c_im;
+            fx = t2;
+        }
+        n++;
+        double f2 = fx * fx + fy * fy;
+        if (f2 > r2) {
+            out[0] = fx, out[1] = fy, out[2] = dr, out[3] = di, out[4] = prev2, out[5] = f2;
+            *de = dx;
+            free(buf);
+            return (int64_t)n;
+        }
+        if (f2 > 4294967296.0) fixed = 0;
+        prev2 = f2;
+    }
+    free(buf);
+    return -1;
+}
+
 /* reference.rs:102-120 nucleus_period */
 uint64_t fdo_nucleus_period(const double *re, const double *im, uint64_t len) {
     if (len < 2) return 0;

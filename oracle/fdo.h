@@ -187,6 +187,10 @@ int fdo_guarded_build(const double *re, const double *im, const double *q, uint6
 int fdo_external_escape_ok(uint64_t n, double zr, double zi, double dr, double di, int64_t dexp,
                            double ez, double ed, double px_m, int64_t px_e);
 
+/* direct iteration of z^2 + c at `bits` bits with dz/dc (test hook for the direct fixture) */
+int64_t fdo_direct(const char *re, const char *im, uint64_t bits, uint64_t max_iter, double r2,
+                   double *out, int64_t *de);
+
 /* sample.rs:1119-1122 zero_step on explicit operands (test hook) */
 void fdo_zero_step(double wr, double wi, int64_t e, double ar, double ai, int64_t ae,
                    double *out, int64_t *oe);
