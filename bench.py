@@ -102,7 +102,10 @@ def cpu_baseline(view, params, route_bla, target_seconds, threads, orbit=None, r
                        f"threads of {allowed} allowed / {online} online cores")
 
 
-def measure(fa, view, params, route, steps, warmup, label, form=None, keep=None, tune=True):
+LAUNCH = "auto"   # --launch: auto (autotune each frame), default, or G,D,C
+
+
+def measure(fa, view, params, route, steps, warmup, label, form=None, keep=None):
     """Open the frame (orbit resident in HBM), autotune its launch (untimed), warm up, time
     `steps` passes.  keep: a list that receives the frame's columns and raw outcomes."""
     form = fa.FORM_F64 if form is None else form
@@ -110,7 +113,11 @@ def measure(fa, view, params, route, steps, warmup, label, form=None, keep=None,
     frame = fa.ResidentFrame(view, params, route, form, fa.EPS)
     log(f"{label}: frame open (host reference orbit + upload) {time.perf_counter() - t:.2f}s")
     launch = None
-    if tune:
+    if LAUNCH not in ("auto", "default"):
+        g, d, c = (int(x) for x in LAUNCH.split(","))
+        frame.launch(g, d, c)
+        launch = dict(chosen=dict(groups_per_cu=g, claim_div=d, claim_cap=c), given=True)
+    if LAUNCH == "auto":
         t = time.perf_counter()
         chosen, tuned_ms, default_ms = frame.tune(reps=2)
         launch = dict(chosen=chosen, tuned_kernel_ms=tuned_ms, default_kernel_ms=default_ms,
@@ -560,9 +567,15 @@ def main():
                          "series-study frames, companion mode only)")
     ap.add_argument("--forms", action="store_true",
                     help="with companions only: also run the FE32 and mixed forms")
+    ap.add_argument("--launch", default="auto",
+                    help="launch of the frames' perturbation kernel: auto (the autotuner, "
+                         "untimed, per frame), default, or G,D,C = groups per CU, claim "
+                         "divisor, claim cap (for profiling a chosen launch)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device copy (the traffic counters' calibration)")
     a = ap.parse_args()
+    global LAUNCH
+    LAUNCH = a.launch
     if a.copy_probe:
         copy_probe()
         return

@@ -189,3 +189,46 @@ def test_film_jet_matches_oracle(centre):
         exact = oracle.render(centre[0], centre[1], ws[0], 0.0, NX, NY, ITERS, columns=COLS,
                               ss=2, bla=True, outcomes=False)["columns"]
         assert_same(g[1]["nu"], exact["nu"], "nu (mask cleared)")
+
+
+@pytest.mark.gpu
+def test_config4_film_all_keyframes_row_sampled():
+    # the benchmark film itself: 64 keyframes of 960x540, widths 1e-10 .. 1e-300 toward the
+    # period-16116 nucleus, max_iter 2e5, one shared orbit, per-frame tables, one batched
+    # launch; four rows of every keyframe (different rows per frame) against the oracle
+    # rendering the same rows from the same orbit, bit for bit
+    from fractodactyl_amd.scenes import film_widths, scene
+    sc = scene(4)
+    cfg = sc["config"]
+    ws = film_widths(sc)
+    nx, ny, it = cfg["nx"], cfg["ny"], cfg["max_iter"]
+    c = (sc["center_re"], sc["center_im"])
+    zr, zi, bits = oracle.reference(c[0], c[1], ws[-1], 0.0, nx, ny, it, cfg["escape_radius"])
+    views = [fa.View(c[0], c[1], w) for w in ws]
+    escaped = 0
+    with fa.Film(views, fa.Params(nx, ny, 1, it, cfg["escape_radius"], COLS)) as film:
+        film.run(fa.LAUNCH_BATCH)
+        for k, w in enumerate(ws):
+            rows = [(k * 7 + r * 131) % ny for r in range(4)]
+            mask = np.zeros(nx * ny, np.uint8)
+            for j in rows:
+                mask[j * nx:(j + 1) * nx] = 1
+            f64 = oracle.plane(c[0], c[1], w, 0.0, nx, ny)["tier"] == oracle.TIER_F64
+            want = oracle.render(c[0], c[1], w, 0.0, nx, ny, it, cfg["escape_radius"],
+                                 columns=COLS, bla=True, mask=mask,
+                                 orbit=None if f64 else (zr, zi, bits))
+            g = film.read(k, raw=True)
+            assert g[0]["kernel"] == want["kernel"] + " gpu-hip-f64/1"
+            sel = mask.astype(bool)
+            for key in ("class", "nu", "de", "normal"):
+                assert_same(g[1][key][sel], want["columns"][key][sel], f"frame {k} {key}")
+            o, r = want["outcomes"], g[1]["raw"]
+            assert_same(r["kind"][sel], o["kind"][sel], f"frame {k} kind")
+            esc = sel & (o["kind"] == 0)
+            done = sel & (o["kind"] != 2)
+            assert_same(r["n"][done], o["n"][done], f"frame {k} n")
+            for key in ("zr", "zi", "dr", "di", "dexp"):
+                assert_same(r[key][esc], o[key][esc], f"frame {k} {key}")
+            escaped += int(esc.sum())
+    print(f"config 4: 64 keyframes x 4 rows, {escaped} escaped samples compared", flush=True)
+    assert escaped > 0
