@@ -362,6 +362,9 @@ __device__ inline int advance32(Lane32& s, const Hot32& h, int32_t aexp, float& 
     float fr = scale2(zl.re, zl.e), fi = scale2(zl.im, zl.e);
     // certified below the escape radius: |z| + e < R (|z| bounded by its L1 norm)
     if (CERT) s.safe = s.safe && fabsf(fr) + fabsf(fi) + exp2f(s.le) < h.escR;
+    // Once its bound reaches the escape radius the sample is recomputed in FP64 whatever
+    // it does next (escaped or not), so the FP32 pass stops it here (3: abandoned).
+    if (CERT && !s.safe) return 3;
     if (fr * fr + fi * fi > h.esc2) {
         ofr = fr;
         ofi = fi;
@@ -595,7 +598,9 @@ __global__ void __launch_bounds__(kBlock32, 4) perturb_fe32(const FrameDev* __re
                     bool keep = !esc && (!CERT || s.safe);
                     if (CERT) fp->cert[idx] = keep ? 1 : 0;
                     if (CERT && !esc && !keep) atomicAdd(&stat[ST_UNCERT], 1ull);
-                    o_kind[idx] = (uint8_t)k;
+                    // (an abandoned sample's fields are placeholders the FP64 pass
+                    // overwrites)
+                    o_kind[idx] = (uint8_t)(k == 3 ? 2 : k);
                     o_n[idx] = s.n;
                     o_ops[idx] = s.plain + s.blocks;
                     o_zr[idx] = esc ? (double)fr : 0.0;

@@ -113,9 +113,10 @@ typedef struct {
     double kernel_seconds;        /* GPU perturbation kernel (device time) */
     double transfer_seconds;      /* uploads + outcome readback */
     double encode_seconds;        /* Store::put into the columns (host) */
-    uint64_t uncertified_samples; /* FP32 forms: non-escaped samples whose error bound
-                                     reached the escape radius (the mixed form recomputes
-                                     them, and the escaped ones, in FP64) */
+    uint64_t uncertified_samples; /* mixed form: samples whose FP32 error bound reached
+                                     the escape radius before they ended (the FP32 pass
+                                     stops them there; they are recomputed in FP64 with
+                                     the escaped ones) */
 } fdg_stats;
 
 int fdg_abi_version(void);
