@@ -321,9 +321,11 @@ __device__ inline bool normalise(double& re, double& im, int32_t& e) {
     return !tie;
 }
 
-// sample.rs:1094-1114 sum2 over two terms
-__device__ inline void sum2(double t0r, double t0i, int64_t t0e, double t1r, double t1i,
-                            int64_t t1e, int64_t zero_e, double& ore, double& oim, int32_t& oe) {
+// sample.rs:1094-1114 sum2 over two terms, for any input (non-finite terms, exponents near
+// the i64 range: Rust's saturating float cast and wrapping adds)
+__device__ inline void sum2_general(double t0r, double t0i, int64_t t0e, double t1r,
+                                          double t1i, int64_t t1e, int64_t zero_e, double& ore,
+                                          double& oim, int32_t& oe) {
     bool have = false;
     int64_t top = 0;
     // (a NaN term passes the != 0 test and takes exponent t.e, as in the reference)
@@ -352,6 +354,44 @@ __device__ inline void sum2(double t0r, double t0i, int64_t t0e, double t1r, dou
     ore = r;
     oim = i;
     oe = (int32_t)top;
+}
+
+// sum2 as the scaled tier's rebases call it (three times per rebase onto Z_0 far below the
+// f64 range: the sum, then the step from Z_0 for dz/dc and for the delta).  For finite
+// terms with exponents inside +-2^29 the reference's i64 arithmetic never wraps or
+// saturates, so it runs in 32 bits, and floor(log2(a)) comes from the frexp exponent
+// (floor_log2_fast: equal to the libm expression for every finite a > 0); anything else
+// takes sum2_general.  x * 2^(k/2) * 2^(k - k/2) is one ldexp per factor for an int32 k
+// (exp2i's clamp at +-4096 only matters beyond int32).
+__device__ inline void sum2(double t0r, double t0i, int64_t t0e, double t1r, double t1i,
+                            int64_t t1e, int64_t zero_e, double& ore, double& oim, int32_t& oe) {
+    double a0 = fmax(fabs(t0r), fabs(t0i)), a1 = fmax(fabs(t1r), fabs(t1i));
+    constexpr int64_t kLim = int64_t(1) << 29;
+    bool fast = isfinite(a0) && isfinite(a1) && t0e > -kLim && t0e < kLim && t1e > -kLim &&
+                t1e < kLim;
+    if (!fast) {
+        sum2_general(t0r, t0i, t0e, t1r, t1i, t1e, zero_e, ore, oim, oe);
+        return;
+    }
+    if (a0 == 0.0 && a1 == 0.0) {
+        ore = 0.0;
+        oim = 0.0;
+        oe = (int32_t)zero_e;
+        return;
+    }
+    const int32_t e0 = (int32_t)t0e, e1 = (int32_t)t1e;
+    int32_t g0 = a0 != 0.0 ? floor_log2_fast(a0) + e0 : INT32_MIN;
+    int32_t g1 = a1 != 0.0 ? floor_log2_fast(a1) + e1 : INT32_MIN;
+    int32_t top = g0 > g1 ? g0 : g1;
+    int32_t k0 = e0 - top, k1 = e1 - top;
+    double r = 0.0, i = 0.0;
+    r = r + (k0 < -2046 ? 0.0 : t0r * exp2i32(k0 / 2) * exp2i32(k0 - k0 / 2));
+    i = i + (k0 < -2046 ? 0.0 : t0i * exp2i32(k0 / 2) * exp2i32(k0 - k0 / 2));
+    r = r + (k1 < -2046 ? 0.0 : t1r * exp2i32(k1 / 2) * exp2i32(k1 - k1 / 2));
+    i = i + (k1 < -2046 ? 0.0 : t1i * exp2i32(k1 / 2) * exp2i32(k1 - k1 / 2));
+    ore = r;
+    oim = i;
+    oe = top;
 }
 
 // Per-lane state that only the periodicity confirmation and the nucleus heuristic use
