@@ -37,23 +37,30 @@ __device__ inline int find_cap(uint32_t k, uint32_t nlevels, uint32_t budget) {
 //   0 (the builder writes it), so a block past the table's end simply fails the test.  A
 //   lookup happens at m <= len - 2 (k <= steps - 1), so k >> (i + 1) stays inside level
 //   i's slot of stride >> (i + 1) entries.  Indices are 32-bit (tables below 2^32 blocks).
+//   Every level searched divides k by 2^(i+1), so block k >> (i + 1) of level i sits at
+//   tb + stride - (stride >> i) + (k >> (i + 1)) = (tb + stride) - ((stride - k/2) >> i):
+//   one shift and one subtraction per level.
+__device__ inline uint32_t bla_block_index(uint32_t tb, uint32_t stride, uint32_t k, int i) {
+    return (tb + stride) - ((stride - (k >> 1)) >> i);
+}
+
 template <class P>
 __device__ inline int find_level_down(P r2, uint32_t tb, uint32_t stride, uint32_t k, int cap,
                                       double d2) {
+    const uint32_t top = tb + stride, span = stride - (k >> 1);
     int lvl = 0;
     for (int hi = cap; hi >= 1; hi -= kFindGroup) {
         double rr[kFindGroup];
 #pragma unroll
         for (int q = 0; q < kFindGroup; q++) {
             int lv = hi - q;
-            int lc = lv >= 1 ? lv : 1;  // (levels below 1 load nothing)
-            uint32_t at = tb + (stride - (stride >> lc)) + (k >> (lc + 1));
-            rr[q] = lv >= 1 ? r2[at] : -1.0;
+            int lc = lv >= 1 ? lv : 1;  // (levels below 1 are loaded at level 1, not taken)
+            rr[q] = r2[top - (span >> lc)];
         }
         int got = -1;
 #pragma unroll
         for (int q = kFindGroup - 1; q >= 0; q--)
-            got = d2 < rr[q] ? hi - q : got;
+            got = (d2 < rr[q] && hi - q >= 1) ? hi - q : got;
         if (got >= 1) {
             lvl = got;
             break;

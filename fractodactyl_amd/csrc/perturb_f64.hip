@@ -715,7 +715,7 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         double a_r = 0.0, a_i = 0.0, b_r = 1.0, b_i = 0.0;
         uint32_t blen = 1;
         if (blk) {
-            uint32_t at = tb + (h.stride32 - (h.stride32 >> lvl)) + ((s.m - 1) >> (lvl + 1));
+            uint32_t at = bla_block_index(tb, h.stride32, s.m - 1, lvl);
             double4 ab = ld4(h.ab, at);
             a_r = ab.x;
             a_i = ab.y;
@@ -757,7 +757,7 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         s.skipped += blk ? blen : 0u;
         s.plain += blk ? 0u : 1u;
     } else if (lvl >= 0) {
-        uint32_t at = tb + (h.stride32 - (h.stride32 >> lvl)) + ((s.m - 1) >> (lvl + 1));
+        uint32_t at = bla_block_index(tb, h.stride32, s.m - 1, lvl);
         double4 ab = ld4(h.ab, at);
         double a_r = ab.x, a_i = ab.y, b_r = ab.z, b_i = ab.w;
         uint32_t blen = 2u << lvl;

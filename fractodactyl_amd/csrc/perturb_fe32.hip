@@ -237,7 +237,7 @@ __device__ inline int advance32(Lane32& s, const Hot32& h, int32_t aexp, float& 
     float mr, mi;  // this step's multiplier (for the contraction product), times 2^mexp
     int32_t mexp = 0;
     if (lvl >= 0) {
-        uint32_t at = ((uint32_t)h.stride - ((uint32_t)h.stride >> lvl)) + ((s.m - 1) >> (lvl + 1));
+        uint32_t at = bla_block_index(0u, (uint32_t)h.stride, s.m - 1, lvl);
         v4f ab = h.ab[at];
         v2i ex = h.abexp[at];
         uint32_t blen = 2u << lvl;

@@ -93,3 +93,23 @@ def test_top_down_search_equals_find(centre, width, max_iter, eps, dcs):
             assert got == want, (d2, budget)
             checked += sum(g is not None for g in got)
     assert checked > 1000
+
+
+def kernel_block_index(tb, stride, k, i):
+    """bla_find.hpp's bla_block_index: (tb + stride) - ((stride - k/2) >> i), 32-bit."""
+    return ((tb + stride) - ((stride - (k >> 1)) >> i)) & 0xFFFFFFFF
+
+
+def test_block_index_is_the_level_major_position():
+    # The kernels address block k >> (i + 1) of level i, which sits at
+    # tb + stride - (stride >> i) + (k >> (i + 1)) in the level-major table, with one shift:
+    # exact wherever 2^(i+1) divides k, the only starts a level-i lookup or block reads.
+    for lg in range(1, 13):
+        stride = 1 << lg
+        for tb in (0, 3 * stride):
+            for k in range(0, stride, 2):
+                tz = lg if k == 0 else (k & -k).bit_length() - 1
+                for i in range(0, min(tz, lg)):
+                    want = tb + stride - (stride >> i) + (k >> (i + 1))
+                    assert kernel_block_index(tb, stride, k, i) == want, (stride, tb, k, i)
+                    assert want < tb + stride  # inside this table's slots
