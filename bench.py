@@ -535,6 +535,34 @@ def measure_config5(fa, sc, cols, steps, warmup):
                                                               ("unresolved", 2))})
 
 
+def measure_plain_fp64(fa, view, params, label, row_stride, forms_ms):
+    """The plain FP64 GPU form: perturbation with FP64 deltas and no BLA (ROUTE_PLAIN),
+    every step executed, on every row_stride-th row of the same frame (a sample mask; one
+    pass), its kernel time scaled to the whole frame by the rows computed.  forms_ms: the
+    BLA forms' kernel ms on the whole frame, reported as speed-ups over this form."""
+    import numpy as np
+    t = time.perf_counter()
+    frame = fa.ResidentFrame(view, params, fa.ROUTE_PLAIN, fa.FORM_F64, fa.EPS)
+    log(f"{label}-plain: frame open {time.perf_counter() - t:.2f}s")
+    mask = np.zeros((params.ny, params.nx), np.uint8)
+    mask[::row_stride] = 1
+    frame.mask(mask.ravel())
+    st = frame.run()
+    frame.close()
+    rows = int(mask[:, 0].sum())
+    ms = st["kernel_seconds"] * 1e3
+    est = ms * params.ny / rows
+    log(f"{label}-plain: {rows} rows, kernel {ms:.1f} ms, frame estimate {est:.0f} ms")
+    out = {"route": "plain perturbation (no BLA), FP64 deltas",
+           "sample": f"{rows} of {params.ny} rows (every {row_stride}th) x {params.nx} samples, "
+                     f"one pass",
+           "kernel_ms_sample": ms, "ms_per_frame_estimate": est,
+           "value": st["equivalent"] / st["kernel_seconds"],
+           "equivalent_iterations_sample": st["equivalent"], "plain_steps_sample": st["plain_steps"]}
+    out["speedup_of_bla_forms"] = {k: est / v for k, v in forms_ms.items()}
+    return out
+
+
 def fe32_roofline(m, forms):
     """The FE32 form's kernel against the FP32 vector peak (same algorithmic flops)."""
     r = m["roofline"]
@@ -656,6 +684,10 @@ def main():
                                     cpu_s / 2)
     if "config5" in frames:
         comp["config5_frame"] = measure_config5(fa, get_scene(5), cols, min(a.steps, 3), 1)
+    # the plain FP64 GPU form (no BLA) on a row sample of the headline frame
+    plain = measure_plain_fp64(fa, view, params, f"config{a.config}", 16 if a.config == 3 else 1,
+                               {"fp64_bla": m["kernel_ms"], "fe32_bla": forms["fe32"]["kernel_ms"],
+                                "mixed_bla": forms["mixed"]["kernel_ms"]})
     out = {
         "metric": "effective pixel-iter/s at 1080p deep zoom (BLA-skipped steps included)",
         "value": m["value"],
@@ -675,6 +707,7 @@ def main():
             **{k: v for k, v in m.items() if k not in ("value", "ms_per_step", "roofline")},
             "precision_forms": forms,
             "precision_forms_roofline": fe32_roofline(m, forms),
+            "plain_fp64_gpu": plain,
             **comp,
         },
         "roofline": dict(m["roofline"], traffic=measured_traffic(a.config)),
