@@ -69,4 +69,35 @@ __device__ inline int find_level_down(P r2, uint32_t tb, uint32_t stride, uint32
     return lvl;
 }
 
+// The same search when every searching lane of the wave has the same table, start and cap
+// (lanes in lockstep: deep frames' samples take the same blocks at the same orbit indices).
+// The radii are then the same words for every lane, so they are loaded once through the
+// scalar unit (r2 in the constant address space, indices in SGPRs) and only the
+// comparisons with each lane's |delta|^2 run on the vector unit.  tb, k and cap must be
+// wave-uniform.
+typedef const __attribute__((address_space(4))) double* ConstR2;
+
+__device__ inline int find_level_down_uniform(ConstR2 r2, uint32_t tb, uint32_t stride,
+                                              uint32_t k, int cap, double d2) {
+    const uint32_t top = tb + stride, span = stride - (k >> 1);
+    int lvl = 0;
+    for (int hi = cap; hi >= 1; hi -= kFindGroup) {
+        double rr[kFindGroup];
+#pragma unroll
+        for (int q = 0; q < kFindGroup; q++) {
+            int lv = hi - q;
+            rr[q] = lv >= 1 ? r2[top - (span >> lv)] : -1.0;
+        }
+        int got = -1;
+#pragma unroll
+        for (int q = kFindGroup - 1; q >= 0; q--)
+            got = d2 < rr[q] ? hi - q : got;
+        if (got >= 1) {
+            lvl = got;
+            break;
+        }
+    }
+    return lvl;
+}
+
 }  // namespace fdg

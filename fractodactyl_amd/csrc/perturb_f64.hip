@@ -478,6 +478,7 @@ __device__ inline double4 ld4(const FDG_GLOBAL v4d* p, uint64_t i) {
 struct Hot {
     const FDG_GLOBAL v2d* orbit;
     const FDG_GLOBAL double* r2;
+    ConstR2 r2c;                        // the same radii, for scalar loads
     const FDG_GLOBAL v4d* ab;
     uint32_t last, max_iter, nlevels, steps;
     uint64_t stride;
@@ -700,8 +701,19 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
                 uint32_t nb = s.nucleus ? (s.n == 0 ? 1u : s.nucleus + 1) : 0xffffffffu;
                 b = b < nb ? b : nb;
                 b = b > s.n ? b - s.n : 0;
-                int cap = find_cap(k, h.nlevels, b);
-                if (cap >= 0) lvl = find_level_down(h.r2, tb, h.stride32, k, cap, d2);
+                // Lanes in lockstep (the same table, start and budget in every searching
+                // lane) take the cap and the radii through the scalar unit.
+                const uint32_t k0 = __builtin_amdgcn_readfirstlane(k),
+                               b0 = __builtin_amdgcn_readfirstlane(b),
+                               t0 = B ? __builtin_amdgcn_readfirstlane(tb) : 0u;
+                if (__ballot(k != k0 || b != b0 || (B && tb != t0)) == 0) {
+                    int cap = find_cap(k0, h.nlevels, b0);
+                    if (cap >= 0)
+                        lvl = find_level_down_uniform(h.r2c, t0, h.stride32, k0, cap, d2);
+                } else {
+                    int cap = find_cap(k, h.nlevels, b);
+                    if (cap >= 0) lvl = find_level_down(h.r2, tb, h.stride32, k, cap, d2);
+                }
             }
         }
     }
@@ -962,6 +974,7 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
     Hot h;
     h.orbit = global_ptr(reinterpret_cast<const v2d*>(fp->orbit));
     h.r2 = global_ptr(fp->bla.r2);
+    h.r2c = (ConstR2)fp->bla.r2;
     h.ab = global_ptr(reinterpret_cast<const v4d*>(fp->bla.ab));
     h.last = (uint32_t)fp->len - 1;
     h.len = (uint32_t)fp->len;
