@@ -200,7 +200,9 @@ def end_to_end(fa, view, params, route, reps=2):
                                           "transfer_seconds", "encode_seconds")}
     w = sum(walls) / len(walls)
     return dict(ms_per_frame=w * 1e3, value=st["equivalent"] / w,
-                parts_ms={k.replace("_seconds", ""): v for k, v in parts.items()})
+                parts_ms={k.replace("_seconds", ""): v for k, v in parts.items()},
+                reference_period=st["reference_period"],
+                reference_repeat_at=st["reference_repeat_at"])
 
 
 def end_to_end_device_encode(fa, view, params, route, reps=2):
@@ -668,6 +670,8 @@ def main():
         mo["workload"] = workload_name(other, so, po)
         if cpu_s > 0:
             mo["cpu_baseline"] = cpu_baseline(vo, po, True, cpu_s / 2, threads, runs=1)
+        mo["end_to_end"] = end_to_end(fa, vo, po, fa.ROUTE_BLA)
+        mo["end_to_end_device_encode"] = end_to_end_device_encode(fa, vo, po, fa.ROUTE_BLA)
         comp[f"config{other}_frame"] = mo
     if "escape" in frames:
         keep0 = []

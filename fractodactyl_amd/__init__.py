@@ -103,7 +103,8 @@ class _Stats(C.Structure):
                 ("max_sample_ops", C.c_uint64), ("wave_iterations", C.c_uint64),
                 ("lane_iterations", C.c_uint64), ("reference_seconds", C.c_double), ("bla_seconds", C.c_double),
                 ("kernel_seconds", C.c_double), ("transfer_seconds", C.c_double),
-                ("encode_seconds", C.c_double), ("uncertified_samples", C.c_uint64)]
+                ("encode_seconds", C.c_double), ("uncertified_samples", C.c_uint64),
+                ("reference_period", C.c_uint64), ("reference_repeat_at", C.c_uint64)]
 
 
 class _SeriesConfig(C.Structure):

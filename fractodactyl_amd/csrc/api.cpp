@@ -307,6 +307,7 @@ struct fdg_frame {
     double dc = 0;
     int64_t dc_exp = 0;
     double reference_seconds = 0, transfer_seconds = 0;
+    uint64_t reference_period = 0, reference_repeat_at = 0;  // the host orbit's exact repeat
     bool use_bla = false, ran = false;
     bool masked = false;  // a sample mask is set (fdg_frame_mask)
     DevBuf mask;          // 1 byte per sample on the device
@@ -612,6 +613,8 @@ static int frame_open(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int 
         if (rc) return rc;
     }
     f->reference_seconds = now_s() - t0;
+    f->reference_period = o.cycle_period;
+    f->reference_repeat_at = o.cycle_at;
     // The loop reads Z_1 from its first step, and the scaled tier's rebase of a delta
     // below 1e-270 continues from Z_1 to Z_2 (sample.rs:422-449): shorter orbits are
     // refused here, where the reference would index past its orbit and panic
@@ -670,6 +673,8 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
         fdg_stats s;
         std::memset(&s, 0, sizeof(s));
         s.reference_seconds = f->reference_seconds;
+        s.reference_period = f->reference_period;
+        s.reference_repeat_at = f->reference_repeat_at;
         s.reference_len = f->len;
         s.reference_bits = f->bits;
         s.nucleus_period = f->nucleus;

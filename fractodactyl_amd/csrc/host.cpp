@@ -578,6 +578,47 @@ struct SpinBarrier {
     }
 };
 
+// Exact repeats of the orbit's fixed-point state (x, y).  The truncated fixed-point map is
+// deterministic, so once a state recurs every later point repeats with the same period, and
+// the rest of the orbit is a copy of earlier points — bit for bit what the squarings would
+// give.  Two watches: Z_i == Z_1 = c, which a nucleus centre reaches at i = P + 1 once Z_P
+// squares to zero at the working precision, and Brent's state saved at each power-of-two
+// index, which finds any other cycle (an attracting cycle the truncated orbit settles on)
+// by the index max(tail, period) rounded up to a power of two, plus the period.  Signs are
+// canonical (zero is non-negative), so equal values have equal limbs.
+struct CycleWatch {
+    size_t l;
+    const uint64_t *cr, *ci;
+    bool crn, cin;
+    uint64_t* saved;                   // 2l limbs: x, then y
+    bool sxn = false, syn = false;
+    uint64_t save_at = 0, next_save = 1;
+    CycleWatch(size_t limbs, const Fixed& re, const Fixed& im, uint64_t* buf)
+        : l(limbs), cr(re.mag.data()), ci(im.mag.data()), saved(buf) {
+        crn = re.neg && std::any_of(cr, cr + l, [](uint64_t v) { return v != 0; });
+        cin = im.neg && std::any_of(ci, ci + l, [](uint64_t v) { return v != 0; });
+        std::fill(saved, saved + 2 * l, 0);   // Z_0 = 0
+    }
+    // Z_i's state (i >= 1); the period when Z_i equals an earlier point, else 0.
+    uint64_t check(uint64_t i, bool xn, const uint64_t* x, bool yn, const uint64_t* y) {
+        if (i >= 2 && xn == crn && yn == cin && std::equal(x, x + l, cr) &&
+            std::equal(y, y + l, ci))
+            return i - 1;
+        if (xn == sxn && yn == syn && std::equal(x, x + l, saved) &&
+            std::equal(y, y + l, saved + l))
+            return i - save_at;
+        if (i == next_save) {
+            std::copy(x, x + l, saved);
+            std::copy(y, y + l, saved + l);
+            sxn = xn;
+            syn = yn;
+            save_at = i;
+            next_save = 2 * i;
+        }
+        return 0;
+    }
+};
+
 // One thread's share of the fixed-point orbit (orbit.rs:14-43: x' = x^2 - y^2 + cr,
 // y' = (x + y)^2 - (x^2 + y^2) + ci).  Thread `role` squares x (0), y (1) or x + y (2)
 // into its slot of the step's buffer; after the barrier every thread combines the three
@@ -598,9 +639,10 @@ struct OrbitShared {
     std::atomic<int> gate{0};       // 0 wait, 1 run, 2 leave
     SpinBarrier bar;
     Orbit* out;
+    uint64_t cycle_at = 0, cycle_period = 0;  // set by the emitting worker
     OrbitShared(size_t limbs, uint32_t threads) : l(limbs), sq(6 * limbs), bar(threads) {
         for (uint32_t w = 0; w < threads; w++)
-            buf[w].assign(6 * limbs + limbs::sqr_scratch(limbs), 0);
+            buf[w].assign(8 * limbs + limbs::sqr_scratch(limbs), 0);
     }
 };
 
@@ -614,7 +656,8 @@ void orbit_worker(OrbitShared* sh, int first_role, int last_role, bool emit) {
     }
     std::vector<uint64_t>& buf = sh->buf[first_role];
     uint64_t *x = &buf[0], *y = &buf[l], *s = &buf[2 * l], *t = &buf[3 * l],
-             *prod = &buf[4 * l], *scratch = &buf[6 * l];
+             *prod = &buf[4 * l], *scratch = &buf[8 * l];
+    CycleWatch watch(l, *sh->cr, *sh->ci, &buf[6 * l]);
     bool xn = false, yn = false;
     for (uint64_t k = 0; k < sh->steps; k++) {
         uint64_t* sq = &sh->sq[(k & 1) * 3 * l];
@@ -642,6 +685,13 @@ void orbit_worker(OrbitShared* sh, int first_role, int last_role, bool emit) {
         }
         // every thread holds the same (x, y), so all stop at the same step
         if (fx * fx + fy * fy > 4294967296.0) break;
+        if (uint64_t p = watch.check(k + 1, xn, x, yn, y)) {
+            if (emit) {
+                sh->cycle_at = k + 1;
+                sh->cycle_period = p;
+            }
+            break;
+        }
     }
 }
 
@@ -718,7 +768,8 @@ struct Small {
         return an && nonzero(o);
     }
     static void orbit(const Fixed& cr, const Fixed& ci, uint64_t steps, Orbit* out) {
-        uint64_t x[L] = {}, y[L] = {}, x2[L], y2[L], s[L], s2[L], t[L];
+        uint64_t x[L] = {}, y[L] = {}, x2[L], y2[L], s[L], s2[L], t[L], saved[2 * L];
+        CycleWatch watch(L, cr, ci, saved);
         bool xn = false, yn = false;
         for (uint64_t k = 0; k < steps; k++) {
             sqr(x, x2);
@@ -734,6 +785,11 @@ struct Small {
             out->re.push_back(fx);
             out->im.push_back(fy);
             if (fx * fx + fy * fy > 4294967296.0) break;
+            if (uint64_t p = watch.check(k + 1, xn, x, yn, y)) {
+                out->cycle_at = k + 1;
+                out->cycle_period = p;
+                break;
+            }
         }
     }
 };
@@ -761,6 +817,7 @@ bool Orbit::fixed(const std::string& re, const std::string& im, uint64_t bits, u
     o.im.push_back(0.0);
     if (orbit_small(l, cr, ci, steps, &o, std::make_index_sequence<14>())) {  // 3..16 limbs
         o.bits = (uint32_t)bits;
+        o.tile(steps);
         *out = std::move(o);
         return true;
     }
@@ -787,6 +844,8 @@ bool Orbit::fixed(const std::string& re, const std::string& im, uint64_t bits, u
             orbit_worker(&sh, 0, 0, true);
             t1.join();
             t2.join();
+            o.cycle_at = sh.cycle_at;
+            o.cycle_period = sh.cycle_period;
         }
     }
     if (!threaded) {
@@ -796,10 +855,23 @@ bool Orbit::fixed(const std::string& re, const std::string& im, uint64_t bits, u
         sh.ci = &ci;
         sh.out = &o;
         orbit_worker(&sh, 0, 2, true);
+        o.cycle_at = sh.cycle_at;
+        o.cycle_period = sh.cycle_period;
     }
     o.bits = (uint32_t)bits;
+    o.tile(steps);
     *out = std::move(o);
     return true;
+}
+
+void Orbit::tile(uint64_t steps) {
+    if (!cycle_period) return;
+    // Z_j = Z_{j - p} for every j past the repeat (cycle_at): the copy the squarings would
+    // have produced, up to the same last point
+    for (uint64_t j = cycle_at + 1; j <= steps; j++) {
+        re.push_back(re[j - cycle_period]);
+        im.push_back(im[j - cycle_period]);
+    }
 }
 
 uint64_t Orbit::nucleus_period_of(const double* re, const double* im, uint64_t len) {

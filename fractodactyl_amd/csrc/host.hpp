@@ -81,6 +81,10 @@ struct Plane {
 struct Orbit {
     std::vector<double> re, im;
     uint32_t bits = 53;
+    // The fixed-point orbit's state at point cycle_at equalled the one cycle_period points
+    // earlier (an exact repeat: a nucleus centre to the working precision, or an attracting
+    // cycle the truncated orbit settled on), so the points after it are copies (0: none).
+    uint64_t cycle_at = 0, cycle_period = 0;
     static constexpr uint64_t kMaxLen = 1ull << 20;
     // f64 orbit at an f64 centre, stopping at the escape radius.
     static Orbit f64(double c_re, double c_im, uint64_t max_iter, double radius,
@@ -90,6 +94,8 @@ struct Orbit {
     static bool fixed(const std::string& re, const std::string& im, uint64_t bits,
                       uint64_t max_iter, Orbit* out, uint64_t max_len = kMaxLen);
     uint64_t len() const { return re.size(); }
+    // Appends the copies of the points after an exact repeat, up to point `steps`.
+    void tile(uint64_t steps);
     // Reference::nucleus_period (reference.rs:102-120); 0 for None.
     uint64_t nucleus_period() const { return nucleus_period_of(re.data(), im.data(), len()); }
     static uint64_t nucleus_period_of(const double* re, const double* im, uint64_t len);

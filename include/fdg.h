@@ -117,6 +117,12 @@ typedef struct {
                                      the escape radius before they ended (the FP32 pass
                                      stops them there; they are recomputed in FP64 with
                                      the escaped ones) */
+    uint64_t reference_period;    /* the host fixed-point orbit's state repeated exactly
+                                     with this period at point reference_repeat_at, so its
+                                     later points were copied, not computed (a nucleus
+                                     centre, or an attracting cycle the truncated orbit
+                                     settled on); 0: no exact repeat */
+    uint64_t reference_repeat_at;
 } fdg_stats;
 
 int fdg_abi_version(void);

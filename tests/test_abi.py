@@ -97,6 +97,27 @@ def test_threaded_karatsuba_orbit_matches_oracle(width, iters):
     assert np.array_equal(a_im.view(np.int64), b_im.view(np.int64))
 
 
+@pytest.mark.parametrize("k", [1, 3])
+def test_nucleus_orbit_repeats_exactly_and_equals_the_oracle(k):
+    # At a nucleus centre known to the working precision, Z_P squares to zero in the
+    # truncated fixed point, so Z_{P+1} = c = Z_1 exactly and the host copies the cycle
+    # instead of squaring (the 1e-1000 frame: 16,117 of 1,000,001 points computed); the
+    # orbit must still be the oracle's directly iterated one, bit for bit, to the last point
+    from fractodactyl_amd.scenes import scene
+    sc = scene(k)
+    c = sc["config"]
+    v = fa.View(sc["center_re"], sc["center_im"], c["width"], 0.0)
+    p = fa.Params(64, 36, max_iter=c["max_iter"])
+    a_re, a_im, a_bits = _host_reference(v, p)
+    b_re, b_im, b_bits = oracle.reference(sc["center_re"], sc["center_im"], c["width"], 0.0,
+                                          64, 36, c["max_iter"], 1e10)
+    assert a_bits == b_bits and len(a_re) == len(b_re) == c["max_iter"] + 1
+    assert np.array_equal(a_re.view(np.int64), b_re.view(np.int64))
+    assert np.array_equal(a_im.view(np.int64), b_im.view(np.int64))
+    P = sc["nucleus"]["period"]
+    assert a_re[P + 1] == a_re[1] and a_im[P + 1] == a_im[1]
+
+
 def test_host_reference_rejects_bad_requests():
     v, p = fa._view(fa.View("abc", "0", "1e-3")), fa._params(fa.Params(8, 8))
     ln, bits = C.c_uint64(), C.c_uint32()

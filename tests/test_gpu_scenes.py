@@ -41,6 +41,9 @@ def test_plain_scene_480x270(ctx):
     g, c = both(ctx, 1)
     compare_frames(g, c)
     assert "bla/1" not in g[0]["kernel"]
+    # the nucleus centre's fixed-point orbit repeated exactly after one period
+    assert g[2]["reference_period"] == scene(1)["nucleus"]["period"]
+    assert g[2]["reference_repeat_at"] == g[2]["reference_period"] + 1
     assert (g[1]["class"] & 3 == 0).any() and (g[1]["class"] & 3 == 2).any()
 
 
@@ -56,3 +59,5 @@ def test_scaled_scene_1e1000_reduced_grid(ctx):
     compare_frames(g, c)
     assert g[0]["kernel"].startswith("pert-fx-scaled/1 bits=")
     assert (g[1]["class"] & 3 == 0).any()
+    assert g[2]["reference_period"] == scene(3)["nucleus"]["period"]
+    assert g[2]["reference_repeat_at"] == g[2]["reference_period"] + 1
