@@ -28,8 +28,9 @@ U = 2.220446049250313e-16 / 2
 #  * nu differs by exactly one ulp where the Windows CRT's log2 differs from glibc's
 #    (each is reproduced by a +-1 ulp change of the inner log2; the outcome z is equal);
 #  * two seahorse samples are Interior in the bundle but Unresolved by sample.rs's PERF-08
-#    rule as written (an attracting 162-cycle whose 81-step near-returns alternate
-#    |mu| > 1 and |mu| < 1, so two consecutive Newton predictions never occur).
+#    rule as written (attracting cycles of 162 and 232 steps whose 81- and 58-step
+#    near-returns alternate |mu| > 1 and |mu| < 1, so two consecutive Newton predictions
+#    never occur; traced step by step in test_seahorse_trace.py).
 KERN05_NU_ULP = {"seahorse-fx": {2790, 11521}, "seahorse-scaled": {19530},
                  "i-1e-20": {2420, 3812, 11792, 15765, 16739}}
 KERN05_CLASS = {"seahorse": {765, 12093}, "seahorse-fx": {765, 12093}}
