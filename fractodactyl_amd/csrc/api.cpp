@@ -186,6 +186,129 @@ int read_encode(fdg_ctx* ctx, const Buffers& b, size_t off, size_t n, const Plan
     return FDG_OK;
 }
 
+std::vector<uint32_t> mask_to_list(const uint8_t* mask, size_t n) {
+    // per-chunk counts, then each chunk writes its own slice
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<size_t> cnt(nt + 1, 0);
+    run_parts(nt, [&](unsigned w) {
+        size_t lo = n * w / nt, hi = n * (w + 1) / nt, c = 0;
+        for (size_t i = lo; i < hi; i++) c += mask[i] != 0;
+        cnt[w + 1] = c;
+    });
+    for (unsigned w = 0; w < nt; w++) cnt[w + 1] += cnt[w];
+    std::vector<uint32_t> list(cnt[nt]);
+    run_parts(nt, [&](unsigned w) {
+        size_t lo = n * w / nt, hi = n * (w + 1) / nt, o = cnt[w];
+        for (size_t i = lo; i < hi; i++)
+            if (mask[i]) list[o++] = (uint32_t)i;
+    });
+    return list;
+}
+
+int read_encode_list(fdg_ctx* ctx, Buffers& b, const std::vector<uint32_t>& list, size_t n,
+                     const Plane& pl, const fdg_params* p, fdg_columns* out, fdg_outcomes* raw,
+                     bool keep_unset, double* transfer_s, double* encode_s) {
+    double t1 = now_s();
+    const size_t m = list.size();
+    const bool want_ops = raw && raw->ops;
+    size_t kbytes = (m + 7) & ~(size_t)7;
+    size_t packed = kbytes + 6 * 8 * m + (want_ops ? 4 * m : 0);
+    size_t need = std::max(packed, 4 * m + 8);
+    int rc = ctx->staging(need);
+    if (rc) return rc;
+    uint8_t* base = static_cast<uint8_t*>(ctx->pinned);
+    if (m > 0) {
+        FDG_TRY(ctx, b.list.ensure(4 * m), "hipMalloc");
+        FDG_TRY(ctx, b.packed.ensure(packed), "hipMalloc");
+        // the list goes up through the pinned staging area, then the packed outcomes come
+        // back through it (stream order keeps the two uses apart)
+        std::memcpy(base, list.data(), 4 * m);
+        FDG_TRY(ctx, hipMemcpyAsync(b.list.p, base, 4 * m, hipMemcpyHostToDevice, ctx->stream),
+                "list upload");
+        uint8_t* dp = static_cast<uint8_t*>(b.packed.p);
+        GatherDev g{};
+        g.list = b.list.as<uint32_t>();
+        g.m = (uint32_t)m;
+        g.kind = b.o_kind.as<uint8_t>();
+        g.n = b.o_n.as<uint64_t>();
+        g.zr = b.o_zr.as<double>();
+        g.zi = b.o_zi.as<double>();
+        g.dr = b.o_dr.as<double>();
+        g.di = b.o_di.as<double>();
+        g.dexp = b.o_dexp.as<int64_t>();
+        g.ops = b.o_ops.as<uint32_t>();
+        g.kind_c = dp;
+        g.n_c = reinterpret_cast<uint64_t*>(dp + kbytes);
+        g.zr_c = reinterpret_cast<double*>(g.n_c + m);
+        g.zi_c = g.zr_c + m;
+        g.dr_c = g.zi_c + m;
+        g.di_c = g.dr_c + m;
+        g.dexp_c = reinterpret_cast<int64_t*>(g.di_c + m);
+        g.ops_c = want_ops ? reinterpret_cast<uint32_t*>(g.dexp_c + m) : nullptr;
+        FDG_TRY(ctx, launch_gather_outcomes(g, ctx->cus, ctx->stream), "gather");
+        FDG_TRY(ctx, hipMemcpyAsync(base, dp, packed, hipMemcpyDeviceToHost, ctx->stream),
+                "readback");
+    }
+    FDG_TRY(ctx, hipStreamSynchronize(ctx->stream), "readback");
+    *transfer_s += now_s() - t1;
+    const uint8_t* kind = base;
+    const uint64_t* nn = reinterpret_cast<const uint64_t*>(base + kbytes);
+    const double* zr = reinterpret_cast<const double*>(nn + m);
+    const double* zi = zr + m;
+    const double* dr = zi + m;
+    const double* di = dr + m;
+    const int64_t* dexp = reinterpret_cast<const int64_t*>(di + m);
+    const uint32_t* ops = want_ops ? reinterpret_cast<const uint32_t*>(dexp + m) : nullptr;
+    double t2 = now_s();
+    Encoder enc(pl, p->ss);
+    double* nu = (p->columns & FDG_COL_NU) ? out->nu : nullptr;
+    float* de = (p->columns & FDG_COL_DE) ? out->de : nullptr;
+    uint16_t* nm = (p->columns & FDG_COL_NORMAL) ? out->normal : nullptr;
+    unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    if (!keep_unset) {
+        // every sample outside the list reads as an unset one: class 0, zero columns
+        run_parts(nt, [&](unsigned w) {
+            size_t lo = n * w / nt, hi = n * (w + 1) / nt;
+            std::memset(out->class_ + lo, 0, hi - lo);
+            if (nu) std::memset(nu + lo, 0, (hi - lo) * 8);
+            if (de) std::memset(de + lo, 0, (hi - lo) * 4);
+            if (nm) std::memset(nm + lo, 0, (hi - lo) * 2);
+            if (raw) {
+                if (raw->kind) std::memset(raw->kind + lo, kKindUnset, hi - lo);
+                for (void* a : {(void*)raw->n, (void*)raw->zr, (void*)raw->zi, (void*)raw->dr,
+                                (void*)raw->di, (void*)raw->dexp})
+                    if (a) std::memset(static_cast<uint8_t*>(a) + lo * 8, 0, (hi - lo) * 8);
+                if (raw->ops) std::memset(raw->ops + lo, 0, (hi - lo) * 4);
+            }
+        });
+    }
+    run_parts(nt, [&](unsigned w) {
+        size_t lo = m * w / nt, hi = m * (w + 1) / nt;
+        for (size_t t = lo; t < hi; t++) {
+            const size_t i = list[t];
+            // Outcome::Unresolved carries no iterate (sample.rs:21-40)
+            const uint64_t ni = kind[t] == KIND_UNRESOLVED ? 0 : nn[t];
+            if (nu) nu[i] = 0.0;
+            if (de) de[i] = 0.0f;
+            if (nm) nm[i] = 0;
+            enc.put(i, kind[t], ni, zr[t], zi[t], dr[t], di[t], dexp[t], out->class_, nu, de,
+                    nm);
+            if (raw) {
+                if (raw->kind) raw->kind[i] = kind[t];
+                if (raw->n) raw->n[i] = ni;
+                if (raw->zr) raw->zr[i] = zr[t];
+                if (raw->zi) raw->zi[i] = zi[t];
+                if (raw->dr) raw->dr[i] = dr[t];
+                if (raw->di) raw->di[i] = di[t];
+                if (raw->dexp) raw->dexp[i] = dexp[t];
+                if (raw->ops) raw->ops[i] = ops[t];
+            }
+        }
+    });
+    *encode_s = now_s() - t2;
+    return FDG_OK;
+}
+
 }  // namespace fdg
 
 using namespace fdg;
@@ -893,8 +1016,14 @@ int fdg_frame_read(fdg_frame* f, fdg_columns* out, fdg_outcomes* raw, fdg_stats*
             return ctx->fail(FDG_EINVAL, "a requested column has no buffer");
         FDG_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
         fdg_stats s = f->last;
-        int rc = read_encode(ctx, f->buf, 0, f->count, f->fr.pl, p, out, raw, &s.transfer_seconds,
-                             &s.encode_seconds, nullptr, nullptr, nullptr);
+        // A masked pass reads back only the samples it computed, gathered on the device
+        int rc = f->masked
+                     ? read_encode_list(ctx, f->buf, mask_to_list(f->mask_host.data(), f->count),
+                                        f->count, f->fr.pl, p, out, raw, false,
+                                        &s.transfer_seconds, &s.encode_seconds)
+                     : read_encode(ctx, f->buf, 0, f->count, f->fr.pl, p, out, raw,
+                                   &s.transfer_seconds, &s.encode_seconds, nullptr, nullptr,
+                                   nullptr);
         if (rc) return rc;
         if (st) *st = s;
         if (kernel_id && kernel_cap) {
@@ -1363,21 +1492,19 @@ int fdg_frame_jet(fdg_frame* f, fdg_columns* out, fdg_stats* st, char* kernel_id
         uint64_t rest = 0;
         for (uint8_t v : todo) rest += v;
         if (rest > 0) {
-            // pass 2: the rejected predictions, then merged over the rebuilt frame
-            std::vector<uint8_t> c2(n);
-            std::vector<double> nu2(n);
-            std::vector<float> de2(n);
-            std::vector<uint16_t> nm2(n);
-            fdg_columns second{c2.data(), nu2.data(), de2.data(), nm2.data()};
+            // pass 2: the rejected predictions, read back packed and encoded straight into
+            // their places over the rebuilt frame (jetaa::merge, jetaa.rs:248-261, without
+            // a second set of columns: every other sample keeps its pass-1 or rebuilt value)
             rc = fdg_frame_mask(f, todo.data());
-            if (!rc) rc = fdg_frame_run(f, nullptr);
-            if (!rc) rc = fdg_frame_read(f, &second, nullptr, &s2, nullptr, 0);
+            if (!rc) rc = fdg_frame_run(f, &s2);
+            if (!rc)
+                rc = read_encode_list(ctx, f->buf, mask_to_list(todo.data(), n), n, f->fr.pl, p,
+                                      out, nullptr, true, &s2.transfer_seconds,
+                                      &s2.encode_seconds);
             if (rc) {
                 (void)restore();
                 return rc;
             }
-            jetaa_merge(out->class_, out->nu, out->de, out->normal, second.class_, second.nu,
-                        second.de, second.normal, todo.data(), n);
         }
         rc = restore();
         if (rc) return rc;

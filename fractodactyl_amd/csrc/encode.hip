@@ -63,7 +63,33 @@ __global__ void __launch_bounds__(256) encode_columns(EncodeDev e) {
     }
 }
 
+// The outcomes of the samples list[0 .. m) packed contiguously (a masked pass's computed
+// samples): the readback then moves 53 bytes per computed sample instead of per sample.
+__global__ void __launch_bounds__(256) gather_outcomes(GatherDev g) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < g.m;
+         t += gridDim.x * blockDim.x) {
+        uint32_t i = g.list[t];
+        g.kind_c[t] = g.kind[i];
+        g.n_c[t] = g.n[i];
+        g.zr_c[t] = g.zr[i];
+        g.zi_c[t] = g.zi[i];
+        g.dr_c[t] = g.dr[i];
+        g.di_c[t] = g.di[i];
+        g.dexp_c[t] = g.dexp[i];
+        if (g.ops_c) g.ops_c[t] = g.ops[i];
+    }
+}
+
 }  // namespace
+
+hipError_t launch_gather_outcomes(const GatherDev& g, int cus, hipStream_t stream) {
+    if (g.m == 0) return hipSuccess;
+    uint64_t blocks = (g.m + 255) / 256;
+    uint64_t cap = (uint64_t)cus * 16;
+    hipLaunchKernelGGL(gather_outcomes, dim3((unsigned)(blocks < cap ? blocks : cap)), dim3(256),
+                       0, stream, g);
+    return hipGetLastError();
+}
 
 hipError_t launch_encode_columns(const EncodeDev& e, int cus, hipStream_t stream) {
     if (e.count == 0) return hipSuccess;

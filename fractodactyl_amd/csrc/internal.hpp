@@ -46,6 +46,7 @@ hipError_t launch_perturb_f64_series(const FrameDev* f, bool deriv, bool bla, in
                                      hipStream_t stream);
 // Store::put on the device for e.count samples (encode.hip).
 hipError_t launch_encode_columns(const EncodeDev& e, int cus, hipStream_t stream);
+hipError_t launch_gather_outcomes(const GatherDev& g, int cus, hipStream_t stream);
 // cert: the mixed form's certifying pass (error bound per sample, FrameDev::cert written).
 hipError_t launch_perturb_fe32(const FrameDev* f, bool deriv, bool bla, int grid,
                                hipStream_t stream, bool cert = false);
@@ -147,6 +148,7 @@ struct Buffers {
     DevBuf qrad, cert;              // FP32 forms: orbit error radii, per-sample certificates
     DevBuf list, counters2, fdev2;  // the mixed form's FP64 refinement pass
     DevBuf c_cls, c_nu, c_de, c_nm;  // columns encoded on the device
+    DevBuf packed;                  // a masked pass's computed samples, gathered for readback
     uint64_t bla_stride = 0, bla_steps = 0;
     // bla_valid holds zeros (a pass set-up cleared the flags after reading them), so the
     // next build needs no clearing of its own
@@ -157,7 +159,7 @@ struct Buffers {
                           &o_kind, &o_n, &o_zr, &o_zi, &o_dr, &o_di, &o_dexp, &o_ops,
                           &counters, &fdev, &orbit32, &ab32, &abexp32, &qrad, &cert, &list, &counters2,
                           &fdev2,
-                          &c_cls, &c_nu, &c_de, &c_nm})
+                          &c_cls, &c_nu, &c_de, &c_nm, &packed})
             b->release();
     }
     // Per-sample outcome arrays for n samples, the counters and one FrameDev.
@@ -186,5 +188,14 @@ int read_encode(fdg_ctx* ctx, const Buffers& B, size_t off, size_t n, const Plan
                 const fdg_params* p, fdg_columns* out, fdg_outcomes* raw, double* transfer_s,
                 double* encode_s, std::vector<uint64_t>* n_out, std::vector<uint8_t>* kind_out,
                 std::vector<uint32_t>* ops_out);
+// The set samples of a mask, in order.
+std::vector<uint32_t> mask_to_list(const uint8_t* mask, size_t n);
+// read_encode for a masked pass: only the samples list[0 .. m) (in order) are gathered on
+// the device, read back packed and encoded into their places; every other sample reads as
+// unset (class 0, zero columns, raw kind 0xff and zeros), or is left as it is with
+// keep_unset (a second pass merged over a first one's columns).
+int read_encode_list(fdg_ctx* ctx, Buffers& b, const std::vector<uint32_t>& list, size_t n,
+                     const Plane& pl, const fdg_params* p, fdg_columns* out, fdg_outcomes* raw,
+                     bool keep_unset, double* transfer_s, double* encode_s);
 
 }  // namespace fdg

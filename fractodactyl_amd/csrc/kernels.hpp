@@ -122,4 +122,21 @@ struct EncodeDev {
     double cos_r, sin_r;
 };
 
+// A masked pass's computed samples (list[0 .. m), sample indices) gathered from the
+// outcome arrays into packed ones for the readback.
+struct GatherDev {
+    const uint32_t* list;
+    uint32_t m;
+    const uint8_t* kind;
+    const uint64_t* n;
+    const double *zr, *zi, *dr, *di;
+    const int64_t* dexp;
+    const uint32_t* ops;            // (read when ops_c is set)
+    uint8_t* kind_c;
+    uint64_t* n_c;
+    double *zr_c, *zi_c, *dr_c, *di_c;
+    int64_t* dexp_c;
+    uint32_t* ops_c;                // nullable
+};
+
 }  // namespace fdg
