@@ -12,6 +12,7 @@
 // reference's operation order, so the rebuilt columns are its bytes.
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <thread>
 #include <vector>
 
@@ -44,9 +45,9 @@ struct Off {
 
 // A pixel's jet: its sample's nu and F', F'', F''' with Re F scaled to 1 (jetaa.rs:78-103).
 struct Jet {
-    bool ok = false;
-    double nu = 0;
-    Cx f1{0, 0}, f2{0, 0}, f3{0, 0};
+    bool ok;
+    double nu;
+    Cx f1, f2, f3;
     // (nu, de, normal x, normal y) at the offset o; false where the expansion fails
     bool at(const Off& o, double* nu_o, float* de_o, double* nx, double* ny) const {
         double re = 1.0 + mul(f1, o.d).re + mul(f2, o.d2h).re + mul(f3, o.d3s).re;
@@ -120,9 +121,11 @@ void jetaa_rebuild(uint8_t* cls, double* nu, float* de, uint16_t* normal, size_t
     auto real = [&](size_t q) { return idx(q % w, q / w, 0); };
     auto kind = [&](size_t i) { return (uint8_t)(cls[i] & 3); };  // 3: invalid encoding
     // per pixel sample: (nu, 2 F' / Re F) = (nu, 2 e^(i normal) / de), escaped only
-    std::vector<uint8_t> g_ok(n);
-    std::vector<double> g_nu(n);
-    std::vector<Cx> g_c(n);
+    // Per-pixel scratch that each pixel's own iteration writes before any read (the host
+    // frames are large: 8.3M pixels at 3840x2160, so nothing is zero-filled on one thread)
+    std::unique_ptr<uint8_t[]> g_ok(new uint8_t[n]);
+    std::unique_ptr<double[]> g_nu(new double[n]);
+    std::unique_ptr<Cx[]> g_c(new Cx[n]);
     par_for(n, [&](size_t q) {
         size_t i = real(q);
         g_ok[q] = kind(i) == KIND_ESCAPED;
@@ -132,9 +135,10 @@ void jetaa_rebuild(uint8_t* cls, double* nu, float* de, uint16_t* normal, size_t
         g_nu[q] = nu[i];
         g_c[q] = scale(Cx{(double)x, (double)y}, 2.0 / (double)de[i]);
     });
-    std::vector<Jet> jets(n);
+    std::unique_ptr<Jet[]> jets(new Jet[n]);
     par_for(n, [&](size_t q) {
         size_t px = q % w, py = q / w;
+        jets[q].ok = false;
         if (px == 0 || py == 0 || px + 1 >= w || py + 1 >= h || !g_ok[q]) return;
         const double nu0 = g_nu[q];
         // F' of a neighbour relative to Re F here: Re F scales as 2^-nu
@@ -160,10 +164,9 @@ void jetaa_rebuild(uint8_t* cls, double* nu, float* de, uint16_t* normal, size_t
     const Off own[3] = {Off(0.5, 0.0), Off(0.0, 0.5), Off(0.5, 0.5)};
     const Off other[3] = {Off(-0.5, 0.0), Off(0.0, -0.5), Off(-0.5, -0.5)};
     const double max_dnu = kM * 1e-3, max_dlnde = kM * 1e-2, max_cross = std::sin(kM * 3e-3);
-    std::vector<Pred> preds(3 * n);
-    par_for(n, [&](size_t q) {
-        size_t px = q % w, py = q / w;
-        Pred* pred = &preds[3 * q];
+    // A pixel's predictions for its samples 1..3 (jetaa.rs:123-245): none (compute it),
+    // the pixel's own non-escaped kind, or an escaped sample's columns.
+    auto predict = [&](size_t q, size_t px, size_t py, Pred* pred) {
         if (px == 0 || py == 0 || px + 2 >= w || py + 2 >= h) return;
         size_t r0 = real(q);
         if (!g_ok[q]) {
@@ -198,27 +201,31 @@ void jetaa_rebuild(uint8_t* cls, double* nu, float* de, uint16_t* normal, size_t
                 pred[k].normal = angle_of(ua.re + ub.re, ua.im + ub.im);
             }
         }
-    });
-    std::memset(todo, 0, 4 * n);
-    for (size_t q = 0; q < n; q++) {
+    };
+    // Each pixel predicts its samples 1..3 and writes them and their todo entries itself:
+    // the reads are of pixels' sample 0 and of the jets, the writes of samples 1..3 (and
+    // todo), so no pixel reads what another writes and the order does not matter.
+    par_for(n, [&](size_t q) {
         size_t px = q % w, py = q / w;
-        uint8_t inside = cls[real(q)];
+        Pred pred[3] = {};
+        predict(q, px, py, pred);
+        const uint8_t inside = cls[real(q)];
+        todo[idx(px, py, 0)] = 0;
         for (int k = 0; k < 3; k++) {
             size_t i = idx(px, py, k + 1);
-            const Pred& p = preds[3 * q + k];
-            if (p.what == 0) {
-                todo[i] = 1;
-            } else if (p.what == 1) {
+            const Pred& p = pred[k];
+            todo[i] = p.what == 0;
+            if (p.what == 1) {
                 // Class::new(kind, Heuristic), or the invalid class as it is
                 cls[i] = (inside & 3) != 3 ? (uint8_t)(inside & 3) : inside;
-            } else {
+            } else if (p.what == 2) {
                 cls[i] = KIND_ESCAPED;  // escaped, heuristic evidence
                 if (nu) nu[i] = p.nu;
                 if (de) de[i] = p.de;
                 if (normal) normal[i] = p.normal;
             }
         }
-    }
+    });
 }
 
 void jetaa_merge(uint8_t* cls, double* nu, float* de, uint16_t* normal, const uint8_t* fcls,
