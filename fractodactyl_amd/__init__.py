@@ -134,6 +134,9 @@ SIGNATURES = {
     "fdg_render": (C.c_int, [C.c_void_p, C.POINTER(_View), C.POINTER(_Params), C.c_int, C.c_int,
                              C.c_double, C.POINTER(_Columns), C.POINTER(_Outcomes),
                              C.POINTER(_Stats), C.c_char_p, C.c_size_t]),
+    "fdg_nucleus": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_uint64, C.c_uint32,
+                              C.c_uint32, C.c_char_p, C.c_char_p, C.c_size_t,
+                              C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
     "fdg_reference": (C.c_int, [C.c_void_p, C.POINTER(_View), C.POINTER(_Params), C.c_void_p,
                                 C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64),
                                 C.POINTER(C.c_uint32)]),
@@ -727,6 +730,21 @@ def jetaa_merge(to, frm, mask):
 def render_bla(view, params, form=FORM_F64, bla_eps=EPS, ctx=None, raw=False):
     """perturb_gpu::render_bla (frame.rs:78-84)."""
     return render(view, params, ROUTE_BLA, form, bla_eps, ctx, raw)
+
+
+def nucleus(re, im, period, bits, max_iters=60):
+    """Newton on z_P(c) = 0 from (re, im) at `bits` fraction bits (fdg_nucleus, host-only;
+    the reference's zone_build.rs:215-233 iteration): returns (re, im, Newton steps, log2
+    of the last step), the nucleus as decimal strings; FdgError when it escapes or does not
+    converge."""
+    cap = int(bits * 0.302) + 64
+    ro, io = C.create_string_buffer(cap), C.create_string_buffer(cap)
+    its, step = C.c_uint32(), C.c_double()
+    rc = lib().fdg_nucleus(None, re.encode(), im.encode(), int(period), int(bits),
+                           int(max_iters), ro, io, cap, C.byref(its), C.byref(step))
+    if rc < 0:
+        raise FdgError(rc, lib().fdg_last_error(None).decode())
+    return ro.value.decode(), io.value.decode(), its.value, step.value
 
 
 def reference(view, params, ctx=None):

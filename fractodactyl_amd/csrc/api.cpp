@@ -584,6 +584,38 @@ int fdg_reference(fdg_ctx* ctx_in, const fdg_view* v, const fdg_params* p, doubl
     });
 }
 
+int fdg_nucleus(fdg_ctx* ctx_in, const char* re, const char* im, uint64_t period,
+                uint32_t bits, uint32_t max_iters, char* re_out, char* im_out, size_t cap,
+                uint32_t* iterations, double* log2_step) {
+    return guarded<int>(ctx_in ? &ctx_in->err : nullptr, [&]() -> int {
+        // Host-only: a NULL context is allowed (errors then go to fdg_last_error(NULL)).
+        fdg_ctx local;
+        fdg_ctx* ctx = ctx_in ? ctx_in : &local;
+        struct Report {
+            fdg_ctx* c;
+            bool own;
+            ~Report() {
+                if (own && !c->err.empty()) g_create_error = c->err;
+            }
+        } report{ctx, ctx_in == nullptr};
+        if (!re || !im) return ctx->fail(FDG_EINVAL, "null start for the nucleus search");
+        if (bits > (1u << 20)) return ctx->fail(FDG_EINVAL, "nucleus search precision too large");
+        Nucleus n;
+        std::string err;
+        bool ok = nucleus_newton(re, im, period, bits, max_iters, &n, &err);
+        if (iterations) *iterations = n.iterations;
+        if (log2_step) *log2_step = n.log2_step;
+        if (!ok) return ctx->fail(FDG_EINVAL, err);
+        if (re_out && im_out) {
+            if (cap < n.re.size() + 1 || cap < n.im.size() + 1)
+                return ctx->fail(FDG_EINVAL, "nucleus output buffers too small");
+            std::memcpy(re_out, n.re.c_str(), n.re.size() + 1);
+            std::memcpy(im_out, n.im.c_str(), n.im.size() + 1);
+        }
+        return FDG_OK;
+    });
+}
+
 int fdg_build_bla(fdg_ctx* ctx, const double* re, const double* im, uint64_t len, double eps,
                   double dc_max, int64_t dc_exp, double* blocks, uint64_t cap_blocks,
                   uint32_t* levels) {

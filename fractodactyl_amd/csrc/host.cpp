@@ -874,6 +874,173 @@ void Orbit::tile(uint64_t steps) {
     }
 }
 
+// ------------------------------------------------------------------ nucleus
+
+namespace {
+
+// A complex value (re, im) 2^e with the larger part near 1: the Newton derivative, which
+// outgrows f64 within a few hundred steps of a deep period.
+struct CxExp {
+    double re, im;
+    int64_t e;
+    void renorm() {
+        double a = std::fmax(std::fabs(re), std::fabs(im));
+        if (!(a > 0.0) || !std::isfinite(a)) return;
+        int k;
+        std::frexp(a, &k);
+        re = std::ldexp(re, -k);
+        im = std::ldexp(im, -k);
+        e += k;
+    }
+};
+
+// v 2^ve (v finite) as a fixed-point magnitude of l limbs (64 (l - 1) fraction bits): v's
+// 53-bit integer mantissa placed at its bit position, the bits below the last limb dropped;
+// false when the value reaches 2^64.  Zero (also after truncation) is non-negative.
+bool fixed_from_scaled(double v, int64_t ve, size_t l, bool* neg, std::vector<uint64_t>* mag) {
+    mag->assign(l, 0);
+    *neg = false;
+    if (v == 0.0) return true;
+    int k;
+    double f = std::frexp(std::fabs(v), &k);                  // |v| = f 2^k, f in [0.5, 1)
+    uint64_t m = (uint64_t)std::ldexp(f, 53);                  // exact 53-bit integer
+    int64_t pos = ve + k - 53 + 64 * ((int64_t)l - 1);         // bit position of m's lsb
+    if (pos + 53 > 64 * (int64_t)l) return false;
+    if (pos <= -64) return true;
+    if (pos < 0) {
+        m >>= -pos;
+        pos = 0;
+    }
+    size_t limb = (size_t)(pos / 64), bit = (size_t)(pos % 64);
+    (*mag)[limb] |= m << bit;
+    if (bit && limb + 1 < l) (*mag)[limb + 1] |= m >> (64 - bit);
+    *neg = v < 0.0 && limbs::nonzero(mag->data(), l);
+    return true;
+}
+
+// A fixed-point value as a decimal with `digits` fraction digits (truncated).
+std::string fixed_to_decimal(bool neg, const uint64_t* mag, size_t l, size_t digits) {
+    std::string out = neg ? "-" : "";
+    out += std::to_string(mag[l - 1]);
+    out += '.';
+    std::vector<uint64_t> frac(mag, mag + (l - 1));
+    while (digits > 0) {
+        unsigned k = (unsigned)std::min<size_t>(digits, 19);
+        uint64_t p = pow10(k), carry = 0;
+        for (auto& x : frac) {
+            u128 t = (u128)x * p + carry;
+            x = (uint64_t)t;
+            carry = (uint64_t)(t >> 64);
+        }
+        std::string chunk = std::to_string(carry);
+        out += std::string(k - chunk.size(), '0') + chunk;
+        digits -= k;
+    }
+    return out;
+}
+
+}  // namespace
+
+bool nucleus_newton(const std::string& re, const std::string& im, uint64_t period, uint64_t bits,
+                    unsigned max_iters, Nucleus* out, std::string* err) {
+    if (period < 1 || bits < 64) {
+        *err = "the nucleus search needs a period >= 1 and at least 64 bits";
+        return false;
+    }
+    const size_t l = Fixed::limbs_for(bits);
+    Fixed c[2];
+    if (!Fixed::parse(re, l, &c[0]) || !Fixed::parse(im, l, &c[1])) {
+        *err = "bad decimal for the nucleus search's start";
+        return false;
+    }
+    std::vector<uint64_t> buf(8 * l + limbs::sqr_scratch(l));
+    uint64_t *x = &buf[0], *y = &buf[l], *s = &buf[2 * l], *t = &buf[3 * l],
+             *prod = &buf[4 * l], *x2 = &buf[6 * l], *y2 = &buf[7 * l],
+             *scratch = &buf[8 * l];
+    std::vector<uint64_t> s2(l);
+    const double limit = -((double)bits - 8.0);  // converged: |step| < 2^-(bits - 8)
+    out->iterations = 0;
+    out->converged = false;
+    out->log2_step = INFINITY;
+    for (unsigned it = 0; it < max_iters; it++) {
+        // P steps of z' = z^2 + c in the orbit's truncated fixed point (orbit.rs:14-43) and
+        // dz' = 2 z dz + 1 in floating point with a carried exponent, from z = dz = 0
+        std::fill(x, x + l, 0);
+        std::fill(y, y + l, 0);
+        bool xn = false, yn = false;
+        CxExp dz{0.0, 0.0, 0};
+        for (uint64_t k = 0; k < period; k++) {
+            double zr = to_f64_limbs(xn, x, l), zi = to_f64_limbs(yn, y, l);
+            if (zr * zr + zi * zi > 4294967296.0) {
+                *err = "the nucleus search's orbit escaped: the start is too far from a "
+                       "nucleus of this period";
+                return false;
+            }
+            CxExp p{2.0 * (zr * dz.re - zi * dz.im), 2.0 * (zr * dz.im + zi * dz.re), dz.e};
+            if (p.e > 100) {
+                dz = p;  // (the + 1 lies below the mantissa)
+            } else if (p.e < -100) {
+                dz = CxExp{1.0, 0.0, 0};
+            } else {
+                dz = CxExp{std::ldexp(p.re, (int)p.e) + 1.0, std::ldexp(p.im, (int)p.e), 0};
+            }
+            dz.renorm();
+            limbs::sqr(x, prod, x2, l, scratch);
+            limbs::sqr(y, prod, y2, l, scratch);
+            limbs::add_signed(xn, x, yn, y, s, l);
+            limbs::sqr(s, prod, s2.data(), l, scratch);
+            bool tn = limbs::add_signed(false, x2, true, y2, t, l);
+            xn = limbs::add_signed(tn, t, c[0].neg, c[0].mag.data(), x, l);
+            limbs::add(x2, y2, s, l);
+            tn = limbs::add_signed(false, s2.data(), true, s, t, l);
+            yn = limbs::add_signed(tn, t, c[1].neg, c[1].mag.data(), y, l);
+        }
+        // step = z_P / z_P', with z_P's parts brought to one exponent
+        double mr = 0, mi = 0;
+        int64_t er = INT64_MIN, ei = INT64_MIN;
+        bool hr = frexp_limbs(xn, x, l, &mr, &er), hi = frexp_limbs(yn, y, l, &mi, &ei);
+        out->iterations = it + 1;
+        if (!hr && !hi) {  // z_P is exactly 0: c is the nucleus to the last bit
+            out->converged = true;
+            out->log2_step = -INFINITY;
+            break;
+        }
+        int64_t E = std::max(hr ? er : INT64_MIN, hi ? ei : INT64_MIN);
+        double ar = hr ? std::ldexp(mr, (int)std::max<int64_t>(er - E, -2000)) : 0.0;
+        double ai = hi ? std::ldexp(mi, (int)std::max<int64_t>(ei - E, -2000)) : 0.0;
+        double den = dz.re * dz.re + dz.im * dz.im;
+        double qr = (ar * dz.re + ai * dz.im) / den, qi = (ai * dz.re - ar * dz.im) / den;
+        int64_t qe = E - dz.e;
+        if (!std::isfinite(qr) || !std::isfinite(qi)) {
+            *err = "the nucleus search's derivative vanished";
+            return false;
+        }
+        double mag = std::sqrt(qr * qr + qi * qi);
+        out->log2_step = std::log2(mag) + (double)qe;
+        // c -= step, in the fixed point (the part of the step below the last limb drops)
+        for (int part = 0; part < 2; part++) {
+            bool sn;
+            std::vector<uint64_t> sm, nc(l);
+            if (!fixed_from_scaled(part == 0 ? qr : qi, qe, l, &sn, &sm)) {
+                *err = "the nucleus search diverged";
+                return false;
+            }
+            c[part].neg = limbs::add_signed(c[part].neg, c[part].mag.data(), !sn, sm.data(),
+                                            nc.data(), l);
+            c[part].mag = std::move(nc);
+        }
+        if (out->log2_step < limit) {
+            out->converged = true;
+            break;
+        }
+    }
+    const size_t digits = (size_t)std::ceil((double)bits * 0.30102999566398120) + 4;
+    out->re = fixed_to_decimal(c[0].neg, c[0].mag.data(), l, digits);
+    out->im = fixed_to_decimal(c[1].neg, c[1].mag.data(), l, digits);
+    if (!out->converged) *err = "the nucleus search did not converge";
+    return out->converged;
+}
+
 uint64_t Orbit::nucleus_period_of(const double* re, const double* im, uint64_t len) {
     if (len < 2) return 0;
     double record = hypot_ref(re[1], im[1]);

@@ -101,6 +101,19 @@ struct Orbit {
     static uint64_t nucleus_period_of(const double* re, const double* im, uint64_t len);
 };
 
+// Newton on the period-P nucleus equation z_P(c) = 0 from c (decimal strings) at `bits`
+// fraction bits (zone_build.rs:215-233's iteration): z in the orbit's truncated fixed point,
+// the derivative z_P' in floating point with a carried exponent, c -= z_P / z_P' until
+// the step is below 2^-(bits - 8) (converged) or max_iters steps have run.
+struct Nucleus {
+    std::string re, im;      // the last c, as decimals to the working precision
+    unsigned iterations = 0;
+    bool converged = false;
+    double log2_step = 0;    // log2 |z_P / z_P'| of the last step
+};
+bool nucleus_newton(const std::string& re, const std::string& im, uint64_t period, uint64_t bits,
+                    unsigned max_iters, Nucleus* out, std::string* err);
+
 // Reference::error_radius (reference.rs:122-148): q[m] >= |A_m - Z_m|; bits < 0 for an f64
 // orbit from (c_re, c_im), else the fixed-point fraction bits.
 void error_radius(const double* re, const double* im, uint64_t len, int64_t bits, double c_re,

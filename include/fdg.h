@@ -379,6 +379,19 @@ int fdg_film_jet(fdg_film *film, fdg_columns *outs, uint64_t *computed);
 int fdg_reference(fdg_ctx *ctx, const fdg_view *view, const fdg_params *params, double *re,
                   double *im, uint64_t cap, uint64_t *len, uint32_t *bits);
 
+/* Newton on the period-P nucleus equation z_P(c) = 0 from the start (re, im), host-only
+ * (ctx may be NULL): the iteration of the reference's nucleus finder
+ * (crates/fd-cli/src/zone_build.rs:215-233), with z in the reference orbit's truncated
+ * fixed point at `bits` fraction bits (fd_fixed::orbit, orbit.rs:14-43) and the derivative
+ * in floating point with a carried exponent.  Stops once a step is below 2^-(bits - 8)
+ * and writes the nucleus as decimals with ceil(bits log10 2) + 4 fraction digits into
+ * re_out / im_out (cap bytes each, NUL-terminated), the Newton steps taken and log2 of the
+ * last step.  FDG_EINVAL for a bad start, an orbit that escapes, a vanishing derivative, or
+ * no convergence within max_iters (iterations and log2_step are written then too). */
+int fdg_nucleus(fdg_ctx *ctx, const char *re, const char *im, uint64_t period, uint32_t bits,
+                uint32_t max_iters, char *re_out, char *im_out, size_t cap, uint32_t *iterations,
+                double *log2_step);
+
 /* Bla::build_scaled (crates/fd-kernel/src/bla.rs:161-181) computed on the GPU as a
  * level-by-level tree reduction.  Blocks of levels 1.. are written level-major, 7 doubles
  * each (a.re a.im b.re b.im r alpha beta, Block::to_array order, bla.rs:321-325), level j

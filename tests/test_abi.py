@@ -118,6 +118,36 @@ def test_nucleus_orbit_repeats_exactly_and_equals_the_oracle(k):
     assert a_re[P + 1] == a_re[1] and a_im[P + 1] == a_im[1]
 
 
+@pytest.mark.parametrize("k,digits", [(1, 25), (3, 1010)])
+def test_newton_nucleus_reproduces_the_scene_nuclei(k, digits):
+    # fdg_nucleus from a truncated start converges to the nucleus the scene holds: config 1's
+    # seeded Newton nucleus (period 1611, 256 bits, found by Python-integer Newton at 288
+    # bits) and config 3's ladder nucleus from the reference's own rung list (period 16116,
+    # 3461 bits, the reference's decimals carry ~3720 bits)
+    from decimal import Decimal, getcontext
+    from fractodactyl_amd.scenes import scene
+    getcontext().prec = 1500
+    sc = scene(k)
+    nuc = sc["nucleus"]
+    re0, im0 = (nuc["re"], nuc["im"]) if k == 1 else (sc["center_re"], sc["center_im"])
+    bits = {1: 256, 3: 3461}[k]
+    cut = lambda s: s[:s.index(".") + 1 + digits]
+    r, i, its, step = fa.nucleus(cut(re0), cut(im0), nuc["period"], bits)
+    assert its <= 10 and step < -(bits - 8)
+    err = max(abs(Decimal(r) - Decimal(re0)), abs(Decimal(i) - Decimal(im0)))
+    assert err < Decimal(2) ** -(bits - 8), float(err)
+
+
+def test_newton_nucleus_refuses_what_it_cannot_converge():
+    # a start outside the period-16116 minibrot's basin escapes before the period
+    from fractodactyl_amd.scenes import scene
+    sc = scene(3)
+    with pytest.raises(fa.FdgError, match="escaped"):
+        fa.nucleus(sc["center_re"][:40], sc["center_im"][:40], 16116, 3461)
+    with pytest.raises(fa.FdgError):
+        fa.nucleus("0.25", "0", 0, 256)  # no period
+
+
 def test_host_reference_rejects_bad_requests():
     v, p = fa._view(fa.View("abc", "0", "1e-3")), fa._params(fa.Params(8, 8))
     ln, bits = C.c_uint64(), C.c_uint32()
