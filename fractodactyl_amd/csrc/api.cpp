@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <new>
 #include <string>
 #include <thread>
 #include <vector>
@@ -528,6 +529,10 @@ void fdg_destroy(fdg_ctx* c) {
         (void)hipStreamSynchronize(c->stream);
         for (auto& e : c->ev)
             if (e) (void)hipEventDestroy(e);
+        if (c->spare) {
+            c->spare->release();
+            delete c->spare;
+        }
         if (c->pinned) (void)hipHostFree(c->pinned);
         if (c->small) (void)hipHostFree(c->small);
         (void)hipStreamDestroy(c->stream);
@@ -783,6 +788,11 @@ static int frame_open(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int 
     f->len = o.len();
     f->bits = o.bits;
     f->nucleus = o.nucleus_period();
+    if (ctx->spare) {
+        f->buf.adopt(*ctx->spare);  // the last closed frame's allocations, grown as needed
+        delete ctx->spare;
+        ctx->spare = nullptr;
+    }
     rc = upload_orbit(ctx, &f->buf.orbit, o, &f->transfer_seconds);
     if (!rc) rc = f->buf.outputs(ctx, f->count);
     if (!rc && form != FDG_FORM_F64) {
@@ -1266,6 +1276,14 @@ void fdg_frame_close(fdg_frame* f) {
         if (!f) return;
         (void)hipSetDevice(f->ctx->device);  // teardown: nothing to report to
         (void)hipStreamSynchronize(f->ctx->stream);
+        if (!f->ctx->spare) {
+            // kept for the next frame on this context (freed by fdg_destroy)
+            Buffers* keep = new (std::nothrow) Buffers();
+            if (keep) {
+                keep->adopt(f->buf);
+                f->ctx->spare = keep;
+            }
+        }
         f->buf.release();
         f->mask.release();
         f->series_b.release();

@@ -92,8 +92,16 @@ inline double now_s() {
 
 }  // namespace fdg
 
+namespace fdg {
+struct Buffers;
+}
+
 struct fdg_ctx {
     int device = 0;
+    // The device buffers of the last frame closed on this context, handed to the next frame
+    // opened (an end-to-end caller that opens and closes a frame per render reuses them
+    // instead of allocating ~53 bytes per sample each time); freed with the context.
+    fdg::Buffers* spare = nullptr;
     int cus = 256;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -164,6 +172,14 @@ struct Buffers {
     }
     // Per-sample outcome arrays for n samples, the counters and one FrameDev.
     int outputs(fdg_ctx* ctx, size_t n);
+    // Take over `from`'s allocations (from then owns none); the table-build state starts
+    // afresh.
+    void adopt(Buffers& from) {
+        *this = from;
+        from = Buffers();
+        bla_stride = bla_steps = 0;
+        valid_clean = false;
+    }
 };
 
 struct Frame {

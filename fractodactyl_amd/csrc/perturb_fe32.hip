@@ -120,6 +120,7 @@ struct Hot32 {
     const FDG_GLOBAL v4f* ab;       // A, B mantissas: (A.re, A.im) 2^ea, (B.re, B.im) 2^eb
     const FDG_GLOBAL v2i* abexp;    // (ea, eb): |A|, |B| exceed FP32's range on long blocks
     const FDG_GLOBAL double* r2;
+    ConstR2 r2c;                    // the same radii, for scalar loads
     const FDG_GLOBAL double* alpha;  // the table's state envelopes (bla.rs:91-111)
     const FDG_GLOBAL double* beta;
     uint32_t last, max_iter, nlevels, steps, count0;
@@ -230,8 +231,20 @@ __device__ inline int advance32(Lane32& s, const Hot32& h, int32_t aexp, float& 
             uint32_t nb = s.nucleus ? (s.n == 0 ? 1u : s.nucleus + 1) : 0xffffffffu;
             b = b < nb ? b : nb;
             b = b > s.n ? b - s.n : 0;
-            int cap = find_cap(k, h.nlevels, b);
-            if (cap >= 0 && d2 < s.R0) lvl = find_level_down(h.r2, 0u, (uint32_t)h.stride, k, cap, d2);
+            if (d2 < s.R0) {
+                // lanes in lockstep (one start and budget in every searching lane) take the
+                // cap and the radii through the scalar unit
+                const uint32_t k0 = __builtin_amdgcn_readfirstlane(k),
+                               b0 = __builtin_amdgcn_readfirstlane(b);
+                if (__ballot(k != k0 || b != b0) == 0) {
+                    int cap = find_cap(k0, h.nlevels, b0);
+                    if (cap >= 0)
+                        lvl = find_level_down_uniform(h.r2c, 0u, (uint32_t)h.stride, k0, cap, d2);
+                } else {
+                    int cap = find_cap(k, h.nlevels, b);
+                    if (cap >= 0) lvl = find_level_down(h.r2, 0u, (uint32_t)h.stride, k, cap, d2);
+                }
+            }
         }
     }
     float mr, mi;  // this step's multiplier (for the contraction product), times 2^mexp
@@ -483,6 +496,7 @@ __global__ void __launch_bounds__(kBlock32, 4) perturb_fe32(const FrameDev* __re
     h.ab = (const FDG_GLOBAL v4f*)(fp->ab32);
     h.abexp = (const FDG_GLOBAL v2i*)(fp->abexp32);
     h.r2 = (const FDG_GLOBAL double*)(fp->bla.r2);
+    h.r2c = (ConstR2)fp->bla.r2;
     h.alpha = (const FDG_GLOBAL double*)(fp->bla_alpha);
     h.beta = (const FDG_GLOBAL double*)(fp->bla_beta);
     h.last = (uint32_t)fp->len - 1;
