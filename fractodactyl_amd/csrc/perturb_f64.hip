@@ -424,6 +424,7 @@ struct Lane {
     int32_t delta_exp, derivative_exp;  // scaled tier only
     uint32_t m, n, sn, hold, chk, nucleus, blocks, skipped, plain;
     uint32_t flags;                     // kPending | kPredicted | kConStarted | returns << 4
+    uint32_t bend;                      // where a lookup's step budget ends (budget_end)
     uint32_t tb;                        // batches: first block of this frame's BLA table
     int32_t oe;                         // batches: this frame's offset exponent h_e
 };
@@ -435,6 +436,21 @@ constexpr uint32_t kPending = 1, kPredicted = 2, kConStarted = 4;
 constexpr uint32_t kCanonX = 256, kCanonD = 512;
 // series_carry's mark of a sample running from the series start
 constexpr uint32_t kSeriesTry = 0xffffffffu;
+
+// The iterate where Bla::find's step budget ends (sample.rs:253-261): the hold's end while
+// a near-return is awaited (ordinary tiers), no bound while a deferred save is pending, else
+// the next scheduled save; at most max_iter and the nucleus test iterate p + 1.  (The
+// reference's budget of 1 at n = 0 never meets a lookup: n = 0 has m = 0, where none is
+// made.)  It reads only state that changes at a near-return, a save, a hold's expiry, a
+// deferred save and the nucleus test, where it is recomputed, so a lookup subtracts n.
+template <bool EXT>
+__device__ inline uint32_t budget_end(const Lane& s, uint32_t max_iter) {
+    uint32_t b = EXT ? s.chk
+                     : (s.hold > 0 ? s.sn + s.hold : ((s.flags & kPending) ? 0xffffffffu : s.chk));
+    b = b < max_iter ? b : max_iter;
+    uint32_t nb = s.nucleus ? s.nucleus + 1 : 0xffffffffu;
+    return b < nb ? b : nb;
+}
 
 __device__ inline uint32_t np2_above(uint32_t n) {
     // next_power_of_two(n + 1) for 1 <= n < 2^31
@@ -526,6 +542,7 @@ __device__ inline void lane_start(Lane& s, Rare& r, const Hot& h, uint32_t nucle
     r.conr = 0.0;
     r.coni = 0.0;
     r.cone = 0;
+    s.bend = budget_end<EXT>(s, h.max_iter);
 }
 
 // Re-derive dc in the delta's units after its exponent changed (sample.rs:300-302).
@@ -586,6 +603,7 @@ __device__ inline void series_start(Lane& s, const Hot& h, const FrameDev* __res
     uint32_t chk = np2_above(m);
     s.chk = chk > 16u ? chk : 16u;
     s.nucleus = 0;
+    s.bend = budget_end<false>(s, h.max_iter);
 }
 
 enum : int { RUNNING = -1 };
@@ -642,6 +660,7 @@ __device__ void ext_rebase(Lane& s, const Hot& h, int32_t oe, double R00, double
     s.Z1i = h.Z1i;
     if (scale_of<true>(s) < kTiny) {
         s.nucleus = 0;
+        s.bend = budget_end<true>(s, h.max_iter);
         if (D) {
             double qr = 2.0 * (s.xr * s.dr - s.xi * s.di);
             double qi = 2.0 * (s.xr * s.di + s.xi * s.dr);
@@ -694,13 +713,7 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
             // whose lanes all fail it skips the budget arithmetic
             if (d2 < s.R0) {
                 // holds and deferred saves exist on the ordinary tiers only
-                uint32_t b = EXT ? s.chk
-                                 : (s.hold > 0 ? s.sn + s.hold
-                                               : ((s.flags & kPending) ? 0xffffffffu : s.chk));
-                b = b < h.max_iter ? b : h.max_iter;
-                uint32_t nb = s.nucleus ? (s.n == 0 ? 1u : s.nucleus + 1) : 0xffffffffu;
-                b = b < nb ? b : nb;
-                b = b > s.n ? b - s.n : 0;
+                uint32_t b = s.bend > s.n ? s.bend - s.n : 0u;
                 // Lanes in lockstep (the same table, start and budget in every searching
                 // lane) take the cap and the radii through the scalar unit.
                 const uint32_t k0 = __builtin_amdgcn_readfirstlane(k),
@@ -835,14 +848,26 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
     double pr = scale_of<EXT>(s) * s.xr, pi = scale_of<EXT>(s) * s.xi;
     double fr = Zn.x + pr, fi = Zn.y + pi;
     double f2 = fr * fr + fi * fi;
-    if (f2 > h.esc2) {
+    const bool escaped = f2 > h.esc2;
+    if (EXT && escaped) {
         ofr = fr;
         ofi = fi;
         return 0;  // Escaped
     }
+    // The ordinary tiers leave by one exit: a lane that escaped (or is found interior)
+    // here runs the checks below too, which change only state its outcome does not read
+    // (the orbit position, the Brent/hold state and the LDS slot; n, the operation counts
+    // and dz/dc are final), and keeps the first outcome.
+    int out = escaped ? 0 : RUNNING;
+    ofr = fr;
+    ofi = fi;
     if (s.nucleus && s.n == s.nucleus + 1) {
-        if ((s.flags & kConStarted) && scaled_below(r.conr, r.coni, r.cone, 1e-6)) return 1;
+        if ((s.flags & kConStarted) && scaled_below(r.conr, r.coni, r.cone, 1e-6)) {
+            if (EXT) return 1;
+            out = escaped ? 0 : 1;
+        }
         s.nucleus = 0;
+        s.bend = budget_end<EXT>(s, h.max_iter);
     }
     double delta2 = pr * pr + pi * pi;
     bool rebase = (EXT ? f2 <= delta2 : f2 < delta2) || s.m == h.last;
@@ -892,34 +917,43 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
             s.si = fi;
             s.sn = s.n;
             s.chk = s.chk * 2;
+            s.bend = budget_end<true>(s, h.max_iter);
         }
         return RUNNING;
     }
-    // ordinary tiers: rebase onto Z_0 = 0 (sample.rs:451), by selects
-    s.xr = rebase ? fr : s.xr;
-    s.xi = rebase ? fi : s.xi;
-    s.m = rebase ? 0u : s.m;
-    s.Zr = rebase ? 0.0 : Zn.x;
-    s.Zi = rebase ? 0.0 : Zn.y;
-    s.Z1r = rebase ? h.Z1r : Znext.x;
-    s.Z1i = rebase ? h.Z1i : Znext.y;
+    // ordinary tiers: rebase onto Z_0 = 0 (sample.rs:451)
+    s.Zr = Zn.x;
+    s.Zi = Zn.y;
+    s.Z1r = Znext.x;
+    s.Z1i = Znext.y;
     s.R0 = r0next;  // after a rebase m = 0 and no block is looked up
+    if (rebase) {
+        s.xr = fr;
+        s.xi = fi;
+        s.m = 0u;
+        s.Zr = 0.0;
+        s.Zi = 0.0;
+        s.Z1r = h.Z1r;
+        s.Z1i = h.Z1i;
+    }
     // |delta|^2 after the rebase select: f2 when rebased, else delta2 (same expressions)
     bool res = (rebase ? f2 : delta2) > kResolvable * kResolvable * f2;
     double dist = fabs(fr - s.sr) + fabs(fi - s.si);
     double size = fabs(s.sr) + fabs(s.si);
-    if (res && dist < 1e-3 * size) {
-        if (near_return(s, r, fr, fi, size)) return 1;  // Interior
-        return RUNNING;
-    }
+    const bool near = res && dist < 1e-3 * size;
+    if (near && near_return(s, r, fr, fi, size) && out == RUNNING) out = 1;  // Interior
+    // A lane that met a near-return now holds from this iterate (hold = its period >= 1,
+    // sn = n): none of the updates below applies to it.  (Sequential rather than if/else,
+    // so each branch updates the state in place.)
     uint32_t h0 = s.hold;
-    bool expired = h0 > 0 && s.n >= s.sn + h0;
+    bool expired = !near && h0 > 0 && s.n >= s.sn + h0;
     if (expired) {
         s.hold = 0;
         s.flags = (s.flags & ~kPredicted) | kPending;
     }
     // !res: a save that falls on an unjudgeable iterate is deferred (pending)
     bool defer = !res && h0 == 0 && s.n >= s.chk;
+    bool newly = defer && !(s.flags & kPending);
     s.flags |= defer ? kPending : 0u;
     // res: save at the deferred or scheduled point unless a return is being awaited
     bool save = res && (h0 == 0 || expired) && ((s.flags & kPending) || s.n >= s.chk);
@@ -931,7 +965,8 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         s.sn = s.n;
         s.flags &= ~kPending;
     }
-    return RUNNING;
+    if (near || expired || newly || save) s.bend = budget_end<false>(s, h.max_iter);
+    return out;
 }
 
 __device__ inline uint32_t lanes_below(uint64_t mask) {
