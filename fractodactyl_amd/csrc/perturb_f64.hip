@@ -849,23 +849,17 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
     double fr = Zn.x + pr, fi = Zn.y + pi;
     double f2 = fr * fr + fi * fi;
     const bool escaped = f2 > h.esc2;
-    if (EXT && escaped) {
-        ofr = fr;
-        ofi = fi;
-        return 0;  // Escaped
-    }
-    // The ordinary tiers leave by one exit: a lane that escaped (or is found interior)
-    // here runs the checks below too, which change only state its outcome does not read
-    // (the orbit position, the Brent/hold state and the LDS slot; n, the operation counts
-    // and dz/dc are final), and keeps the first outcome.
+    // One exit: a lane that escaped (or is found interior) here runs the checks below
+    // too, which change only state its outcome does not read (the orbit position, the
+    // Brent/hold state and the LDS slot; n, the operation counts and dz/dc are final:
+    // the scaled tier's rebase, which can change dz/dc, skips it), and keeps the first
+    // outcome.
     int out = escaped ? 0 : RUNNING;
     ofr = fr;
     ofi = fi;
     if (s.nucleus && s.n == s.nucleus + 1) {
-        if ((s.flags & kConStarted) && scaled_below(r.conr, r.coni, r.cone, 1e-6)) {
-            if (EXT) return 1;
+        if ((s.flags & kConStarted) && scaled_below(r.conr, r.coni, r.cone, 1e-6))
             out = escaped ? 0 : 1;
-        }
         s.nucleus = 0;
         s.bend = budget_end<EXT>(s, h.max_iter);
     }
@@ -877,11 +871,14 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         s.Z1r = Znext.x;
         s.Z1i = Znext.y;
         s.R0 = r0next;
-        if (rebase) {
+        const bool rb = rebase && !escaped;
+        if (rb) {
             double R00 = B ? (L && h.count0 > 0 ? h.r2[tb] : 0.0) : h.R00;
             ext_rebase<D>(s, h, oe, R00, fr, fi, f2, Zn.x, Zn.y);
             s.flags &= ~(kCanonX | kCanonD);
-        } else if (!(s.flags & kCanonX)) {
+        }
+        // (in sequence rather than if/else, so each branch updates the state in place)
+        if (!rb && !(s.flags & kCanonX)) {
             // (a block's canonical delta has its larger part in [1, 2): nothing to rescale)
             double magnitude = fmax(fabs(s.xr), fabs(s.xi));
             int32_t k = 0;
@@ -906,8 +903,9 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
             double size = fabs(s.sr) + fabs(s.si);
             if (dist < 1e-13 * size + 1e-300) {
                 if (contracting(reinterpret_cast<const double*>((const v2d*)h.orbit), h.len, s.m, pr, pi,
-                                s.ar * exp2i(oe), s.ai * exp2i(oe), s.n - s.sn))
-                    return 2;  // Unresolved
+                                s.ar * exp2i(oe), s.ai * exp2i(oe), s.n - s.sn) &&
+                    out == RUNNING)
+                    out = 2;  // Unresolved
                 s.flags += 16;
             }
         }
@@ -919,7 +917,7 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
             s.chk = s.chk * 2;
             s.bend = budget_end<true>(s, h.max_iter);
         }
-        return RUNNING;
+        return out;
     }
     // ordinary tiers: rebase onto Z_0 = 0 (sample.rs:451)
     s.Zr = Zn.x;
