@@ -732,50 +732,49 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
     }
     double Zmr = s.Zr, Zmi = s.Zi;
     if (!EXT) {
-        // One instruction stream for both a block (delta' = A delta + B dc, D' = A D + B)
-        // and a plain step (delta' = 2 Z delta + delta^2 + dc, D' = 2 z D + 1): the
-        // operands are selected per lane so that each expression rounds exactly as the
-        // reference's (multiplying by c = 1 or by 1.0, and adding -0.0, are exact).
-        bool blk = lvl >= 0;
-        double a_r = 0.0, a_i = 0.0, b_r = 1.0, b_i = 0.0;
+        // A block (delta' = A delta + B dc, D' = A D + B, bla.rs) or a plain step
+        // (delta' = 2 Z delta + delta^2 + dc, D' = 2 z D + 1, sample.rs:343-381), each in the
+        // reference's operation order.  A wave whose lanes take both runs both branches.
+        const bool blk = lvl >= 0;
+        const double xr = s.xr, xi = s.xi;
+        const double fr = Zmr + xr, fi = Zmi + xi;  // z = Z_m + delta (plain step)
+        double mr, mi;                              // the step's multiplier
         uint32_t blen = 1;
         if (blk) {
             uint32_t at = bla_block_index(tb, h.stride32, s.m - 1, lvl);
             double4 ab = ld4(h.ab, at);
-            a_r = ab.x;
-            a_i = ab.y;
-            b_r = ab.z;
-            b_i = ab.w;
+            const double a_r = ab.x, a_i = ab.y, b_r = ab.z, b_i = ab.w;
             blen = 2u << lvl;
             Zn = ld2(h.orbit, s.m + blen);
             Znext = ld2(h.orbit, s.m + blen + 1);
             uint32_t i0 = (s.m + blen - 1) >> 1;
             r0next = i0 < h.count0 ? h.r2[tb + i0] : 0.0;
+            mr = a_r;
+            mi = a_i;
+            if (D) {
+                double dr = s.dr, di = s.di;
+                s.dr = a_r * dr - a_i * di + b_r;
+                s.di = a_r * di + a_i * dr + b_i;
+            }
+            s.xr = a_r * xr - a_i * xi + b_r * s.dc_r - b_i * s.dc_i;
+            s.xi = a_r * xi + a_i * xr + b_r * s.dc_i + b_i * s.dc_r;
+        } else {
+            mr = 2.0 * fr;
+            mi = 2.0 * fi;
+            if (D) {
+                double dr = s.dr, di = s.di;
+                s.dr = 2.0 * (fr * dr - fi * di) + 1.0;
+                s.di = 2.0 * (fr * di + fi * dr);
+            }
+            s.xr = 2.0 * (Zmr * xr - Zmi * xi) + xr * xr - xi * xi + s.dc_r;
+            s.xi = 2.0 * (Zmr * xi + Zmi * xr + xr * xi) + s.dc_i;
         }
-        double xr = s.xr, xi = s.xi;
-        double fr = Zmr + xr, fi = Zmi + xi;           // z = Z_m + delta (plain step)
-        double c = blk ? 1.0 : 2.0;
-        double mr = blk ? a_r : 2.0 * fr, mi = blk ? a_i : 2.0 * fi;  // step multiplier
         if (s.nucleus && s.n >= 1) {
             bool st = s.flags & kConStarted;
             con_multiply(r.conr, r.coni, r.cone, st, mr, mi);
             s.flags |= kConStarted;
         }
         if (s.hold > 0) mu_multiply(r.mur, r.mui, r.mue, mr, mi);
-        if (D) {
-            double p2r = blk ? a_r : fr, p2i = blk ? a_i : fi;
-            double dr = s.dr, di = s.di;
-            s.dr = c * (p2r * dr - p2i * di) + b_r * inverse_of<EXT>(s);
-            s.di = c * (p2r * di + p2i * dr) + (blk ? b_i : -0.0) * inverse_of<EXT>(s);
-        }
-        double pr_ = blk ? a_r : Zmr, pi_ = blk ? a_i : Zmi;
-        double ur = blk ? b_r : xr, vr = blk ? s.dc_r : xr;
-        double wr = blk ? b_i : xi, qr = blk ? s.dc_i : xi;
-        double yr = blk ? -0.0 : s.dc_r;
-        double ui = blk ? b_r : xr, vi = blk ? s.dc_i : xi;
-        double wi = blk ? b_i : s.dc_i, xi2 = blk ? s.dc_r : 1.0;
-        s.xr = c * (pr_ * xr - pi_ * xi) + ur * vr - wr * qr + yr;
-        s.xi = c * (pr_ * xi + pi_ * xr + ui * vi) + wi * xi2;
         s.m += blen;
         s.n += blen;
         s.blocks += blk ? 1u : 0u;
