@@ -713,7 +713,7 @@ static int frame_open(fdg_ctx* ctx, const fdg_view* v, const fdg_params* p, int 
     // by one chunk per wave.
     if (count64 >= (1ull << 31)) return ctx->fail(FDG_EINVAL, "grid too large");
     // The kernels count iterates in 32 bits.
-    if (p->max_iter >= (1ull << 31)) return ctx->fail(FDG_EINVAL, "max_iter must be < 2^31");
+    if (p->max_iter >= (1ull << 27)) return ctx->fail(FDG_EINVAL, "max_iter must be < 2^27");
     if (route == FDG_ROUTE_BLA) {
         // grid.rs:378-413 bla_checks, and bla.rs:27-33 check
         if (std::isnan(p->escape_radius) || p->escape_radius < 4.0)

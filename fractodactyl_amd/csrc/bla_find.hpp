@@ -9,6 +9,23 @@
 
 namespace fdg {
 
+// p[i] for an array in global memory below 4 GiB: the byte offset is formed in 32 bits, so
+// the access takes the scalar-base + 32-bit vector-offset form (p wave-uniform) instead of
+// a 64-bit address computed per lane.
+template <class T>
+__device__ inline T load32(const __attribute__((address_space(1))) T* p, uint32_t i) {
+    typedef const __attribute__((address_space(1))) char* Bytes;
+    typedef const __attribute__((address_space(1))) T* Elem;
+    return *(Elem)((Bytes)p + i * (uint32_t)sizeof(T));
+}
+
+// p[i] for a table that may exceed 4 GiB (WIDE: batches of frames, each with its own
+// table, in one array), else load32.
+template <bool WIDE, class T>
+__device__ inline T table_load(const __attribute__((address_space(1))) T* p, uint32_t i) {
+    return WIDE ? p[(uint64_t)i] : load32(p, i);
+}
+
 // Levels whose radii one lookup step loads together.
 constexpr int kFindGroup = 4;
 
@@ -44,8 +61,8 @@ __device__ inline uint32_t bla_block_index(uint32_t tb, uint32_t stride, uint32_
     return (tb + stride) - ((stride - (k >> 1)) >> i);
 }
 
-template <class P>
-__device__ inline int find_level_down(P r2, uint32_t tb, uint32_t stride, uint32_t k, int cap,
+template <bool WIDE>
+__device__ inline int find_level_down(const __attribute__((address_space(1))) double* r2, uint32_t tb, uint32_t stride, uint32_t k, int cap,
                                       double d2) {
     const uint32_t top = tb + stride, span = stride - (k >> 1);
     int lvl = 0;
@@ -55,7 +72,7 @@ __device__ inline int find_level_down(P r2, uint32_t tb, uint32_t stride, uint32
         for (int q = 0; q < kFindGroup; q++) {
             int lv = hi - q;
             int lc = lv >= 1 ? lv : 1;  // (levels below 1 are loaded at level 1, not taken)
-            rr[q] = r2[top - (span >> lc)];
+            rr[q] = table_load<WIDE>(r2, top - (span >> lc));
         }
         int got = -1;
 #pragma unroll

@@ -317,7 +317,7 @@ int fdg_film_open(fdg_ctx* ctx, const fdg_view* views, uint32_t nframes, const f
             return ctx->fail(FDG_EINVAL, "unknown route");
         if (tables != FDG_TABLES_PER_FRAME && tables != FDG_TABLES_SHARED)
             return ctx->fail(FDG_EINVAL, "unknown table mode");
-        if (p->max_iter >= (1ull << 31)) return ctx->fail(FDG_EINVAL, "max_iter must be < 2^31");
+        if (p->max_iter >= (1ull << 27)) return ctx->fail(FDG_EINVAL, "max_iter must be < 2^27");
         uint64_t per = (uint64_t)p->nx * p->ny;
         if (per * nframes >= (1ull << 31)) return ctx->fail(FDG_EINVAL, "film too large");
         if (route == FDG_ROUTE_BLA) {

@@ -111,7 +111,7 @@ extern "C" int fdg_render_orbit_noalloc(fdg_ctx* ctx, const fdg_view* v, const f
             return ctx->fail(FDG_EINVAL, "unknown route");
         const uint64_t count64 = (uint64_t)p->nx * p->ny;
         if (count64 >= (1ull << 31)) return ctx->fail(FDG_EINVAL, "grid too large");
-        if (p->max_iter >= (1ull << 31)) return ctx->fail(FDG_EINVAL, "max_iter must be < 2^31");
+        if (p->max_iter >= (1ull << 27)) return ctx->fail(FDG_EINVAL, "max_iter must be < 2^27");
         if (route == FDG_ROUTE_BLA) {
             if (std::isnan(p->escape_radius) || p->escape_radius < 4.0)
                 return ctx->fail(FDG_EINVAL, "escape radius is below 4: BLA needs >= 4");

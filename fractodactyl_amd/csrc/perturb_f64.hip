@@ -481,12 +481,16 @@ __device__ inline FDG_GLOBAL T* global_ptr(T* p) {
 // and double4 classes cannot be copied out of an address-space-qualified lvalue).
 typedef double v2d __attribute__((ext_vector_type(2)));
 typedef double v4d __attribute__((ext_vector_type(4)));
-__device__ inline double2 ld2(const FDG_GLOBAL v2d* p, uint64_t i) {
-    v2d t = p[i];
+// (32-bit element indices.  The orbit stays below 4 GiB (max_iter < 2^27, checked by the
+// host), so its byte offsets are formed in 32 bits: load32.  So do a single frame's
+// tables'; a batch's tables, one per frame in one array, may exceed 4 GiB: table_load.)
+__device__ inline double2 ld2(const FDG_GLOBAL v2d* p, uint32_t i) {
+    v2d t = load32(p, i);
     return make_double2(t.x, t.y);
 }
-__device__ inline double4 ld4(const FDG_GLOBAL v4d* p, uint64_t i) {
-    v4d t = p[i];
+template <bool WIDE>
+__device__ inline double4 ld4(const FDG_GLOBAL v4d* p, uint32_t i) {
+    v4d t = table_load<WIDE>(p, i);
     return make_double4(t.x, t.y, t.z, t.w);
 }
 
@@ -697,7 +701,7 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
     // orbit buffer is padded past its end.)
     double2 pz2 = ld2(h.orbit, s.m + 2);
     double pr0 = 0.0;
-    if (L && (s.m & 1u) == 0 && (s.m >> 1) < h.count0) pr0 = h.r2[tb + (s.m >> 1)];
+    if (L && (s.m & 1u) == 0 && (s.m >> 1) < h.count0) pr0 = table_load<B>(h.r2, tb + (s.m >> 1));
     double2 Zn = make_double2(s.Z1r, s.Z1i);  // a plain step's landing point
     double2 Znext = pz2;
     double r0next = pr0;
@@ -725,7 +729,7 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
                         lvl = find_level_down_uniform(h.r2c, t0, h.stride32, k0, cap, d2);
                 } else {
                     int cap = find_cap(k, h.nlevels, b);
-                    if (cap >= 0) lvl = find_level_down(h.r2, tb, h.stride32, k, cap, d2);
+                    if (cap >= 0) lvl = find_level_down<B>(h.r2, tb, h.stride32, k, cap, d2);
                 }
             }
         }
@@ -742,13 +746,13 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         uint32_t blen = 1;
         if (blk) {
             uint32_t at = bla_block_index(tb, h.stride32, s.m - 1, lvl);
-            double4 ab = ld4(h.ab, at);
+            double4 ab = ld4<B>(h.ab, at);
             const double a_r = ab.x, a_i = ab.y, b_r = ab.z, b_i = ab.w;
             blen = 2u << lvl;
             Zn = ld2(h.orbit, s.m + blen);
             Znext = ld2(h.orbit, s.m + blen + 1);
             uint32_t i0 = (s.m + blen - 1) >> 1;
-            r0next = i0 < h.count0 ? h.r2[tb + i0] : 0.0;
+            r0next = i0 < h.count0 ? table_load<B>(h.r2, tb + i0) : 0.0;
             mr = a_r;
             mi = a_i;
             if (D) {
@@ -782,13 +786,13 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         s.plain += blk ? 0u : 1u;
     } else if (lvl >= 0) {
         uint32_t at = bla_block_index(tb, h.stride32, s.m - 1, lvl);
-        double4 ab = ld4(h.ab, at);
+        double4 ab = ld4<B>(h.ab, at);
         double a_r = ab.x, a_i = ab.y, b_r = ab.z, b_i = ab.w;
         uint32_t blen = 2u << lvl;
         Zn = ld2(h.orbit, s.m + blen);
         Znext = ld2(h.orbit, s.m + blen + 1);
         uint32_t i0 = (s.m + blen - 1) >> 1;
-        r0next = L && i0 < h.count0 ? h.r2[tb + i0] : 0.0;
+        r0next = L && i0 < h.count0 ? table_load<B>(h.r2, tb + i0) : 0.0;
         if (s.nucleus && s.n >= 1) {
             bool st = s.flags & kConStarted;
             con_multiply(r.conr, r.coni, r.cone, st, a_r, a_i);
@@ -986,9 +990,7 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
     __shared__ uint32_t series_carry[SER ? kBlock : 1];
     // Statistics accumulate in LDS at each sample's end (rare next to the steps), not in
     // per-lane registers held across the loop: per-sample counts in the lane's own slots,
-    // folded into the block's counters `stat` at the end.
-    __shared__ unsigned long long stat[ST_COUNT];
-    if (threadIdx.x < ST_COUNT) stat[threadIdx.x] = 0;
+    // folded into the block's counters at the end.
     __shared__ unsigned long long lane_slots[kLaneStats * kBlock];
     LaneStats<kBlock> ls{lane_slots};
     ls.clear();
@@ -1214,6 +1216,12 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
             }
         } while (!__ballot(!active) || (drained && __ballot(active)));
     }
+    // The block's counters take the LDS of the lanes' rare state, which no lane reads
+    // after the barrier.
+    __syncthreads();
+    unsigned long long* stat = reinterpret_cast<unsigned long long*>(rare.qr);
+    if (threadIdx.x < ST_COUNT) stat[threadIdx.x] = 0;
+    __syncthreads();
     ls.fold(stat);
     if (lane == 0) {
         atomicAdd(&stat[ST_WAVE_ITERS], witers);

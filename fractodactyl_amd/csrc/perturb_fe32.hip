@@ -242,7 +242,7 @@ __device__ inline int advance32(Lane32& s, const Hot32& h, int32_t aexp, float& 
                         lvl = find_level_down_uniform(h.r2c, 0u, (uint32_t)h.stride, k0, cap, d2);
                 } else {
                     int cap = find_cap(k, h.nlevels, b);
-                    if (cap >= 0) lvl = find_level_down(h.r2, 0u, (uint32_t)h.stride, k, cap, d2);
+                    if (cap >= 0) lvl = find_level_down<false>(h.r2, 0u, (uint32_t)h.stride, k, cap, d2);
                 }
             }
         }

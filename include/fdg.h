@@ -69,7 +69,7 @@ typedef struct {
  * parameters have no meaning here. */
 typedef struct {
     uint32_t nx, ny, ss;
-    uint64_t max_iter;
+    uint64_t max_iter;  /* below 2^27: orbit and table offsets are formed in 32 bits */
     double escape_radius;
     uint32_t columns;
     int32_t tier;
