@@ -334,7 +334,8 @@ int launch_table(fdg_ctx* ctx, Buffers* bf, uint64_t len, double eps, double dc_
     *built = false;
     if (steps < 2) return FDG_OK;
     FDG_TRY(ctx, B.bla_r.ensure(stride * 8), "hipMalloc");
-    FDG_TRY(ctx, B.bla_r2.ensure(stride * 8), "hipMalloc");
+    // (one entry past the table: the level search's untaken loads may land there)
+    FDG_TRY(ctx, B.bla_r2.ensure((stride + 1) * 8), "hipMalloc");
     FDG_TRY(ctx, B.bla_ab.ensure(stride * 32), "hipMalloc");
     FDG_TRY(ctx, B.bla_alpha.ensure(stride * 8), "hipMalloc");
     FDG_TRY(ctx, B.bla_beta.ensure(stride * 8), "hipMalloc");

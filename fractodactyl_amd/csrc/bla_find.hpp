@@ -33,10 +33,12 @@ constexpr int kFindGroup = 4;
 // where 2^(i+1) divides k (level_top = min(trailing zeros of k, stored levels); k = 0
 // allows every level), is at most `budget` steps long (2 << i <= budget, budget >= 2), and
 // exists below the table's end (checked by the loads).  -1 when no level qualifies.
+// (Tables hold at most 26 levels, below the 31 trailing zeros that k = 0 is given here;
+// budget | 1 gives 0 and 1 the same floor(log2) of 0, hence -1.)
 __device__ inline int find_cap(uint32_t k, uint32_t nlevels, uint32_t budget) {
-    int top = k == 0 ? (int)nlevels : __builtin_ctz(k);
-    top = top < (int)nlevels ? top : (int)nlevels;
-    int by_budget = budget >= 2u ? 30 - __builtin_clz(budget) : -1;
+    uint32_t tz = (uint32_t)__builtin_ctz(k | 0x80000000u);
+    int top = (int)(tz < nlevels ? tz : nlevels);
+    int by_budget = 30 - __builtin_clz(budget | 1u);
     int cap = top - 1;
     return cap < by_budget ? cap : by_budget;
 }
@@ -54,6 +56,9 @@ __device__ inline int find_cap(uint32_t k, uint32_t nlevels, uint32_t budget) {
 //   0 (the builder writes it), so a block past the table's end simply fails the test.  A
 //   lookup happens at m <= len - 2 (k <= steps - 1), so k >> (i + 1) stays inside level
 //   i's slot of stride >> (i + 1) entries.  Indices are 32-bit (tables below 2^32 blocks).
+//   The last group of a search may reach below level 1: those loads are not taken, and
+//   their shift counts (the level's low five bits) land them on a level-0 block or on
+//   entry tb + stride, one past the table, which every r2 buffer allocates.
 //   Every level searched divides k by 2^(i+1), so block k >> (i + 1) of level i sits at
 //   tb + stride - (stride >> i) + (k >> (i + 1)) = (tb + stride) - ((stride - k/2) >> i):
 //   one shift and one subtraction per level.
@@ -62,7 +67,8 @@ __device__ inline uint32_t bla_block_index(uint32_t tb, uint32_t stride, uint32_
 }
 
 template <bool WIDE>
-__device__ inline int find_level_down(const __attribute__((address_space(1))) double* r2, uint32_t tb, uint32_t stride, uint32_t k, int cap,
+__device__ inline int find_level_down(const __attribute__((address_space(1))) double* r2,
+                                      uint32_t tb, uint32_t stride, uint32_t k, int cap,
                                       double d2) {
     const uint32_t top = tb + stride, span = stride - (k >> 1);
     int lvl = 0;
@@ -70,9 +76,8 @@ __device__ inline int find_level_down(const __attribute__((address_space(1))) do
         double rr[kFindGroup];
 #pragma unroll
         for (int q = 0; q < kFindGroup; q++) {
-            int lv = hi - q;
-            int lc = lv >= 1 ? lv : 1;  // (levels below 1 are loaded at level 1, not taken)
-            rr[q] = table_load<WIDE>(r2, top - (span >> lc));
+            uint32_t lv = (uint32_t)(hi - q) & 31u;  // (levels below 1: not taken)
+            rr[q] = table_load<WIDE>(r2, top - (span >> lv));
         }
         int got = -1;
 #pragma unroll

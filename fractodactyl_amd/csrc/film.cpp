@@ -144,7 +144,8 @@ int build_orbit_tables(fdg_film* f, int o, std::vector<uint32_t>* valid_base,
         }
         f->tb[k] = t * (uint32_t)ob.stride;
     }
-    FDG_TRY(ctx, ob.r2.ensure(ntab * ob.stride * 8), "hipMalloc tables");
+    // (one entry past the last table: the level search's untaken loads may land there)
+    FDG_TRY(ctx, ob.r2.ensure((ntab * ob.stride + 1) * 8), "hipMalloc tables");
     FDG_TRY(ctx, ob.ab.ensure(ntab * ob.stride * 32), "hipMalloc tables");
     Buffers& B = f->buf;
     for (uint32_t t = 0; t < ntab; t++) {

@@ -29,7 +29,7 @@ constexpr uint64_t kOffStageRe = kOffValid + 256;              // the caller's o
 constexpr uint64_t kOffStageIm = kOffStageRe + align256(8 * kMaxPoints);
 constexpr uint64_t kOffOrbit = kOffStageIm + align256(8 * kMaxPoints);  // (re, im) + 4 zero points
 constexpr uint64_t kOffR2 = kOffOrbit + align256(16 * (kMaxPoints + 4));
-constexpr uint64_t kOffAb = kOffR2 + 8 * kMaxStride;
+constexpr uint64_t kOffAb = kOffR2 + 8 * kMaxStride;  // (r2's entry past the end lies in ab)
 constexpr uint64_t kOffR = kOffAb + 32 * kMaxStride;
 constexpr uint64_t kOffAlpha = kOffR + 8 * kMaxStride;
 constexpr uint64_t kOffBeta = kOffAlpha + 8 * kMaxStride;

@@ -425,6 +425,7 @@ struct Lane {
     uint32_t m, n, sn, hold, chk, nucleus, blocks, skipped, plain;
     uint32_t flags;                     // kPending | kPredicted | kConStarted | returns << 4
     uint32_t bend;                      // where a lookup's step budget ends (budget_end)
+    uint32_t ev;                        // ordinary tiers: the next Brent event (event_at)
     uint32_t tb;                        // batches: first block of this frame's BLA table
     int32_t oe;                         // batches: this frame's offset exponent h_e
 };
@@ -450,6 +451,13 @@ __device__ inline uint32_t budget_end(const Lane& s, uint32_t max_iter) {
     b = b < max_iter ? b : max_iter;
     uint32_t nb = s.nucleus ? s.nucleus + 1 : 0xffffffffu;
     return b < nb ? b : nb;
+}
+
+// Ordinary tiers: the first iterate at which a hold can expire (sn + hold), a save be
+// deferred or made (chk; 0 while a deferred save is pending): before it the updates after
+// a step that met no near-return change nothing.  Recomputed with budget_end.
+__device__ inline uint32_t event_at(const Lane& s) {
+    return s.hold > 0 ? s.sn + s.hold : ((s.flags & kPending) ? 0u : s.chk);
 }
 
 __device__ inline uint32_t np2_above(uint32_t n) {
@@ -547,6 +555,7 @@ __device__ inline void lane_start(Lane& s, Rare& r, const Hot& h, uint32_t nucle
     r.coni = 0.0;
     r.cone = 0;
     s.bend = budget_end<EXT>(s, h.max_iter);
+    s.ev = event_at(s);
 }
 
 // Re-derive dc in the delta's units after its exponent changed (sample.rs:300-302).
@@ -608,6 +617,7 @@ __device__ inline void series_start(Lane& s, const Hot& h, const FrameDev* __res
     s.chk = chk > 16u ? chk : 16u;
     s.nucleus = 0;
     s.bend = budget_end<false>(s, h.max_iter);
+    s.ev = event_at(s);
 }
 
 enum : int { RUNNING = -1 };
@@ -743,12 +753,12 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
         const double xr = s.xr, xi = s.xi;
         const double fr = Zmr + xr, fi = Zmi + xi;  // z = Z_m + delta (plain step)
         double mr, mi;                              // the step's multiplier
-        uint32_t blen = 1;
+        const bool contract = s.nucleus && s.n >= 1;  // (n before the step)
         if (blk) {
             uint32_t at = bla_block_index(tb, h.stride32, s.m - 1, lvl);
             double4 ab = ld4<B>(h.ab, at);
             const double a_r = ab.x, a_i = ab.y, b_r = ab.z, b_i = ab.w;
-            blen = 2u << lvl;
+            const uint32_t blen = 2u << lvl;
             Zn = ld2(h.orbit, s.m + blen);
             Znext = ld2(h.orbit, s.m + blen + 1);
             uint32_t i0 = (s.m + blen - 1) >> 1;
@@ -762,6 +772,10 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
             }
             s.xr = a_r * xr - a_i * xi + b_r * s.dc_r - b_i * s.dc_i;
             s.xi = a_r * xi + a_i * xr + b_r * s.dc_i + b_i * s.dc_r;
+            s.m += blen;
+            s.n += blen;
+            s.blocks += 1;
+            s.skipped += blen;
         } else {
             mr = 2.0 * fr;
             mi = 2.0 * fi;
@@ -772,18 +786,16 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
             }
             s.xr = 2.0 * (Zmr * xr - Zmi * xi) + xr * xr - xi * xi + s.dc_r;
             s.xi = 2.0 * (Zmr * xi + Zmi * xr + xr * xi) + s.dc_i;
+            s.m += 1;
+            s.n += 1;
+            s.plain += 1;
         }
-        if (s.nucleus && s.n >= 1) {
+        if (contract) {
             bool st = s.flags & kConStarted;
             con_multiply(r.conr, r.coni, r.cone, st, mr, mi);
             s.flags |= kConStarted;
         }
         if (s.hold > 0) mu_multiply(r.mur, r.mui, r.mue, mr, mi);
-        s.m += blen;
-        s.n += blen;
-        s.blocks += blk ? 1u : 0u;
-        s.skipped += blk ? blen : 0u;
-        s.plain += blk ? 0u : 1u;
     } else if (lvl >= 0) {
         uint32_t at = bla_block_index(tb, h.stride32, s.m - 1, lvl);
         double4 ab = ld4<B>(h.ab, at);
@@ -943,30 +955,35 @@ __device__ inline int lane_advance(Lane& s, Rare& r, const Hot& h, double& ofr, 
     double size = fabs(s.sr) + fabs(s.si);
     const bool near = res && dist < 1e-3 * size;
     if (near && near_return(s, r, fr, fi, size) && out == RUNNING) out = 1;  // Interior
-    // A lane that met a near-return now holds from this iterate (hold = its period >= 1,
-    // sn = n): none of the updates below applies to it.  (Sequential rather than if/else,
-    // so each branch updates the state in place.)
-    uint32_t h0 = s.hold;
-    bool expired = !near && h0 > 0 && s.n >= s.sn + h0;
-    if (expired) {
-        s.hold = 0;
-        s.flags = (s.flags & ~kPredicted) | kPending;
+    // Otherwise a hold's expiry and the saves happen only from n = ev on (a lane that met a
+    // near-return holds from this iterate).  (Sequential rather than if/else, so each
+    // branch updates the state in place.)
+    const bool upd = !near && s.n >= s.ev;
+    if (upd) {
+        uint32_t h0 = s.hold;
+        bool expired = h0 > 0 && s.n >= s.sn + h0;
+        if (expired) {
+            s.hold = 0;
+            s.flags = (s.flags & ~kPredicted) | kPending;
+        }
+        // !res: a save that falls on an unjudgeable iterate is deferred (pending)
+        bool defer = !res && h0 == 0 && s.n >= s.chk;
+        s.flags |= defer ? kPending : 0u;
+        // res: save at the deferred or scheduled point unless a return is being awaited
+        bool save = res && (h0 == 0 || expired) && ((s.flags & kPending) || s.n >= s.chk);
+        if (save) {
+            uint32_t np = np2_above(s.n);
+            s.chk = s.chk > np ? s.chk : np;
+            s.sr = fr;
+            s.si = fi;
+            s.sn = s.n;
+            s.flags &= ~kPending;
+        }
     }
-    // !res: a save that falls on an unjudgeable iterate is deferred (pending)
-    bool defer = !res && h0 == 0 && s.n >= s.chk;
-    bool newly = defer && !(s.flags & kPending);
-    s.flags |= defer ? kPending : 0u;
-    // res: save at the deferred or scheduled point unless a return is being awaited
-    bool save = res && (h0 == 0 || expired) && ((s.flags & kPending) || s.n >= s.chk);
-    if (save) {
-        uint32_t np = np2_above(s.n);
-        s.chk = s.chk > np ? s.chk : np;
-        s.sr = fr;
-        s.si = fi;
-        s.sn = s.n;
-        s.flags &= ~kPending;
+    if (near || upd) {
+        s.bend = budget_end<false>(s, h.max_iter);
+        s.ev = event_at(s);
     }
-    if (near || expired || newly || save) s.bend = budget_end<false>(s, h.max_iter);
     return out;
 }
 
