@@ -114,17 +114,20 @@ def measure(fa, view, params, route, steps, warmup, label, form=None, keep=None)
     log(f"{label}: frame open (host reference orbit + upload) {time.perf_counter() - t:.2f}s")
     launch = None
     if LAUNCH not in ("auto", "default"):
-        g, d, c = (int(x) for x in LAUNCH.split(","))
-        frame.launch(g, d, c)
-        launch = dict(chosen=dict(groups_per_cu=g, claim_div=d, claim_cap=c), given=True)
+        g, d, c, *rf = (int(x) for x in LAUNCH.split(","))
+        rf = rf[0] if rf else 0
+        frame.launch(g, d, c, rf)
+        launch = dict(chosen=dict(groups_per_cu=g, claim_div=d, claim_cap=c, refill=rf),
+                      given=True)
     if LAUNCH == "auto":
         t = time.perf_counter()
         chosen, tuned_ms, default_ms = frame.tune(reps=2)
         launch = dict(chosen=chosen, tuned_kernel_ms=tuned_ms, default_kernel_ms=default_ms,
                       tuning_seconds=time.perf_counter() - t,
                       candidates="groups/CU {4,8,16} x claim divisor {2,4,8} x claim cap "
-                                 "{256,1024,4096}, plus the default (8, 4, 1024); min of 2 "
-                                 "runs each")
+                                 "{256,1024,4096} with whole-wave refills, plus the default "
+                                 "(8, 4, 1024, 64), then refills at 1 or 16 idle lanes under "
+                                 "the best; min of 2 runs each")
         log(f"{label}: tuned launch {chosen} {tuned_ms:.3f} ms (default {default_ms:.3f} ms)")
     for i in range(warmup):
         st = frame.run()
@@ -599,8 +602,8 @@ def main():
                     help="with companions only: also run the FE32 and mixed forms")
     ap.add_argument("--launch", default="auto",
                     help="launch of the frames' perturbation kernel: auto (the autotuner, "
-                         "untimed, per frame), default, or G,D,C = groups per CU, claim "
-                         "divisor, claim cap (for profiling a chosen launch)")
+                         "untimed, per frame), default, or G,D,C[,R] = groups per CU, claim "
+                         "divisor, claim cap, refill (for profiling a chosen launch)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device copy (the traffic counters' calibration)")
     a = ap.parse_args()

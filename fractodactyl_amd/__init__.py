@@ -91,7 +91,7 @@ class _EscapedState(C.Structure):
 
 class _Launch(C.Structure):
     _fields_ = [("groups_per_cu", C.c_uint32), ("claim_div", C.c_uint32),
-                ("claim_cap", C.c_uint32)]
+                ("claim_cap", C.c_uint32), ("refill", C.c_uint32)]
 
 
 class _Stats(C.Structure):
@@ -406,10 +406,10 @@ class ResidentFrame:
             raise FdgError(-1, "sample mask must cover the sample grid")
         self.ctx._check(lib().fdg_frame_mask(self._h, _p(self._mask)))
 
-    def launch(self, groups_per_cu=0, claim_div=0, claim_cap=0):
+    def launch(self, groups_per_cu=0, claim_div=0, claim_cap=0, refill=0):
         """Launch tuning of the following runs (fdg_frame_launch; 0 = the default)."""
-        self.ctx._check(lib().fdg_frame_launch(self._h, C.byref(_Launch(groups_per_cu,
-                                                                        claim_div, claim_cap))))
+        self.ctx._check(lib().fdg_frame_launch(self._h, C.byref(_Launch(
+            groups_per_cu, claim_div, claim_cap, refill))))
 
     def tune(self, reps=2):
         """The autotuner (fdg_frame_tune): times each candidate launch over `reps` runs and

@@ -467,6 +467,7 @@ int check_launch(fdg_ctx* ctx, const fdg_launch& l) {
     // claim per wave after it is exhausted: 2^15 waves x 2^14 stays far from wrapping)
     if (l.claim_cap != 0 && (l.claim_cap % 64 != 0 || l.claim_cap > 16384))
         return ctx->fail(FDG_EINVAL, "claim_cap must be a multiple of 64 in [64, 16384]");
+    if (l.refill > 64) return ctx->fail(FDG_EINVAL, "refill must be in [1, 64]");
     return FDG_OK;
 }
 
@@ -911,6 +912,7 @@ int fdg_frame_run(fdg_frame* f, fdg_stats* st) {
         if (grid < 1) grid = 1;
         d.claim_div = f->launch.claim_div;
         d.claim_cap = f->launch.claim_cap;
+        d.refill = f->launch.refill;
         static_assert(sizeof(FrameDev) <= 4096, "kernel descriptor slot");
         // counters: words 0-7 the work index, 8-23 the statistics (d.stats), word 24 (bytes
         // 192-195) the table's level count for the host
@@ -1226,21 +1228,25 @@ int fdg_frame_tune(fdg_frame* f, uint32_t reps, fdg_launch* best, double* best_m
         fdg_launch win = dflt;
         double wms = dms;
         static const uint32_t per_cu[] = {4, 8, 16}, divs[] = {2, 4, 8},
-                              caps[] = {256, 1024, 4096};
+                              caps[] = {256, 1024, 4096}, refills[] = {1, 16};
+        // a candidate replaces the default only when it is 1% faster, so that timing noise
+        // alone never moves the choice away from the default
+        auto consider = [&](const fdg_launch& l) {
+            double ms = 0.0;
+            rc = time_of(l, &ms);
+            if (!rc && ms < (win.groups_per_cu == 0 ? 0.99 * wms : wms)) {
+                win = l;
+                wms = ms;
+            }
+        };
         for (uint32_t pc : per_cu)
             for (uint32_t dv : divs)
-                for (uint32_t cp : caps) {
-                    if (rc) break;
-                    const fdg_launch l{pc, dv, cp};
-                    double ms = 0.0;
-                    rc = time_of(l, &ms);
-                    // a candidate replaces the default only when it is 1% faster, so that
-                    // timing noise alone never moves the choice away from the default
-                    if (!rc && ms < (win.groups_per_cu == 0 ? 0.99 * wms : wms)) {
-                        win = l;
-                        wms = ms;
-                    }
-                }
+                for (uint32_t cp : caps)
+                    if (!rc) consider(fdg_launch{pc, dv, cp, 0});
+        // then partial refills (lanes refilled as they free up) under the best claim policy
+        const fdg_launch base = win;
+        for (uint32_t rf : refills)
+            if (!rc) consider(fdg_launch{base.groups_per_cu, base.claim_div, base.claim_cap, rf});
         if (rc) {
             f->launch = before;
             return rc;

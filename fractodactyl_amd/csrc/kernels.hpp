@@ -101,9 +101,22 @@ struct FrameDev {
     const BatchFrameDev* frames;    // nframes entries
     uint32_t nframes, per_frame;    // count = nframes * per_frame
     // claim policy (0 = default): a wave's claim is the work left / (claim_div x waves),
-    // at most claim_cap samples (a multiple of 64), at least its idle lanes
-    uint32_t claim_div, claim_cap;
+    // at most claim_cap samples (a multiple of 64), at least its idle lanes; a wave refills
+    // once `refill` (1-64, default 64) of its lanes are idle
+    uint32_t claim_div, claim_cap, refill;
 };
+
+// The sample (row-major index in an nx x ny grid) that claimed index `my` stands for.
+// Claims hand out whole groups of 64 indices; a group is an 8 x 8 tile of the grid (tiles in
+// row-major order) when both sides are multiples of 8, else 64 consecutive samples of the
+// row-major scan.  A wave refilled all at once then starts its lanes on neighbouring
+// samples, whose orbits stay close (the same blocks at the same orbit indices) for most of
+// their iterations.
+__device__ inline uint32_t sample_of(uint32_t my, uint32_t nx, uint32_t ny) {
+    if ((nx | ny) & 7u) return my;
+    const uint32_t t = my >> 6, q = my & 63u, tiles_x = nx >> 3;
+    return ((t / tiles_x) * 8u + (q >> 3)) * nx + (t % tiles_x) * 8u + (q & 7u);
+}
 
 // One frame pass's outcomes and the columns the device encoder writes from them
 // (Store::put, store.rs:66-153); nu / de / normal are null when not requested.

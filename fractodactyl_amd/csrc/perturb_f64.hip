@@ -1067,6 +1067,9 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
     // claim policy (the launch's tuning; 0 = the defaults)
     const uint32_t cdiv = fp->claim_div ? fp->claim_div : 4u;
     const uint32_t ccap = fp->claim_cap ? fp->claim_cap : 1024u;
+    // Refilling only once every lane is idle keeps a wave's 64 samples (a tile) in lockstep:
+    // the same lookups (through the scalar unit) and the same branches, most of their way.
+    const uint32_t refill = fp->refill ? fp->refill : 64u;
     for (;;) {
         uint64_t idle = __ballot(!active);
         if (idle && !drained) {
@@ -1106,10 +1109,13 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
             if (!active && have) {
                 // batches: the mask (when set) covers every frame's samples, frame-major
                 uint32_t bf = B ? my / fp->per_frame : 0u;
-                bool bmask = B && fp->mask && !fp->mask[fp->frames[bf].out0 + (my - bf * fp->per_frame)];
+                uint32_t loc = B ? sample_of(my - bf * fp->per_frame, nx, fp->ny) : 0u;
+                bool bmask = B && fp->mask && !fp->mask[fp->frames[bf].out0 + loc];
+                // single frames: the list entry, or the sample the claimed index stands for
+                uint32_t one = !B ? (LIST ? fp->list[my] : sample_of(my, nx, fp->ny)) : 0u;
                 if (B && !bmask) {
                     took = true;
-                    uint32_t f = bf, loc = my - f * fp->per_frame;
+                    uint32_t f = bf;
                     const BatchFrameDev& g = fp->frames[f];
                     idx = g.out0 + loc;
                     uint32_t i = loc % nx, j = loc / nx;
@@ -1135,9 +1141,9 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
                         s.oe = (int32_t)g.h_e;
                         active = true;
                     }
-                } else if (!B && (!fp->mask || fp->mask[my])) {
+                } else if (!B && (LIST || !fp->mask || fp->mask[one])) {
                     took = true;
-                    idx = LIST ? fp->list[my] : my;
+                    idx = one;
                     uint32_t i = idx % nx, j = idx / nx;
                     double x = (double)i + 0.5 - fp->half_x;
                     double y = (double)j + 0.5 - fp->half_y;
@@ -1175,11 +1181,11 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
             }
             if (!LIST) finished += __popcll(__ballot(took));
         }
-        if (!__ballot(active)) {
-            if (drained) break;
-            continue;
-        }
-        // Step until some lane finishes (then refill) or, once drained, all are done.
+        // Fill the wave before stepping: a masked-out index or a closed-form interior
+        // leaves its lane idle, and the steps below refill only at `refill` idle lanes.
+        if (__ballot(!active) && !drained) continue;
+        if (!__ballot(active)) break;  // (drained)
+        // Step until `refill` lanes are idle (then refill) or, once drained, all are done.
         do {
             witers += 1;
             liters += __popcll(__ballot(active));
@@ -1231,7 +1237,7 @@ __global__ void __launch_bounds__(kBlock, 4) perturb_f64(const FrameDev* __restr
                     active = false;
                 }
             }
-        } while (!__ballot(!active) || (drained && __ballot(active)));
+        } while (__popcll(__ballot(!active)) < refill || (drained && __ballot(active)));
     }
     // The block's counters take the LDS of the lanes' rare state, which no lane reads
     // after the barrier.

@@ -539,6 +539,8 @@ __global__ void __launch_bounds__(kBlock32, 4) perturb_fe32(const FrameDev* __re
     // claim policy (the launch's tuning; 0 = the defaults)
     const uint32_t cdiv = fp->claim_div ? fp->claim_div : 4u;
     const uint32_t ccap = fp->claim_cap ? fp->claim_cap : 1024u;
+    // (whole-wave refills by default: a wave's tile of samples stays in lockstep)
+    const uint32_t refill = fp->refill ? fp->refill : 64u;
     for (;;) {
         uint64_t idle = __ballot(!active);
         if (idle && !drained) {
@@ -573,9 +575,10 @@ __global__ void __launch_bounds__(kBlock32, 4) perturb_fe32(const FrameDev* __re
                 ce = ne;
                 if (nb >= count) drained = true;
             }
-            bool took = !active && have && (!fp->mask || fp->mask[my]);
+            const uint32_t one = sample_of(my, nx, fp->ny);
+            bool took = !active && have && (!fp->mask || fp->mask[one]);
             if (took) {
-                idx = my;
+                idx = one;
                 uint32_t i = idx % nx, j = idx / nx;
                 double x = (double)i + 0.5 - fp->half_x;
                 double y = (double)j + 0.5 - fp->half_y;
@@ -596,10 +599,10 @@ __global__ void __launch_bounds__(kBlock32, 4) perturb_fe32(const FrameDev* __re
             }
             finished += __popcll(__ballot(took));
         }
-        if (!__ballot(active)) {
-            if (drained) break;
-            continue;
-        }
+        // Fill the wave before stepping: a masked-out index or a closed-form interior
+        // leaves its lane idle, and the steps below refill only at `refill` idle lanes.
+        if (__ballot(!active) && !drained) continue;
+        if (!__ballot(active)) break;  // (drained)
         do {
             witers += 1;
             liters += __popcll(__ballot(active));
@@ -638,7 +641,7 @@ __global__ void __launch_bounds__(kBlock32, 4) perturb_fe32(const FrameDev* __re
                     active = false;
                 }
             }
-        } while (!__ballot(!active) || (drained && __ballot(active)));
+        } while (__popcll(__ballot(!active)) < refill || (drained && __ballot(active)));
     }
     ls.fold(stat);
     if (lane == 0) {

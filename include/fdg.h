@@ -198,6 +198,9 @@ typedef struct fdg_launch {
                                default 4 */
     uint32_t claim_cap;     /* at most this many samples per claim, a multiple of 64 in
                                [64, 16384]; default 1024 */
+    uint32_t refill;        /* a wave takes new samples once this many of its 64 lanes are
+                               idle, 1-64; default 64 (whole waves: a wave's samples, an
+                               8 x 8 tile of the grid, run in lockstep) */
 } fdg_launch;
 
 /* Set the launch tuning of the frame's following runs (NULL: the defaults). */

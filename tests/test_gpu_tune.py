@@ -74,7 +74,8 @@ def test_tuner_keeps_a_launch_no_slower_than_the_default():
 @pytest.mark.parametrize("launch,what", [((33, 0, 0), "groups_per_cu"),
                                          ((0, 65, 0), "claim_div"),
                                          ((0, 0, 100), "claim_cap"),
-                                         ((0, 0, 16448), "claim_cap")])
+                                         ((0, 0, 16448), "claim_cap"),
+                                         ((0, 0, 0, 65), "refill")])
 def test_bad_launches_are_refused(launch, what):
     v, p = frame()
     with fa.ResidentFrame(v, fa.Params(96, 54, 1, 1000, 1e10, COLS), fa.ROUTE_PLAIN) as f:
